@@ -1,0 +1,335 @@
+// GPUAlgebra for ugcore on MI355X (gfx950): the C++ host classes that sit
+// above the ugx_* C ABI (include/ugcore_amd.h).
+//
+// They mirror the reference's algebra and operator interfaces so the bridge
+// can register them beside CPUAlgebra under the "GPU1" suffix:
+//   GPUAlgebra                  lib_algebra/cpu_algebra_types.h:76-121 (CPUAlgebra / commented GPUAlgebra)
+//   GPUVector                   lib_algebra/cpu_algebra/vector.h:90-176
+//   GPUSparseMatrix             lib_algebra/cpu_algebra/sparsematrix.h:98-752
+//   ILinearOperator             lib_algebra/operator/interface/linear_operator.h:78-143
+//   ILinearIterator             lib_algebra/operator/interface/linear_iterator.h:79-198
+//   MatrixOperator              lib_algebra/operator/interface/matrix_operator.h:47
+//   StdConvCheck                lib_algebra/operator/convergence_check.h:178-262
+//   Jacobi                      lib_algebra/operator/preconditioner/jacobi.h:89-311
+//   CG, BiCGStab                lib_algebra/operator/linear_solver/cg.h:64-304, bicgstab.h:112-385
+//   AssembledMultiGridCycle     lib_disc/operator/linear_operator/multi_grid_solver/mg_solver.h:82-590
+//   DomainDiscretization        lib_disc/spatial_disc/domain_disc.h:77,649 (structured device assembly)
+//
+// Device memory is owned by the objects (hipMalloc at set-up, never inside
+// an iteration); all work is ordered on one HIP stream per process.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ug {
+namespace gpu {
+
+struct UGError : public std::runtime_error {
+    explicit UGError(const std::string& m) : std::runtime_error(m) {}
+};
+
+// ---------------------------------------------------------------------------
+// device context and buffers
+// ---------------------------------------------------------------------------
+struct DeviceContext {
+    hipStream_t stream = nullptr;
+    double* red_partial = nullptr;   // 1024 partial sums
+    double* red_out = nullptr;       // 8 results
+    double* red_host = nullptr;      // pinned host mirror of red_out
+    int* flag = nullptr;             // device error flag
+};
+DeviceContext& context();            // initialises the device on first use
+void sync();
+
+template <typename T>
+class DeviceBuffer {
+   public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(int64_t n) { resize(n); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void resize(int64_t n);
+    void release();
+    T* get() const { return p_; }
+    int64_t size() const { return n_; }
+    void upload(const T* h, int64_t n, int64_t offset = 0);
+    void download(T* h, int64_t n, int64_t offset = 0) const;
+    void zero();
+
+   private:
+    T* p_ = nullptr;
+    int64_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// algebra types
+// ---------------------------------------------------------------------------
+class GPUVector {
+   public:
+    typedef double value_type;
+    GPUVector() = default;
+    explicit GPUVector(int64_t n) { resize(n); }
+    GPUVector(const GPUVector& o);
+    GPUVector& operator=(const GPUVector& o);
+    void resize(int64_t n);
+    int64_t size() const { return n_; }
+    void set(double v);
+    double norm() const;
+    double dotprod(const GPUVector& b) const;
+    GPUVector& operator+=(const GPUVector& b);
+    GPUVector& operator-=(const GPUVector& b);
+    GPUVector& operator*=(double a);
+    std::unique_ptr<GPUVector> clone() const { return std::unique_ptr<GPUVector>(new GPUVector(*this)); }
+    std::unique_ptr<GPUVector> clone_without_values() const { return std::unique_ptr<GPUVector>(new GPUVector(n_)); }
+    double* data() { return buf_.get(); }
+    const double* data() const { return buf_.get(); }
+    void upload(const double* h) { buf_.upload(h, n_); }
+    void download(double* h) const { buf_.download(h, n_); }
+
+   private:
+    int64_t n_ = 0;
+    DeviceBuffer<double> buf_;
+};
+
+// VecScaleAdd (operations_vec.h:55-63, 162-175)
+void VecScaleAdd(GPUVector& dest, double a1, const GPUVector& v1, double a2, const GPUVector& v2);
+void VecScaleAdd(GPUVector& dest, double a1, const GPUVector& v1, double a2, const GPUVector& v2, double a3,
+                 const GPUVector& v3);
+
+class GPUSparseMatrix {
+   public:
+    typedef double value_type;
+    GPUSparseMatrix() = default;
+    int64_t num_rows() const { return rows_; }
+    int64_t num_cols() const { return cols_; }
+    int64_t total_num_connections() const { return nnz_; }
+    // create from host CSR (row pointers int32, columns sorted per row)
+    void set_from_host_csr(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col, const double* val);
+    // allocate the device pattern (values zero)
+    void allocate(int64_t rows, int64_t cols, int64_t nnz);
+    void download_csr(int32_t* rowptr, int32_t* col, double* val) const;
+    // dest = a1*v1 + b1*A*w1 (sparsematrix_impl.h:291-339)
+    void axpy(GPUVector& dest, double a1, const GPUVector& v1, double b1, const GPUVector& w1) const;
+    void apply(GPUVector& dest, const GPUVector& w) const { axpy(dest, 0.0, dest, 1.0, w); }
+    void matmul_minus(GPUVector& dest, const GPUVector& w) const { axpy(dest, 1.0, dest, -1.0, w); }
+    void apply_ignore_zero_rows(GPUVector& dest, double beta, const GPUVector& w) const;
+    int32_t* rowptr() const { return rowptr_.get(); }
+    int32_t* col() const { return col_.get(); }
+    double* val() const { return val_.get(); }
+    int lanes() const { return lanes_; }
+
+   private:
+    int64_t rows_ = 0, cols_ = 0, nnz_ = 0;
+    int lanes_ = 8;
+    DeviceBuffer<int32_t> rowptr_, col_;
+    DeviceBuffer<double> val_;
+};
+
+struct AlgebraType {
+    enum Type { CPU = 0, GPU = 1 };
+    Type type;
+    int blocksize;
+};
+
+struct GPUAlgebra {
+    typedef GPUSparseMatrix matrix_type;
+    typedef GPUVector vector_type;
+    static const int blockSize = 1;
+    static AlgebraType get_type() { return AlgebraType{AlgebraType::GPU, 1}; }
+};
+
+// ---------------------------------------------------------------------------
+// operator interfaces
+// ---------------------------------------------------------------------------
+class ILinearOperator {
+   public:
+    virtual ~ILinearOperator() = default;
+    virtual void init() {}
+    virtual void apply(GPUVector& f, const GPUVector& u) = 0;       // f = L*u
+    virtual void apply_sub(GPUVector& f, const GPUVector& u) = 0;   // f -= L*u
+};
+
+class MatrixOperator : public ILinearOperator {
+   public:
+    explicit MatrixOperator(std::shared_ptr<GPUSparseMatrix> A) : A_(std::move(A)) {}
+    void apply(GPUVector& f, const GPUVector& u) override { A_->apply(f, u); }
+    void apply_sub(GPUVector& f, const GPUVector& u) override { A_->matmul_minus(f, u); }
+    GPUSparseMatrix& get_matrix() { return *A_; }
+    std::shared_ptr<GPUSparseMatrix> matrix_ptr() { return A_; }
+
+   private:
+    std::shared_ptr<GPUSparseMatrix> A_;
+};
+
+class ILinearIterator {
+   public:
+    virtual ~ILinearIterator() = default;
+    virtual const char* name() const = 0;
+    virtual bool init(std::shared_ptr<MatrixOperator> J) = 0;
+    virtual bool apply(GPUVector& c, const GPUVector& d) = 0;    // c = B*d
+    virtual bool apply_update_defect(GPUVector& c, GPUVector& d);  // c = B*d; d -= A*c
+   protected:
+    std::shared_ptr<MatrixOperator> op_;
+};
+
+class StdConvCheck {
+   public:
+    StdConvCheck(int maxSteps = 100, double minDefect = 1e-50, double relReduction = 1e-10, bool verbose = false)
+        : max_steps_(maxSteps), min_defect_(minDefect), rel_reduction_(relReduction), verbose_(verbose) {}
+    void set_maximum_steps(int s) { max_steps_ = s; }
+    void set_minimum_defect(double d) { min_defect_ = d; }
+    void set_reduction(double r) { rel_reduction_ = r; }
+    void set_verbose(bool v) { verbose_ = v; }
+    void start_defect(double d);
+    void start(const GPUVector& d) { start_defect(d.norm()); }
+    void update_defect(double d);
+    void update(const GPUVector& d) { update_defect(d.norm()); }
+    bool iteration_ended() const;
+    bool post() const;
+    int step() const { return step_; }
+    double defect() const { return current_; }
+    double reduction() const { return current_ / initial_; }
+    const std::vector<double>& defects() const { return defects_; }
+    void set_name(const std::string& n) { name_ = n; }
+
+   private:
+    int max_steps_;
+    double min_defect_, rel_reduction_;
+    bool verbose_;
+    double initial_ = 0, current_ = 0, last_ = 0;
+    int step_ = 0;
+    std::vector<double> defects_;
+    std::string name_ = "Iteration";
+};
+
+class Jacobi : public ILinearIterator {
+   public:
+    explicit Jacobi(double omega = 1.0) : omega_(omega) {}
+    const char* name() const override { return "Jacobi"; }
+    bool init(std::shared_ptr<MatrixOperator> J) override;
+    bool apply(GPUVector& c, const GPUVector& d) override;
+    void set_damp(double w) { omega_ = w; }
+
+   private:
+    double omega_;
+    DeviceBuffer<double> dinv_;
+};
+
+class ILinearOperatorInverse {
+   public:
+    virtual ~ILinearOperatorInverse() = default;
+    virtual const char* name() const = 0;
+    virtual bool init(std::shared_ptr<MatrixOperator> J);
+    virtual bool apply(GPUVector& x, const GPUVector& b);
+    virtual bool apply_return_defect(GPUVector& x, GPUVector& b) = 0;
+    void set_preconditioner(std::shared_ptr<ILinearIterator> p) { precond_ = std::move(p); }
+    void set_convergence_check(std::shared_ptr<StdConvCheck> c) { conv_ = std::move(c); }
+    StdConvCheck& convergence_check() { return *conv_; }
+    int step() const { return conv_->step(); }
+
+   protected:
+    std::shared_ptr<MatrixOperator> op_;
+    std::shared_ptr<ILinearIterator> precond_;
+    std::shared_ptr<StdConvCheck> conv_ = std::make_shared<StdConvCheck>();
+};
+
+class CG : public ILinearOperatorInverse {
+   public:
+    const char* name() const override { return "CG"; }
+    bool apply_return_defect(GPUVector& x, GPUVector& b) override;
+};
+
+class BiCGStab : public ILinearOperatorInverse {
+   public:
+    const char* name() const override { return "BiCGStab"; }
+    bool apply_return_defect(GPUVector& x, GPUVector& b) override;
+};
+
+// ---------------------------------------------------------------------------
+// structured discretisation + multigrid
+// ---------------------------------------------------------------------------
+struct LevelGrid {
+    int dim = 3;
+    int n = 0;            // cells per side
+    int W = 0;            // points per side (n+1)
+    int64_t N = 0;        // points (DoFs)
+    int64_t n_inner = 0;  // DoFs [0, n_inner) are not Dirichlet
+    DeviceBuffer<int32_t> dof_of_lat, lat_of_dof;
+    DeviceBuffer<uint8_t> dir;   // Dirichlet flag per lattice point
+};
+
+// Domain of lattice topology: kind 0 = unit cube from one hexahedron (Q1 FE,
+// -div(k grad u) = f), kind 1 = unit square from two triangles (FV1).
+class DomainDiscretization {
+   public:
+    DomainDiscretization(int kind, int num_refs);
+    int kind() const { return kind_; }
+    int dim() const { return kind_ == 0 ? 3 : 2; }
+    int num_levels() const { return (int)levels_.size(); }
+    int top_level() const { return num_levels() - 1; }
+    LevelGrid& level(int l) { return *levels_[l]; }
+    // coefficient: theta per fine element (lattice order), k = exp(theta); empty = 1
+    void set_theta(const double* theta_host);
+    const double* theta_device() const { return has_theta_ ? theta_.get() : nullptr; }
+    void set_source_constant(double f) { f_const_ = f; }
+    void set_source_dofs(const double* f_host);   // FV1: per top-level DoF
+    // assemble the top-level (surface) CSR in DoF numbering and the rhs
+    void assemble_linear(GPUSparseMatrix& A, GPUVector& b);
+    // assemble a level operator in lattice numbering as DIA values (27 or 9
+    // diagonals of length N, FP64), Dirichlet rows set
+    void assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd, std::vector<int32_t>& offsets);
+    double assemble_seconds() const { return t_assemble_; }
+
+   private:
+    int kind_;
+    int refs_;
+    std::vector<std::unique_ptr<LevelGrid>> levels_;
+    DeviceBuffer<double> theta_, f_dof_;
+    bool has_theta_ = false, has_f_ = false;
+    double f_const_ = 1.0;
+    double t_assemble_ = 0.0;
+};
+
+enum class Precision { F64 = 0, F32 = 1 };
+
+class AssembledMultiGridCycle : public ILinearIterator {
+   public:
+    explicit AssembledMultiGridCycle(std::shared_ptr<DomainDiscretization> disc);
+    ~AssembledMultiGridCycle() override;
+    const char* name() const override { return "GMG"; }
+    void set_base_level(int l) { base_lev_ = l; }
+    void set_num_presmooth(int n) { nu1_ = n; }
+    void set_num_postsmooth(int n) { nu2_ = n; }
+    void set_cycle_type(int c) { cycle_ = c; }
+    void set_smoother_damp(double w) { omega_ = w; }
+    void set_precision(Precision p) { prec_ = p; }
+    bool init(std::shared_ptr<MatrixOperator> J) override;
+    bool apply(GPUVector& c, const GPUVector& d) override;
+    // testing access: level matrix (FP64 DIA values in lattice numbering)
+    void download_level_dia(int lev, double* out) const;
+    int level_num_diags(int lev) const;
+    // nsteps fused smoothing steps on level `lev` (timing / inspection)
+    void smooth_level(int lev, int nsteps);
+    struct Impl;
+
+   private:
+    std::shared_ptr<DomainDiscretization> disc_;
+    int base_lev_ = 0, nu1_ = 2, nu2_ = 2, cycle_ = 1;
+    double omega_ = 0.66;
+    Precision prec_ = Precision::F64;
+    Impl* impl_ = nullptr;
+};
+
+}  // namespace gpu
+}  // namespace ug

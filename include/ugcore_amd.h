@@ -1,0 +1,161 @@
+/*
+ * ugcore_amd C ABI: the drop-in boundary of the MI355X discretize-and-solve
+ * path for ugcore. Plain pointers, sizes and opaque handles; no C++ or torch
+ * types. Every function returns 0 (or a non-negative count/handle) on success
+ * and a negative value on failure; ugx_last_error() then describes the error.
+ *
+ * Two layers:
+ *  1. Kernel entry points (device pointers, stream ordered) in the style of the
+ *     reference's C wrappers ugbase/lib_algebra/gpu_algebra/cuda/common_cuda.h:36-49
+ *     (extern "C" bool CUDA_VecAdd2(int len, double a, double* x, double b, const double* y)).
+ *  2. Object entry points (handles) over the GPUAlgebra C++ classes in
+ *     include/ug_gpu/gpu_algebra.hpp, which the ugcore bridge registers as the
+ *     "GPU1" algebra; the Python bindings (ugcore_amd/__init__.py) bind these.
+ *
+ * `stream` arguments may be NULL, meaning the library's own stream.
+ */
+#ifndef UGCORE_AMD_H
+#define UGCORE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------ */
+int ugx_init(void);                                  /* selects/initialises the device */
+const char* ugx_last_error(void);
+void* ugx_stream(void);                              /* the library's hipStream_t */
+int ugx_synchronize(void);
+
+/* ---- device buffers and timing (plumbing for callers without a GPU framework) */
+void* ugx_malloc(int64_t bytes);
+int ugx_free(void* p);
+int ugx_memcpy_htod(void* dst_dev, const void* src_host, int64_t bytes);   /* synchronous */
+int ugx_memcpy_dtoh(void* dst_host, const void* src_dev, int64_t bytes);   /* synchronous */
+void* ugx_event_create(void);
+int ugx_event_destroy(void* ev);
+int ugx_event_record(void* ev, void* stream);
+/* waits for `end` and returns the elapsed milliseconds between the two events */
+int ugx_event_elapsed_ms(void* begin, void* end, double* ms);
+
+/* ---- kernel entry points: CSR (int32 row pointers/columns, sorted rows) ---- */
+
+/* dest = alpha*v1 + beta*A*w. Replaces SparseMatrix::axpy
+ * (lib_algebra/cpu_algebra/sparsematrix_impl.h:291-339) and the dead
+ * cusparseDcsrmv call (lib_algebra/gpu_algebra/gpusparsematrix_impl.h:156-164).
+ * v1 may alias dest; w must not. alpha == 0 ignores v1 (empty rows give 0). */
+int ugx_spmv_csr_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, double alpha,
+                     const double* v1, double beta, const double* w, double* dest, void* stream);
+int ugx_spmv_csr_f32(int64_t nrows, const int32_t* rowptr, const int32_t* col, const float* val, double alpha,
+                     const float* v1, double beta, const float* w, float* dest, void* stream);
+/* dest[i] = beta*(A*w)[i] for rows with entries; empty rows untouched.
+ * Replaces SparseMatrix::apply_ignore_zero_rows (sparsematrix_impl.h:271-288). */
+int ugx_spmv_csr_ignore_zero_rows_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val,
+                                      double beta, const double* w, double* dest, void* stream);
+
+/* ---- kernel entry points: vectors ---------------------------------------- */
+/* VecScaleAdd 2-/3-term (lib_algebra/common/operations_vec.h:55-63, 162-175),
+ * replacing CUDA_VecAdd_2/CUDA_VecAdd_3 (gpu_algebra/cuda/common_cuda.cu:86-100). */
+int ugx_vec_scale_add2_f64(int64_t n, double* dest, double a1, const double* v1, double a2, const double* v2,
+                           void* stream);
+int ugx_vec_scale_add3_f64(int64_t n, double* dest, double a1, const double* v1, double a2, const double* v2,
+                           double a3, const double* v3, void* stream);
+int ugx_vec_set_f64(int64_t n, double* x, double value, void* stream);
+/* Vector::dotprod / norm (cpu_algebra/vector_impl.h:70-79, 323-329), replacing
+ * cublasDdot/cublasDnrm2 (gpu_algebra/gpuvector.h:200-214). Deterministic
+ * two-pass reduction; synchronises and writes the result to host memory. */
+int ugx_dot_f64(int64_t n, const double* a, const double* b, double* result_host, void* stream);
+int ugx_norm_f64(int64_t n, const double* a, double* result_host, void* stream);
+
+/* ---- kernel entry points: Jacobi (lib_algebra/operator/preconditioner/jacobi.h:155-249,
+ * replacing GPUJacobi::preprocess + CUDA_JacobiApply, gpujacobi.h:113-144, common_cuda.cu:46-52) */
+/* dinv_i = 1/(a_ii*(1/omega)); returns -2 if a diagonal entry is zero */
+int ugx_jacobi_diag_inv_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, double omega,
+                            double* dinv, void* stream);
+/* c = dinv .* d */
+int ugx_jacobi_apply_f64(int64_t n, double* c, const double* dinv, const double* d, void* stream);
+
+/* ---- object entry points --------------------------------------------------- */
+
+/* Host-only (no device use): LEVEL DoF numbering of a structured hierarchy as
+ * produced by lib_grid's GlobalMultiGridRefiner + DoFDistribution
+ * (global_multi_grid_refiner.cpp:135-455, dof_distribution.cpp:1401-1421).
+ * out_host receives (2^level+1)^dim entries: lattice point (x fastest) -> DoF.
+ * n_inner receives the number of DoFs of subset "Inner" (the rest are Dirichlet). */
+int ugx_hierarchy_numbering(int kind, int num_refs, int level, int32_t* out_host, int64_t* n_inner);
+
+/* structured domain + DoF distribution + DomainDiscretization
+ * (lib_disc/spatial_disc/domain_disc_impl.h:469-948 for the assembly; the DoF
+ * numbering follows lib_disc/dof_manager/dof_distribution.cpp:1401-1421).
+ * kind 0: unit cube from one hexahedron, Q1 FE for -div(k grad u) = f;
+ * kind 1: unit square from two triangles, FV1 box method. */
+void* ugx_disc_create(int kind, int num_refs);
+void ugx_disc_destroy(void* disc);
+int ugx_disc_num_levels(void* disc);
+int64_t ugx_disc_num_dofs(void* disc, int level);
+int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out_host);   /* (n+1)^dim entries */
+int ugx_disc_set_theta(void* disc, const double* theta_host);            /* k = exp(theta) per fine element */
+int ugx_disc_set_source_const(void* disc, double f);
+int ugx_disc_set_source_dofs(void* disc, const double* f_host);          /* per top-level DoF (FV1) */
+/* DomainDiscretization::assemble_linear(A, b) (domain_disc_impl.h:835-948) incl. Dirichlet rows */
+int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
+double ugx_disc_assemble_seconds(void* disc);
+
+/* GPUSparseMatrix (matrix_type) wrapped as MatrixOperator (matrix_operator.h:47) */
+void* ugx_matrix_create(void);
+void ugx_matrix_destroy(void* m);
+int64_t ugx_matrix_num_rows(void* m);
+int64_t ugx_matrix_nnz(void* m);
+int ugx_matrix_upload_csr(void* m, int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
+                          const double* val);
+int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val);
+int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val);
+int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w);   /* vectors are handles */
+int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
+
+/* GPUVector (vector_type) */
+void* ugx_vector_create(int64_t n);
+void ugx_vector_destroy(void* v);
+int64_t ugx_vector_size(void* v);
+double* ugx_vector_device_ptr(void* v);
+int ugx_vector_upload(void* v, const double* host);
+int ugx_vector_download(void* v, double* host);
+int ugx_vector_set(void* v, double value);
+
+/* AssembledMultiGridCycle (mg_solver_impl.hpp:175-2136): V(nu1,nu2) or W cycle,
+ * damped Jacobi smoother, P1 transfer, dense LU on the base level.
+ * precision: 0 = FP64 level operators, 1 = FP32 level operators (base LU FP64). */
+void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision);
+void ugx_gmg_destroy(void* g);
+int ugx_gmg_init(void* g, void* matrix);
+int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
+/* nsteps fused damped-Jacobi smoothing steps on one level's current defect
+ * (st = dinv.*sd; sd -= A*st; sc += st), as in presmooth_and_restriction
+ * (mg_solver_impl.hpp:1702-1730); used to time the smoother in isolation */
+int ugx_gmg_smooth_level(void* g, int level, int nsteps);
+int ugx_gmg_level_num_diags(void* g, int level);
+int ugx_gmg_level_dia(void* g, int level, double* out_host);  /* ndiag x (n+1)^dim values, lattice numbering */
+
+/* Jacobi as a stand-alone preconditioner on a matrix handle */
+void* ugx_jacobi_create(double omega);
+void ugx_jacobi_destroy(void* j);
+
+/* CG (cg.h:103-242) = 0, BiCGStab (bicgstab.h:112-385) = 1, with StdConvCheck */
+void* ugx_solver_create(int type);
+void ugx_solver_destroy(void* s);
+/* precond_kind: 0 none, 1 GMG handle, 2 Jacobi handle */
+int ugx_solver_set_preconditioner(void* s, int precond_kind, void* precond);
+int ugx_solver_set_conv_check(void* s, int max_steps, double min_defect, double rel_reduction, int verbose);
+int ugx_solver_init(void* s, void* matrix);
+/* returns 1 if converged, 0 if not, <0 on error */
+int ugx_solver_apply(void* s, void* x, void* b);
+int ugx_solver_steps(void* s);
+int ugx_solver_defects(void* s, double* out_host, int max_n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UGCORE_AMD_H */

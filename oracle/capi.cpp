@@ -1,0 +1,308 @@
+// Oracle (test infrastructure only): C entry points used by tests/ (ctypes)
+// and by the CPU-baseline leg of bench.py.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "oracle.hpp"
+
+using namespace orc;
+
+namespace {
+
+struct Problem {
+    MultiGrid mg;
+    int kind = 0;
+    int n_fine = 0;
+    std::vector<double> theta;          // per fine element (lattice order)
+    std::vector<double> f_top;          // FV1 source per top-level DoF (kind 1)
+    std::vector<CSR> A, P, R;
+    std::vector<std::vector<double>> b;
+    std::vector<double> x;
+    SolveResult last;
+    double t_assemble = 0, t_setup = 0, t_solve = 0;
+    std::string err;
+};
+
+thread_local std::string g_err;
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void copy_csr(const CSR& A, int64_t* rowptr, int32_t* col, double* val) {
+    if (rowptr) std::memcpy(rowptr, A.rowptr.data(), sizeof(int64_t) * A.rowptr.size());
+    if (col) std::memcpy(col, A.col.data(), sizeof(int32_t) * A.col.size());
+    if (val) std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+// ---- mesh hierarchy -------------------------------------------------------------
+
+void* orc_problem_create(int kind, int num_refs) {
+    try {
+        auto* p = new Problem();
+        p->kind = kind;
+        p->mg = build_hierarchy(kind, num_refs);
+        p->n_fine = 1 << num_refs;
+        p->A.resize(num_refs + 1);
+        p->P.resize(num_refs + 1);
+        p->R.resize(num_refs + 1);
+        p->b.resize(num_refs + 1);
+        return p;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_problem_destroy(void* h) { delete (Problem*)h; }
+
+int orc_num_levels(void* h) { return (int)((Problem*)h)->mg.levels.size(); }
+
+int64_t orc_num_dofs(void* h, int lev) { return ((Problem*)h)->mg.levels[lev].n_dof; }
+
+int64_t orc_num_elements(void* h, int lev) {
+    auto* p = (Problem*)h;
+    const Level& L = p->mg.levels[lev];
+    return p->mg.dim == 3 ? (int64_t)L.vols.size() : (int64_t)L.faces.size();
+}
+
+// lattice coordinates (units of the finest lattice) per DoF, 3 int64 each
+void orc_dof_lattice(void* h, int lev, int64_t* out) {
+    const Level& L = ((Problem*)h)->mg.levels[lev];
+    for (int d = 0; d < L.n_dof; ++d) {
+        const auto& p = L.vpos[L.vtx_of_dof[d]];
+        out[3 * d] = p[0];
+        out[3 * d + 1] = p[1];
+        out[3 * d + 2] = p[2];
+    }
+}
+
+void orc_dirichlet_mask(void* h, int lev, uint8_t* out) {
+    auto m = dirichlet_mask(((Problem*)h)->mg, lev);
+    std::memcpy(out, m.data(), m.size());
+}
+
+// element -> DoF table in element creation order, local vertex order
+void orc_elements(void* h, int lev, int32_t* out) {
+    auto* p = (Problem*)h;
+    const Level& L = p->mg.levels[lev];
+    if (p->mg.dim == 3) {
+        for (size_t e = 0; e < L.vols.size(); ++e)
+            for (int i = 0; i < 8; ++i) out[8 * e + i] = L.dof[L.vols[e][i]];
+    } else {
+        for (size_t e = 0; e < L.faces.size(); ++e)
+            for (int i = 0; i < 3; ++i) out[3 * e + i] = L.dof[L.faces[e][i]];
+    }
+}
+
+// ---- synthetic data ---------------------------------------------------------------
+
+void orc_make_theta(int n, uint64_t seed, int passes, double* out) {
+    auto t = make_theta(n, seed, passes);
+    std::memcpy(out, t.data(), sizeof(double) * t.size());
+}
+
+double orc_gauss(uint64_t seed, uint64_t counter) { return gauss_from_counter(seed, counter); }
+
+void orc_set_theta(void* h, const double* theta) {
+    auto* p = (Problem*)h;
+    const int64_t n = (int64_t)p->n_fine * p->n_fine * (p->mg.dim == 3 ? p->n_fine : 1);
+    p->theta.assign(theta, theta + n);
+}
+
+void orc_set_source_top(void* h, const double* f) {
+    auto* p = (Problem*)h;
+    const int top = (int)p->mg.levels.size() - 1;
+    p->f_top.assign(f, f + p->mg.levels[top].n_dof);
+}
+
+// ---- assembly -------------------------------------------------------------------------
+
+int orc_assemble(void* h, int lev, double f_const) {
+    auto* p = (Problem*)h;
+    try {
+        Coefficient k;
+        k.n_fine = p->n_fine;
+        k.dim = p->mg.dim;
+        k.theta = p->theta.empty() ? nullptr : p->theta.data();
+        double t0 = now();
+        if (p->kind == 0) {
+            assemble_q1_hex(p->mg, lev, k, f_const, p->A[lev], p->b[lev]);
+        } else {
+            const int top = (int)p->mg.levels.size() - 1;
+            std::vector<double> f(p->mg.levels[lev].n_dof, 0.0);
+            if (lev == top && !p->f_top.empty()) f = p->f_top;
+            assemble_fv1_tri(p->mg, lev, k, f, p->A[lev], p->b[lev]);
+        }
+        p->t_assemble += now() - t0;
+        if (lev > 0) {
+            p->P[lev] = prolongation_p1(p->mg, lev);
+            p->R[lev] = restriction_from(p->mg, lev, p->P[lev]);
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int64_t orc_matrix_nnz(void* h, int which, int lev) {
+    auto* p = (Problem*)h;
+    const CSR& A = which == 0 ? p->A[lev] : which == 1 ? p->P[lev] : p->R[lev];
+    return A.nnz();
+}
+
+int64_t orc_matrix_rows(void* h, int which, int lev) {
+    auto* p = (Problem*)h;
+    const CSR& A = which == 0 ? p->A[lev] : which == 1 ? p->P[lev] : p->R[lev];
+    return A.n_rows;
+}
+
+// which: 0 = level matrix A, 1 = prolongation P (lev-1 -> lev), 2 = restriction R
+void orc_matrix_get(void* h, int which, int lev, int64_t* rowptr, int32_t* col, double* val) {
+    auto* p = (Problem*)h;
+    const CSR& A = which == 0 ? p->A[lev] : which == 1 ? p->P[lev] : p->R[lev];
+    copy_csr(A, rowptr, col, val);
+}
+
+void orc_rhs_get(void* h, int lev, double* out) {
+    auto* p = (Problem*)h;
+    std::memcpy(out, p->b[lev].data(), sizeof(double) * p->b[lev].size());
+}
+
+// ---- solve ------------------------------------------------------------------------------
+// solver: 0 = CG, 1 = BiCGStab. precond: 0 = none, 1 = GMG.
+int orc_solve(void* h, int solver, int precond, int base_lev, int nu1, int nu2, int cycle,
+              double omega, double rel_red, double min_def, int max_steps) {
+    auto* p = (Problem*)h;
+    try {
+        const int top = (int)p->mg.levels.size() - 1;
+        double t0 = now();
+        GMG g;
+        if (precond) {
+            g.base_lev = base_lev;
+            g.top_lev = top;
+            g.nu1 = nu1;
+            g.nu2 = nu2;
+            g.cycle = cycle;
+            g.omega = omega;
+            g.A = p->A;
+            g.P = p->P;
+            g.R = p->R;
+            g.init();
+        }
+        double t1 = now();
+        p->x.assign(p->A[top].n_rows, 0.0);
+        ConvCheck cc;
+        cc.max_steps = max_steps;
+        cc.min_defect = min_def;
+        cc.rel_reduction = rel_red;
+        p->last = solver == 0 ? cg_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc)
+                              : bicgstab_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc);
+        double t2 = now();
+        p->t_setup = t1 - t0;
+        p->t_solve = t2 - t1;
+        return p->last.ok ? p->last.steps : -1 - p->last.steps;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100000;
+    }
+}
+
+// one application c = GMG(d) on the top level (AssembledMultiGridCycle::apply)
+int orc_gmg_apply(void* h, int base_lev, int nu1, int nu2, int cycle, double omega, const double* d, double* c) {
+    auto* p = (Problem*)h;
+    try {
+        const int top = (int)p->mg.levels.size() - 1;
+        GMG g;
+        g.base_lev = base_lev;
+        g.top_lev = top;
+        g.nu1 = nu1;
+        g.nu2 = nu2;
+        g.cycle = cycle;
+        g.omega = omega;
+        g.A = p->A;
+        g.P = p->P;
+        g.R = p->R;
+        g.init();
+        const int64_t n = p->A[top].n_rows;
+        std::vector<double> dv(d, d + n), cv(n, 0.0);
+        g.apply(cv, dv);
+        std::memcpy(c, cv.data(), sizeof(double) * n);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void orc_solution_get(void* h, double* out) {
+    auto* p = (Problem*)h;
+    std::memcpy(out, p->x.data(), sizeof(double) * p->x.size());
+}
+
+int orc_defects_get(void* h, double* out, int max_n) {
+    auto* p = (Problem*)h;
+    int n = (int)p->last.defects.size();
+    if (out)
+        for (int i = 0; i < n && i < max_n; ++i) out[i] = p->last.defects[i];
+    return n;
+}
+
+void orc_timings(void* h, double* out3) {
+    auto* p = (Problem*)h;
+    out3[0] = p->t_assemble;
+    out3[1] = p->t_setup;
+    out3[2] = p->t_solve;
+}
+
+// ---- CSR semantics probes (insertion, transpose, SpMV variants) -------------------------
+
+void* orc_builder_create(int64_t rows, int64_t cols) {
+    auto* b = new RowBuilder();
+    b->resize_and_clear(rows, cols);
+    return b;
+}
+void orc_builder_set(void* h, int64_t r, int64_t c, double v) { ((RowBuilder*)h)->at(r, c) = v; }
+void orc_builder_add(void* h, int64_t r, int64_t c, double v) { ((RowBuilder*)h)->at(r, c) += v; }
+void orc_builder_destroy(void* h) { delete (RowBuilder*)h; }
+
+// mode: 0 = as built, 1 = transpose keeping zeros, 2 = transpose dropping zeros
+int64_t orc_builder_csr(void* h, int mode, int64_t* rowptr, int32_t* col, double* val) {
+    CSR A = defragment(*(RowBuilder*)h);
+    if (mode == 1) A = transpose_keep_zeros(A);
+    else if (mode == 2) A = transpose_drop_zeros(A);
+    copy_csr(A, rowptr, col, val);
+    return A.nnz();
+}
+
+// op: 0 = axpy (dest = a*v1 + b*A*w), 1 = apply_ignore_zero_rows (dest = b*A*w, empty rows untouched),
+//     2 = axpy_transposed (dest = a*v1 + b*A^T*w)
+void orc_csr_op(int op, int64_t rows, int64_t cols, const int64_t* rowptr, const int32_t* col,
+                const double* val, double alpha, const double* v1, double beta, const double* w,
+                double* dest) {
+    CSR A;
+    A.n_rows = rows;
+    A.n_cols = cols;
+    A.rowptr.assign(rowptr, rowptr + rows + 1);
+    A.col.assign(col, col + rowptr[rows]);
+    A.val.assign(val, val + rowptr[rows]);
+    const int64_t nd = op == 2 ? cols : rows;
+    const int64_t nw = op == 2 ? rows : cols;
+    std::vector<double> d(dest, dest + nd), vv(v1 ? v1 : dest, (v1 ? v1 : dest) + nd), ww(w, w + nw);
+    if (op == 0) axpy(A, d, alpha, vv, beta, ww);
+    else if (op == 1) apply_ignore_zero_rows(A, d, beta, ww);
+    else axpy_transposed(A, d, alpha, vv, beta, ww);
+    std::memcpy(dest, d.data(), sizeof(double) * nd);
+}
+
+}  // extern "C"

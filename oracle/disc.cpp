@@ -1,0 +1,248 @@
+// Oracle (test infrastructure only): element discretisations, Dirichlet
+// constraints and grid transfer.
+//
+// The ConvectionDiffusion element discretisation is an external plugin (not
+// part of ugcore); the element matrices here follow its documented operator
+// built from ugcore's building blocks:
+//  * Q1 FE: LagrangeP1 shape functions on the reference hexahedron
+//    (local_finite_element/lagrange/lagrangep1.h:52-96,
+//    reference_element/reference_element.h ReferenceHexahedron corners),
+//    trilinear reference mapping (reference_mapping.h:1099), 2x2x2 Gauss rule
+//    (quadrature/gauss/gauss_quad_hexahedron.cpp, order 3).
+//  * FV1: box method on P1 triangles; scvf from edge midpoint to barycentre
+//    (disc_util/fv1_geom.cpp:258-463), normal oriented from `from` to `to`
+//    (fv_util.h:194-262); diffusion flux k grad(u).n subtracted from `from`
+//    and added to `to`; source f(corner)*|scv| (fv1_geom.h:311-314).
+//  * Scatter: AddLocalMatrixToGlobal (lib_disc/common/local_algebra.h:766-789)
+//    through SparseMatrix::operator() insertion semantics.
+//  * Dirichlet: SetDirichletRow (algebra_common/sparsematrix_util.h:850-861),
+//    rhs value 0 (lagrange_dirichlet_boundary_impl.h:753-1035).
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+namespace {
+
+const double REF_HEX[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                              {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+void q1_shape(const double* xi, double* phi, double (*dphi)[3]) {
+    for (int i = 0; i < 8; ++i) {
+        double f[3], g[3];
+        for (int d = 0; d < 3; ++d) {
+            f[d] = REF_HEX[i][d] > 0.5 ? xi[d] : 1.0 - xi[d];
+            g[d] = REF_HEX[i][d] > 0.5 ? 1.0 : -1.0;
+        }
+        phi[i] = f[0] * f[1] * f[2];
+        dphi[i][0] = g[0] * f[1] * f[2];
+        dphi[i][1] = f[0] * g[1] * f[2];
+        dphi[i][2] = f[0] * f[1] * g[2];
+    }
+}
+
+double inv3(const double J[3][3], double Ji[3][3]) {
+    double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double id = 1.0 / det;
+    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return det;
+}
+
+}  // namespace
+
+std::vector<uint8_t> dirichlet_mask(const MultiGrid& mg, int lev) {
+    const Level& L = mg.levels[lev];
+    std::vector<uint8_t> m(L.n_dof, 0);
+    for (size_t v = 0; v < L.vpos.size(); ++v)
+        if (L.vsub[v] == 1) m[L.dof[v]] = 1;
+    return m;
+}
+
+void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double f_const,
+                     CSR& A, std::vector<double>& b) {
+    const Level& L = mg.levels[lev];
+    const double scale = 1.0 / (double)mg.finest_scale;
+    RowBuilder B;
+    B.resize_and_clear(L.n_dof, L.n_dof);
+    b.assign(L.n_dof, 0.0);
+    const double g = 0.5 / std::sqrt(3.0);
+    const double qp1[2] = {0.5 - g, 0.5 + g};
+    for (size_t e = 0; e < L.vols.size(); ++e) {
+        if (L.volsub[e] != 0) continue;   // element disc defined on subset "Inner"
+        double X[8][3];
+        int ind[8];
+        for (int i = 0; i < 8; ++i) {
+            int v = L.vols[e][i];
+            ind[i] = L.dof[v];
+            for (int d = 0; d < 3; ++d) X[i][d] = (double)L.vpos[v][d] * scale;
+        }
+        double Ke[8][8] = {{0}};
+        double fe[8] = {0};
+        for (int qz = 0; qz < 2; ++qz)
+            for (int qy = 0; qy < 2; ++qy)
+                for (int qx = 0; qx < 2; ++qx) {
+                    const double xi[3] = {qp1[qx], qp1[qy], qp1[qz]};
+                    double phi[8], dphi[8][3];
+                    q1_shape(xi, phi, dphi);
+                    double J[3][3] = {{0}}, Ji[3][3], xq[3] = {0, 0, 0};
+                    for (int i = 0; i < 8; ++i)
+                        for (int r = 0; r < 3; ++r) {
+                            xq[r] += phi[i] * X[i][r];
+                            for (int c = 0; c < 3; ++c) J[r][c] += X[i][r] * dphi[i][c];
+                        }
+                    const double det = inv3(J, Ji);
+                    const double w = 0.125 * std::fabs(det);
+                    const double kq = k.eval(xq);
+                    double grad[8][3];
+                    for (int i = 0; i < 8; ++i)
+                        for (int r = 0; r < 3; ++r)
+                            grad[i][r] = Ji[0][r] * dphi[i][0] + Ji[1][r] * dphi[i][1] + Ji[2][r] * dphi[i][2];
+                    for (int i = 0; i < 8; ++i) {
+                        for (int j = 0; j < 8; ++j)
+                            Ke[i][j] += w * kq * (grad[i][0] * grad[j][0] + grad[i][1] * grad[j][1] + grad[i][2] * grad[j][2]);
+                        fe[i] += w * f_const * phi[i];
+                    }
+                }
+        for (int i = 0; i < 8; ++i) {
+            for (int j = 0; j < 8; ++j) B.at(ind[i], ind[j]) += Ke[i][j];
+            b[ind[i]] += fe[i];
+        }
+    }
+    A = defragment(B);
+    auto dm = dirichlet_mask(mg, lev);
+    for (int64_t r = 0; r < A.n_rows; ++r)
+        if (dm[r]) {
+            set_row(A, r, 0.0);
+            double* d = find_entry(A, r, r);
+            if (!d) throw std::runtime_error("oracle: Dirichlet row without diagonal");
+            *d = 1.0;
+            b[r] = 0.0;
+        }
+}
+
+void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
+                      const std::vector<double>& f_at_dof, CSR& A, std::vector<double>& b) {
+    const Level& L = mg.levels[lev];
+    const double scale = 1.0 / (double)mg.finest_scale;
+    RowBuilder B;
+    B.resize_and_clear(L.n_dof, L.n_dof);
+    b.assign(L.n_dof, 0.0);
+    for (size_t f = 0; f < L.faces.size(); ++f) {
+        if (L.fsub[f] != 0 || L.fnv[f] != 3) continue;
+        double X[3][2];
+        int ind[3];
+        for (int i = 0; i < 3; ++i) {
+            int v = L.faces[f][i];
+            ind[i] = L.dof[v];
+            X[i][0] = (double)L.vpos[v][0] * scale;
+            X[i][1] = (double)L.vpos[v][1] * scale;
+        }
+        // P1 gradients: grad phi_i = J^{-T} grad_ref phi_i
+        const double J00 = X[1][0] - X[0][0], J01 = X[2][0] - X[0][0];
+        const double J10 = X[1][1] - X[0][1], J11 = X[2][1] - X[0][1];
+        const double det = J00 * J11 - J01 * J10;
+        const double area = 0.5 * std::fabs(det);
+        const double gr[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+        double grad[3][2];
+        for (int i = 0; i < 3; ++i) {
+            grad[i][0] = (J11 * gr[i][0] - J10 * gr[i][1]) / det;
+            grad[i][1] = (-J01 * gr[i][0] + J00 * gr[i][1]) / det;
+        }
+        const double bary[2] = {(X[0][0] + X[1][0] + X[2][0]) / 3.0, (X[0][1] + X[1][1] + X[2][1]) / 3.0};
+        double Ke[3][3] = {{0}};
+        for (int e = 0; e < 3; ++e) {
+            const int from = e, to = (e + 1) % 3;
+            const double m[2] = {0.5 * (X[from][0] + X[to][0]), 0.5 * (X[from][1] + X[to][1])};
+            // scvf segment midpoint->barycentre, normal = rotated segment, oriented from->to
+            double n[2] = {bary[1] - m[1], -(bary[0] - m[0])};
+            const double dir[2] = {X[to][0] - X[from][0], X[to][1] - X[from][1]};
+            if (n[0] * dir[0] + n[1] * dir[1] < 0) { n[0] = -n[0]; n[1] = -n[1]; }
+            const double ip[2] = {0.5 * (m[0] + bary[0]), 0.5 * (m[1] + bary[1])};
+            const double kq = k.eval(ip);
+            for (int sh = 0; sh < 3; ++sh) {
+                const double flux = kq * (grad[sh][0] * n[0] + grad[sh][1] * n[1]);
+                Ke[from][sh] -= flux;
+                Ke[to][sh] += flux;
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) B.at(ind[i], ind[j]) += Ke[i][j];
+            b[ind[i]] += f_at_dof[ind[i]] * (area / 3.0);
+        }
+    }
+    A = defragment(B);
+    auto dm = dirichlet_mask(mg, lev);
+    for (int64_t r = 0; r < A.n_rows; ++r)
+        if (dm[r]) {
+            set_row(A, r, 0.0);
+            double* d = find_entry(A, r, r);
+            if (!d) throw std::runtime_error("oracle: Dirichlet row without diagonal");
+            *d = 1.0;
+            b[r] = 0.0;
+        }
+}
+
+CSR prolongation_p1(const MultiGrid& mg, int fine_lev) {
+    const Level& F = mg.levels[fine_lev];
+    const Level& C = mg.levels[fine_lev - 1];
+    RowBuilder B;
+    B.resize_and_clear(F.n_dof, C.n_dof);
+    for (int si = 0; si < mg.num_subsets; ++si)
+        for (size_t v = 0; v < F.vpos.size(); ++v) {
+            if (F.vsub[v] != si) continue;
+            const int row = F.dof[v];
+            const int p = F.vparent[v];
+            switch (F.vparent_type[v]) {
+                case OT_VERTEX: B.at(row, C.dof[p]) = 1.0; break;
+                case OT_EDGE:
+                    for (int i = 0; i < 2; ++i) B.at(row, C.dof[C.edges[p][i]]) = 0.5;
+                    break;
+                case OT_FACE:
+                    if (C.fnv[p] != 4) throw std::runtime_error("oracle: face parent must be a quadrilateral");
+                    for (int i = 0; i < 4; ++i) B.at(row, C.dof[C.faces[p][i]]) = 0.25;
+                    break;
+                case OT_VOLUME:
+                    for (int i = 0; i < 8; ++i) B.at(row, C.dof[C.vols[p][i]]) = 0.125;
+                    break;
+                default: throw std::runtime_error("oracle: vertex without parent");
+            }
+        }
+    CSR P = defragment(B);
+    // adjust_prolongation: fine Dirichlet rows zeroed, vertex-copy children keep
+    // the injection 1 at their parent
+    for (size_t v = 0; v < F.vpos.size(); ++v) {
+        if (F.vsub[v] != 1) continue;
+        const int row = F.dof[v];
+        set_row(P, row, 0.0);
+        if (F.vparent_type[v] == OT_VERTEX) *find_entry(P, row, C.dof[F.vparent[v]]) = 1.0;
+    }
+    return P;
+}
+
+CSR restriction_from(const MultiGrid& mg, int fine_lev, const CSR& P) {
+    const Level& F = mg.levels[fine_lev];
+    const Level& C = mg.levels[fine_lev - 1];
+    CSR R = transpose_keep_zeros(P);
+    for (size_t v = 0; v < F.vpos.size(); ++v) {
+        if (F.vsub[v] != 1 || F.vparent_type[v] != OT_VERTEX) continue;
+        const int crow = C.dof[F.vparent[v]];
+        set_row(R, crow, 0.0);
+        *find_entry(R, crow, F.dof[v]) = 1.0;
+    }
+    return R;
+}
+
+}  // namespace orc

@@ -1,0 +1,268 @@
+// Oracle (test infrastructure only): restatement of lib_grid's global multigrid
+// refinement order and of the LEVEL DoF numbering.
+//
+// Rules followed:
+//  * GlobalMultiGridRefiner::perform_refinement
+//    (lib_grid/refinement/global_multi_grid_refiner.cpp:135-455): on the new
+//    level, vertices are created as clones of the old level's vertices (storage
+//    order), then one midpoint per old edge (edge order; each edge is split into
+//    (v0', mid), (mid, v1'), grid_objects_1d.cpp:39-54), then one centre per old
+//    quadrilateral (face order), then one centre per old hexahedron (volume
+//    order). New faces/volumes are registered in creation order.
+//  * Grid::register_face / register_volume
+//    (lib_grid/grid/grid_connection_managment.cpp:922-1010, 1440-1560): a
+//    registered volume first autogenerates its missing faces in local face order
+//    (Hexahedron::create_face uses vertices f0,f3,f2,f1 of FACE_VRT_INDS,
+//    grid_objects_3d.cpp:566-601); every registered face autogenerates its
+//    missing edges (w_i, w_{i+1}) in local edge order.
+//  * Child rules: quadrilateral (grid_objects_2d.cpp:650-662), triangle
+//    (grid_objects_2d.cpp:230-236), hexahedron regular rule
+//    (hexahedron_rules.cpp:385-440) with the local index space
+//    [8 corners, 12 edge vertices, 6 face vertices, centre]
+//    (grid_objects_3d.cpp:162-237).
+//  * Subset inheritance: every new object takes its parent's subset
+//    (tools/subset_handler_interface.cpp:975-1005, non-strict inheritance).
+//  * LEVEL DoF numbering: per subset ascending, in subset-list (= creation)
+//    order (lib_disc/dof_manager/dof_distribution.cpp:1401-1421).
+//
+// The base grids (one unit hexahedron; the unit square as two triangles) are
+// defined here as if their elements were registered with autogeneration, with
+// subset 0 = "Inner" (volume/faces of the domain and inner edges) and
+// subset 1 = "Boundary" (all boundary vertices, edges, faces).
+#include <algorithm>
+#include <cassert>
+#include <stdexcept>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+namespace {
+
+// hexahedron_rules.h: EDGE_VRT_INDS, FACE_VRT_INDS
+const int HEX_EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                             {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
+                            {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
+
+uint64_t edge_key(int a, int b) {
+    if (a > b) std::swap(a, b);
+    return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+Key4 face_key(const int* v, int nv) {
+    Key4 k;
+    for (int i = 0; i < 4; ++i) k.v[i] = i < nv ? v[i] : -1;
+    std::sort(k.v, k.v + nv);
+    return k;
+}
+
+struct Builder {
+    Level& L;
+    explicit Builder(Level& l) : L(l) {}
+
+    int add_vertex(const std::array<int64_t, 3>& p, int si, int8_t ptype, int parent) {
+        L.vpos.push_back(p);
+        L.vsub.push_back(si);
+        L.vparent_type.push_back(ptype);
+        L.vparent.push_back(parent);
+        return (int)L.vpos.size() - 1;
+    }
+    int find_edge(int a, int b) const {
+        auto it = L.edge_map.find(edge_key(a, b));
+        return it == L.edge_map.end() ? -1 : it->second;
+    }
+    int add_edge(int a, int b, int si) {
+        int id = (int)L.edges.size();
+        L.edges.push_back({a, b});
+        L.esub.push_back(si);
+        L.edge_map.emplace(edge_key(a, b), id);
+        return id;
+    }
+    // Grid::register_face with FACEOPT_AUTOGENERATE_EDGES
+    int register_face(const int* v, int nv, int si) {
+        int id = (int)L.faces.size();
+        std::array<int, 4> fv{-1, -1, -1, -1};
+        for (int i = 0; i < nv; ++i) fv[i] = v[i];
+        L.faces.push_back(fv);
+        L.fnv.push_back((int8_t)nv);
+        L.fsub.push_back(si);
+        L.face_map.emplace(face_key(v, nv), id);
+        std::array<int, 4> fe{-1, -1, -1, -1};
+        for (int i = 0; i < nv; ++i) {
+            int a = v[i], b = v[(i + 1) % nv];
+            int e = find_edge(a, b);
+            if (e < 0) e = add_edge(a, b, si);
+            fe[i] = e;
+        }
+        L.face_edges.push_back(fe);
+        return id;
+    }
+    int find_face(const int* v, int nv) const {
+        auto it = L.face_map.find(face_key(v, nv));
+        return it == L.face_map.end() ? -1 : it->second;
+    }
+    // Grid::register_volume with VOLOPT_AUTOGENERATE_FACES (hexahedron)
+    int register_hex(const int* h, int si) {
+        int id = (int)L.vols.size();
+        std::array<int, 8> hv;
+        for (int i = 0; i < 8; ++i) hv[i] = h[i];
+        L.vols.push_back(hv);
+        L.volnv.push_back(8);
+        L.volsub.push_back(si);
+        std::array<int, 6> vf;
+        for (int i = 0; i < 6; ++i) {
+            const int* f = HEX_FACE[i];
+            int fd[4] = {h[f[0]], h[f[3]], h[f[2]], h[f[1]]};   // Hexahedron::face_desc
+            int fid = find_face(fd, 4);
+            if (fid < 0) fid = register_face(fd, 4, si);
+            vf[i] = fid;
+        }
+        L.vol_faces.push_back(vf);
+        std::array<int, 12> ve;
+        for (int i = 0; i < 12; ++i) {
+            int e = find_edge(h[HEX_EDGE[i][0]], h[HEX_EDGE[i][1]]);
+            if (e < 0) e = add_edge(h[HEX_EDGE[i][0]], h[HEX_EDGE[i][1]], si);
+            ve[i] = e;
+        }
+        L.vol_edges.push_back(ve);
+        return id;
+    }
+};
+
+std::array<int64_t, 3> avg(const Level& L, const int* v, int n) {
+    std::array<int64_t, 3> s{0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) s[d] += L.vpos[v[i]][d];
+    for (int d = 0; d < 3; ++d) {
+        if (s[d] % n != 0) throw std::runtime_error("oracle grid: non-lattice midpoint");
+        s[d] /= n;
+    }
+    return s;
+}
+
+Level refine(const Level& C) {
+    Level F;
+    F.dim = C.dim;
+    Builder B(F);
+    const int nv = (int)C.vpos.size();
+    std::vector<int> cv(nv), me(C.edges.size()), cf(C.faces.size(), -1);
+    // 1. clones of old vertices
+    for (int v = 0; v < nv; ++v) cv[v] = B.add_vertex(C.vpos[v], C.vsub[v], OT_VERTEX, v);
+    // 2. edge midpoints and child edges
+    for (size_t e = 0; e < C.edges.size(); ++e) {
+        int ab[2] = {C.edges[e][0], C.edges[e][1]};
+        int m = B.add_vertex(avg(C, ab, 2), C.esub[e], OT_EDGE, (int)e);
+        me[e] = m;
+        B.add_edge(cv[ab[0]], m, C.esub[e]);
+        B.add_edge(m, cv[ab[1]], C.esub[e]);
+    }
+    // 3. faces
+    for (size_t f = 0; f < C.faces.size(); ++f) {
+        const int n = C.fnv[f];
+        int vr[4], ev[4];
+        for (int j = 0; j < n; ++j) {
+            vr[j] = cv[C.faces[f][j]];
+            ev[j] = me[C.face_edges[f][j]];
+        }
+        const int si = C.fsub[f];
+        if (n == 4) {
+            int c = B.add_vertex(avg(C, C.faces[f].data(), 4), si, OT_FACE, (int)f);
+            cf[f] = c;
+            int q[4][4] = {{vr[0], ev[0], c, ev[3]},
+                           {vr[1], ev[1], c, ev[0]},
+                           {vr[2], ev[2], c, ev[1]},
+                           {vr[3], ev[3], c, ev[2]}};
+            for (auto& qq : q) B.register_face(qq, 4, si);
+        } else {
+            int t[4][3] = {{vr[0], ev[0], ev[2]},
+                           {vr[1], ev[1], ev[0]},
+                           {vr[2], ev[2], ev[1]},
+                           {ev[0], ev[1], ev[2]}};
+            for (auto& tt : t) B.register_face(tt, 3, si);
+        }
+    }
+    // 4. volumes (hexahedra, regular rule)
+    for (size_t v = 0; v < C.vols.size(); ++v) {
+        if (C.volnv[v] != 8) throw std::runtime_error("oracle grid: only hexahedra supported");
+        int all[27];
+        for (int j = 0; j < 8; ++j) all[j] = cv[C.vols[v][j]];
+        for (int j = 0; j < 12; ++j) all[8 + j] = me[C.vol_edges[v][j]];
+        for (int j = 0; j < 6; ++j) all[20 + j] = cf[C.vol_faces[v][j]];
+        const int si = C.volsub[v];
+        all[26] = B.add_vertex(avg(C, C.vols[v].data(), 8), si, OT_VOLUME, (int)v);
+        const int E = 8, Fo = 20, V = 26;
+        const int rule[8][8] = {
+            {0, E, Fo, E + 3, E + 4, Fo + 1, V, Fo + 4},
+            {E, 1, E + 1, Fo, Fo + 1, E + 5, Fo + 2, V},
+            {Fo, E + 1, 2, E + 2, V, Fo + 2, E + 6, Fo + 3},
+            {E + 3, Fo, E + 2, 3, Fo + 4, V, Fo + 3, E + 7},
+            {E + 4, Fo + 1, V, Fo + 4, 4, E + 8, Fo + 5, E + 11},
+            {Fo + 1, E + 5, Fo + 2, V, E + 8, 5, E + 9, Fo + 5},
+            {V, Fo + 2, E + 6, Fo + 3, Fo + 5, E + 9, 6, E + 10},
+            {Fo + 4, V, Fo + 3, E + 7, E + 11, Fo + 5, E + 10, 7}};
+        for (int c = 0; c < 8; ++c) {
+            int h[8];
+            for (int j = 0; j < 8; ++j) h[j] = all[rule[c][j]];
+            B.register_hex(h, si);
+        }
+    }
+    return F;
+}
+
+}  // namespace
+
+void number_dofs(Level& L, int num_subsets) {
+    const int nv = (int)L.vpos.size();
+    L.dof.assign(nv, -1);
+    L.vtx_of_dof.assign(nv, -1);
+    int k = 0;
+    for (int si = 0; si < num_subsets; ++si)
+        for (int v = 0; v < nv; ++v)
+            if (L.vsub[v] == si) {
+                L.dof[v] = k;
+                L.vtx_of_dof[k] = v;
+                ++k;
+            }
+    L.n_dof = k;
+}
+
+MultiGrid build_hierarchy(int kind, int num_refs) {
+    MultiGrid mg;
+    mg.num_subsets = 2;
+    mg.finest_scale = (int64_t)1 << num_refs;
+    const int64_t S = mg.finest_scale;
+    Level L0;
+    Builder B(L0);
+    if (kind == 0) {
+        mg.dim = L0.dim = 3;
+        const int64_t c[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (auto& p : c) B.add_vertex({p[0] * S, p[1] * S, p[2] * S}, 1, -1, -1);
+        int h[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+        B.register_hex(h, 0);
+        // every face, edge and vertex of the single cube lies on the boundary
+        for (auto& s : L0.fsub) s = 1;
+        for (auto& s : L0.esub) s = 1;
+    } else if (kind == 1) {
+        mg.dim = L0.dim = 2;
+        const int64_t c[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        for (auto& p : c) B.add_vertex({p[0] * S, p[1] * S, 0}, 1, -1, -1);
+        int t0[3] = {0, 1, 2}, t1[3] = {0, 2, 3};
+        B.register_face(t0, 3, 0);
+        B.register_face(t1, 3, 0);
+        for (size_t e = 0; e < L0.edges.size(); ++e) {
+            int a = L0.edges[e][0], b = L0.edges[e][1];
+            bool diag = (std::min(a, b) == 0 && std::max(a, b) == 2);
+            L0.esub[e] = diag ? 0 : 1;
+        }
+    } else {
+        throw std::runtime_error("oracle grid: unknown base grid kind");
+    }
+    mg.levels.push_back(std::move(L0));
+    for (int r = 0; r < num_refs; ++r) mg.levels.push_back(refine(mg.levels.back()));
+    for (auto& L : mg.levels) number_dofs(L, mg.num_subsets);
+    return mg;
+}
+
+}  // namespace orc

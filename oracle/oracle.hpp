@@ -1,0 +1,228 @@
+// CPU oracle: a restatement of ugcore's CPUAlgebra discretize-and-solve path.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the shipped library (ugcore_amd/) may
+// link, import or call this code; only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py use it, as the checker and as the timed
+// single-core CPU baseline.
+//
+// Each function cites the reference file:line it follows (paths relative to
+// the ugcore checkout). Parity pins are listed per part in the tests.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <functional>
+#include <unordered_map>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Multigrid mesh restatement (lib_grid/refinement/global_multi_grid_refiner.cpp)
+// ---------------------------------------------------------------------------
+enum ObjType : int8_t { OT_VERTEX = 0, OT_EDGE = 1, OT_FACE = 2, OT_VOLUME = 3 };
+
+struct Key4 {
+    int v[4];
+    bool operator==(const Key4& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3]; }
+};
+struct Key4Hash {
+    size_t operator()(const Key4& k) const {
+        uint64_t h = 1469598103934665603ull;
+        for (int i = 0; i < 4; ++i) { h ^= (uint64_t)(uint32_t)k.v[i]; h *= 1099511628211ull; }
+        return (size_t)(h ^ (h >> 29));
+    }
+};
+
+struct Level {
+    int dim = 3;
+    // vertices in creation (storage) order
+    std::vector<std::array<int64_t, 3>> vpos;   // integer lattice coords at finest resolution
+    std::vector<int> vsub;                      // subset index
+    std::vector<int8_t> vparent_type;           // parent object type at level-1 (-1 for level 0)
+    std::vector<int> vparent;                   // parent object id at level-1
+    // edges
+    std::vector<std::array<int, 2>> edges;
+    std::vector<int> esub;
+    // faces (tri: 3 verts, quad: 4 verts)
+    std::vector<std::array<int, 4>> faces;
+    std::vector<int8_t> fnv;
+    std::vector<int> fsub;
+    std::vector<std::array<int, 4>> face_edges;
+    // volumes (hex: 8 verts, tet: 4 verts)
+    std::vector<std::array<int, 8>> vols;
+    std::vector<int8_t> volnv;
+    std::vector<int> volsub;
+    std::vector<std::array<int, 6>> vol_faces;
+    std::vector<std::array<int, 12>> vol_edges;
+    // lookup by vertex set (sorted ids), like Grid::find_edge/face_in_associated_*
+    std::unordered_map<uint64_t, int> edge_map;
+    std::unordered_map<Key4, int, Key4Hash> face_map;
+    // DoF numbering: dof[v] for each vertex (subset-major, creation order within subset)
+    std::vector<int> dof;
+    std::vector<int> vtx_of_dof;
+    int n_dof = 0;
+};
+
+struct MultiGrid {
+    int dim = 3;
+    int num_subsets = 2;           // 0 = "Inner", 1 = "Boundary"
+    int64_t finest_scale = 1;      // lattice units per base-element edge at the top level
+    std::vector<Level> levels;
+};
+
+// kind: 0 = unit cube as 1 hexahedron; 1 = unit square as 2 triangles
+MultiGrid build_hierarchy(int kind, int num_refs);
+void number_dofs(Level& L, int num_subsets);
+
+// ---------------------------------------------------------------------------
+// CSR with SparseMatrix insertion semantics (lib_algebra/cpu_algebra/sparsematrix_impl.h)
+// ---------------------------------------------------------------------------
+struct RowBuilder {
+    // one sorted row per matrix row; entries are created on first access even
+    // when the value stays 0 (get_index, sparsematrix_impl.h:611-700)
+    std::vector<std::vector<int>> cols;
+    std::vector<std::vector<double>> vals;
+    int64_t n_cols = 0;
+    void resize_and_clear(int64_t r, int64_t c);
+    double& at(int64_t r, int64_t c);
+};
+
+struct CSR {
+    int64_t n_rows = 0, n_cols = 0;
+    std::vector<int64_t> rowptr;
+    std::vector<int> col;
+    std::vector<double> val;
+    int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+CSR defragment(const RowBuilder& B);                       // copyToNewSize, sparsematrix_impl.h:703-750
+CSR transpose_keep_zeros(const CSR& A);                    // set_as_transpose_of, sparsematrix_impl.h:148-183
+CSR transpose_drop_zeros(const CSR& A);                    // set_as_transpose_of2, sparsematrix_impl.h:185-254
+void set_row(CSR& A, int64_t r, double v);                 // SetRow, sparsematrix_util.h:759-765
+double* find_entry(CSR& A, int64_t r, int64_t c);
+
+// y = alpha1*v1 + beta1*A*w1 (SparseMatrix::axpy, sparsematrix_impl.h:291-339)
+void axpy(const CSR& A, std::vector<double>& dest, double alpha1, const std::vector<double>& v1,
+          double beta1, const std::vector<double>& w1);
+// dest[i] = beta*sum_k A_ik w_k for non-empty rows only (sparsematrix_impl.h:271-288)
+void apply_ignore_zero_rows(const CSR& A, std::vector<double>& dest, double beta, const std::vector<double>& w);
+// dest = alpha1*v1 + beta1*A^T*w1 (sparsematrix_impl.h:341-370)
+void axpy_transposed(const CSR& A, std::vector<double>& dest, double alpha1, const std::vector<double>& v1,
+                     double beta1, const std::vector<double>& w1);
+// d -= A x  (MatrixOperator::apply_sub -> matmul_minus, sparsematrix.h:184-206)
+void apply_sub(const CSR& A, std::vector<double>& d, const std::vector<double>& x);
+
+double dot(const std::vector<double>& a, const std::vector<double>& b);   // vector_impl.h:70-79
+double norm(const std::vector<double>& a);                                // vector_impl.h:323-329
+
+// ---------------------------------------------------------------------------
+// Synthetic data
+// ---------------------------------------------------------------------------
+uint64_t splitmix64(uint64_t x);
+double gauss_from_counter(uint64_t seed, uint64_t counter);
+// theta per fine element of an n^3 element lattice, i.i.d. N(0,1) then `passes`
+// rounds of the 27-neighbour box average (clipped at the domain boundary)
+std::vector<double> make_theta(int n, uint64_t seed, int passes);
+
+// ---------------------------------------------------------------------------
+// Discretisation
+// ---------------------------------------------------------------------------
+struct Coefficient {
+    // diffusion k(x) = exp(theta[fine element containing x]) on a n_fine^dim lattice
+    int n_fine = 0;
+    int dim = 3;
+    const double* theta = nullptr;   // nullptr => k = 1
+    double eval(const double* x) const;
+};
+
+// Q1 FE Laplace-type operator -div(k grad u) = f on hexahedra, 2x2x2 Gauss,
+// assembled in element creation order and scattered with AddLocalMatrixToGlobal
+// semantics (lib_disc/common/local_algebra.h:766-789); Dirichlet rows via
+// SetDirichletRow (lagrange_dirichlet_boundary_impl.h:753-915).
+void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double f_const,
+                     CSR& A, std::vector<double>& b);
+// FV1 (vertex-centred box method) on P1 triangles, diffusion k=1 (or k(x)),
+// source given per vertex (evaluated at the SCV integration point = corner,
+// fv1_geom.h:311-314).
+void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
+                      const std::vector<double>& f_at_dof, CSR& A, std::vector<double>& b);
+std::vector<uint8_t> dirichlet_mask(const MultiGrid& mg, int lev);
+
+// StdTransfer::assemble_prolongation_p1 (std_transfer_impl.h:45-168) followed by
+// DirichletBoundary::adjust_prolongation (lagrange_dirichlet_boundary_impl.h:485-606)
+CSR prolongation_p1(const MultiGrid& mg, int fine_lev);
+// R = P^T keeping zeros (std_transfer_impl.h:667-668) then adjust_restriction
+// (lagrange_dirichlet_boundary_impl.h:617-740)
+CSR restriction_from(const MultiGrid& mg, int fine_lev, const CSR& P);
+
+// ---------------------------------------------------------------------------
+// Solvers
+// ---------------------------------------------------------------------------
+struct ConvCheck {
+    // StdConvCheck (convergence_check.h:178-262, convergence_check_impl.h:134-169)
+    int max_steps = 100;
+    double min_defect = 1e-50;
+    double rel_reduction = 1e-10;
+    double initial = 0, current = 0, last = 0;
+    int step = 0;
+    std::vector<double> defects;
+    void start(double d) { initial = current = d; step = 0; defects.assign(1, d); }
+    void update(double d) { last = current; current = d; ++step; defects.push_back(d); }
+    bool valid() const;
+    bool iteration_ended() const;
+    bool post() const;
+};
+
+struct Jacobi {
+    // jacobi.h:155-249: diagInv_i = 1 / (a_ii * (1/omega))
+    double omega = 0.66;
+    std::vector<double> diag_inv;
+    bool init(const CSR& A);
+    void apply(std::vector<double>& c, const std::vector<double>& d) const;
+};
+
+struct DenseLU {
+    // lu.h:122-143, 203-225 with no_lapack/lu_decomp.h:47-200 (LUDecomp/SolveLU)
+    int n = 0;
+    std::vector<double> a;            // row-major LU factors
+    std::vector<size_t> piv;
+    bool init(const CSR& A);
+    void apply(std::vector<double>& x, const std::vector<double>& b) const;
+};
+
+struct GMG {
+    // AssembledMultiGridCycle (mg_solver_impl.hpp), full refinement, LEVEL view
+    int base_lev = 0, top_lev = 0;
+    int nu1 = 2, nu2 = 2;
+    int cycle = 1;                    // 1 = V, 2 = W
+    double omega = 0.66;
+    std::vector<CSR> A;               // level matrices
+    std::vector<CSR> P, R;            // P[l]: l-1 -> l (rows fine), R[l]: l -> l-1
+    std::vector<Jacobi> smoother;
+    DenseLU base;
+    // level vectors sc, sd, st
+    std::vector<std::vector<double>> sc, sd, st;
+    void init();
+    void apply(std::vector<double>& c, const std::vector<double>& d);
+    void lmgc(int lev);
+    void presmooth_and_restriction(int lev);
+    void prolongation_and_postsmooth(int lev);
+    void base_solve(int lev);
+    int full_ref_level() const { return top_lev; }
+};
+
+struct SolveResult {
+    bool ok = false;
+    int steps = 0;
+    std::vector<double> defects;
+};
+
+// CG::apply_return_defect (linear_solver/cg.h:103-242)
+SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
+                     ConvCheck cc);
+// BiCGStab::apply_return_defect (linear_solver/bicgstab.h:112-385), no restarts
+SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
+                           ConvCheck cc);
+
+}  // namespace orc

@@ -1,0 +1,416 @@
+"""ugcore_amd: MI355X-native discretize-and-solve path for ugcore (GPU1 algebra).
+
+Python bindings over the C ABI in include/ugcore_amd.h (the reference's
+ug4py module is absent; its CMake option USE_PYBIND11 exists at
+cmake/ug/pybind11.cmake:33-55). Names follow the ugcore Lua/registry API:
+DomainDiscretization, GeometricMultiGrid (AssembledMultiGridCycle), CG,
+BiCGStab, Jacobi, ConvCheck (StdConvCheck), SparseMatrix, Vector.
+
+The shared library is built in-tree (ugcore_amd/libugcore_amd.so). If it is
+missing or cannot be loaded the package raises; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libugcore_amd.so")
+
+_lib = None
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_D = C.c_double
+_I = C.c_int
+
+_SIGNATURES = {
+    "ugx_init": (_I, []),
+    "ugx_last_error": (C.c_char_p, []),
+    "ugx_stream": (_P, []),
+    "ugx_synchronize": (_I, []),
+    "ugx_malloc": (_P, [_I64]),
+    "ugx_free": (_I, [_P]),
+    "ugx_memcpy_htod": (_I, [_P, _P, _I64]),
+    "ugx_memcpy_dtoh": (_I, [_P, _P, _I64]),
+    "ugx_event_create": (_P, []),
+    "ugx_event_destroy": (_I, [_P]),
+    "ugx_event_record": (_I, [_P, _P]),
+    "ugx_event_elapsed_ms": (_I, [_P, _P, _P]),
+    "ugx_spmv_csr_f64": (_I, [_I64, _P, _P, _P, _D, _P, _D, _P, _P, _P]),
+    "ugx_spmv_csr_f32": (_I, [_I64, _P, _P, _P, _D, _P, _D, _P, _P, _P]),
+    "ugx_spmv_csr_ignore_zero_rows_f64": (_I, [_I64, _P, _P, _P, _D, _P, _P, _P]),
+    "ugx_vec_scale_add2_f64": (_I, [_I64, _P, _D, _P, _D, _P, _P]),
+    "ugx_vec_scale_add3_f64": (_I, [_I64, _P, _D, _P, _D, _P, _D, _P, _P]),
+    "ugx_vec_set_f64": (_I, [_I64, _P, _D, _P]),
+    "ugx_dot_f64": (_I, [_I64, _P, _P, _P, _P]),
+    "ugx_norm_f64": (_I, [_I64, _P, _P, _P]),
+    "ugx_jacobi_diag_inv_f64": (_I, [_I64, _P, _P, _P, _D, _P, _P]),
+    "ugx_jacobi_apply_f64": (_I, [_I64, _P, _P, _P, _P]),
+    "ugx_hierarchy_numbering": (_I, [_I, _I, _I, _P, _P]),
+    "ugx_disc_create": (_P, [_I, _I]),
+    "ugx_disc_destroy": (None, [_P]),
+    "ugx_disc_num_levels": (_I, [_P]),
+    "ugx_disc_num_dofs": (_I64, [_P, _I]),
+    "ugx_disc_dof_of_lattice": (_I, [_P, _I, _P]),
+    "ugx_disc_set_theta": (_I, [_P, _P]),
+    "ugx_disc_set_source_const": (_I, [_P, _D]),
+    "ugx_disc_set_source_dofs": (_I, [_P, _P]),
+    "ugx_disc_assemble_linear": (_I, [_P, _P, _P]),
+    "ugx_disc_assemble_seconds": (_D, [_P]),
+    "ugx_matrix_create": (_P, []),
+    "ugx_matrix_destroy": (None, [_P]),
+    "ugx_matrix_num_rows": (_I64, [_P]),
+    "ugx_matrix_nnz": (_I64, [_P]),
+    "ugx_matrix_upload_csr": (_I, [_P, _I64, _I64, _P, _P, _P]),
+    "ugx_matrix_download_csr": (_I, [_P, _P, _P, _P]),
+    "ugx_matrix_device_csr": (_I, [_P, _P, _P, _P]),
+    "ugx_matrix_axpy": (_I, [_P, _P, _D, _P, _D, _P]),
+    "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
+    "ugx_vector_create": (_P, [_I64]),
+    "ugx_vector_destroy": (None, [_P]),
+    "ugx_vector_size": (_I64, [_P]),
+    "ugx_vector_device_ptr": (_P, [_P]),
+    "ugx_vector_upload": (_I, [_P, _P]),
+    "ugx_vector_download": (_I, [_P, _P]),
+    "ugx_vector_set": (_I, [_P, _D]),
+    "ugx_gmg_create": (_P, [_P, _I, _I, _I, _I, _D, _I]),
+    "ugx_gmg_destroy": (None, [_P]),
+    "ugx_gmg_init": (_I, [_P, _P]),
+    "ugx_gmg_apply": (_I, [_P, _P, _P]),
+    "ugx_gmg_smooth_level": (_I, [_P, _I, _I]),
+    "ugx_gmg_level_num_diags": (_I, [_P, _I]),
+    "ugx_gmg_level_dia": (_I, [_P, _I, _P]),
+    "ugx_jacobi_create": (_P, [_D]),
+    "ugx_jacobi_destroy": (None, [_P]),
+    "ugx_solver_create": (_P, [_I]),
+    "ugx_solver_destroy": (None, [_P]),
+    "ugx_solver_set_preconditioner": (_I, [_P, _I, _P]),
+    "ugx_solver_set_conv_check": (_I, [_P, _I, _D, _D, _I]),
+    "ugx_solver_init": (_I, [_P, _P]),
+    "ugx_solver_apply": (_I, [_P, _P, _P]),
+    "ugx_solver_steps": (_I, [_P]),
+    "ugx_solver_defects": (_I, [_P, _P, _I]),
+}
+
+
+class UGError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libugcore_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UGError(f"{LIB_PATH} not found: build it with `python -m ugcore_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        raise UGError(f"{what}: {lib().ugx_last_error().decode()}")
+    return rc
+
+
+def _handle(h, what):
+    if not h:
+        raise UGError(f"{what}: {lib().ugx_last_error().decode()}")
+    return h
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def init():
+    _check(lib().ugx_init(), "ugx_init")
+
+
+def synchronize():
+    _check(lib().ugx_synchronize(), "ugx_synchronize")
+
+
+class DeviceArray:
+    """A raw device buffer owned by the library (for the kernel entry points)."""
+
+    def __init__(self, host):
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = max(self.host.nbytes, 8)
+        self.ptr = _handle(lib().ugx_malloc(self.nbytes), "ugx_malloc")
+        if self.host.nbytes:
+            _check(lib().ugx_memcpy_htod(self.ptr, _ptr(self.host), self.host.nbytes), "ugx_memcpy_htod")
+
+    def numpy(self):
+        out = np.empty_like(self.host)
+        if out.nbytes:
+            _check(lib().ugx_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes), "ugx_memcpy_dtoh")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.ugx_free(self.ptr)
+            self.ptr = None
+
+
+def time_launches(fn, reps, stream=None):
+    """Average seconds per call of fn (which enqueues work on the library
+    stream), measured with HIP events recorded on that stream."""
+    L = lib()
+    e0 = _handle(L.ugx_event_create(), "event")
+    e1 = _handle(L.ugx_event_create(), "event")
+    try:
+        fn()
+        synchronize()
+        _check(L.ugx_event_record(e0, stream), "event_record")
+        for _ in range(reps):
+            fn()
+        _check(L.ugx_event_record(e1, stream), "event_record")
+        ms = C.c_double()
+        _check(L.ugx_event_elapsed_ms(e0, e1, C.byref(ms)), "event_elapsed")
+        return ms.value * 1e-3 / reps
+    finally:
+        L.ugx_event_destroy(e0)
+        L.ugx_event_destroy(e1)
+
+
+def hierarchy_numbering(kind, num_refs, level):
+    """LEVEL DoF numbering (lattice point -> DoF) computed on the host; returns
+    (dof_of_lattice, n_inner)."""
+    dim = 3 if kind == 0 else 2
+    out = np.empty((2 ** level + 1) ** dim, np.int32)
+    ninner = C.c_int64(0)
+    _check(lib().ugx_hierarchy_numbering(kind, num_refs, level, _ptr(out), C.byref(ninner)), "hierarchy_numbering")
+    return out, ninner.value
+
+
+class Vector:
+    """GPUVector (vector_type of the GPU1 algebra), FP64 device storage."""
+
+    def __init__(self, n=None, values=None):
+        if values is not None:
+            values = np.ascontiguousarray(values, np.float64)
+            n = values.size
+        self.h = _handle(lib().ugx_vector_create(int(n)), "Vector")
+        if values is not None:
+            self.upload(values)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_vector_destroy(self.h)
+            self.h = None
+
+    def size(self):
+        return lib().ugx_vector_size(self.h)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.size()
+        _check(lib().ugx_vector_upload(self.h, _ptr(a)), "Vector.upload")
+
+    def numpy(self):
+        out = np.empty(self.size(), np.float64)
+        _check(lib().ugx_vector_download(self.h, _ptr(out)), "Vector.download")
+        return out
+
+    def set(self, v):
+        _check(lib().ugx_vector_set(self.h, float(v)), "Vector.set")
+
+    def device_ptr(self):
+        return lib().ugx_vector_device_ptr(self.h)
+
+
+class SparseMatrix:
+    """GPUSparseMatrix (matrix_type of the GPU1 algebra): device CSR, int32 indices."""
+
+    def __init__(self):
+        self.h = _handle(lib().ugx_matrix_create(), "SparseMatrix")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_matrix_destroy(self.h)
+            self.h = None
+
+    def num_rows(self):
+        return lib().ugx_matrix_num_rows(self.h)
+
+    def nnz(self):
+        return lib().ugx_matrix_nnz(self.h)
+
+    def upload_csr(self, rowptr, col, val, n_cols=None):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        rows = rowptr.size - 1
+        _check(lib().ugx_matrix_upload_csr(self.h, rows, n_cols if n_cols is not None else rows, _ptr(rowptr),
+                                           _ptr(col), _ptr(val)), "SparseMatrix.upload_csr")
+
+    def download_csr(self):
+        n = self.num_rows()
+        nnz = self.nnz()
+        rp = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        _check(lib().ugx_matrix_download_csr(self.h, _ptr(rp), _ptr(col), _ptr(val)), "SparseMatrix.download")
+        return rp, col, val
+
+    def device_csr(self):
+        rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        lib().ugx_matrix_device_csr(self.h, C.byref(rp), C.byref(col), C.byref(val))
+        return rp.value, col.value, val.value
+
+    def axpy(self, dest, alpha, v1, beta, w):
+        _check(lib().ugx_matrix_axpy(self.h, dest.h, float(alpha), v1.h, float(beta), w.h), "SparseMatrix.axpy")
+
+    def apply(self, dest, w):
+        self.axpy(dest, 0.0, dest, 1.0, w)
+
+    def apply_sub(self, d, x):
+        _check(lib().ugx_matrix_apply_sub(self.h, d.h, x.h), "SparseMatrix.apply_sub")
+
+
+class DomainDiscretization:
+    """Structured domain (kind 0: unit cube from one hexahedron, Q1 FE;
+    kind 1: unit square from two triangles, FV1) with device assembly."""
+
+    def __init__(self, kind, num_refs):
+        self.kind = kind
+        self.num_refs = num_refs
+        self.dim = 3 if kind == 0 else 2
+        self.h = _handle(lib().ugx_disc_create(int(kind), int(num_refs)), "DomainDiscretization")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_disc_destroy(self.h)
+            self.h = None
+
+    def num_levels(self):
+        return lib().ugx_disc_num_levels(self.h)
+
+    def num_dofs(self, level=None):
+        if level is None:
+            level = self.num_levels() - 1
+        return lib().ugx_disc_num_dofs(self.h, level)
+
+    def dof_of_lattice(self, level):
+        n = (2 ** level + 1) ** self.dim
+        out = np.empty(n, np.int32)
+        _check(lib().ugx_disc_dof_of_lattice(self.h, level, _ptr(out)), "dof_of_lattice")
+        return out
+
+    def set_theta(self, theta):
+        theta = np.ascontiguousarray(theta, np.float64)
+        assert theta.size == (2 ** self.num_refs) ** self.dim
+        _check(lib().ugx_disc_set_theta(self.h, _ptr(theta)), "set_theta")
+
+    def set_source(self, f):
+        if np.isscalar(f):
+            _check(lib().ugx_disc_set_source_const(self.h, float(f)), "set_source")
+        else:
+            f = np.ascontiguousarray(f, np.float64)
+            _check(lib().ugx_disc_set_source_dofs(self.h, _ptr(f)), "set_source")
+
+    def assemble_linear(self, A, b):
+        _check(lib().ugx_disc_assemble_linear(self.h, A.h, b.h), "assemble_linear")
+
+    def assemble_seconds(self):
+        return lib().ugx_disc_assemble_seconds(self.h)
+
+
+class GeometricMultiGrid:
+    """AssembledMultiGridCycle: V(nu1,nu2) (cycle=1) or W (cycle=2), damped
+    Jacobi smoother, P1 transfer, dense FP64 LU on the base level.
+    precision 'f64' or 'f32' for the level operators."""
+
+    def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64"):
+        self.disc = disc
+        p = 1 if precision in ("f32", "float32", 1) else 0
+        self.h = _handle(lib().ugx_gmg_create(disc.h, base_level, nu1, nu2, cycle, float(omega), p),
+                         "GeometricMultiGrid")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_gmg_destroy(self.h)
+            self.h = None
+
+    def init(self, A):
+        _check(lib().ugx_gmg_init(self.h, A.h), "GMG.init")
+
+    def apply(self, c, d):
+        _check(lib().ugx_gmg_apply(self.h, c.h, d.h), "GMG.apply")
+
+    def smooth_level(self, level, nsteps=1):
+        _check(lib().ugx_gmg_smooth_level(self.h, level, nsteps), "GMG.smooth_level")
+
+    def level_dia(self, level):
+        nd = lib().ugx_gmg_level_num_diags(self.h, level)
+        n = (2 ** level + 1) ** self.disc.dim
+        out = np.empty(nd * n, np.float64)
+        _check(lib().ugx_gmg_level_dia(self.h, level, _ptr(out)), "GMG.level_dia")
+        return out.reshape(nd, n)
+
+
+class Jacobi:
+    def __init__(self, omega=1.0):
+        self.h = _handle(lib().ugx_jacobi_create(float(omega)), "Jacobi")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_jacobi_destroy(self.h)
+            self.h = None
+
+
+class _Solver:
+    _type = 0
+
+    def __init__(self, precond=None, max_steps=100, min_defect=1e-50, reduction=1e-10, verbose=False):
+        self.h = _handle(lib().ugx_solver_create(self._type), type(self).__name__)
+        self._precond = None
+        self.set_conv_check(max_steps, min_defect, reduction, verbose)
+        if precond is not None:
+            self.set_preconditioner(precond)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_solver_destroy(self.h)
+            self.h = None
+
+    def set_preconditioner(self, p):
+        kind = 1 if isinstance(p, GeometricMultiGrid) else 2 if isinstance(p, Jacobi) else 0
+        self._precond = p
+        _check(lib().ugx_solver_set_preconditioner(self.h, kind, p.h if p is not None else None), "set_preconditioner")
+
+    def set_conv_check(self, max_steps=100, min_defect=1e-50, reduction=1e-10, verbose=False):
+        _check(lib().ugx_solver_set_conv_check(self.h, int(max_steps), float(min_defect), float(reduction),
+                                               1 if verbose else 0), "set_conv_check")
+
+    def init(self, A):
+        self._A = A
+        _check(lib().ugx_solver_init(self.h, A.h), "solver.init")
+
+    def apply(self, x, b):
+        rc = _check(lib().ugx_solver_apply(self.h, x.h, b.h), "solver.apply")
+        return rc == 1
+
+    def steps(self):
+        return lib().ugx_solver_steps(self.h)
+
+    def defects(self):
+        n = lib().ugx_solver_defects(self.h, None, 0)
+        out = np.empty(n, np.float64)
+        lib().ugx_solver_defects(self.h, _ptr(out), n)
+        return out
+
+
+class CG(_Solver):
+    _type = 0
+
+
+class BiCGStab(_Solver):
+    _type = 1

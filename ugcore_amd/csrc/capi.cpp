@@ -1,0 +1,461 @@
+// C ABI (include/ugcore_amd.h) over the kernels and the GPUAlgebra classes.
+// No exception crosses this boundary: failures return negative status codes
+// and leave a message for ugx_last_error().
+#include "ugcore_amd.h"
+
+#include <memory>
+#include <string>
+
+#include "ug_gpu/gpu_algebra.hpp"
+#include "hierarchy.hpp"
+#include "ugx_internal.hpp"
+
+#include <algorithm>
+
+using namespace ug::gpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+hipStream_t S(void* s) { return s ? (hipStream_t)s : context().stream; }
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+template <typename F>
+void* guard_ptr(F&& f) {
+    try {
+        return f();
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+int launched(int rc) {
+    if (rc != 0) {
+        g_err = "kernel launch failed";
+        return -1;
+    }
+    return 0;
+}
+
+struct MatrixH {
+    std::shared_ptr<GPUSparseMatrix> A = std::make_shared<GPUSparseMatrix>();
+    std::shared_ptr<MatrixOperator> op = std::make_shared<MatrixOperator>(A);
+};
+struct VectorH {
+    GPUVector v;
+};
+struct DiscH {
+    std::shared_ptr<DomainDiscretization> d;
+};
+struct GmgH {
+    std::shared_ptr<AssembledMultiGridCycle> g;
+};
+struct JacobiH {
+    std::shared_ptr<Jacobi> j;
+};
+struct SolverH {
+    std::shared_ptr<ILinearOperatorInverse> s;
+};
+
+}  // namespace
+
+extern "C" {
+
+int ugx_init(void) {
+    return guard([] {
+        context();
+        return 0;
+    });
+}
+const char* ugx_last_error(void) { return g_err.c_str(); }
+void* ugx_stream(void) {
+    try {
+        return (void*)context().stream;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+int ugx_synchronize(void) {
+    return guard([] {
+        sync();
+        return 0;
+    });
+}
+
+// ---- device buffers / events ---------------------------------------------------
+
+void* ugx_malloc(int64_t bytes) {
+    return guard_ptr([&] {
+        context();
+        void* p = nullptr;
+        if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) throw UGError("ugx_malloc failed");
+        return p;
+    });
+}
+int ugx_free(void* p) {
+    return guard([&] {
+        if (p && hipFree(p) != hipSuccess) throw UGError("ugx_free failed");
+        return 0;
+    });
+}
+int ugx_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        auto& c = context();
+        if (hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess)
+            throw UGError("ugx_memcpy_htod failed");
+        return 0;
+    });
+}
+int ugx_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        auto& c = context();
+        if (hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess)
+            throw UGError("ugx_memcpy_dtoh failed");
+        return 0;
+    });
+}
+void* ugx_event_create(void) {
+    return guard_ptr([] {
+        context();
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) throw UGError("ugx_event_create failed");
+        return (void*)e;
+    });
+}
+int ugx_event_destroy(void* ev) {
+    return guard([&] { return hipEventDestroy((hipEvent_t)ev) == hipSuccess ? 0 : -1; });
+}
+int ugx_event_record(void* ev, void* stream) {
+    return guard([&] {
+        if (hipEventRecord((hipEvent_t)ev, S(stream)) != hipSuccess) throw UGError("ugx_event_record failed");
+        return 0;
+    });
+}
+int ugx_event_elapsed_ms(void* begin, void* end, double* ms) {
+    return guard([&] {
+        float f = 0.f;
+        if (hipEventSynchronize((hipEvent_t)end) != hipSuccess ||
+            hipEventElapsedTime(&f, (hipEvent_t)begin, (hipEvent_t)end) != hipSuccess)
+            throw UGError("ugx_event_elapsed_ms failed");
+        *ms = (double)f;
+        return 0;
+    });
+}
+
+// ---- kernels -----------------------------------------------------------------
+
+int ugx_spmv_csr_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, double alpha,
+                     const double* v1, double beta, const double* w, double* dest, void* stream) {
+    return guard([&] {
+        if (nrows < 0 || !rowptr || !col || !val || !w || !dest) throw UGError("ugx_spmv_csr_f64: bad arguments");
+        return launched(ugamd::spmv_csr<double, double>(nrows, rowptr, col, val, alpha, v1 ? v1 : dest, beta, w, dest,
+                                                        0, 16, S(stream)));
+    });
+}
+int ugx_spmv_csr_f32(int64_t nrows, const int32_t* rowptr, const int32_t* col, const float* val, double alpha,
+                     const float* v1, double beta, const float* w, float* dest, void* stream) {
+    return guard([&] {
+        if (nrows < 0 || !rowptr || !col || !val || !w || !dest) throw UGError("ugx_spmv_csr_f32: bad arguments");
+        return launched(ugamd::spmv_csr<float, float>(nrows, rowptr, col, val, alpha, v1 ? v1 : dest, beta, w, dest,
+                                                      0, 16, S(stream)));
+    });
+}
+int ugx_spmv_csr_ignore_zero_rows_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val,
+                                      double beta, const double* w, double* dest, void* stream) {
+    return guard([&] {
+        if (nrows < 0 || !rowptr || !col || !val || !w || !dest) throw UGError("ugx_spmv_csr_ignore_zero_rows_f64: bad arguments");
+        return launched(ugamd::spmv_csr<double, double>(nrows, rowptr, col, val, 0.0, dest, beta, w, dest, 1, 16,
+                                                        S(stream)));
+    });
+}
+
+int ugx_vec_scale_add2_f64(int64_t n, double* dest, double a1, const double* v1, double a2, const double* v2,
+                           void* stream) {
+    return guard([&] { return launched(ugamd::vec_scale_add<double>(n, dest, a1, v1, a2, v2, 0.0, nullptr, S(stream))); });
+}
+int ugx_vec_scale_add3_f64(int64_t n, double* dest, double a1, const double* v1, double a2, const double* v2,
+                           double a3, const double* v3, void* stream) {
+    return guard([&] { return launched(ugamd::vec_scale_add<double>(n, dest, a1, v1, a2, v2, a3, v3, S(stream))); });
+}
+int ugx_vec_set_f64(int64_t n, double* x, double value, void* stream) {
+    return guard([&] { return launched(ugamd::vec_set<double>(n, x, value, S(stream))); });
+}
+int ugx_dot_f64(int64_t n, const double* a, const double* b, double* result_host, void* stream) {
+    return guard([&] {
+        auto& c = context();
+        hipStream_t s = S(stream);
+        if (ugamd::dot<double>(n, a, b, c.red_partial, c.red_out, false, s) != 0) throw UGError("dot launch failed");
+        if (hipMemcpyAsync(c.red_host, c.red_out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            throw UGError("dot readback failed");
+        *result_host = c.red_host[0];
+        return 0;
+    });
+}
+int ugx_norm_f64(int64_t n, const double* a, double* result_host, void* stream) {
+    return guard([&] {
+        auto& c = context();
+        hipStream_t s = S(stream);
+        if (ugamd::dot<double>(n, a, a, c.red_partial, c.red_out, true, s) != 0) throw UGError("norm launch failed");
+        if (hipMemcpyAsync(c.red_host, c.red_out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            throw UGError("norm readback failed");
+        *result_host = c.red_host[0];
+        return 0;
+    });
+}
+int ugx_jacobi_diag_inv_f64(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, double omega,
+                            double* dinv, void* stream) {
+    return guard([&] {
+        auto& c = context();
+        hipStream_t s = S(stream);
+        if (hipMemsetAsync(c.flag, 0, sizeof(int), s) != hipSuccess) throw UGError("flag reset failed");
+        if (ugamd::jacobi_diag_csr<double, double>(nrows, rowptr, col, val, omega, dinv, c.flag, s) != 0)
+            throw UGError("jacobi launch failed");
+        int bad = 0;
+        if (hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            throw UGError("jacobi readback failed");
+        if (bad) {
+            g_err = "Jacobi: zero diagonal entry";
+            return -2;
+        }
+        return 0;
+    });
+}
+int ugx_jacobi_apply_f64(int64_t n, double* c, const double* dinv, const double* d, void* stream) {
+    return guard([&] { return launched(ugamd::diag_mult<double, double>(n, c, dinv, d, S(stream))); });
+}
+
+// ---- domain discretisation ---------------------------------------------------------
+
+int ugx_hierarchy_numbering(int kind, int num_refs, int level, int32_t* out, int64_t* n_inner) {
+    return guard([&] {
+        if (num_refs < 0 || num_refs > 10 || level < 0 || level > num_refs)
+            throw UGError("ugx_hierarchy_numbering: level out of range");
+        auto nums = ugamd::build_level_numbering(kind, level);
+        const auto& L = nums.back();
+        std::copy(L.dof_of_lattice.begin(), L.dof_of_lattice.end(), out);
+        if (n_inner) *n_inner = L.n_inner;
+        return 0;
+    });
+}
+
+void* ugx_disc_create(int kind, int num_refs) {
+    return guard_ptr([&] {
+        auto* h = new DiscH();
+        h->d = std::make_shared<DomainDiscretization>(kind, num_refs);
+        return (void*)h;
+    });
+}
+void ugx_disc_destroy(void* disc) { delete (DiscH*)disc; }
+int ugx_disc_num_levels(void* disc) { return ((DiscH*)disc)->d->num_levels(); }
+int64_t ugx_disc_num_dofs(void* disc, int level) {
+    auto* h = (DiscH*)disc;
+    if (level < 0 || level >= h->d->num_levels()) return -1;
+    return h->d->level(level).N;
+}
+int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out) {
+    return guard([&] {
+        auto& L = ((DiscH*)disc)->d->level(level);
+        L.dof_of_lat.download(out, L.N);
+        return 0;
+    });
+}
+int ugx_disc_set_theta(void* disc, const double* theta) {
+    return guard([&] {
+        ((DiscH*)disc)->d->set_theta(theta);
+        return 0;
+    });
+}
+int ugx_disc_set_source_const(void* disc, double f) {
+    ((DiscH*)disc)->d->set_source_constant(f);
+    return 0;
+}
+int ugx_disc_set_source_dofs(void* disc, const double* f) {
+    return guard([&] {
+        ((DiscH*)disc)->d->set_source_dofs(f);
+        return 0;
+    });
+}
+int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs) {
+    return guard([&] {
+        ((DiscH*)disc)->d->assemble_linear(*((MatrixH*)matrix)->A, ((VectorH*)rhs)->v);
+        return 0;
+    });
+}
+double ugx_disc_assemble_seconds(void* disc) { return ((DiscH*)disc)->d->assemble_seconds(); }
+
+// ---- matrices / vectors --------------------------------------------------------------
+
+void* ugx_matrix_create(void) {
+    return guard_ptr([] { return (void*)new MatrixH(); });
+}
+void ugx_matrix_destroy(void* m) { delete (MatrixH*)m; }
+int64_t ugx_matrix_num_rows(void* m) { return ((MatrixH*)m)->A->num_rows(); }
+int64_t ugx_matrix_nnz(void* m) { return ((MatrixH*)m)->A->total_num_connections(); }
+int ugx_matrix_upload_csr(void* m, int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
+                          const double* val) {
+    return guard([&] {
+        ((MatrixH*)m)->A->set_from_host_csr(rows, cols, rowptr, col, val);
+        return 0;
+    });
+}
+int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val) {
+    return guard([&] {
+        ((MatrixH*)m)->A->download_csr(rowptr, col, val);
+        return 0;
+    });
+}
+int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val) {
+    auto& A = *((MatrixH*)m)->A;
+    *rowptr = A.rowptr();
+    *col = A.col();
+    *val = A.val();
+    return 0;
+}
+int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w) {
+    return guard([&] {
+        ((MatrixH*)m)->A->axpy(((VectorH*)dest)->v, alpha, ((VectorH*)v1)->v, beta, ((VectorH*)w)->v);
+        return 0;
+    });
+}
+int ugx_matrix_apply_sub(void* m, void* d, void* x) {
+    return guard([&] {
+        ((MatrixH*)m)->A->matmul_minus(((VectorH*)d)->v, ((VectorH*)x)->v);
+        return 0;
+    });
+}
+
+void* ugx_vector_create(int64_t n) {
+    return guard_ptr([&] {
+        auto* h = new VectorH();
+        h->v.resize(n);
+        h->v.set(0.0);
+        return (void*)h;
+    });
+}
+void ugx_vector_destroy(void* v) { delete (VectorH*)v; }
+int64_t ugx_vector_size(void* v) { return ((VectorH*)v)->v.size(); }
+double* ugx_vector_device_ptr(void* v) { return ((VectorH*)v)->v.data(); }
+int ugx_vector_upload(void* v, const double* host) {
+    return guard([&] {
+        ((VectorH*)v)->v.upload(host);
+        return 0;
+    });
+}
+int ugx_vector_download(void* v, double* host) {
+    return guard([&] {
+        ((VectorH*)v)->v.download(host);
+        return 0;
+    });
+}
+int ugx_vector_set(void* v, double value) {
+    return guard([&] {
+        ((VectorH*)v)->v.set(value);
+        return 0;
+    });
+}
+
+// ---- GMG / Jacobi / solvers ---------------------------------------------------------------
+
+void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision) {
+    return guard_ptr([&] {
+        auto* h = new GmgH();
+        h->g = std::make_shared<AssembledMultiGridCycle>(((DiscH*)disc)->d);
+        h->g->set_base_level(base_level);
+        h->g->set_num_presmooth(nu1);
+        h->g->set_num_postsmooth(nu2);
+        h->g->set_cycle_type(cycle);
+        h->g->set_smoother_damp(omega);
+        h->g->set_precision(precision == 1 ? Precision::F32 : Precision::F64);
+        return (void*)h;
+    });
+}
+void ugx_gmg_destroy(void* g) { delete (GmgH*)g; }
+int ugx_gmg_init(void* g, void* matrix) {
+    return guard([&] { return ((GmgH*)g)->g->init(((MatrixH*)matrix)->op) ? 0 : -1; });
+}
+int ugx_gmg_apply(void* g, void* c, void* d) {
+    return guard([&] { return ((GmgH*)g)->g->apply(((VectorH*)c)->v, ((VectorH*)d)->v) ? 0 : -1; });
+}
+int ugx_gmg_smooth_level(void* g, int level, int nsteps) {
+    return guard([&] {
+        ((GmgH*)g)->g->smooth_level(level, nsteps);
+        return 0;
+    });
+}
+int ugx_gmg_level_num_diags(void* g, int level) { return ((GmgH*)g)->g->level_num_diags(level); }
+int ugx_gmg_level_dia(void* g, int level, double* out) {
+    return guard([&] {
+        ((GmgH*)g)->g->download_level_dia(level, out);
+        return 0;
+    });
+}
+
+void* ugx_jacobi_create(double omega) {
+    return guard_ptr([&] {
+        auto* h = new JacobiH();
+        h->j = std::make_shared<Jacobi>(omega);
+        return (void*)h;
+    });
+}
+void ugx_jacobi_destroy(void* j) { delete (JacobiH*)j; }
+
+void* ugx_solver_create(int type) {
+    return guard_ptr([&] {
+        auto* h = new SolverH();
+        if (type == 0) h->s = std::make_shared<CG>();
+        else if (type == 1) h->s = std::make_shared<BiCGStab>();
+        else throw UGError("ugx_solver_create: unknown solver type");
+        return (void*)h;
+    });
+}
+void ugx_solver_destroy(void* s) { delete (SolverH*)s; }
+int ugx_solver_set_preconditioner(void* s, int kind, void* p) {
+    return guard([&] {
+        auto& solver = *((SolverH*)s)->s;
+        if (kind == 0 || !p) solver.set_preconditioner(nullptr);
+        else if (kind == 1) solver.set_preconditioner(((GmgH*)p)->g);
+        else if (kind == 2) solver.set_preconditioner(((JacobiH*)p)->j);
+        else throw UGError("ugx_solver_set_preconditioner: unknown kind");
+        return 0;
+    });
+}
+int ugx_solver_set_conv_check(void* s, int max_steps, double min_defect, double rel_reduction, int verbose) {
+    return guard([&] {
+        ((SolverH*)s)->s->set_convergence_check(
+            std::make_shared<StdConvCheck>(max_steps, min_defect, rel_reduction, verbose != 0));
+        return 0;
+    });
+}
+int ugx_solver_init(void* s, void* matrix) {
+    return guard([&] { return ((SolverH*)s)->s->init(((MatrixH*)matrix)->op) ? 0 : -1; });
+}
+int ugx_solver_apply(void* s, void* x, void* b) {
+    return guard([&] { return ((SolverH*)s)->s->apply(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });
+}
+int ugx_solver_steps(void* s) { return ((SolverH*)s)->s->step(); }
+int ugx_solver_defects(void* s, double* out, int max_n) {
+    const auto& d = ((SolverH*)s)->s->convergence_check().defects();
+    const int n = (int)d.size();
+    if (out)
+        for (int i = 0; i < n && i < max_n; ++i) out[i] = d[i];
+    return n;
+}
+
+}  // extern "C"

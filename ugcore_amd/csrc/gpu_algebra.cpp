@@ -1,0 +1,722 @@
+// Host implementation of the GPUAlgebra classes (include/ug_gpu/gpu_algebra.hpp).
+#include "ug_gpu/gpu_algebra.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <type_traits>
+
+#include "hierarchy.hpp"
+#include "ugx_internal.hpp"
+
+namespace ug {
+namespace gpu {
+
+using namespace ugamd;
+
+#define UGX_HIP(cmd)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (cmd);                                                                            \
+        if (e_ != hipSuccess) throw UGError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                                            __FILE__ + ":" + std::to_string(__LINE__));                   \
+    } while (0)
+#define UGX_LAUNCH(cmd)                                                                          \
+    do {                                                                                         \
+        if ((cmd) != 0) throw UGError(std::string("kernel launch failed: ") + #cmd + " at line " + \
+                                      std::to_string(__LINE__));                                 \
+    } while (0)
+
+static double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// context / buffers
+// ---------------------------------------------------------------------------
+DeviceContext& context() {
+    static DeviceContext ctx;
+    static bool init = false;
+    if (!init) {
+        int n = 0;
+        UGX_HIP(hipGetDeviceCount(&n));
+        if (n < 1) throw UGError("no HIP device available");
+        UGX_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+        UGX_HIP(hipMalloc(&ctx.red_partial, 1024 * sizeof(double)));
+        UGX_HIP(hipMalloc(&ctx.red_out, 8 * sizeof(double)));
+        UGX_HIP(hipHostMalloc(&ctx.red_host, 8 * sizeof(double), hipHostMallocDefault));
+        UGX_HIP(hipMalloc(&ctx.flag, sizeof(int)));
+        UGX_HIP(hipMemset(ctx.flag, 0, sizeof(int)));
+        init = true;
+    }
+    return ctx;
+}
+
+void sync() { UGX_HIP(hipStreamSynchronize(context().stream)); }
+
+template <typename T>
+void DeviceBuffer<T>::resize(int64_t n) {
+    if (n == n_) return;
+    release();
+    if (n > 0) {
+        context();
+        UGX_HIP(hipMalloc(&p_, sizeof(T) * (size_t)n));
+    }
+    n_ = n;
+}
+template <typename T>
+void DeviceBuffer<T>::release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+template <typename T>
+void DeviceBuffer<T>::upload(const T* h, int64_t n, int64_t offset) {
+    if (n <= 0) return;
+    if (offset + n > n_) throw UGError("DeviceBuffer::upload out of range");
+    UGX_HIP(hipMemcpyAsync(p_ + offset, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, context().stream));
+    sync();
+}
+template <typename T>
+void DeviceBuffer<T>::download(T* h, int64_t n, int64_t offset) const {
+    if (n <= 0) return;
+    if (offset + n > n_) throw UGError("DeviceBuffer::download out of range");
+    UGX_HIP(hipMemcpyAsync(h, p_ + offset, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, context().stream));
+    sync();
+}
+template <typename T>
+void DeviceBuffer<T>::zero() {
+    if (n_ > 0) UGX_HIP(hipMemsetAsync(p_, 0, sizeof(T) * (size_t)n_, context().stream));
+}
+template class DeviceBuffer<double>;
+template class DeviceBuffer<float>;
+template class DeviceBuffer<int32_t>;
+template class DeviceBuffer<uint8_t>;
+
+// ---------------------------------------------------------------------------
+// GPUVector
+// ---------------------------------------------------------------------------
+GPUVector::GPUVector(const GPUVector& o) {
+    resize(o.n_);
+    if (n_) UGX_HIP(hipMemcpyAsync(buf_.get(), o.buf_.get(), sizeof(double) * n_, hipMemcpyDeviceToDevice, context().stream));
+}
+GPUVector& GPUVector::operator=(const GPUVector& o) {
+    if (this == &o) return *this;
+    resize(o.n_);
+    if (n_) UGX_HIP(hipMemcpyAsync(buf_.get(), o.buf_.get(), sizeof(double) * n_, hipMemcpyDeviceToDevice, context().stream));
+    return *this;
+}
+void GPUVector::resize(int64_t n) {
+    buf_.resize(n);
+    n_ = n;
+}
+void GPUVector::set(double v) { UGX_LAUNCH(vec_set<double>(n_, buf_.get(), v, context().stream)); }
+
+static double reduce_to_host(int slot) {
+    auto& c = context();
+    UGX_HIP(hipMemcpyAsync(c.red_host + slot, c.red_out + slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    sync();
+    return c.red_host[slot];
+}
+
+double GPUVector::norm() const {
+    auto& c = context();
+    UGX_LAUNCH(dot<double>(n_, buf_.get(), buf_.get(), c.red_partial, c.red_out, true, c.stream));
+    return reduce_to_host(0);
+}
+double GPUVector::dotprod(const GPUVector& b) const {
+    if (b.n_ != n_) throw UGError("GPUVector::dotprod size mismatch");
+    auto& c = context();
+    UGX_LAUNCH(dot<double>(n_, buf_.get(), b.buf_.get(), c.red_partial, c.red_out, false, c.stream));
+    return reduce_to_host(0);
+}
+GPUVector& GPUVector::operator+=(const GPUVector& b) {
+    UGX_LAUNCH(vec_scale_add<double>(n_, buf_.get(), 1.0, buf_.get(), 1.0, b.buf_.get(), 0.0, nullptr, context().stream));
+    return *this;
+}
+GPUVector& GPUVector::operator-=(const GPUVector& b) {
+    UGX_LAUNCH(vec_scale_add<double>(n_, buf_.get(), 1.0, buf_.get(), -1.0, b.buf_.get(), 0.0, nullptr, context().stream));
+    return *this;
+}
+GPUVector& GPUVector::operator*=(double a) {
+    UGX_LAUNCH(vec_scale_add<double>(n_, buf_.get(), a, buf_.get(), 0.0, buf_.get(), 0.0, nullptr, context().stream));
+    return *this;
+}
+
+void VecScaleAdd(GPUVector& dest, double a1, const GPUVector& v1, double a2, const GPUVector& v2) {
+    UGX_LAUNCH(vec_scale_add<double>(dest.size(), dest.data(), a1, v1.data(), a2, v2.data(), 0.0, nullptr,
+                                     context().stream));
+}
+void VecScaleAdd(GPUVector& dest, double a1, const GPUVector& v1, double a2, const GPUVector& v2, double a3,
+                 const GPUVector& v3) {
+    UGX_LAUNCH(vec_scale_add<double>(dest.size(), dest.data(), a1, v1.data(), a2, v2.data(), a3, v3.data(),
+                                     context().stream));
+}
+
+// ---------------------------------------------------------------------------
+// GPUSparseMatrix
+// ---------------------------------------------------------------------------
+static int lanes_for(int64_t rows, int64_t nnz) {
+    const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
+    if (avg <= 6) return 4;
+    if (avg <= 14) return 8;
+    if (avg <= 40) return 16;
+    return 32;
+}
+
+void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
+    if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix: nnz exceeds int32 range");
+    rows_ = rows;
+    cols_ = cols;
+    nnz_ = nnz;
+    rowptr_.resize(rows + 1);
+    col_.resize(nnz);
+    val_.resize(nnz);
+    lanes_ = lanes_for(rows, nnz);
+}
+
+void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
+                                        const double* val) {
+    allocate(rows, cols, rowptr[rows]);
+    rowptr_.upload(rowptr, rows + 1);
+    col_.upload(col, nnz_);
+    val_.upload(val, nnz_);
+}
+
+void GPUSparseMatrix::download_csr(int32_t* rowptr, int32_t* col, double* val) const {
+    if (rowptr) rowptr_.download(rowptr, rows_ + 1);
+    if (col) col_.download(col, nnz_);
+    if (val) val_.download(val, nnz_);
+}
+
+void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, double b1, const GPUVector& w1) const {
+    if (dest.size() != rows_ || w1.size() != cols_) throw UGError("GPUSparseMatrix::axpy size mismatch");
+    if (&dest == &w1) throw UGError("GPUSparseMatrix::axpy: dest must not alias the multiplied vector");
+    UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), a1, v1.data(), b1, w1.data(),
+                                         dest.data(), 0, lanes_, context().stream)));
+}
+
+void GPUSparseMatrix::apply_ignore_zero_rows(GPUVector& dest, double beta, const GPUVector& w) const {
+    UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), 0.0, dest.data(), beta,
+                                         w.data(), dest.data(), 1, lanes_, context().stream)));
+}
+
+// ---------------------------------------------------------------------------
+// iterators / solvers
+// ---------------------------------------------------------------------------
+bool ILinearIterator::apply_update_defect(GPUVector& c, GPUVector& d) {
+    if (!apply(c, d)) return false;
+    op_->apply_sub(d, c);
+    return true;
+}
+
+void StdConvCheck::start_defect(double d) {
+    defects_.assign(1, d);
+    initial_ = current_ = d;
+    step_ = 0;
+    if (verbose_) std::printf("%% %s\n  Iter      Defect         Rate\n%4d:    %e      -------\n", name_.c_str(), 0, d);
+}
+void StdConvCheck::update_defect(double d) {
+    last_ = current_;
+    current_ = d;
+    ++step_;
+    defects_.push_back(d);
+    if (verbose_) std::printf("%4d:    %e    %e\n", step_, d, d / last_);
+}
+bool StdConvCheck::iteration_ended() const {
+    if (!std::isfinite(current_)) return true;
+    if (step_ >= max_steps_) return true;
+    if (current_ < min_defect_) return true;
+    if (reduction() < rel_reduction_) return true;
+    return false;
+}
+bool StdConvCheck::post() const { return current_ < min_defect_ || reduction() < rel_reduction_; }
+
+bool Jacobi::init(std::shared_ptr<MatrixOperator> J) {
+    op_ = J;
+    auto& A = J->get_matrix();
+    dinv_.resize(A.num_rows());
+    auto& c = context();
+    UGX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    UGX_LAUNCH((jacobi_diag_csr<double, double>(A.num_rows(), A.rowptr(), A.col(), A.val(), omega_, dinv_.get(),
+                                                c.flag, c.stream)));
+    int bad = 0;
+    UGX_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    sync();
+    return bad == 0;
+}
+
+bool Jacobi::apply(GPUVector& c, const GPUVector& d) {
+    UGX_LAUNCH((diag_mult<double, double>(d.size(), c.data(), dinv_.get(), d.data(), context().stream)));
+    return true;
+}
+
+bool ILinearOperatorInverse::init(std::shared_ptr<MatrixOperator> J) {
+    op_ = J;
+    if (precond_ && !precond_->init(J)) return false;
+    return true;
+}
+
+bool ILinearOperatorInverse::apply(GPUVector& x, const GPUVector& b) {
+    GPUVector r(b);
+    return apply_return_defect(x, r);
+}
+
+bool CG::apply_return_defect(GPUVector& x, GPUVector& b) {
+    // cg.h:103-242
+    GPUVector& r = b;
+    op_->apply_sub(r, x);
+    GPUVector q(r.size()), z(x.size()), p(x.size());
+    if (precond_) {
+        if (!precond_->apply(z, r)) return false;
+    } else {
+        z = r;
+    }
+    conv_->set_name(name());
+    conv_->start(r);
+    p = z;
+    double rho_old = z.dotprod(r), rho;
+    while (!conv_->iteration_ended()) {
+        op_->apply(q, p);
+        double lambda = q.dotprod(p);
+        if (lambda == 0.0) {
+            if (p.size()) return false;
+            lambda = 1.0;
+        }
+        const double alpha = rho_old / lambda;
+        VecScaleAdd(x, 1.0, x, alpha, p);
+        VecScaleAdd(r, 1.0, r, -alpha, q);
+        conv_->update(r);
+        if (conv_->iteration_ended()) break;
+        if (precond_) {
+            if (!precond_->apply(z, r)) return false;
+        } else {
+            z = r;
+        }
+        rho = z.dotprod(r);
+        const double beta = rho / rho_old;
+        VecScaleAdd(p, beta, p, 1.0, z);
+        rho_old = rho;
+    }
+    return conv_->post();
+}
+
+bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
+    // bicgstab.h:112-385 with m_numRestarts = 0 and m_minOrtho = 0
+    op_->apply_sub(b, x);
+    GPUVector& r = b;
+    const int64_t n = r.size();
+    GPUVector r0(n), p(n), v(n), t(n), s(n), q(n);
+    conv_->set_name(name());
+    conv_->start(r);
+    double rho = 1, alpha = 1, omega = 1, norm_r0 = 0.0;
+    bool restart = true;
+    while (!conv_->iteration_ended()) {
+        if (restart) {
+            r0 = r;
+            p.set(0.0);
+            alpha = 0.0;
+            v.set(0.0);
+            omega = 1.0;
+            rho = 1.0;
+            norm_r0 = conv_->defect();
+            restart = false;
+        }
+        const double rho_old = rho;
+        rho = r0.dotprod(r);
+        const double norm_r = conv_->defect();
+        if (std::fabs(rho) / (norm_r * norm_r0) <= 0.0) restart = true;
+        if (rho_old == 0.0) return false;
+        const double beta = (rho / rho_old) * (alpha / omega);
+        VecScaleAdd(p, 1.0, r, beta, p, -beta * omega, v);
+        if (precond_) {
+            if (!precond_->apply(q, p)) return false;
+        } else {
+            q = p;
+        }
+        op_->apply(v, q);
+        alpha = v.dotprod(r0);
+        if (alpha == 0.0) return false;
+        alpha = rho / alpha;
+        VecScaleAdd(x, 1.0, x, alpha, q);
+        VecScaleAdd(s, 1.0, r, -alpha, v);
+        conv_->update(s);
+        if (conv_->iteration_ended()) {
+            r = s;
+            break;
+        }
+        if (precond_) {
+            if (!precond_->apply(q, s)) return false;
+        } else {
+            q = s;
+        }
+        op_->apply(t, q);
+        const double tt = t.dotprod(t);
+        omega = s.dotprod(t);
+        if (tt == 0.0) return false;
+        omega = omega / tt;
+        VecScaleAdd(x, 1.0, x, omega, q);
+        VecScaleAdd(r, 1.0, s, -omega, t);
+        conv_->update(r);
+        if (omega == 0.0) return false;
+    }
+    return conv_->post();
+}
+
+// ---------------------------------------------------------------------------
+// DomainDiscretization (structured)
+// ---------------------------------------------------------------------------
+DomainDiscretization::DomainDiscretization(int kind, int num_refs) : kind_(kind), refs_(num_refs) {
+    if (kind != 0 && kind != 1) throw UGError("DomainDiscretization: unknown domain kind");
+    if (num_refs < 0 || num_refs > (kind == 0 ? 9 : 13)) throw UGError("DomainDiscretization: refinement out of range");
+    auto nums = build_level_numbering(kind, num_refs);
+    auto& c = context();
+    for (auto& ln : nums) {
+        auto L = std::make_unique<LevelGrid>();
+        L->dim = ln.dim;
+        L->n = ln.n;
+        L->W = ln.n + 1;
+        L->N = (int64_t)ln.dof_of_lattice.size();
+        L->n_inner = ln.n_inner;
+        if (ln.n_dof != L->N) throw UGError("DomainDiscretization: numbering does not cover the lattice");
+        {
+            // the numbering must be a permutation of the lattice points
+            std::vector<uint8_t> seen(L->N, 0);
+            for (int32_t d : ln.dof_of_lattice) {
+                if (d < 0 || d >= L->N || seen[d]) throw UGError("DomainDiscretization: numbering is not a permutation");
+                seen[d] = 1;
+            }
+        }
+        L->dof_of_lat.resize(L->N);
+        L->dof_of_lat.upload(ln.dof_of_lattice.data(), L->N);
+        L->lat_of_dof.resize(L->N);
+        UGX_LAUNCH(invert_map(L->N, L->dof_of_lat.get(), L->lat_of_dof.get(), c.stream));
+        L->dir.resize(L->N);
+        UGX_LAUNCH(boundary_flags(L->dim, L->W, L->N, L->dir.get(), c.stream));
+        levels_.push_back(std::move(L));
+    }
+    sync();
+}
+
+void DomainDiscretization::set_theta(const double* theta_host) {
+    const int64_t nf = (int64_t)1 << refs_;
+    const int64_t n = kind_ == 0 ? nf * nf * nf : nf * nf;
+    theta_.resize(n);
+    theta_.upload(theta_host, n);
+    has_theta_ = true;
+}
+
+void DomainDiscretization::set_source_dofs(const double* f_host) {
+    const int64_t n = levels_.back()->N;
+    f_dof_.resize(n);
+    f_dof_.upload(f_host, n);
+    has_f_ = true;
+}
+
+void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
+    auto& c = context();
+    const double t0 = wall();
+    LevelGrid& L = *levels_.back();
+    DeviceBuffer<int32_t> cnt(L.N);
+    UGX_LAUNCH(pattern_count(L.dim, L.W, L.N, L.lat_of_dof.get(), cnt.get(), c.stream));
+    std::vector<int32_t> h(L.N + 1, 0);
+    cnt.download(h.data() + 1, L.N);
+    int64_t acc = 0;
+    for (int64_t i = 1; i <= L.N; ++i) {
+        acc += h[i];
+        if (acc >= ((int64_t)1 << 31)) throw UGError("assemble_linear: nnz exceeds int32 range");
+        h[i] = (int32_t)acc;
+    }
+    A.allocate(L.N, L.N, acc);
+    UGX_HIP(hipMemcpyAsync(A.rowptr(), h.data(), sizeof(int32_t) * (L.N + 1), hipMemcpyHostToDevice, c.stream));
+    UGX_LAUNCH(pattern_fill(L.dim, L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), A.rowptr(), A.col(), c.stream));
+    UGX_HIP(hipMemsetAsync(A.val(), 0, sizeof(double) * acc, c.stream));
+    b.resize(L.N);
+    b.set(0.0);
+    const double hsz = 1.0 / L.n;
+    const int nfine = 1 << refs_;
+    if (kind_ == 0) {
+        UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), A.rowptr(),
+                                           A.col(), A.val(), nullptr, 0, b.data(), c.stream));
+    } else {
+        UGX_LAUNCH(assemble_fv1_tri<double>(L.n, hsz, theta_device(), nfine, has_f_ ? f_dof_.get() : nullptr,
+                                            L.dof_of_lat.get(), A.rowptr(), A.col(), A.val(), nullptr, 0, b.data(),
+                                            c.stream));
+    }
+    UGX_LAUNCH(dirichlet_csr<double>(L.N, L.n_inner, A.rowptr(), A.col(), A.val(), b.data(), c.stream));
+    sync();
+    t_assemble_ = wall() - t0;
+}
+
+void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd,
+                                              std::vector<int32_t>& offsets) {
+    auto& c = context();
+    LevelGrid& L = *levels_[lev];
+    nd = L.dim == 3 ? 27 : 9;
+    offsets.assign(nd, 0);
+    const int64_t W = L.W;
+    for (int d = 0; d < nd; ++d) {
+        const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = L.dim == 3 ? d / 9 - 1 : 0;
+        offsets[d] = (int32_t)(dx + dy * W + dz * W * W);
+    }
+    dia.resize((int64_t)nd * L.N);
+    dia.zero();
+    const double hsz = 1.0 / L.n;
+    const int nfine = 1 << refs_;
+    if (kind_ == 0) {
+        UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), nullptr,
+                                           nullptr, nullptr, dia.get(), L.N, nullptr, c.stream));
+    } else {
+        UGX_LAUNCH(assemble_fv1_tri<double>(L.n, hsz, theta_device(), nfine, nullptr, L.dof_of_lat.get(), nullptr,
+                                            nullptr, nullptr, dia.get(), L.N, nullptr, c.stream));
+    }
+    UGX_LAUNCH(dirichlet_dia<double>(L.N, nd, nd / 2, L.dir.get(), dia.get(), c.stream));
+}
+
+// ---------------------------------------------------------------------------
+// AssembledMultiGridCycle
+// ---------------------------------------------------------------------------
+struct AssembledMultiGridCycle::Impl {
+    virtual ~Impl() = default;
+    virtual void init() = 0;
+    virtual void apply(GPUVector& c, const GPUVector& d) = 0;
+    virtual void download_level_dia(int lev, double* out) const = 0;
+    virtual int num_diags(int lev) const = 0;
+    virtual void smooth(int lev, int nsteps) = 0;
+};
+
+namespace {
+
+// dense LU with partial pivoting (no_lapack/lu_decomp.h:47-200), FP64 on the host
+bool host_lu(int n, std::vector<double>& a, std::vector<int32_t>& piv) {
+    piv.assign(n, 0);
+    auto M = [&](int r, int c) -> double& { return a[(size_t)r * n + c]; };
+    for (int k = 0; k < n; ++k) {
+        int big = k;
+        for (int j = k + 1; j < n; ++j)
+            if (std::fabs(M(big, k)) < std::fabs(M(j, k))) big = j;
+        if (big != k)
+            for (int j = 0; j < n; ++j) std::swap(M(k, j), M(big, j));
+        piv[k] = big;
+        if (std::fabs(M(k, k)) < 1e-10) return false;
+        for (int i = k + 1; i < n; ++i) {
+            M(i, k) = M(i, k) / M(k, k);
+            for (int j = k + 1; j < n; ++j) M(i, j) = M(i, j) - M(i, k) * M(k, j);
+        }
+    }
+    return true;
+}
+
+template <typename T>
+struct GmgLevel {
+    int dim = 3, W = 0, nd = 0;
+    int64_t N = 0, H = 0;
+    std::vector<int32_t> off;
+    DeviceBuffer<T> A;
+    DeviceBuffer<T> b_dinv, b_sc, b_sd, b_sd2, b_st;
+    T *dinv = nullptr, *sc = nullptr, *sd = nullptr, *sd2 = nullptr, *st = nullptr;
+    const uint8_t* dir = nullptr;
+};
+
+template <typename T>
+struct GmgImpl : public AssembledMultiGridCycle::Impl {
+    DomainDiscretization* disc;
+    int base, top, nu1, nu2, cycle;
+    double omega;
+    std::vector<GmgLevel<T>> L;
+    DeviceBuffer<double> lu, lu_work;
+    DeviceBuffer<int32_t> lu_piv, lu_rows;
+    int lu_n = 0;
+
+    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w)
+        : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w) {}
+
+    void alloc_vec(DeviceBuffer<T>& buf, T*& p, int64_t N, int64_t H) {
+        buf.resize(N + 2 * H);
+        buf.zero();
+        p = buf.get() + H;
+    }
+
+    void init() override {
+        auto& c = context();
+        if (base < 0 || base > top) throw UGError("GMG: base level out of range");
+        L.clear();
+        L.resize(top + 1);
+        for (int lev = base; lev <= top; ++lev) {
+            LevelGrid& G = disc->level(lev);
+            GmgLevel<T>& l = L[lev];
+            l.dim = G.dim;
+            l.W = G.W;
+            l.N = G.N;
+            l.dir = G.dir.get();
+            const int64_t W = G.W;
+            l.H = G.dim == 3 ? ((W * W + W + 1 + 63) / 64) * 64 : ((W + 1 + 63) / 64) * 64;
+            DeviceBuffer<double> A64;
+            disc->assemble_level_dia(lev, A64, l.nd, l.off);
+            alloc_vec(l.b_dinv, l.dinv, l.N, l.H);
+            alloc_vec(l.b_sc, l.sc, l.N, l.H);
+            alloc_vec(l.b_sd, l.sd, l.N, l.H);
+            alloc_vec(l.b_sd2, l.sd2, l.N, l.H);
+            alloc_vec(l.b_st, l.st, l.N, l.H);
+            // Jacobi: diagInv = 1/(a_ii * (1/omega)) from the FP64 centre diagonal
+            DeviceBuffer<double> dinv64(l.N);
+            UGX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+            UGX_LAUNCH((dia_jacobi_diag<double, double>(l.N, A64.get() + (int64_t)(l.nd / 2) * l.N, omega,
+                                                        dinv64.get(), c.flag, c.stream)));
+            if (lev == base) {
+                std::vector<double> h((size_t)l.nd * l.N);
+                A64.download(h.data(), (int64_t)h.size());
+                lu_n = (int)l.N;
+                if (lu_n > 4096) throw UGError("GMG: base level too large for the dense LU");
+                std::vector<double> dense((size_t)lu_n * lu_n, 0.0);
+                for (int64_t i = 0; i < l.N; ++i)
+                    for (int d = 0; d < l.nd; ++d) {
+                        const int64_t j = i + l.off[d];
+                        const double v = h[(size_t)d * l.N + i];
+                        if (j < 0 || j >= l.N) continue;
+                        dense[(size_t)i * lu_n + j] += v;
+                    }
+                std::vector<int32_t> piv;
+                if (!host_lu(lu_n, dense, piv)) throw UGError("GMG: base level matrix is singular");
+                lu.resize((int64_t)dense.size());
+                lu.upload(dense.data(), (int64_t)dense.size());
+                lu_piv.resize(lu_n);
+                lu_piv.upload(piv.data(), lu_n);
+                std::vector<int32_t> rows(lu_n);
+                std::iota(rows.begin(), rows.end(), 0);
+                lu_rows.resize(lu_n);
+                lu_rows.upload(rows.data(), lu_n);
+                lu_work.resize(lu_n);
+            }
+            if constexpr (std::is_same<T, double>::value) {
+                l.A = std::move(A64);
+                UGX_HIP(hipMemcpyAsync(l.dinv, dinv64.get(), sizeof(double) * l.N, hipMemcpyDeviceToDevice, c.stream));
+            } else {
+                l.A.resize((int64_t)l.nd * l.N);
+                UGX_LAUNCH((convert<double, T>((int64_t)l.nd * l.N, A64.get(), l.A.get(), c.stream)));
+                UGX_LAUNCH((convert<double, T>(l.N, dinv64.get(), l.dinv, c.stream)));
+            }
+            sync();
+            int bad = 0;
+            UGX_HIP(hipMemcpy(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost));
+            if (bad && lev > base) throw UGError("GMG: Jacobi found a zero diagonal");
+        }
+    }
+
+    void base_solve(int lev) {
+        auto& c = context();
+        GmgLevel<T>& l = L[lev];
+        UGX_LAUNCH(lu_solve<T>(lu_n, lu.get(), lu_piv.get(), lu_rows.get(), l.sd, l.sc, lu_work.get(), c.stream));
+        if (lev >= top) UGX_LAUNCH((dia_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.A.get(), l.sc, l.sd, l.sd, c.stream)));
+    }
+
+    void presmooth_and_restriction(int lev) {
+        auto& c = context();
+        GmgLevel<T>& f = L[lev];
+        GmgLevel<T>& cl = L[lev - 1];
+        for (int nu = 0; nu < nu1; ++nu) {
+            UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+            std::swap(f.sd, f.sd2);
+        }
+        UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
+        UGX_LAUNCH(restrict_defect<T>(f.dim, cl.W, f.W, cl.N, f.sd, cl.dir, f.dir, cl.sd, c.stream));
+    }
+
+    void prolongation_and_postsmooth(int lev) {
+        auto& c = context();
+        GmgLevel<T>& f = L[lev];
+        GmgLevel<T>& cl = L[lev - 1];
+        UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
+        if (nu2 == 0) {
+            if (lev >= top) UGX_LAUNCH((dia_apply_sub<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.st, f.sd, f.sd, c.stream)));
+            return;
+        }
+        UGX_LAUNCH((dia_apply_sub<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.st, f.sd, f.sd, c.stream)));
+        for (int nu = 0; nu < nu2; ++nu) {
+            const bool last = (nu == nu2 - 1);
+            if (!last || lev >= top) {
+                UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+                std::swap(f.sd, f.sd2);
+            } else {
+                UGX_LAUNCH(jacobi_accumulate<T>(f.N, f.dinv, f.sd, f.sc, c.stream));
+            }
+        }
+    }
+
+    void lmgc(int lev) {
+        if (lev == base) {
+            base_solve(top);
+            return;
+        }
+        presmooth_and_restriction(lev);
+        if (lev - 1 == base) base_solve(lev - 1);
+        else for (int i = 0; i < cycle; ++i) lmgc(lev - 1);
+        prolongation_and_postsmooth(lev);
+    }
+
+    void apply(GPUVector& cvec, const GPUVector& d) override {
+        auto& c = context();
+        LevelGrid& G = disc->level(top);
+        GmgLevel<T>& t = L[top];
+        if (d.size() != t.N || cvec.size() != t.N) throw UGError("GMG::apply: size mismatch");
+        UGX_LAUNCH((permute_to<double, T>(t.N, G.lat_of_dof.get(), d.data(), t.sd, c.stream)));
+        cvec.set(0.0);
+        UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
+        lmgc(top);
+        UGX_LAUNCH((permute_add_from<T, double>(t.N, G.lat_of_dof.get(), t.sc, cvec.data(), c.stream)));
+    }
+
+    void download_level_dia(int lev, double* out) const override {
+        const GmgLevel<T>& l = L[lev];
+        const int64_t n = (int64_t)l.nd * l.N;
+        std::vector<T> h(n);
+        l.A.download(h.data(), n);
+        for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i];
+    }
+    int num_diags(int lev) const override { return L[lev].nd; }
+    void smooth(int lev, int nsteps) override {
+        if (lev <= base || lev > top) throw UGError("GMG::smooth_level: level out of range");
+        auto& c = context();
+        GmgLevel<T>& f = L[lev];
+        for (int k = 0; k < nsteps; ++k) {
+            UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+            std::swap(f.sd, f.sd2);
+        }
+    }
+};
+
+}  // namespace
+
+AssembledMultiGridCycle::AssembledMultiGridCycle(std::shared_ptr<DomainDiscretization> disc) : disc_(std::move(disc)) {}
+AssembledMultiGridCycle::~AssembledMultiGridCycle() { delete impl_; }
+
+bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
+    op_ = J;
+    delete impl_;
+    impl_ = nullptr;
+    if (prec_ == Precision::F64) impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_);
+    else impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_);
+    impl_->init();
+    return true;
+}
+
+bool AssembledMultiGridCycle::apply(GPUVector& c, const GPUVector& d) {
+    if (!impl_) throw UGError("GMG::apply: not initialised");
+    impl_->apply(c, d);
+    return true;
+}
+
+void AssembledMultiGridCycle::download_level_dia(int lev, double* out) const {
+    if (!impl_) throw UGError("GMG: not initialised");
+    impl_->download_level_dia(lev, out);
+}
+void AssembledMultiGridCycle::smooth_level(int lev, int nsteps) {
+    if (!impl_) throw UGError("GMG: not initialised");
+    impl_->smooth(lev, nsteps);
+}
+int AssembledMultiGridCycle::level_num_diags(int lev) const { return impl_ ? impl_->num_diags(lev) : 0; }
+
+}  // namespace gpu
+}  // namespace ug

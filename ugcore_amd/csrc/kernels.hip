@@ -1,0 +1,973 @@
+// HIP kernels for gfx950 (MI355X) behind the ugx_* C ABI.
+//
+// Roles (reference files they replace or restate):
+//  * CSR SpMV          SparseMatrix::axpy / apply_ignore_zero_rows
+//                      (lib_algebra/cpu_algebra/sparsematrix_impl.h:271-339)
+//  * vector ops        VecScaleAdd 2/3-term, set, scale (lib_algebra/common/operations_vec.h:55-63,162-202)
+//  * reductions        Vector::dotprod / norm (cpu_algebra/vector_impl.h:70-79, 323-329)
+//  * Jacobi            jacobi.h:155-249 (diagInv = 1/(a_ii/omega), c = diagInv*d)
+//  * level operators   AssembledMultiGridCycle smoothing/transfer stages
+//                      (lib_disc/operator/linear_operator/multi_grid_solver/mg_solver_impl.hpp:1686-1964)
+//                      on diagonal-offset (DIA) level matrices in lattice order
+//  * transfer          StdTransfer P1 prolongation/restriction with Dirichlet
+//                      adjustment (std_transfer_impl.h:45-168, lagrange_dirichlet_boundary_impl.h:485-740)
+//  * assembly          Q1 FE hexahedra / FV1 triangles, colour-ordered scatter
+//
+// All kernels are stream ordered; none allocates. Wavefront size is 64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ugx_internal.hpp"
+
+namespace ugamd {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous range of logical blocks (its L2 then sees neighbouring
+// rows, which share x-vector lines).
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+    const int64_t b = blockIdx.x;
+    if (nblocks < 64) return b;
+    const int64_t per = (nblocks + 7) / 8;
+    const int64_t xcd = b % 8, slot = b / 8;
+    const int64_t lb = xcd * per + slot;
+    return lb;   // may exceed nblocks-1 for the last XCD: callers bound-check rows
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_group(T v, int G) {
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV: dest = alpha*v1 + beta*A*w   (mode 0)
+//           dest = beta*A*w on non-empty rows only (mode 1, apply_ignore_zero_rows)
+// G lanes cooperate on one row; rowptr/col are int32.
+// ---------------------------------------------------------------------------
+template <typename TM, typename TV, int G>
+__global__ __launch_bounds__(256) void k_spmv_csr(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col, const TM* __restrict__ val,
+                                                  double alpha, const TV* v1, double beta,
+                                                  const TV* __restrict__ w, TV* dest, int mode,
+                                                  int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    const int lane = threadIdx.x % G;
+    const int64_t row = (lb * 256 + threadIdx.x) / G;
+    const bool active = (lb < nblocks) && (row < nrows);
+    double s = 0.0;
+    int32_t rb = 0, re = 0;
+    if (active) {
+        rb = rowptr[row];
+        re = rowptr[row + 1];
+        for (int32_t k = rb + lane; k < re; k += G) s += (double)val[k] * (double)w[col[k]];
+    }
+    s = wave_sum_group(s, G);
+    if (active && lane == 0) {
+        if (mode == 1) {
+            if (re > rb) dest[row] = (TV)(beta * s);
+        } else {
+            const double base = (alpha == 0.0) ? 0.0 : alpha * (double)v1[row];
+            dest[row] = (TV)(base + beta * s);
+        }
+    }
+}
+
+template <typename TM, typename TV>
+static int launch_spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* col, const TM* val,
+                           double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode,
+                           int lanes, hipStream_t s) {
+    if (nrows <= 0) return 0;
+    int G = lanes;
+    if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) G = 8;
+    const int64_t rows_per_block = 256 / G;
+    const int64_t nblocks = (nrows + rows_per_block - 1) / rows_per_block;
+    const int64_t grid = nblocks < 64 ? nblocks : ((nblocks + 7) / 8) * 8;
+    switch (G) {
+        case 4: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 4>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
+        case 8: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 8>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
+        case 16: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 16>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
+        case 32: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 32>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
+        default: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 64>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels (grid-stride)
+// ---------------------------------------------------------------------------
+static inline int64_t vec_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    return g;
+}
+
+template <typename T>
+__global__ void k_set(int64_t n, T* x, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+}
+
+// dest = a1*v1 + a2*v2 (+ a3*v3 when v3 != nullptr); any of them may alias dest
+template <typename T>
+__global__ void k_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2, const T* v2, double a3,
+                            const T* v3) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double r = a1 * (double)v1[i] + a2 * (double)v2[i];
+        if (v3) r += a3 * (double)v3[i];
+        dest[i] = (T)r;
+    }
+}
+
+// dest[i] = d[i] * x[i]  (Jacobi step with the damped inverse diagonal)
+template <typename TD, typename TV>
+__global__ void k_diag_mult(int64_t n, TV* dest, const TD* d, const TV* x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dest[i] = (TV)((double)d[i] * (double)x[i]);
+}
+
+// dinv_i = 1 / (a_ii * (1/omega)) from a CSR matrix (Jacobi::preprocess)
+template <typename TM, typename TD>
+__global__ void k_jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const TM* val, double omega,
+                                  TD* dinv, int* bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+            if (col[k] == i) d = (double)val[k];
+        double m = d;
+        m *= 1.0 / omega;
+        if (m == 0.0) { bad[0] = 1; dinv[i] = (TD)0; }
+        else dinv[i] = (TD)(1.0 / m);
+    }
+}
+
+// gather/scatter with precision conversion: level[perm[i]] <- surf[i] and back
+template <typename TS, typename TL>
+__global__ void k_permute_to(int64_t n, const int32_t* __restrict__ lat_of_dof, const TS* __restrict__ src,
+                             TL* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[lat_of_dof[i]] = (TL)src[i];
+}
+template <typename TL, typename TS>
+__global__ void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
+                                   TS* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (TS)((double)dst[i] + (double)src[lat_of_dof[i]]);
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reductions: pass 1 writes one partial per block, pass 2 sums
+// the partials in a fixed order
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                     double* partial) {
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += (double)a[i] * (double)b[i];
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(int np, const double* partial, double* out, int sqrt_flag) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += partial[i];
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        out[0] = sqrt_flag ? sqrt(r) : r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DIA level operators. Vectors carry a zero halo so x[i + off] is always in
+// bounds; values of couplings that leave the lattice are stored as 0.
+// ---------------------------------------------------------------------------
+struct DiaOffsets {
+    int32_t off[27];
+};
+
+// d_out = d_in - A*x  (x may be any vector; d_out may equal d_in)
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, int nd, DiaOffsets o, const TM* __restrict__ A,
+                                                       const TV* __restrict__ x, const TV* d_in, TV* d_out,
+                                                       int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 27; ++d)
+        if (d < nd) s += (double)A[(int64_t)d * n + i] * (double)x[i + o.off[d]];
+    d_out[i] = (TV)((double)d_in[i] - s);
+}
+
+// y = alpha*v + beta*A*x
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, int nd, DiaOffsets o, const TM* __restrict__ A,
+                                                  double alpha, const TV* v, double beta, const TV* __restrict__ x,
+                                                  TV* y, int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 27; ++d)
+        if (d < nd) s += (double)A[(int64_t)d * n + i] * (double)x[i + o.off[d]];
+    y[i] = (TV)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * s);
+}
+
+// One damped-Jacobi smoothing step fused with the defect update and the
+// correction update (mg_solver_impl.hpp:1702-1730 / 1917-1947):
+//   st = dinv .* sd_in ;  sd_out = sd_in - A*st ;  sc += st
+// sd_out must not alias sd_in (neighbours read the old defect).
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, int nd, DiaOffsets o,
+                                                         const TM* __restrict__ A, const TV* __restrict__ dinv,
+                                                         const TV* __restrict__ sd_in, TV* __restrict__ sd_out,
+                                                         TV* __restrict__ sc, int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 27; ++d)
+        if (d < nd) {
+            const int64_t j = i + o.off[d];
+            s += (double)A[(int64_t)d * n + i] * ((double)dinv[j] * (double)sd_in[j]);
+        }
+    const double st = (double)dinv[i] * (double)sd_in[i];
+    sd_out[i] = (TV)((double)sd_in[i] - s);
+    sc[i] = (TV)((double)sc[i] + st);
+}
+
+// sc += dinv .* sd  (last post-smoothing step on levels below the top, where
+// the final defect update is skipped: mg_solver_impl.hpp:1954)
+template <typename TV>
+__global__ void k_jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        sc[i] = (TV)((double)sc[i] + (double)dinv[i] * (double)sd[i]);
+}
+
+// dinv from the DIA centre diagonal
+template <typename TM, typename TV>
+__global__ void k_dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double m = (double)diag[i];
+        m *= 1.0 / omega;
+        if (m == 0.0) { bad[0] = 1; dinv[i] = (TV)0; }
+        else dinv[i] = (TV)(1.0 / m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free P1 transfer between lattice levels (3D: hexahedra, 2D: triangles
+// with diagonals along (1,1)). Weights are the exact dyadic P entries; the
+// Dirichlet adjustment is applied exactly as on the assembled matrices.
+// ---------------------------------------------------------------------------
+
+// prolongation with accumulation: st_f = sum_c P(f,c) sc_c ; sc_f += st_f
+template <typename TV>
+__global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* __restrict__ scc,
+                                                     const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
+                                                     TV* __restrict__ sc) {
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= n_f) return;
+    const int x = (int)(i % Wf);
+    const int y = (int)((i / Wf) % Wf);
+    const int z = dim == 3 ? (int)(i / ((int64_t)Wf * Wf)) : 0;
+    const int x0 = x >> 1, y0 = y >> 1, z0 = z >> 1;
+    const int ox = x & 1, oy = y & 1, oz = z & 1;
+    double v = 0.0;
+    if (dim == 3) {
+        const int nodd = ox + oy + oz;
+        if (dir_f[i] && nodd != 0) {
+            v = 0.0;   // Dirichlet rows of P are zeroed except vertex copies
+        } else {
+            const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+            for (int dz = 0; dz <= oz; ++dz)
+                for (int dy = 0; dy <= oy; ++dy)
+                    for (int dx = 0; dx <= ox; ++dx)
+                        v += w * (double)scc[((int64_t)(z0 + dz) * Wc + (y0 + dy)) * Wc + (x0 + dx)];
+        }
+    } else {
+        // triangle mesh: odd coordinates lie on axis edges or on the (1,1) diagonal
+        const int nodd = ox + oy;
+        if (dir_f[i] && nodd != 0) {
+            v = 0.0;
+        } else if (nodd == 0) {
+            v = (double)scc[(int64_t)y0 * Wc + x0];
+        } else if (nodd == 1) {
+            v = 0.5 * (double)scc[(int64_t)y0 * Wc + x0] + 0.5 * (double)scc[(int64_t)(y0 + oy) * Wc + (x0 + ox)];
+        } else {
+            v = 0.5 * (double)scc[(int64_t)y0 * Wc + x0] + 0.5 * (double)scc[(int64_t)(y0 + 1) * Wc + (x0 + 1)];
+        }
+    }
+    st[i] = (TV)v;
+    sc[i] = (TV)((double)sc[i] + v);
+}
+
+// restriction: sdc_c = sum_f R(c,f) sd_f with R = P^T (Dirichlet coarse rows:
+// injection from the vertex copy)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64_t n_c, const TV* __restrict__ sdf,
+                                                  const uint8_t* __restrict__ dir_c, const uint8_t* __restrict__ dir_f,
+                                                  TV* __restrict__ sdc) {
+    const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (c >= n_c) return;
+    const int x = (int)(c % Wc);
+    const int y = (int)((c / Wc) % Wc);
+    const int z = dim == 3 ? (int)(c / ((int64_t)Wc * Wc)) : 0;
+    const int fx = 2 * x, fy = 2 * y, fz = 2 * z;
+    const int nf = Wf - 1;
+    if (dir_c[c]) {
+        const int64_t f = dim == 3 ? ((int64_t)fz * Wf + fy) * Wf + fx : (int64_t)fy * Wf + fx;
+        sdc[c] = sdf[f];
+        return;
+    }
+    double s = 0.0;
+    if (dim == 3) {
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int zz = fz + dz;
+            if (zz < 0 || zz > nf) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int yy = fy + dy;
+                if (yy < 0 || yy > nf) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = fx + dx;
+                    if (xx < 0 || xx > nf) continue;
+                    const int nodd = (dx != 0) + (dy != 0) + (dz != 0);
+                    const int64_t f = ((int64_t)zz * Wf + yy) * Wf + xx;
+                    if (dir_f[f] && nodd != 0) continue;   // zeroed P rows
+                    const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+                    s += w * (double)sdf[f];
+                }
+            }
+        }
+    } else {
+        const int dxs[7] = {0, 1, -1, 0, 0, 1, -1};
+        const int dys[7] = {0, 0, 0, 1, -1, 1, -1};
+        for (int k = 0; k < 7; ++k) {
+            const int xx = fx + dxs[k], yy = fy + dys[k];
+            if (xx < 0 || xx > nf || yy < 0 || yy > nf) continue;
+            const int64_t f = (int64_t)yy * Wf + xx;
+            if (dir_f[f] && k != 0) continue;
+            s += (k == 0 ? 1.0 : 0.5) * (double)sdf[f];
+        }
+    }
+    sdc[c] = (TV)s;
+}
+
+// ---------------------------------------------------------------------------
+// dense LU solve for the base level (one workgroup; factors in FP64)
+// ---------------------------------------------------------------------------
+template <typename TV>
+__global__ __launch_bounds__(256) void k_lu_solve(int n, const double* __restrict__ LU, const int32_t* __restrict__ piv,
+                                                  const int32_t* __restrict__ lat_of_row, const TV* __restrict__ b,
+                                                  TV* __restrict__ x, double* work) {
+    // work holds n doubles (device scratch owned by the LU object)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) work[i] = (double)b[lat_of_row[i]];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i)
+            if (i < piv[i]) { double t = work[i]; work[i] = work[piv[i]]; work[piv[i]] = t; }
+    }
+    __syncthreads();
+    // forward substitution (unit lower)
+    for (int i = 0; i < n; ++i) {
+        __shared__ double red[4];
+        double s = 0.0;
+        for (int k = threadIdx.x; k < i; k += blockDim.x) s += LU[(int64_t)i * n + k] * work[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) work[i] = work[i] - ((red[0] + red[1]) + (red[2] + red[3]));
+        __syncthreads();
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        __shared__ double red2[4];
+        double s = 0.0;
+        for (int k = i + 1 + threadIdx.x; k < n; k += blockDim.x) s += LU[(int64_t)i * n + k] * work[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if ((threadIdx.x & 63) == 0) red2[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) work[i] = (work[i] - ((red2[0] + red2[1]) + (red2[2] + red2[3]))) / LU[(int64_t)i * n + i];
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[lat_of_row[i]] = (TV)work[i];
+}
+
+// ---------------------------------------------------------------------------
+// Assembly
+// ---------------------------------------------------------------------------
+
+// lattice -> DoF inverse map
+__global__ void k_invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        lat_of_dof[dof_of_lat[i]] = (int32_t)i;
+}
+
+// number of lattice neighbours (including self) of every DoF row: 27-point box
+// (hexahedra) or 7-point (triangles with (1,1) diagonals)
+__global__ void k_pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t* cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = lat_of_dof[r];
+        const int x = (int)(l % W), y = (int)((l / W) % W);
+        const int nx = 1 + (x > 0) + (x < W - 1), ny = 1 + (y > 0) + (y < W - 1);
+        if (dim == 3) {
+            const int z = (int)(l / ((int64_t)W * W));
+            const int nz = 1 + (z > 0) + (z < W - 1);
+            cnt[r] = nx * ny * nz;
+        } else {
+            cnt[r] = nx * ny - ((x > 0 && y < W - 1) ? 1 : 0) - ((x < W - 1 && y > 0) ? 1 : 0);
+        }
+    }
+}
+
+__global__ void k_pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat,
+                               const int32_t* rowptr, int32_t* col) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = lat_of_dof[r];
+        const int x = (int)(l % W), y = (int)((l / W) % W);
+        const int z = dim == 3 ? (int)(l / ((int64_t)W * W)) : 0;
+        int32_t c[27];
+        int k = 0;
+        for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = x + dx, yy = y + dy, zz = z + dz;
+                    if (xx < 0 || xx >= W || yy < 0 || yy >= W || zz < 0 || zz >= W) continue;
+                    if (dim == 2 && dx * dy == -1) continue;   // no anti-diagonal edges
+                    c[k++] = dof_of_lat[((int64_t)zz * W + yy) * W + xx];
+                }
+        // insertion sort (rows are short)
+        for (int a = 1; a < k; ++a) {
+            int32_t v = c[a];
+            int b = a - 1;
+            while (b >= 0 && c[b] > v) { c[b + 1] = c[b]; --b; }
+            c[b + 1] = v;
+        }
+        const int32_t base = rowptr[r];
+        for (int a = 0; a < k; ++a) col[base + a] = c[a];
+    }
+}
+
+__device__ __forceinline__ int32_t csr_slot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                            int32_t r, int32_t c) {
+    int32_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;   // the pattern contains (r,c) by construction
+}
+
+// Q1 FE element matrix of a hexahedron with trilinear geometry, 2x2x2 Gauss,
+// k(x) = exp(theta[fine element containing x]); rhs f_const * int phi_i.
+// Local vertex order is the reference hexahedron's (x fastest, bottom then top).
+__device__ void q1_hex_element(const double X[8][3], const double* __restrict__ theta, int nfine, double f_const,
+                               double Ke[8][8], double fe[8]) {
+    const double g = 0.28867513459481287;   // 0.5/sqrt(3)
+    for (int a = 0; a < 8; ++a) {
+        fe[a] = 0.0;
+        for (int b = 0; b < 8; ++b) Ke[a][b] = 0.0;
+    }
+    for (int q = 0; q < 8; ++q) {
+        const double xi[3] = {(q & 1) ? 0.5 + g : 0.5 - g, (q & 2) ? 0.5 + g : 0.5 - g, (q & 4) ? 0.5 + g : 0.5 - g};
+        double phi[8], dphi[8][3];
+        for (int a = 0; a < 8; ++a) {
+            const int bx = (a == 1 || a == 2 || a == 5 || a == 6), by = (a == 2 || a == 3 || a == 6 || a == 7), bz = (a >= 4);
+            const double fx = bx ? xi[0] : 1.0 - xi[0], fy = by ? xi[1] : 1.0 - xi[1], fz = bz ? xi[2] : 1.0 - xi[2];
+            const double gx = bx ? 1.0 : -1.0, gy = by ? 1.0 : -1.0, gz = bz ? 1.0 : -1.0;
+            phi[a] = fx * fy * fz;
+            dphi[a][0] = gx * fy * fz;
+            dphi[a][1] = fx * gy * fz;
+            dphi[a][2] = fx * fy * gz;
+        }
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, xq[3] = {0, 0, 0};
+        for (int a = 0; a < 8; ++a)
+            for (int r = 0; r < 3; ++r) {
+                xq[r] += phi[a] * X[a][r];
+                for (int c = 0; c < 3; ++c) J[r][c] += X[a][r] * dphi[a][c];
+            }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        const double id = 1.0 / det;
+        double Ji[3][3];
+        Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        const double w = 0.125 * fabs(det);
+        double kq = 1.0;
+        if (theta) {
+            int cx = (int)floor(xq[0] * nfine), cy = (int)floor(xq[1] * nfine), cz = (int)floor(xq[2] * nfine);
+            cx = cx < 0 ? 0 : (cx >= nfine ? nfine - 1 : cx);
+            cy = cy < 0 ? 0 : (cy >= nfine ? nfine - 1 : cy);
+            cz = cz < 0 ? 0 : (cz >= nfine ? nfine - 1 : cz);
+            kq = exp(theta[((int64_t)cz * nfine + cy) * nfine + cx]);
+        }
+        double grad[8][3];
+        for (int a = 0; a < 8; ++a)
+            for (int r = 0; r < 3; ++r)
+                grad[a][r] = Ji[0][r] * dphi[a][0] + Ji[1][r] * dphi[a][1] + Ji[2][r] * dphi[a][2];
+        const double wk = w * kq;
+        for (int a = 0; a < 8; ++a) {
+            for (int b = 0; b < 8; ++b)
+                Ke[a][b] += wk * (grad[a][0] * grad[b][0] + grad[a][1] * grad[b][1] + grad[a][2] * grad[b][2]);
+            fe[a] += w * f_const * phi[a];
+        }
+    }
+}
+
+// corner c of lattice element (ex,ey,ez): reference hexahedron order
+__device__ __forceinline__ void hex_corner(int c, int& dx, int& dy, int& dz) {
+    dx = (c == 1 || c == 2 || c == 5 || c == 6);
+    dy = (c == 2 || c == 3 || c == 6 || c == 7);
+    dz = (c >= 4);
+}
+
+// One colour (ex%2, ey%2, ez%2) == (cx,cy,cz): elements of one colour share no
+// vertex, so the scatter needs no atomics.
+// Output target: CSR in DoF numbering (rowptr != nullptr) or DIA in lattice
+// numbering (dia != nullptr). rhs (DoF or lattice numbering) may be null.
+template <typename TM>
+__global__ __launch_bounds__(128) void k_assemble_q1_hex(int n, int color, double h, const double* __restrict__ theta,
+                                                         int nfine, double f_const, const int32_t* __restrict__ dof_of_lat,
+                                                         const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                         TM* __restrict__ csr_val, TM* __restrict__ dia, int64_t dia_n,
+                                                         double* __restrict__ rhs) {
+    const int hn = (n + 1) / 2;   // elements per dim of this colour (upper bound)
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int cx = color & 1, cy = (color >> 1) & 1, cz = (color >> 2) & 1;
+    const int64_t nper = (int64_t)hn * hn * hn;
+    if (e >= nper) return;
+    const int ex = 2 * (int)(e % hn) + cx, ey = 2 * (int)((e / hn) % hn) + cy, ez = 2 * (int)(e / ((int64_t)hn * hn)) + cz;
+    if (ex >= n || ey >= n || ez >= n) return;
+    const int W = n + 1;
+    double X[8][3];
+    int64_t lat[8];
+    for (int a = 0; a < 8; ++a) {
+        int dx, dy, dz;
+        hex_corner(a, dx, dy, dz);
+        X[a][0] = (ex + dx) * h;
+        X[a][1] = (ey + dy) * h;
+        X[a][2] = (ez + dz) * h;
+        lat[a] = ((int64_t)(ez + dz) * W + (ey + dy)) * W + (ex + dx);
+    }
+    double Ke[8][8], fe[8];
+    q1_hex_element(X, theta, nfine, f_const, Ke, fe);
+    if (rowptr) {
+        int32_t dof[8];
+        for (int a = 0; a < 8; ++a) dof[a] = dof_of_lat[lat[a]];
+        for (int a = 0; a < 8; ++a) {
+            for (int b = 0; b < 8; ++b) {
+                const int32_t k = csr_slot(rowptr, col, dof[a], dof[b]);
+                csr_val[k] = (TM)((double)csr_val[k] + Ke[a][b]);
+            }
+            if (rhs) rhs[dof[a]] += fe[a];
+        }
+    } else {
+        for (int a = 0; a < 8; ++a) {
+            int ax, ay, az;
+            hex_corner(a, ax, ay, az);
+            for (int b = 0; b < 8; ++b) {
+                int bx, by, bz;
+                hex_corner(b, bx, by, bz);
+                const int d = ((bz - az + 1) * 3 + (by - ay + 1)) * 3 + (bx - ax + 1);
+                TM* p = dia + (int64_t)d * dia_n + lat[a];
+                *p = (TM)((double)*p + Ke[a][b]);
+            }
+            if (rhs) rhs[lat[a]] += fe[a];
+        }
+    }
+}
+
+// FV1 on P1 triangles (box method), lattice squares split along (1,1).
+// Colour = (sx%2, sy%2, which triangle).
+template <typename TM>
+__device__ void fv1_tri_element(const double X[3][2], const double* __restrict__ theta, int nfine, double Ke[3][3],
+                                double& area) {
+    const double J00 = X[1][0] - X[0][0], J01 = X[2][0] - X[0][0];
+    const double J10 = X[1][1] - X[0][1], J11 = X[2][1] - X[0][1];
+    const double det = J00 * J11 - J01 * J10;
+    area = 0.5 * fabs(det);
+    const double gr[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+    double grad[3][2];
+    for (int i = 0; i < 3; ++i) {
+        grad[i][0] = (J11 * gr[i][0] - J10 * gr[i][1]) / det;
+        grad[i][1] = (-J01 * gr[i][0] + J00 * gr[i][1]) / det;
+    }
+    const double bx = (X[0][0] + X[1][0] + X[2][0]) / 3.0, by = (X[0][1] + X[1][1] + X[2][1]) / 3.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Ke[a][b] = 0.0;
+    for (int e = 0; e < 3; ++e) {
+        const int fr = e, to = (e + 1) % 3;
+        const double mx = 0.5 * (X[fr][0] + X[to][0]), my = 0.5 * (X[fr][1] + X[to][1]);
+        double nx = by - my, ny = -(bx - mx);
+        if (nx * (X[to][0] - X[fr][0]) + ny * (X[to][1] - X[fr][1]) < 0) { nx = -nx; ny = -ny; }
+        double kq = 1.0;
+        if (theta) {
+            const double ipx = 0.5 * (mx + bx), ipy = 0.5 * (my + by);
+            int cx = (int)floor(ipx * nfine), cy = (int)floor(ipy * nfine);
+            cx = cx < 0 ? 0 : (cx >= nfine ? nfine - 1 : cx);
+            cy = cy < 0 ? 0 : (cy >= nfine ? nfine - 1 : cy);
+            kq = exp(theta[(int64_t)cy * nfine + cx]);
+        }
+        for (int sh = 0; sh < 3; ++sh) {
+            const double flux = kq * (grad[sh][0] * nx + grad[sh][1] * ny);
+            Ke[fr][sh] -= flux;
+            Ke[to][sh] += flux;
+        }
+    }
+}
+
+template <typename TM>
+__global__ __launch_bounds__(128) void k_assemble_fv1_tri(int n, int color, double h, const double* __restrict__ theta,
+                                                          int nfine, const double* __restrict__ f_at_dof,
+                                                          const int32_t* __restrict__ dof_of_lat,
+                                                          const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                          TM* __restrict__ csr_val, TM* __restrict__ dia, int64_t dia_n,
+                                                          double* __restrict__ rhs) {
+    const int hn = (n + 1) / 2;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)hn * hn) return;
+    const int sx = 2 * (int)(e % hn) + (color & 1), sy = 2 * (int)(e / hn) + ((color >> 1) & 1);
+    const int upper = (color >> 2) & 1;
+    if (sx >= n || sy >= n) return;
+    const int W = n + 1;
+    // lower: (0,0),(1,0),(1,1); upper: (0,0),(1,1),(0,1)
+    const int cxs[2][3] = {{0, 1, 1}, {0, 1, 0}};
+    const int cys[2][3] = {{0, 0, 1}, {0, 1, 1}};
+    double X[3][2];
+    int64_t lat[3];
+    for (int a = 0; a < 3; ++a) {
+        const int xx = sx + cxs[upper][a], yy = sy + cys[upper][a];
+        X[a][0] = xx * h;
+        X[a][1] = yy * h;
+        lat[a] = (int64_t)yy * W + xx;
+    }
+    double Ke[3][3], area;
+    fv1_tri_element<TM>(X, theta, nfine, Ke, area);
+    if (rowptr) {
+        int32_t dof[3];
+        for (int a = 0; a < 3; ++a) dof[a] = dof_of_lat[lat[a]];
+        for (int a = 0; a < 3; ++a) {
+            for (int b = 0; b < 3; ++b) {
+                const int32_t k = csr_slot(rowptr, col, dof[a], dof[b]);
+                csr_val[k] = (TM)((double)csr_val[k] + Ke[a][b]);
+            }
+            if (rhs && f_at_dof) rhs[dof[a]] += f_at_dof[dof[a]] * (area / 3.0);
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            for (int b = 0; b < 3; ++b) {
+                const int dx = (int)((lat[b] % W) - (lat[a] % W)), dy = (int)((lat[b] / W) - (lat[a] / W));
+                const int d = (dy + 1) * 3 + (dx + 1);
+                TM* p = dia + (int64_t)d * dia_n + lat[a];
+                *p = (TM)((double)*p + Ke[a][b]);
+            }
+            if (rhs && f_at_dof) rhs[lat[a]] += f_at_dof[dof_of_lat[lat[a]]] * (area / 3.0);
+        }
+    }
+}
+
+// Dirichlet rows in the CSR (DoF numbering): all values 0, diagonal 1, rhs 0
+template <typename TM>
+__global__ void k_dirichlet_csr(int64_t n, int64_t n_inner, const int32_t* rowptr, const int32_t* col, TM* val,
+                                double* rhs) {
+    for (int64_t r = n_inner + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) val[k] = (TM)(col[k] == r ? 1.0 : 0.0);
+        if (rhs) rhs[r] = 0.0;
+    }
+}
+
+// Dirichlet rows in the DIA (lattice numbering)
+template <typename TM>
+__global__ void k_dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!dir[i]) continue;
+        for (int d = 0; d < nd; ++d) dia[(int64_t)d * n + i] = (TM)(d == centre ? 1.0 : 0.0);
+    }
+}
+
+// convert DIA values between precisions
+template <typename TA, typename TB>
+__global__ void k_convert(int64_t n, const TA* a, TB* b) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (TB)a[i];
+}
+
+// Dirichlet flag per lattice point of a level: boundary of the unit box/square
+__global__ void k_boundary_flags(int dim, int W, int64_t n, uint8_t* dir) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)((i / W) % W);
+        bool b = x == 0 || y == 0 || x == W - 1 || y == W - 1;
+        if (dim == 3) {
+            const int z = (int)(i / ((int64_t)W * W));
+            b = b || z == 0 || z == W - 1;
+        }
+        dir[i] = b ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exported launchers
+// ---------------------------------------------------------------------------
+
+#define UGX_CHECK_LAUNCH() return hipGetLastError() == hipSuccess ? 0 : -1
+
+template <typename TM, typename TV>
+int spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* col, const TM* val, double alpha, const TV* v1,
+             double beta, const TV* w, TV* dest, int mode, int lanes, hipStream_t s) {
+    return launch_spmv_csr<TM, TV>(nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, lanes, s);
+}
+template int spmv_csr<double, double>(int64_t, const int32_t*, const int32_t*, const double*, double, const double*, double, const double*, double*, int, int, hipStream_t);
+template int spmv_csr<float, float>(int64_t, const int32_t*, const int32_t*, const float*, double, const float*, double, const float*, float*, int, int, hipStream_t);
+template int spmv_csr<float, double>(int64_t, const int32_t*, const int32_t*, const float*, double, const double*, double, const double*, double*, int, int, hipStream_t);
+
+template <typename T>
+int vec_set(int64_t n, T* x, double v, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_set<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, (T)v);
+    UGX_CHECK_LAUNCH();
+}
+template int vec_set<double>(int64_t, double*, double, hipStream_t);
+template int vec_set<float>(int64_t, float*, double, hipStream_t);
+template int vec_set<uint8_t>(int64_t, uint8_t*, double, hipStream_t);
+template int vec_set<int32_t>(int64_t, int32_t*, double, hipStream_t);
+
+template <typename T>
+int vec_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2, const T* v2, double a3, const T* v3,
+                  hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_scale_add<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, dest, a1, v1, a2, v2, a3, v3);
+    UGX_CHECK_LAUNCH();
+}
+template int vec_scale_add<double>(int64_t, double*, double, const double*, double, const double*, double, const double*, hipStream_t);
+template int vec_scale_add<float>(int64_t, float*, double, const float*, double, const float*, double, const float*, hipStream_t);
+
+template <typename TD, typename TV>
+int diag_mult(int64_t n, TV* dest, const TD* d, const TV* x, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_diag_mult<TD, TV>), dim3(vec_grid(n)), dim3(256), 0, s, n, dest, d, x);
+    UGX_CHECK_LAUNCH();
+}
+template int diag_mult<double, double>(int64_t, double*, const double*, const double*, hipStream_t);
+template int diag_mult<float, float>(int64_t, float*, const float*, const float*, hipStream_t);
+
+template <typename TM, typename TD>
+int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const TM* val, double omega, TD* dinv,
+                    int* bad, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_jacobi_diag_csr<TM, TD>), dim3(vec_grid(n)), dim3(256), 0, s, n, rowptr, col, val, omega, dinv, bad);
+    UGX_CHECK_LAUNCH();
+}
+template int jacobi_diag_csr<double, double>(int64_t, const int32_t*, const int32_t*, const double*, double, double*, int*, hipStream_t);
+
+template <typename T>
+int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s) {
+    int64_t g = (n + 255) / 256;
+    if (g > kRedBlocks) g = kRedBlocks;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(256), 0, s, n, a, b, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (int)g, partial, out, take_sqrt ? 1 : 0);
+    UGX_CHECK_LAUNCH();
+}
+template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
+template int dot<float>(int64_t, const float*, const float*, double*, double*, bool, hipStream_t);
+
+template <typename TS, typename TL>
+int permute_to(int64_t n, const int32_t* lat_of_dof, const TS* src, TL* dst, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_permute_to<TS, TL>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    UGX_CHECK_LAUNCH();
+}
+template int permute_to<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
+template int permute_to<double, float>(int64_t, const int32_t*, const double*, float*, hipStream_t);
+
+template <typename TL, typename TS>
+int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_permute_add_from<TL, TS>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    UGX_CHECK_LAUNCH();
+}
+template int permute_add_from<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
+template int permute_add_from<float, double>(int64_t, const int32_t*, const float*, double*, hipStream_t);
+
+static DiaOffsets make_offsets(const int32_t* off, int nd) {
+    DiaOffsets o;
+    for (int d = 0; d < 27; ++d) o.off[d] = d < nd ? off[d] : 0;
+    return o;
+}
+
+static inline int64_t remapped_grid(int64_t nblocks) { return nblocks < 64 ? nblocks : ((nblocks + 7) / 8) * 8; }
+
+template <typename TM, typename TV>
+int dia_apply_sub(int64_t n, int nd, const int32_t* off, const TM* A, const TV* x, const TV* d_in, TV* d_out,
+                  hipStream_t s) {
+    if (n <= 0) return 0;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_dia_apply_sub<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, x, d_in, d_out, nb);
+    UGX_CHECK_LAUNCH();
+}
+template int dia_apply_sub<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, hipStream_t);
+template int dia_apply_sub<float, float>(int64_t, int, const int32_t*, const float*, const float*, const float*, float*, hipStream_t);
+
+template <typename TM, typename TV>
+int dia_axpy(int64_t n, int nd, const int32_t* off, const TM* A, double alpha, const TV* v, double beta, const TV* x,
+             TV* y, hipStream_t s) {
+    if (n <= 0) return 0;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_dia_axpy<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, alpha, v, beta, x, y, nb);
+    UGX_CHECK_LAUNCH();
+}
+template int dia_axpy<double, double>(int64_t, int, const int32_t*, const double*, double, const double*, double, const double*, double*, hipStream_t);
+template int dia_axpy<float, float>(int64_t, int, const int32_t*, const float*, double, const float*, double, const float*, float*, hipStream_t);
+
+template <typename TM, typename TV>
+int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV* dinv, const TV* sd_in, TV* sd_out,
+                    TV* sc, hipStream_t s) {
+    if (n <= 0) return 0;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_dia_jacobi_step<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, dinv, sd_in, sd_out, sc, nb);
+    UGX_CHECK_LAUNCH();
+}
+template int dia_jacobi_step<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, double*, hipStream_t);
+template int dia_jacobi_step<float, float>(int64_t, int, const int32_t*, const float*, const float*, const float*, float*, float*, hipStream_t);
+
+template <typename TV>
+int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_jacobi_accumulate<TV>, dim3(vec_grid(n)), dim3(256), 0, s, n, dinv, sd, sc);
+    UGX_CHECK_LAUNCH();
+}
+template int jacobi_accumulate<double>(int64_t, const double*, const double*, double*, hipStream_t);
+template int jacobi_accumulate<float>(int64_t, const float*, const float*, float*, hipStream_t);
+
+template <typename TM, typename TV>
+int dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_dia_jacobi_diag<TM, TV>), dim3(vec_grid(n)), dim3(256), 0, s, n, diag, omega, dinv, bad);
+    UGX_CHECK_LAUNCH();
+}
+template int dia_jacobi_diag<double, double>(int64_t, const double*, double, double*, int*, hipStream_t);
+template int dia_jacobi_diag<float, float>(int64_t, const float*, double, float*, int*, hipStream_t);
+
+template <typename TV>
+int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
+                hipStream_t s) {
+    if (n_f <= 0) return 0;
+    hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, scc, dir_f, st, sc);
+    UGX_CHECK_LAUNCH();
+}
+template int prolong_add<double>(int, int, int, int64_t, const double*, const uint8_t*, double*, double*, hipStream_t);
+template int prolong_add<float>(int, int, int, int64_t, const float*, const uint8_t*, float*, float*, hipStream_t);
+
+template <typename TV>
+int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
+                    TV* sdc, hipStream_t s) {
+    if (n_c <= 0) return 0;
+    hipLaunchKernelGGL(k_restrict<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_c, sdf, dir_c, dir_f, sdc);
+    UGX_CHECK_LAUNCH();
+}
+template int restrict_defect<double>(int, int, int, int64_t, const double*, const uint8_t*, const uint8_t*, double*, hipStream_t);
+template int restrict_defect<float>(int, int, int, int64_t, const float*, const uint8_t*, const uint8_t*, float*, hipStream_t);
+
+template <typename TV>
+int lu_solve(int n, const double* LU, const int32_t* piv, const int32_t* lat_of_row, const TV* b, TV* x, double* work,
+             hipStream_t s) {
+    hipLaunchKernelGGL(k_lu_solve<TV>, dim3(1), dim3(256), 0, s, n, LU, piv, lat_of_row, b, x, work);
+    UGX_CHECK_LAUNCH();
+}
+template int lu_solve<double>(int, const double*, const int32_t*, const int32_t*, const double*, double*, double*, hipStream_t);
+template int lu_solve<float>(int, const double*, const int32_t*, const int32_t*, const float*, float*, double*, hipStream_t);
+
+int invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof, hipStream_t s) {
+    hipLaunchKernelGGL(k_invert_map, dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, lat_of_dof);
+    UGX_CHECK_LAUNCH();
+}
+
+int pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_pattern_count, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, lat_of_dof, cnt);
+    UGX_CHECK_LAUNCH();
+}
+
+int pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const int32_t* rowptr,
+                 int32_t* col, hipStream_t s) {
+    hipLaunchKernelGGL(k_pattern_fill, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, lat_of_dof, dof_of_lat, rowptr, col);
+    UGX_CHECK_LAUNCH();
+}
+
+template <typename TM>
+int assemble_q1_hex(int n, double h, const double* theta, int nfine, double f_const, const int32_t* dof_of_lat,
+                    const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
+                    hipStream_t s) {
+    const int hn = (n + 1) / 2;
+    const int64_t per = (int64_t)hn * hn * hn;
+    for (int color = 0; color < 8; ++color)
+        hipLaunchKernelGGL(k_assemble_q1_hex<TM>, dim3((per + 127) / 128), dim3(128), 0, s, n, color, h, theta, nfine, f_const, dof_of_lat, rowptr, col, csr_val, dia, dia_n, rhs);
+    UGX_CHECK_LAUNCH();
+}
+template int assemble_q1_hex<double>(int, double, const double*, int, double, const int32_t*, const int32_t*, const int32_t*, double*, double*, int64_t, double*, hipStream_t);
+
+template <typename TM>
+int assemble_fv1_tri(int n, double h, const double* theta, int nfine, const double* f_at_dof, const int32_t* dof_of_lat,
+                     const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
+                     hipStream_t s) {
+    const int hn = (n + 1) / 2;
+    const int64_t per = (int64_t)hn * hn;
+    for (int color = 0; color < 8; ++color)
+        hipLaunchKernelGGL(k_assemble_fv1_tri<TM>, dim3((per + 127) / 128), dim3(128), 0, s, n, color, h, theta, nfine, f_at_dof, dof_of_lat, rowptr, col, csr_val, dia, dia_n, rhs);
+    UGX_CHECK_LAUNCH();
+}
+template int assemble_fv1_tri<double>(int, double, const double*, int, const double*, const int32_t*, const int32_t*, const int32_t*, double*, double*, int64_t, double*, hipStream_t);
+
+template <typename TM>
+int dirichlet_csr(int64_t n, int64_t n_inner, const int32_t* rowptr, const int32_t* col, TM* val, double* rhs,
+                  hipStream_t s) {
+    if (n - n_inner <= 0) return 0;
+    hipLaunchKernelGGL(k_dirichlet_csr<TM>, dim3(vec_grid(n - n_inner)), dim3(256), 0, s, n, n_inner, rowptr, col, val, rhs);
+    UGX_CHECK_LAUNCH();
+}
+template int dirichlet_csr<double>(int64_t, int64_t, const int32_t*, const int32_t*, double*, double*, hipStream_t);
+
+template <typename TM>
+int dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia, hipStream_t s) {
+    hipLaunchKernelGGL(k_dirichlet_dia<TM>, dim3(vec_grid(n)), dim3(256), 0, s, n, nd, centre, dir, dia);
+    UGX_CHECK_LAUNCH();
+}
+template int dirichlet_dia<double>(int64_t, int, int, const uint8_t*, double*, hipStream_t);
+
+template <typename TA, typename TB>
+int convert(int64_t n, const TA* a, TB* b, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_convert<TA, TB>), dim3(vec_grid(n)), dim3(256), 0, s, n, a, b);
+    UGX_CHECK_LAUNCH();
+}
+template int convert<double, float>(int64_t, const double*, float*, hipStream_t);
+
+int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s) {
+    hipLaunchKernelGGL(k_boundary_flags, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, dir);
+    UGX_CHECK_LAUNCH();
+}
+
+}  // namespace ugamd

@@ -1,0 +1,75 @@
+// Internal launcher declarations (kernels.hip). All are stream ordered, return
+// 0 on success and -1 when the launch failed; none allocates device memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ugamd {
+
+template <typename TM, typename TV>
+int spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* col, const TM* val, double alpha, const TV* v1,
+             double beta, const TV* w, TV* dest, int mode, int lanes, hipStream_t s);
+
+template <typename T>
+int vec_set(int64_t n, T* x, double v, hipStream_t s);
+template <typename T>
+int vec_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2, const T* v2, double a3, const T* v3,
+                  hipStream_t s);
+template <typename TD, typename TV>
+int diag_mult(int64_t n, TV* dest, const TD* d, const TV* x, hipStream_t s);
+template <typename TM, typename TD>
+int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const TM* val, double omega, TD* dinv,
+                    int* bad, hipStream_t s);
+template <typename T>
+int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s);
+template <typename TS, typename TL>
+int permute_to(int64_t n, const int32_t* lat_of_dof, const TS* src, TL* dst, hipStream_t s);
+template <typename TL, typename TS>
+int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s);
+
+template <typename TM, typename TV>
+int dia_apply_sub(int64_t n, int nd, const int32_t* off, const TM* A, const TV* x, const TV* d_in, TV* d_out,
+                  hipStream_t s);
+template <typename TM, typename TV>
+int dia_axpy(int64_t n, int nd, const int32_t* off, const TM* A, double alpha, const TV* v, double beta, const TV* x,
+             TV* y, hipStream_t s);
+template <typename TM, typename TV>
+int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV* dinv, const TV* sd_in, TV* sd_out,
+                    TV* sc, hipStream_t s);
+template <typename TV>
+int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
+template <typename TM, typename TV>
+int dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad, hipStream_t s);
+template <typename TV>
+int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
+                hipStream_t s);
+template <typename TV>
+int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
+                    TV* sdc, hipStream_t s);
+template <typename TV>
+int lu_solve(int n, const double* LU, const int32_t* piv, const int32_t* lat_of_row, const TV* b, TV* x, double* work,
+             hipStream_t s);
+
+int invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof, hipStream_t s);
+int pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t* cnt, hipStream_t s);
+int pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const int32_t* rowptr,
+                 int32_t* col, hipStream_t s);
+template <typename TM>
+int assemble_q1_hex(int n, double h, const double* theta, int nfine, double f_const, const int32_t* dof_of_lat,
+                    const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
+                    hipStream_t s);
+template <typename TM>
+int assemble_fv1_tri(int n, double h, const double* theta, int nfine, const double* f_at_dof, const int32_t* dof_of_lat,
+                     const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
+                     hipStream_t s);
+template <typename TM>
+int dirichlet_csr(int64_t n, int64_t n_inner, const int32_t* rowptr, const int32_t* col, TM* val, double* rhs,
+                  hipStream_t s);
+template <typename TM>
+int dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia, hipStream_t s);
+template <typename TA, typename TB>
+int convert(int64_t n, const TA* a, TB* b, hipStream_t s);
+int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s);
+
+}  // namespace ugamd
