@@ -128,6 +128,10 @@ class GPUSparseMatrix {
     int32_t* col() const { return col_.get(); }
     double* val() const { return val_.get(); }
     int lanes() const { return lanes_; }
+    void set_lanes(int g) {
+        if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64) throw UGError("set_lanes: 4, 8, 16, 32 or 64");
+        lanes_ = g;
+    }
 
    private:
     int64_t rows_ = 0, cols_ = 0, nnz_ = 0;
@@ -273,7 +277,11 @@ struct LevelGrid {
 // -div(k grad u) = f), kind 1 = unit square from two triangles (FV1).
 class DomainDiscretization {
    public:
-    DomainDiscretization(int kind, int num_refs);
+    // ordering 0: the reference's LEVEL numbering (lib_grid creation order,
+    // subset-major); ordering 1: lexicographic lattice order with the
+    // Dirichlet DoFs last (an OrderLex-style renumbering,
+    // lib_disc/ordering_strategies)
+    DomainDiscretization(int kind, int num_refs, int ordering = 0);
     int kind() const { return kind_; }
     int dim() const { return kind_ == 0 ? 3 : 2; }
     int num_levels() const { return (int)levels_.size(); }

@@ -92,6 +92,9 @@ int ugx_hierarchy_numbering(int kind, int num_refs, int level, int32_t* out_host
  * kind 0: unit cube from one hexahedron, Q1 FE for -div(k grad u) = f;
  * kind 1: unit square from two triangles, FV1 box method. */
 void* ugx_disc_create(int kind, int num_refs);
+/* ordering 0: the reference's numbering (as ugx_disc_create); 1: lexicographic
+ * lattice order with Dirichlet DoFs last (OrderLex-style renumbering) */
+void* ugx_disc_create_ordered(int kind, int num_refs, int ordering);
 void ugx_disc_destroy(void* disc);
 int ugx_disc_num_levels(void* disc);
 int64_t ugx_disc_num_dofs(void* disc, int level);
@@ -113,6 +116,8 @@ int ugx_matrix_upload_csr(void* m, int64_t rows, int64_t cols, const int32_t* ro
 int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val);
 int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val);
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w);   /* vectors are handles */
+/* tuning: lanes of a wavefront that share one CSR row in this matrix's SpMV (4..64, power of 2) */
+int ugx_matrix_set_spmv_lanes(void* m, int lanes);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
 
 /* GPUVector (vector_type) */

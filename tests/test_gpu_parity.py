@@ -310,3 +310,27 @@ def test_full_size_128_properties():
         assert rn / np.linalg.norm(b.numpy()) < 1e-10
         steps[prec] = S.steps()
     assert abs(steps["f64"] - steps["f32"]) <= 2
+
+
+def test_lexicographic_ordering_is_a_renumbering():
+    """The OrderLex-style numbering yields the same operator up to the DoF
+    permutation: identical solutions mapped through the lattice."""
+    say("case lex ordering")
+    u = U()
+    refs = 4
+    p, theta, f = oracle_problem(0, refs)
+    sols = {}
+    for ordering in ("reference", "lex"):
+        d = u.DomainDiscretization(0, refs, ordering=ordering)
+        d.set_theta(theta)
+        A = u.SparseMatrix()
+        b = u.Vector(d.num_dofs())
+        d.assemble_linear(A, b)
+        g = u.GeometricMultiGrid(d, base_level=0)
+        S = u.CG(g, reduction=1e-12)
+        S.init(A)
+        x = u.Vector(d.num_dofs())
+        assert S.apply(x, b)
+        dol = d.dof_of_lattice(refs)
+        sols[ordering] = x.numpy()[dol]        # lattice order
+    assert np.linalg.norm(sols["lex"] - sols["reference"]) <= 1e-10 * np.linalg.norm(sols["reference"])

@@ -49,6 +49,7 @@ _SIGNATURES = {
     "ugx_jacobi_apply_f64": (_I, [_I64, _P, _P, _P, _P]),
     "ugx_hierarchy_numbering": (_I, [_I, _I, _I, _P, _P]),
     "ugx_disc_create": (_P, [_I, _I]),
+    "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
     "ugx_disc_destroy": (None, [_P]),
     "ugx_disc_num_levels": (_I, [_P]),
     "ugx_disc_num_dofs": (_I64, [_P, _I]),
@@ -66,6 +67,7 @@ _SIGNATURES = {
     "ugx_matrix_download_csr": (_I, [_P, _P, _P, _P]),
     "ugx_matrix_device_csr": (_I, [_P, _P, _P, _P]),
     "ugx_matrix_axpy": (_I, [_P, _P, _D, _P, _D, _P]),
+    "ugx_matrix_set_spmv_lanes": (_I, [_P, _I]),
     "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
     "ugx_vector_create": (_P, [_I64]),
     "ugx_vector_destroy": (None, [_P]),
@@ -274,16 +276,20 @@ class SparseMatrix:
     def apply_sub(self, d, x):
         _check(lib().ugx_matrix_apply_sub(self.h, d.h, x.h), "SparseMatrix.apply_sub")
 
+    def set_spmv_lanes(self, lanes):
+        _check(lib().ugx_matrix_set_spmv_lanes(self.h, int(lanes)), "SparseMatrix.set_spmv_lanes")
+
 
 class DomainDiscretization:
     """Structured domain (kind 0: unit cube from one hexahedron, Q1 FE;
     kind 1: unit square from two triangles, FV1) with device assembly."""
 
-    def __init__(self, kind, num_refs):
+    def __init__(self, kind, num_refs, ordering="reference"):
         self.kind = kind
         self.num_refs = num_refs
         self.dim = 3 if kind == 0 else 2
-        self.h = _handle(lib().ugx_disc_create(int(kind), int(num_refs)), "DomainDiscretization")
+        o = {"reference": 0, "lex": 1}[ordering]
+        self.h = _handle(lib().ugx_disc_create_ordered(int(kind), int(num_refs), o), "DomainDiscretization")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

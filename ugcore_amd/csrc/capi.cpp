@@ -262,6 +262,13 @@ void* ugx_disc_create(int kind, int num_refs) {
         return (void*)h;
     });
 }
+void* ugx_disc_create_ordered(int kind, int num_refs, int ordering) {
+    return guard_ptr([&] {
+        auto* h = new DiscH();
+        h->d = std::make_shared<DomainDiscretization>(kind, num_refs, ordering);
+        return (void*)h;
+    });
+}
 void ugx_disc_destroy(void* disc) { delete (DiscH*)disc; }
 int ugx_disc_num_levels(void* disc) { return ((DiscH*)disc)->d->num_levels(); }
 int64_t ugx_disc_num_dofs(void* disc, int level) {
@@ -331,6 +338,12 @@ int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w) {
     return guard([&] {
         ((MatrixH*)m)->A->axpy(((VectorH*)dest)->v, alpha, ((VectorH*)v1)->v, beta, ((VectorH*)w)->v);
+        return 0;
+    });
+}
+int ugx_matrix_set_spmv_lanes(void* m, int lanes) {
+    return guard([&] {
+        ((MatrixH*)m)->A->set_lanes(lanes);
         return 0;
     });
 }

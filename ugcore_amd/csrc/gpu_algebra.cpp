@@ -368,10 +368,26 @@ bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
 // ---------------------------------------------------------------------------
 // DomainDiscretization (structured)
 // ---------------------------------------------------------------------------
-DomainDiscretization::DomainDiscretization(int kind, int num_refs) : kind_(kind), refs_(num_refs) {
+DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering) : kind_(kind), refs_(num_refs) {
     if (kind != 0 && kind != 1) throw UGError("DomainDiscretization: unknown domain kind");
     if (num_refs < 0 || num_refs > (kind == 0 ? 9 : 13)) throw UGError("DomainDiscretization: refinement out of range");
     auto nums = build_level_numbering(kind, num_refs);
+    if (ordering == 1) {
+        // lexicographic lattice order, inner DoFs first then Dirichlet DoFs
+        for (auto& ln : nums) {
+            const int64_t W = ln.n + 1, N = (int64_t)ln.dof_of_lattice.size();
+            int32_t k = 0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int64_t l = 0; l < N; ++l) {
+                    const int64_t x = l % W, y = (l / W) % W, z = ln.dim == 3 ? l / (W * W) : 0;
+                    const bool bnd = x == 0 || y == 0 || x == W - 1 || y == W - 1 ||
+                                     (ln.dim == 3 && (z == 0 || z == W - 1));
+                    if (bnd == (pass == 1)) ln.dof_of_lattice[l] = k++;
+                }
+        }
+    } else if (ordering != 0) {
+        throw UGError("DomainDiscretization: unknown ordering");
+    }
     auto& c = context();
     for (auto& ln : nums) {
         auto L = std::make_unique<LevelGrid>();
@@ -526,8 +542,8 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     int base, top, nu1, nu2, cycle;
     double omega;
     std::vector<GmgLevel<T>> L;
-    DeviceBuffer<double> lu, lu_work;
-    DeviceBuffer<int32_t> lu_piv, lu_rows;
+    DeviceBuffer<double> lu_linvT, lu_uinvT;
+    DeviceBuffer<int32_t> lu_perm;
     int lu_n = 0;
 
     GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w)
@@ -580,15 +596,41 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                     }
                 std::vector<int32_t> piv;
                 if (!host_lu(lu_n, dense, piv)) throw UGError("GMG: base level matrix is singular");
-                lu.resize((int64_t)dense.size());
-                lu.upload(dense.data(), (int64_t)dense.size());
-                lu_piv.resize(lu_n);
-                lu_piv.upload(piv.data(), lu_n);
-                std::vector<int32_t> rows(lu_n);
-                std::iota(rows.begin(), rows.end(), 0);
-                lu_rows.resize(lu_n);
-                lu_rows.upload(rows.data(), lu_n);
-                lu_work.resize(lu_n);
+                // row permutation of SolveLU (swap x[i], x[piv[i]] in order)
+                std::vector<int32_t> perm(lu_n);
+                std::iota(perm.begin(), perm.end(), 0);
+                for (int i = 0; i < lu_n; ++i)
+                    if (i < piv[i]) std::swap(perm[i], perm[piv[i]]);
+                // inverses of the unit lower and the upper triangular factor,
+                // stored column-major: LinvT[k*n+i] = Linv(i,k)
+                const int m = lu_n;
+                std::vector<double> linvT((size_t)m * m, 0.0), uinvT((size_t)m * m, 0.0);
+                auto LU = [&](int r, int c) { return dense[(size_t)r * m + c]; };
+                for (int k = 0; k < m; ++k) {
+                    // column k of L^{-1}: solve L y = e_k (unit diagonal)
+                    std::vector<double> y(m, 0.0);
+                    y[k] = 1.0;
+                    for (int i = k + 1; i < m; ++i) {
+                        double acc = 0.0;
+                        for (int j = k; j < i; ++j) acc += LU(i, j) * y[j];
+                        y[i] = -acc;
+                    }
+                    for (int i = 0; i < m; ++i) linvT[(size_t)k * m + i] = y[i];
+                    // column k of U^{-1}: solve U z = e_k
+                    std::vector<double> z(m, 0.0);
+                    for (int i = k; i >= 0; --i) {
+                        double acc = (i == k) ? 1.0 : 0.0;
+                        for (int j = i + 1; j <= k; ++j) acc -= LU(i, j) * z[j];
+                        z[i] = acc / LU(i, i);
+                    }
+                    for (int i = 0; i < m; ++i) uinvT[(size_t)k * m + i] = z[i];
+                }
+                lu_linvT.resize((int64_t)m * m);
+                lu_linvT.upload(linvT.data(), (int64_t)m * m);
+                lu_uinvT.resize((int64_t)m * m);
+                lu_uinvT.upload(uinvT.data(), (int64_t)m * m);
+                lu_perm.resize(m);
+                lu_perm.upload(perm.data(), m);
             }
             if constexpr (std::is_same<T, double>::value) {
                 l.A = std::move(A64);
@@ -608,7 +650,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     void base_solve(int lev) {
         auto& c = context();
         GmgLevel<T>& l = L[lev];
-        UGX_LAUNCH(lu_solve<T>(lu_n, lu.get(), lu_piv.get(), lu_rows.get(), l.sd, l.sc, lu_work.get(), c.stream));
+        UGX_LAUNCH(lu_solve<T>(lu_n, lu_linvT.get(), lu_uinvT.get(), lu_perm.get(), l.sd, l.sc, c.stream));
         if (lev >= top) UGX_LAUNCH((dia_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.A.get(), l.sc, l.sd, l.sd, c.stream)));
     }
 
@@ -661,7 +703,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         LevelGrid& G = disc->level(top);
         GmgLevel<T>& t = L[top];
         if (d.size() != t.N || cvec.size() != t.N) throw UGError("GMG::apply: size mismatch");
-        UGX_LAUNCH((permute_to<double, T>(t.N, G.lat_of_dof.get(), d.data(), t.sd, c.stream)));
+        UGX_LAUNCH((permute_to<double, T>(t.N, G.dof_of_lat.get(), d.data(), t.sd, c.stream)));
         cvec.set(0.0);
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
         lmgc(top);

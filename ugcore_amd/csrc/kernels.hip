@@ -64,7 +64,20 @@ __global__ __launch_bounds__(256) void k_spmv_csr(int64_t nrows, const int32_t* 
     if (active) {
         rb = rowptr[row];
         re = rowptr[row + 1];
-        for (int32_t k = rb + lane; k < re; k += G) s += (double)val[k] * (double)w[col[k]];
+        // two independent (column, value) pairs in flight per lane; the matrix
+        // streams are read once, so they bypass the cache (non-temporal)
+        for (int32_t k = rb + lane; k < re; k += 2 * G) {
+            const int32_t k1 = k + G;
+            const int32_t c0 = __builtin_nontemporal_load(col + k);
+            const double a0 = (double)__builtin_nontemporal_load(val + k);
+            int32_t c1 = c0;
+            double a1 = 0.0;
+            if (k1 < re) {
+                c1 = __builtin_nontemporal_load(col + k1);
+                a1 = (double)__builtin_nontemporal_load(val + k1);
+            }
+            s += a0 * (double)w[c0] + a1 * (double)w[c1];
+        }
     }
     s = wave_sum_group(s, G);
     if (active && lane == 0) {
@@ -145,12 +158,14 @@ __global__ void k_jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_
     }
 }
 
-// gather/scatter with precision conversion: level[perm[i]] <- surf[i] and back
+// surface (DoF order) <-> level (lattice order) copies with precision
+// conversion, written as gathers so the stores are coalesced:
+//   dst_lat[l] = src_dof[dof_of_lat[l]]   and   dst_dof[i] += src_lat[lat_of_dof[i]]
 template <typename TS, typename TL>
-__global__ void k_permute_to(int64_t n, const int32_t* __restrict__ lat_of_dof, const TS* __restrict__ src,
+__global__ void k_permute_to(int64_t n, const int32_t* __restrict__ dof_of_lat, const TS* __restrict__ src,
                              TL* __restrict__ dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[lat_of_dof[i]] = (TL)src[i];
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x)
+        dst[l] = (TL)src[dof_of_lat[l]];
 }
 template <typename TL, typename TS>
 __global__ void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
@@ -198,58 +213,83 @@ struct DiaOffsets {
     int32_t off[27];
 };
 
-// d_out = d_in - A*x  (x may be any vector; d_out may equal d_in)
-template <typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, int nd, DiaOffsets o, const TM* __restrict__ A,
+// Accumulation type: FP32 level operators compute in FP32, FP64 in FP64.
+template <typename T>
+struct Acc {
+    typedef T type;
+};
+
+// d_out = d_in - A*x  (x may be any vector; d_out may equal d_in). ND is the
+// number of stored diagonals (27 in 3D, 9 in 2D); all loads of a row are issued
+// before the multiply-adds.
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, DiaOffsets o, const TM* __restrict__ A,
                                                        const TV* __restrict__ x, const TV* d_in, TV* d_out,
                                                        int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
     const int64_t lb = xcd_block(nblocks);
     const int64_t i = lb * 256 + threadIdx.x;
     if (lb >= nblocks || i >= n) return;
-    double s = 0.0;
+    T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < 27; ++d)
-        if (d < nd) s += (double)A[(int64_t)d * n + i] * (double)x[i + o.off[d]];
-    d_out[i] = (TV)((double)d_in[i] - s);
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], xv[d], s);
+    d_out[i] = (TV)((T)d_in[i] - s);
 }
 
 // y = alpha*v + beta*A*x
-template <typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, int nd, DiaOffsets o, const TM* __restrict__ A,
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, DiaOffsets o, const TM* __restrict__ A,
                                                   double alpha, const TV* v, double beta, const TV* __restrict__ x,
                                                   TV* y, int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
     const int64_t lb = xcd_block(nblocks);
     const int64_t i = lb * 256 + threadIdx.x;
     if (lb >= nblocks || i >= n) return;
-    double s = 0.0;
+    T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < 27; ++d)
-        if (d < nd) s += (double)A[(int64_t)d * n + i] * (double)x[i + o.off[d]];
-    y[i] = (TV)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * s);
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], xv[d], s);
+    y[i] = (TV)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * (double)s);
 }
 
 // One damped-Jacobi smoothing step fused with the defect update and the
 // correction update (mg_solver_impl.hpp:1702-1730 / 1917-1947):
 //   st = dinv .* sd_in ;  sd_out = sd_in - A*st ;  sc += st
 // sd_out must not alias sd_in (neighbours read the old defect).
-template <typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, int nd, DiaOffsets o,
-                                                         const TM* __restrict__ A, const TV* __restrict__ dinv,
-                                                         const TV* __restrict__ sd_in, TV* __restrict__ sd_out,
-                                                         TV* __restrict__ sc, int64_t nblocks) {
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, DiaOffsets o, const TM* __restrict__ A,
+                                                         const TV* __restrict__ dinv, const TV* __restrict__ sd_in,
+                                                         TV* __restrict__ sd_out, TV* __restrict__ sc,
+                                                         int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
     const int64_t lb = xcd_block(nblocks);
     const int64_t i = lb * 256 + threadIdx.x;
     if (lb >= nblocks || i >= n) return;
-    double s = 0.0;
+    T a[ND], dv[ND], sv[ND];
 #pragma unroll
-    for (int d = 0; d < 27; ++d)
-        if (d < nd) {
-            const int64_t j = i + o.off[d];
-            s += (double)A[(int64_t)d * n + i] * ((double)dinv[j] * (double)sd_in[j]);
-        }
-    const double st = (double)dinv[i] * (double)sd_in[i];
-    sd_out[i] = (TV)((double)sd_in[i] - s);
-    sc[i] = (TV)((double)sc[i] + st);
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const int64_t j = i + o.off[d];
+        dv[d] = (T)dinv[j];
+        sv[d] = (T)sd_in[j];
+    }
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], dv[d] * sv[d], s);
+    const T sdi = sv[ND / 2];
+    const T st = dv[ND / 2] * sdi;
+    sd_out[i] = (TV)(sdi - s);
+    sc[i] = (TV)((T)sc[i] + st);
 }
 
 // sc += dinv .* sd  (last post-smoothing step on levels below the top, where
@@ -370,42 +410,30 @@ __global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64
 }
 
 // ---------------------------------------------------------------------------
-// dense LU solve for the base level (one workgroup; factors in FP64)
+// base-level solve with the LU factors of the dense base matrix (FP64):
+// x = U^{-1} L^{-1} P b, the triangular inverses stored column-major so that
+// the threads of a block read consecutive entries. One workgroup.
 // ---------------------------------------------------------------------------
 template <typename TV>
-__global__ __launch_bounds__(256) void k_lu_solve(int n, const double* __restrict__ LU, const int32_t* __restrict__ piv,
-                                                  const int32_t* __restrict__ lat_of_row, const TV* __restrict__ b,
-                                                  TV* __restrict__ x, double* work) {
-    // work holds n doubles (device scratch owned by the LU object)
-    for (int i = threadIdx.x; i < n; i += blockDim.x) work[i] = (double)b[lat_of_row[i]];
+__global__ __launch_bounds__(256) void k_lu_solve(int n, const double* __restrict__ LinvT, const double* __restrict__ UinvT,
+                                                  const int32_t* __restrict__ perm, const TV* __restrict__ b,
+                                                  TV* __restrict__ x) {
+    extern __shared__ double sh[];
+    double* pb = sh;        // permuted rhs
+    double* t = sh + n;     // L^{-1} P b
+    for (int i = threadIdx.x; i < n; i += blockDim.x) pb[i] = (double)b[perm[i]];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < n; ++i)
-            if (i < piv[i]) { double t = work[i]; work[i] = work[piv[i]]; work[piv[i]] = t; }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k <= i; ++k) s += LinvT[(int64_t)k * n + i] * pb[k];
+        t[i] = s;
     }
     __syncthreads();
-    // forward substitution (unit lower)
-    for (int i = 0; i < n; ++i) {
-        __shared__ double red[4];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         double s = 0.0;
-        for (int k = threadIdx.x; k < i; k += blockDim.x) s += LU[(int64_t)i * n + k] * work[k];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) work[i] = work[i] - ((red[0] + red[1]) + (red[2] + red[3]));
-        __syncthreads();
+        for (int k = i; k < n; ++k) s += UinvT[(int64_t)k * n + i] * t[k];
+        x[i] = (TV)s;
     }
-    for (int i = n - 1; i >= 0; --i) {
-        __shared__ double red2[4];
-        double s = 0.0;
-        for (int k = i + 1 + threadIdx.x; k < n; k += blockDim.x) s += LU[(int64_t)i * n + k] * work[k];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if ((threadIdx.x & 63) == 0) red2[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) work[i] = (work[i] - ((red2[0] + red2[1]) + (red2[2] + red2[3]))) / LU[(int64_t)i * n + i];
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) x[lat_of_row[i]] = (TV)work[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -796,9 +824,9 @@ template int dot<double>(int64_t, const double*, const double*, double*, double*
 template int dot<float>(int64_t, const float*, const float*, double*, double*, bool, hipStream_t);
 
 template <typename TS, typename TL>
-int permute_to(int64_t n, const int32_t* lat_of_dof, const TS* src, TL* dst, hipStream_t s) {
+int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_permute_to<TS, TL>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    hipLaunchKernelGGL((k_permute_to<TS, TL>), dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst);
     UGX_CHECK_LAUNCH();
 }
 template int permute_to<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
@@ -821,12 +849,22 @@ static DiaOffsets make_offsets(const int32_t* off, int nd) {
 
 static inline int64_t remapped_grid(int64_t nblocks) { return nblocks < 64 ? nblocks : ((nblocks + 7) / 8) * 8; }
 
+#define UGX_DIA_DISPATCH(KERNEL, ...)                                                                      \
+    do {                                                                                                  \
+        const int64_t nb = (n + 255) / 256;                                                               \
+        if (nd == 27)                                                                                     \
+            hipLaunchKernelGGL((KERNEL<TM, TV, 27>), dim3(remapped_grid(nb)), dim3(256), 0, s, __VA_ARGS__, nb); \
+        else if (nd == 9)                                                                                 \
+            hipLaunchKernelGGL((KERNEL<TM, TV, 9>), dim3(remapped_grid(nb)), dim3(256), 0, s, __VA_ARGS__, nb);  \
+        else                                                                                              \
+            return -1;                                                                                    \
+    } while (0)
+
 template <typename TM, typename TV>
 int dia_apply_sub(int64_t n, int nd, const int32_t* off, const TM* A, const TV* x, const TV* d_in, TV* d_out,
                   hipStream_t s) {
     if (n <= 0) return 0;
-    const int64_t nb = (n + 255) / 256;
-    hipLaunchKernelGGL((k_dia_apply_sub<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, x, d_in, d_out, nb);
+    UGX_DIA_DISPATCH(k_dia_apply_sub, n, make_offsets(off, nd), A, x, d_in, d_out);
     UGX_CHECK_LAUNCH();
 }
 template int dia_apply_sub<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, hipStream_t);
@@ -836,8 +874,7 @@ template <typename TM, typename TV>
 int dia_axpy(int64_t n, int nd, const int32_t* off, const TM* A, double alpha, const TV* v, double beta, const TV* x,
              TV* y, hipStream_t s) {
     if (n <= 0) return 0;
-    const int64_t nb = (n + 255) / 256;
-    hipLaunchKernelGGL((k_dia_axpy<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, alpha, v, beta, x, y, nb);
+    UGX_DIA_DISPATCH(k_dia_axpy, n, make_offsets(off, nd), A, alpha, v, beta, x, y);
     UGX_CHECK_LAUNCH();
 }
 template int dia_axpy<double, double>(int64_t, int, const int32_t*, const double*, double, const double*, double, const double*, double*, hipStream_t);
@@ -847,8 +884,7 @@ template <typename TM, typename TV>
 int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV* dinv, const TV* sd_in, TV* sd_out,
                     TV* sc, hipStream_t s) {
     if (n <= 0) return 0;
-    const int64_t nb = (n + 255) / 256;
-    hipLaunchKernelGGL((k_dia_jacobi_step<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, n, nd, make_offsets(off, nd), A, dinv, sd_in, sd_out, sc, nb);
+    UGX_DIA_DISPATCH(k_dia_jacobi_step, n, make_offsets(off, nd), A, dinv, sd_in, sd_out, sc);
     UGX_CHECK_LAUNCH();
 }
 template int dia_jacobi_step<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, double*, hipStream_t);
@@ -893,13 +929,13 @@ template int restrict_defect<double>(int, int, int, int64_t, const double*, cons
 template int restrict_defect<float>(int, int, int, int64_t, const float*, const uint8_t*, const uint8_t*, float*, hipStream_t);
 
 template <typename TV>
-int lu_solve(int n, const double* LU, const int32_t* piv, const int32_t* lat_of_row, const TV* b, TV* x, double* work,
-             hipStream_t s) {
-    hipLaunchKernelGGL(k_lu_solve<TV>, dim3(1), dim3(256), 0, s, n, LU, piv, lat_of_row, b, x, work);
+int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s) {
+    if (n <= 0 || n > 4096) return -1;
+    hipLaunchKernelGGL(k_lu_solve<TV>, dim3(1), dim3(256), 2 * n * sizeof(double), s, n, LinvT, UinvT, perm, b, x);
     UGX_CHECK_LAUNCH();
 }
-template int lu_solve<double>(int, const double*, const int32_t*, const int32_t*, const double*, double*, double*, hipStream_t);
-template int lu_solve<float>(int, const double*, const int32_t*, const int32_t*, const float*, float*, double*, hipStream_t);
+template int lu_solve<double>(int, const double*, const double*, const int32_t*, const double*, double*, hipStream_t);
+template int lu_solve<float>(int, const double*, const double*, const int32_t*, const float*, float*, hipStream_t);
 
 int invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof, hipStream_t s) {
     hipLaunchKernelGGL(k_invert_map, dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, lat_of_dof);

@@ -24,7 +24,7 @@ int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const 
 template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s);
 template <typename TS, typename TL>
-int permute_to(int64_t n, const int32_t* lat_of_dof, const TS* src, TL* dst, hipStream_t s);
+int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hipStream_t s);
 template <typename TL, typename TS>
 int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s);
 
@@ -47,9 +47,9 @@ int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8
 template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
                     TV* sdc, hipStream_t s);
+// x = U^{-1} L^{-1} P b with column-major triangular inverses; perm[i] = row of b
 template <typename TV>
-int lu_solve(int n, const double* LU, const int32_t* piv, const int32_t* lat_of_row, const TV* b, TV* x, double* work,
-             hipStream_t s);
+int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s);
 
 int invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof, hipStream_t s);
 int pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t* cnt, hipStream_t s);
