@@ -118,16 +118,24 @@ def main():
     nnz = A.nnz()
     log(f"[bench] N={N} nnz={nnz} assembly {t_asm:.3f} s (mesh+numbering {t_mesh:.2f} s)")
 
-    # SpMV set-up: pick the lanes-per-row of the CSR kernel for this matrix
+    # SpMV set-up: CSR kernel lanes-per-row sweep, then the sliced-ELL layout
+    # (the GPUSparseMatrix default); the faster one is used by the solves
     xs = U.Vector(values=np.random.default_rng(0).normal(size=N))
     ys = U.Vector(N)
+    A.set_spmv_layout("csr")
     sweep = {}
     for lanes in (8, 16, 32):
         A.set_spmv_lanes(lanes)
         sweep[lanes] = U.time_launches(lambda: A.apply(ys, xs), max(3, args.kernel_reps // 4))
     best_lanes = min(sweep, key=sweep.get)
     A.set_spmv_lanes(best_lanes)
-    log(f"[bench] CSR SpMV lanes sweep (ms): " + ", ".join(f"{k}: {v*1e3:.3f}" for k, v in sweep.items()))
+    t_csr = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
+    A.set_spmv_layout("sell")
+    t_sell = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
+    layout = "sell" if t_sell <= t_csr else "csr"
+    A.set_spmv_layout(layout)
+    log(f"[bench] CSR SpMV lanes sweep (ms): " + ", ".join(f"{k}: {v*1e3:.3f}" for k, v in sweep.items())
+        + f"; sliced ELL {t_sell*1e3:.3f} ms -> {layout}")
 
     solvers = {}
     setup_s = {}
@@ -168,34 +176,28 @@ def main():
     final_rel_res = float(np.linalg.norm(r.numpy()) / np.linalg.norm(b.numpy()))
 
     # ---- kernel timings with HIP events recorded on the library stream --------
-    def spmv():
-        A.apply(ys, xs)
-
-    t_spmv = U.time_launches(spmv, args.kernel_reps)
+    t_spmv = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
+    spmv_bytes = nnz * 12 + (N + 1) * 4 + N * 8 + N * 8
+    spmv_gbs = spmv_bytes / t_spmv / 1e9
 
     # the same operator with an OrderLex-style DoF numbering (lattice order):
-    # isolates what the reference numbering costs the x-gathers of the CSR SpMV
+    # isolates what the reference numbering costs the x-gathers
     lex = U.DomainDiscretization(0, refs, ordering="lex")
     lex.set_theta(theta)
     A_lex = U.SparseMatrix()
     b_lex = U.Vector(N)
     lex.assemble_linear(A_lex, b_lex)
-    sweep_lex = {}
-    for lanes in (8, 16, 32):
-        A_lex.set_spmv_lanes(lanes)
-        sweep_lex[lanes] = U.time_launches(lambda: A_lex.apply(ys, xs), args.kernel_reps)
-    t_spmv_lex = min(sweep_lex.values())
+    A_lex.set_spmv_layout(layout)
+    t_spmv_lex = U.time_launches(lambda: A_lex.apply(ys, xs), args.kernel_reps)
     del A_lex, b_lex, lex
-    log(f"[bench] CSR SpMV f64 lexicographic numbering: " + ", ".join(f"{k}: {v*1e3:.3f}" for k, v in sweep_lex.items()))
-    spmv_bytes = nnz * 12 + (N + 1) * 4 + N * 8 + N * 8
-    spmv_gbs = spmv_bytes / t_spmv / 1e9
+    log(f"[bench] SpMV f64 ({layout}) lexicographic numbering: {t_spmv_lex*1e3:.3f} ms")
 
     g32 = solvers["f32"][0]
     nd = L.ugx_gmg_level_num_diags(g32.h, refs)
     t_smooth = U.time_launches(lambda: g32.smooth_level(refs, 1), args.kernel_reps)
     smooth_bytes = N * 4 * (nd + 5)      # 27 value streams + sd in/out + dinv + sc read/write
     smooth_gbs = smooth_bytes / t_smooth / 1e9
-    log(f"[bench] CSR SpMV f64: {t_spmv*1e3:.3f} ms  {spmv_gbs:.0f} GB/s; "
+    log(f"[bench] SpMV f64 ({layout}): {t_spmv*1e3:.3f} ms  {spmv_gbs:.0f} GB/s; "
         f"DIA Jacobi step f32: {t_smooth*1e3:.3f} ms  {smooth_gbs:.0f} GB/s")
 
     t32 = float(np.mean(times["f32"]))
@@ -211,7 +213,7 @@ def main():
                 "frac": smooth_gbs / HBM_PEAK_GBS, "traffic": None,
                 "algorithmic_bytes_per_launch": smooth_bytes, "launch_ms": t_smooth * 1e3}
     else:
-        roof = {"kernel": f"k_spmv_csr<double,double,{best_lanes}>", "bound": "hbm",
+        roof = {"kernel": "k_sell_spmv<double,double>" if layout == "sell" else f"k_spmv_csr<double,double,{best_lanes}>", "bound": "hbm",
                 "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
                 "algorithmic_bytes_per_launch": spmv_bytes, "launch_ms": t_spmv * 1e3}
@@ -251,13 +253,16 @@ def main():
             "assembly_elements_per_s": n ** 3 / t_asm,
             "gmg_setup_s": setup_s,
             "mesh_and_numbering_s": t_mesh,
-            "spmv_csr_f64_ms": t_spmv * 1e3,
+            "spmv_layout": layout,
+            "spmv_f64_ms": t_spmv * 1e3,
+            "spmv_csr_direct_ms": t_csr * 1e3,
+            "spmv_sliced_ell_ms": t_sell * 1e3,
             "spmv_csr_lanes": best_lanes,
-            "spmv_csr_f64_lex_ordering_ms": t_spmv_lex * 1e3,
-            "spmv_csr_f64_lex_ordering_GBps": spmv_bytes / t_spmv_lex / 1e9,
+            "spmv_f64_lex_ordering_ms": t_spmv_lex * 1e3,
+            "spmv_f64_lex_ordering_GBps": spmv_bytes / t_spmv_lex / 1e9,
             "spmv_csr_lanes_sweep_ms": {str(k): v * 1e3 for k, v in sweep.items()},
-            "spmv_csr_f64_GBps": spmv_gbs,
-            "spmv_csr_f64_frac_of_8TBps": spmv_gbs / HBM_PEAK_GBS,
+            "spmv_f64_GBps": spmv_gbs,
+            "spmv_f64_frac_of_8TBps": spmv_gbs / HBM_PEAK_GBS,
             "smoother_dia_f32_ms": t_smooth * 1e3,
             "smoother_dia_f32_GBps": smooth_gbs,
         },

@@ -127,6 +127,11 @@ class GPUSparseMatrix {
     int32_t* rowptr() const { return rowptr_.get(); }
     int32_t* col() const { return col_.get(); }
     double* val() const { return val_.get(); }
+    // build the sliced-ELL SpMV layout from the CSR (after the values are set;
+    // assembly and uploads call it); until then SpMV runs on the CSR directly
+    void finalize();
+    bool uses_sliced_ell() const { return sell_valid_; }
+    void set_use_sliced_ell(bool on) { sell_enabled_ = on; if (!on) sell_valid_ = false; }
     int lanes() const { return lanes_; }
     void set_lanes(int g) {
         if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64) throw UGError("set_lanes: 4, 8, 16, 32 or 64");
@@ -138,6 +143,10 @@ class GPUSparseMatrix {
     int lanes_ = 8;
     DeviceBuffer<int32_t> rowptr_, col_;
     DeviceBuffer<double> val_;
+    bool sell_enabled_ = true, sell_valid_ = false;
+    DeviceBuffer<int32_t> sell_width_, sell_col_;
+    DeviceBuffer<int64_t> sell_ptr_;
+    DeviceBuffer<double> sell_val_;
 };
 
 struct AlgebraType {
@@ -298,6 +307,14 @@ class DomainDiscretization {
     // diagonals of length N, FP64), Dirichlet rows set
     void assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd, std::vector<int32_t>& offsets);
     double assemble_seconds() const { return t_assemble_; }
+    // theta from device memory (fine-element lattice order)
+    void set_theta_device(const double* theta_dev);
+    // DirichletBoundary::adjust_defect: zero the Dirichlet entries of a
+    // top-level vector (lagrange_dirichlet_boundary_impl.h:920-1035)
+    void adjust_defect(GPUVector& d);
+    // adjoint gradient g_e = -lambda_e^T (dA_e/dtheta_e) u_e per fine element
+    // (kind 0); grad_dev holds n_fine^3 doubles
+    void adjoint_gradient(const GPUVector& u, const GPUVector& lambda, double* grad_dev);
 
    private:
     int kind_;

@@ -34,6 +34,7 @@ void* ugx_malloc(int64_t bytes);
 int ugx_free(void* p);
 int ugx_memcpy_htod(void* dst_dev, const void* src_host, int64_t bytes);   /* synchronous */
 int ugx_memcpy_dtoh(void* dst_host, const void* src_dev, int64_t bytes);   /* synchronous */
+int ugx_memcpy_dtod(void* dst_dev, const void* src_dev, int64_t bytes);    /* synchronous */
 void* ugx_event_create(void);
 int ugx_event_destroy(void* ev);
 int ugx_event_record(void* ev, void* stream);
@@ -102,6 +103,15 @@ int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out_host);   /* (n+1
 int ugx_disc_set_theta(void* disc, const double* theta_host);            /* k = exp(theta) per fine element */
 int ugx_disc_set_source_const(void* disc, double f);
 int ugx_disc_set_source_dofs(void* disc, const double* f_host);          /* per top-level DoF (FV1) */
+int ugx_disc_set_theta_device(void* disc, const double* theta_dev);
+/* zero the Dirichlet entries of a top-level vector handle (DirichletBoundary::adjust_defect,
+ * lagrange_dirichlet_boundary_impl.h:920-1035) */
+int ugx_disc_adjust_defect(void* disc, void* vec);
+/* Adjoint gradient (no counterpart in the reference): for J(u(theta)) with
+ * A(theta) u = b and the adjoint solution lambda of A^T lambda = dJ/du,
+ * grad_dev[e] = -lambda_e^T (dA_e/dtheta_e) u_e for every fine element e
+ * (lattice order, n_fine^3 doubles, device memory). Q1 hexahedra (kind 0). */
+int ugx_disc_adjoint_gradient(void* disc, void* u, void* lambda, double* grad_dev);
 /* DomainDiscretization::assemble_linear(A, b) (domain_disc_impl.h:835-948) incl. Dirichlet rows */
 int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
 double ugx_disc_assemble_seconds(void* disc);
@@ -118,6 +128,11 @@ int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w);   /* vectors are handles */
 /* tuning: lanes of a wavefront that share one CSR row in this matrix's SpMV (4..64, power of 2) */
 int ugx_matrix_set_spmv_lanes(void* m, int lanes);
+/* SpMV layout: 1 = sliced ELL (64-row slices, built from the CSR; default),
+ * 0 = CSR directly. Call ugx_matrix_finalize after changing values through
+ * the raw CSR pointers of ugx_matrix_device_csr. */
+int ugx_matrix_set_spmv_layout(void* m, int sliced_ell);
+int ugx_matrix_finalize(void* m);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
 
 /* GPUVector (vector_type) */

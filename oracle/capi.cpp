@@ -245,6 +245,49 @@ int orc_gmg_apply(void* h, int base_lev, int nu1, int nu2, int cycle, double ome
     }
 }
 
+// solve A x = rhs on the top level (rhs given; Dirichlet entries zeroed as
+// DirichletBoundary::adjust_defect does), GMG-CG; returns steps (<0 failure)
+int orc_solve_rhs(void* h, const double* rhs, double* x_out, int base_lev, double rel_red) {
+    auto* p = (Problem*)h;
+    try {
+        const int top = (int)p->mg.levels.size() - 1;
+        GMG g;
+        g.base_lev = base_lev;
+        g.top_lev = top;
+        g.A = p->A;
+        g.P = p->P;
+        g.R = p->R;
+        g.init();
+        const int64_t n = p->A[top].n_rows;
+        auto dm = dirichlet_mask(p->mg, top);
+        std::vector<double> b(rhs, rhs + n), x(n, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            if (dm[i]) b[i] = 0.0;
+        ConvCheck cc;
+        cc.max_steps = 200;
+        cc.rel_reduction = rel_red;
+        auto r = cg_solve(p->A[top], &g, x, b, cc);
+        std::memcpy(x_out, x.data(), sizeof(double) * n);
+        return r.ok ? r.steps : -1 - r.steps;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100000;
+    }
+}
+
+void orc_adjoint_gradient(void* h, const double* u, const double* lambda, double* g_out) {
+    auto* p = (Problem*)h;
+    Coefficient k;
+    k.n_fine = p->n_fine;
+    k.dim = 3;
+    k.theta = p->theta.empty() ? nullptr : p->theta.data();
+    const int top = (int)p->mg.levels.size() - 1;
+    const int64_t n = p->mg.levels[top].n_dof;
+    std::vector<double> uu(u, u + n), ll(lambda, lambda + n), g;
+    adjoint_gradient_q1_hex(p->mg, k, uu, ll, g);
+    std::memcpy(g_out, g.data(), sizeof(double) * g.size());
+}
+
 void orc_solution_get(void* h, double* out) {
     auto* p = (Problem*)h;
     std::memcpy(out, p->x.data(), sizeof(double) * p->x.size());

@@ -18,6 +18,7 @@
 //  * Dirichlet: SetDirichletRow (algebra_common/sparsematrix_util.h:850-861),
 //    rhs value 0 (lagrange_dirichlet_boundary_impl.h:753-1035).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <stdexcept>
 
@@ -71,51 +72,61 @@ std::vector<uint8_t> dirichlet_mask(const MultiGrid& mg, int lev) {
     return m;
 }
 
+// Q1 element matrix and load vector of hexahedron `e` on level `lev`
+// (2x2x2 Gauss, trilinear geometry, k at each quadrature point)
+static void q1_element(const MultiGrid& mg, const Level& L, size_t e, const Coefficient& k, double f_const,
+                       double Ke[8][8], double fe[8], int ind[8]) {
+    const double scale = 1.0 / (double)mg.finest_scale;
+    const double g = 0.5 / std::sqrt(3.0);
+    const double qp1[2] = {0.5 - g, 0.5 + g};
+    double X[8][3];
+    for (int i = 0; i < 8; ++i) {
+        int v = L.vols[e][i];
+        ind[i] = L.dof[v];
+        for (int d = 0; d < 3; ++d) X[i][d] = (double)L.vpos[v][d] * scale;
+    }
+    for (int i = 0; i < 8; ++i) {
+        fe[i] = 0.0;
+        for (int j = 0; j < 8; ++j) Ke[i][j] = 0.0;
+    }
+    for (int qz = 0; qz < 2; ++qz)
+        for (int qy = 0; qy < 2; ++qy)
+            for (int qx = 0; qx < 2; ++qx) {
+                const double xi[3] = {qp1[qx], qp1[qy], qp1[qz]};
+                double phi[8], dphi[8][3];
+                q1_shape(xi, phi, dphi);
+                double J[3][3] = {{0}}, Ji[3][3], xq[3] = {0, 0, 0};
+                for (int i = 0; i < 8; ++i)
+                    for (int r = 0; r < 3; ++r) {
+                        xq[r] += phi[i] * X[i][r];
+                        for (int c = 0; c < 3; ++c) J[r][c] += X[i][r] * dphi[i][c];
+                    }
+                const double det = inv3(J, Ji);
+                const double w = 0.125 * std::fabs(det);
+                const double kq = k.eval(xq);
+                double grad[8][3];
+                for (int i = 0; i < 8; ++i)
+                    for (int r = 0; r < 3; ++r)
+                        grad[i][r] = Ji[0][r] * dphi[i][0] + Ji[1][r] * dphi[i][1] + Ji[2][r] * dphi[i][2];
+                for (int i = 0; i < 8; ++i) {
+                    for (int j = 0; j < 8; ++j)
+                        Ke[i][j] += w * kq * (grad[i][0] * grad[j][0] + grad[i][1] * grad[j][1] + grad[i][2] * grad[j][2]);
+                    fe[i] += w * f_const * phi[i];
+                }
+            }
+}
+
 void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double f_const,
                      CSR& A, std::vector<double>& b) {
     const Level& L = mg.levels[lev];
-    const double scale = 1.0 / (double)mg.finest_scale;
     RowBuilder B;
     B.resize_and_clear(L.n_dof, L.n_dof);
     b.assign(L.n_dof, 0.0);
-    const double g = 0.5 / std::sqrt(3.0);
-    const double qp1[2] = {0.5 - g, 0.5 + g};
     for (size_t e = 0; e < L.vols.size(); ++e) {
         if (L.volsub[e] != 0) continue;   // element disc defined on subset "Inner"
-        double X[8][3];
+        double Ke[8][8], fe[8];
         int ind[8];
-        for (int i = 0; i < 8; ++i) {
-            int v = L.vols[e][i];
-            ind[i] = L.dof[v];
-            for (int d = 0; d < 3; ++d) X[i][d] = (double)L.vpos[v][d] * scale;
-        }
-        double Ke[8][8] = {{0}};
-        double fe[8] = {0};
-        for (int qz = 0; qz < 2; ++qz)
-            for (int qy = 0; qy < 2; ++qy)
-                for (int qx = 0; qx < 2; ++qx) {
-                    const double xi[3] = {qp1[qx], qp1[qy], qp1[qz]};
-                    double phi[8], dphi[8][3];
-                    q1_shape(xi, phi, dphi);
-                    double J[3][3] = {{0}}, Ji[3][3], xq[3] = {0, 0, 0};
-                    for (int i = 0; i < 8; ++i)
-                        for (int r = 0; r < 3; ++r) {
-                            xq[r] += phi[i] * X[i][r];
-                            for (int c = 0; c < 3; ++c) J[r][c] += X[i][r] * dphi[i][c];
-                        }
-                    const double det = inv3(J, Ji);
-                    const double w = 0.125 * std::fabs(det);
-                    const double kq = k.eval(xq);
-                    double grad[8][3];
-                    for (int i = 0; i < 8; ++i)
-                        for (int r = 0; r < 3; ++r)
-                            grad[i][r] = Ji[0][r] * dphi[i][0] + Ji[1][r] * dphi[i][1] + Ji[2][r] * dphi[i][2];
-                    for (int i = 0; i < 8; ++i) {
-                        for (int j = 0; j < 8; ++j)
-                            Ke[i][j] += w * kq * (grad[i][0] * grad[j][0] + grad[i][1] * grad[j][1] + grad[i][2] * grad[j][2]);
-                        fe[i] += w * f_const * phi[i];
-                    }
-                }
+        q1_element(mg, L, e, k, f_const, Ke, fe, ind);
         for (int i = 0; i < 8; ++i) {
             for (int j = 0; j < 8; ++j) B.at(ind[i], ind[j]) += Ke[i][j];
             b[ind[i]] += fe[i];
@@ -131,6 +142,34 @@ void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double 
             *d = 1.0;
             b[r] = 0.0;
         }
+}
+
+void adjoint_gradient_q1_hex(const MultiGrid& mg, const Coefficient& k, const std::vector<double>& u,
+                             const std::vector<double>& lambda, std::vector<double>& g) {
+    // dJ/dtheta_e = -lambda^T (dA/dtheta_e) u. On the fine level k is
+    // exp(theta_e) at every quadrature point of element e, so dA_e/dtheta_e =
+    // A_e; Dirichlet rows of A do not depend on theta.
+    const int top = (int)mg.levels.size() - 1;
+    const Level& L = mg.levels[top];
+    const int64_t n = mg.finest_scale;
+    g.assign(n * n * n, 0.0);
+    auto dm = dirichlet_mask(mg, top);
+    for (size_t e = 0; e < L.vols.size(); ++e) {
+        double Ke[8][8], fe[8];
+        int ind[8];
+        q1_element(mg, L, e, k, 0.0, Ke, fe, ind);
+        int64_t mn[3] = {INT64_MAX, INT64_MAX, INT64_MAX};
+        for (int i = 0; i < 8; ++i)
+            for (int d = 0; d < 3; ++d) mn[d] = std::min(mn[d], L.vpos[L.vols[e][i]][d]);
+        double acc = 0.0;
+        for (int a = 0; a < 8; ++a) {
+            if (dm[ind[a]]) continue;
+            double row = 0.0;
+            for (int b2 = 0; b2 < 8; ++b2) row += Ke[a][b2] * u[ind[b2]];
+            acc += lambda[ind[a]] * row;
+        }
+        g[(mn[2] * n + mn[1]) * n + mn[0]] = -acc;
+    }
 }
 
 void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,

@@ -148,6 +148,10 @@ void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double 
 void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
                       const std::vector<double>& f_at_dof, CSR& A, std::vector<double>& b);
 std::vector<uint8_t> dirichlet_mask(const MultiGrid& mg, int lev);
+// adjoint element gradient on the top level: g_e = -lambda_e^T (dA_e/dtheta_e) u_e
+// (interior rows only), g in fine-element lattice order
+void adjoint_gradient_q1_hex(const MultiGrid& mg, const Coefficient& k, const std::vector<double>& u,
+                             const std::vector<double>& lambda, std::vector<double>& g);
 
 // StdTransfer::assemble_prolongation_p1 (std_transfer_impl.h:45-168) followed by
 // DirichletBoundary::adjust_prolongation (lagrange_dirichlet_boundary_impl.h:485-606)

@@ -53,6 +53,8 @@ def lib():
             "orc_solution_get": (None, [P, F64P]),
             "orc_gmg_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, F64P, F64P]),
             "orc_defects_get": (C.c_int, [P, C.c_void_p, C.c_int]),
+            "orc_solve_rhs": (C.c_int, [P, F64P, F64P, C.c_int, C.c_double]),
+            "orc_adjoint_gradient": (None, [P, F64P, F64P, F64P]),
             "orc_timings": (None, [P, F64P]),
             "orc_builder_create": (P, [I64, I64]),
             "orc_builder_set": (None, [P, I64, I64, C.c_double]),
@@ -220,6 +222,20 @@ class Problem:
         if lib().orc_gmg_apply(self.h, base_lev, nu1, nu2, cycle, omega, d, c) != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return c
+
+    def solve_rhs(self, rhs, base_lev=0, rel_red=1e-12):
+        top = self.num_levels - 1
+        x = np.zeros(self.num_dofs(top), np.float64)
+        r = lib().orc_solve_rhs(self.h, np.ascontiguousarray(rhs, np.float64), x, base_lev, rel_red)
+        if r < 0:
+            raise RuntimeError("oracle solve failed")
+        return x
+
+    def adjoint_gradient(self, u, lam):
+        g = np.zeros((2 ** self.num_refs) ** 3, np.float64)
+        lib().orc_adjoint_gradient(self.h, np.ascontiguousarray(u, np.float64),
+                                   np.ascontiguousarray(lam, np.float64), g)
+        return g
 
     def timings(self):
         out = np.zeros(3, np.float64)

@@ -334,3 +334,34 @@ def test_lexicographic_ordering_is_a_renumbering():
         dol = d.dof_of_lattice(refs)
         sols[ordering] = x.numpy()[dol]        # lattice order
     assert np.linalg.norm(sols["lex"] - sols["reference"]) <= 1e-10 * np.linalg.norm(sols["reference"])
+
+
+def test_sliced_ell_and_csr_layouts_agree():
+    """GPUSparseMatrix SpMV through the sliced-ELL layout and through the CSR
+    kernel against the oracle's SparseMatrix::axpy, on a matrix with empty and
+    ragged rows (rows and columns not multiples of 64)."""
+    say("case sliced ell")
+    u = U()
+    rng = np.random.default_rng(11)
+    rows, cols = 1000, 777
+    ent = []
+    for r in range(rows):
+        if r % 9 == 0:
+            continue
+        for _ in range(int(rng.integers(1, 40))):
+            ent.append((r, int(rng.integers(cols)), float(rng.normal())))
+    M = O.builder_csr(ent, rows, cols, mode=0, add=True)
+    A = u.SparseMatrix()
+    A.upload_csr(M.rowptr.astype(np.int32), M.col, M.val, n_cols=cols)
+    w = rng.normal(size=cols)
+    v1 = rng.normal(size=rows)
+    for layout in ("sell", "csr"):
+        A.set_spmv_layout(layout)
+        for alpha, beta in ((0.0, 1.0), (1.0, -1.0), (0.7, 1.3)):
+            dest = u.Vector(values=rng.normal(size=rows))
+            vv = u.Vector(values=v1)
+            ww = u.Vector(values=w)
+            A.axpy(dest, alpha, vv, beta, ww)
+            expect = np.zeros(rows)
+            O.lib().orc_csr_op(0, rows, cols, M.rowptr, M.col, M.val, alpha, v1.ctypes.data, beta, w, expect)
+            assert np.allclose(dest.numpy(), expect, rtol=1e-13, atol=1e-13), layout

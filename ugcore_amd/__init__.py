@@ -33,6 +33,7 @@ _SIGNATURES = {
     "ugx_free": (_I, [_P]),
     "ugx_memcpy_htod": (_I, [_P, _P, _I64]),
     "ugx_memcpy_dtoh": (_I, [_P, _P, _I64]),
+    "ugx_memcpy_dtod": (_I, [_P, _P, _I64]),
     "ugx_event_create": (_P, []),
     "ugx_event_destroy": (_I, [_P]),
     "ugx_event_record": (_I, [_P, _P]),
@@ -55,6 +56,9 @@ _SIGNATURES = {
     "ugx_disc_num_dofs": (_I64, [_P, _I]),
     "ugx_disc_dof_of_lattice": (_I, [_P, _I, _P]),
     "ugx_disc_set_theta": (_I, [_P, _P]),
+    "ugx_disc_set_theta_device": (_I, [_P, _P]),
+    "ugx_disc_adjust_defect": (_I, [_P, _P]),
+    "ugx_disc_adjoint_gradient": (_I, [_P, _P, _P, _P]),
     "ugx_disc_set_source_const": (_I, [_P, _D]),
     "ugx_disc_set_source_dofs": (_I, [_P, _P]),
     "ugx_disc_assemble_linear": (_I, [_P, _P, _P]),
@@ -68,6 +72,8 @@ _SIGNATURES = {
     "ugx_matrix_device_csr": (_I, [_P, _P, _P, _P]),
     "ugx_matrix_axpy": (_I, [_P, _P, _D, _P, _D, _P]),
     "ugx_matrix_set_spmv_lanes": (_I, [_P, _I]),
+    "ugx_matrix_set_spmv_layout": (_I, [_P, _I]),
+    "ugx_matrix_finalize": (_I, [_P]),
     "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
     "ugx_vector_create": (_P, [_I64]),
     "ugx_vector_destroy": (None, [_P]),
@@ -276,6 +282,13 @@ class SparseMatrix:
     def apply_sub(self, d, x):
         _check(lib().ugx_matrix_apply_sub(self.h, d.h, x.h), "SparseMatrix.apply_sub")
 
+    def set_spmv_layout(self, layout):
+        """'sell' (sliced ELL, default) or 'csr'"""
+        _check(lib().ugx_matrix_set_spmv_layout(self.h, 1 if layout == "sell" else 0), "set_spmv_layout")
+
+    def finalize(self):
+        _check(lib().ugx_matrix_finalize(self.h), "SparseMatrix.finalize")
+
     def set_spmv_lanes(self, lanes):
         _check(lib().ugx_matrix_set_spmv_lanes(self.h, int(lanes)), "SparseMatrix.set_spmv_lanes")
 
@@ -327,6 +340,23 @@ class DomainDiscretization:
 
     def assemble_seconds(self):
         return lib().ugx_disc_assemble_seconds(self.h)
+
+    def set_theta_device(self, ptr):
+        _check(lib().ugx_disc_set_theta_device(self.h, ptr), "set_theta_device")
+
+    def adjust_defect(self, v):
+        _check(lib().ugx_disc_adjust_defect(self.h, v.h), "adjust_defect")
+
+    def adjoint_gradient(self, u, lam, grad_ptr=None):
+        """g_e = -lambda_e^T (dA_e/dtheta_e) u_e per fine element; returns a
+        numpy array unless a device pointer for the result is given."""
+        ne = (2 ** self.num_refs) ** 3
+        if grad_ptr is not None:
+            _check(lib().ugx_disc_adjoint_gradient(self.h, u.h, lam.h, grad_ptr), "adjoint_gradient")
+            return None
+        buf = DeviceArray(np.zeros(ne))
+        _check(lib().ugx_disc_adjoint_gradient(self.h, u.h, lam.h, buf.ptr), "adjoint_gradient")
+        return buf.numpy()
 
 
 class GeometricMultiGrid:

@@ -128,6 +128,15 @@ int ugx_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
         return 0;
     });
 }
+int ugx_memcpy_dtod(void* dst, const void* src, int64_t bytes) {
+    return guard([&] {
+        auto& c = context();
+        if (hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess)
+            throw UGError("ugx_memcpy_dtod failed");
+        return 0;
+    });
+}
 void* ugx_event_create(void) {
     return guard_ptr([] {
         context();
@@ -289,6 +298,25 @@ int ugx_disc_set_theta(void* disc, const double* theta) {
         return 0;
     });
 }
+int ugx_disc_set_theta_device(void* disc, const double* theta) {
+    return guard([&] {
+        ((DiscH*)disc)->d->set_theta_device(theta);
+        return 0;
+    });
+}
+int ugx_disc_adjust_defect(void* disc, void* vec) {
+    return guard([&] {
+        ((DiscH*)disc)->d->adjust_defect(((VectorH*)vec)->v);
+        return 0;
+    });
+}
+int ugx_disc_adjoint_gradient(void* disc, void* u, void* lambda, double* grad) {
+    return guard([&] {
+        if (!grad) throw UGError("ugx_disc_adjoint_gradient: null output");
+        ((DiscH*)disc)->d->adjoint_gradient(((VectorH*)u)->v, ((VectorH*)lambda)->v, grad);
+        return 0;
+    });
+}
 int ugx_disc_set_source_const(void* disc, double f) {
     ((DiscH*)disc)->d->set_source_constant(f);
     return 0;
@@ -344,6 +372,20 @@ int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, vo
 int ugx_matrix_set_spmv_lanes(void* m, int lanes) {
     return guard([&] {
         ((MatrixH*)m)->A->set_lanes(lanes);
+        return 0;
+    });
+}
+int ugx_matrix_set_spmv_layout(void* m, int sliced_ell) {
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        A.set_use_sliced_ell(sliced_ell != 0);
+        if (sliced_ell) A.finalize();
+        return 0;
+    });
+}
+int ugx_matrix_finalize(void* m) {
+    return guard([&] {
+        ((MatrixH*)m)->A->finalize();
         return 0;
     });
 }

@@ -94,6 +94,7 @@ template class DeviceBuffer<double>;
 template class DeviceBuffer<float>;
 template class DeviceBuffer<int32_t>;
 template class DeviceBuffer<uint8_t>;
+template class DeviceBuffer<int64_t>;
 
 // ---------------------------------------------------------------------------
 // GPUVector
@@ -166,7 +167,29 @@ static int lanes_for(int64_t rows, int64_t nnz) {
     return 32;
 }
 
+void GPUSparseMatrix::finalize() {
+    sell_valid_ = false;
+    if (!sell_enabled_ || rows_ == 0) return;
+    auto& c = context();
+    const int64_t nsl = (rows_ + 63) / 64;
+    sell_width_.resize(nsl);
+    UGX_LAUNCH(sell_widths(rows_, rowptr_.get(), sell_width_.get(), c.stream));
+    std::vector<int32_t> w(nsl);
+    sell_width_.download(w.data(), nsl);
+    std::vector<int64_t> ptr(nsl + 1, 0);
+    for (int64_t t = 0; t < nsl; ++t) ptr[t + 1] = ptr[t] + (int64_t)64 * w[t];
+    sell_ptr_.resize(nsl + 1);
+    sell_ptr_.upload(ptr.data(), nsl + 1);
+    sell_col_.resize(ptr[nsl]);
+    sell_val_.resize(ptr[nsl]);
+    UGX_LAUNCH(sell_fill<double>(rows_, cols_, rowptr_.get(), col_.get(), val_.get(), sell_ptr_.get(),
+                                 sell_width_.get(), sell_col_.get(), sell_val_.get(), c.stream));
+    sync();
+    sell_valid_ = true;
+}
+
 void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
+    sell_valid_ = false;
     if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix: nnz exceeds int32 range");
     rows_ = rows;
     cols_ = cols;
@@ -183,6 +206,7 @@ void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_
     rowptr_.upload(rowptr, rows + 1);
     col_.upload(col, nnz_);
     val_.upload(val, nnz_);
+    finalize();
 }
 
 void GPUSparseMatrix::download_csr(int32_t* rowptr, int32_t* col, double* val) const {
@@ -194,13 +218,23 @@ void GPUSparseMatrix::download_csr(int32_t* rowptr, int32_t* col, double* val) c
 void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, double b1, const GPUVector& w1) const {
     if (dest.size() != rows_ || w1.size() != cols_) throw UGError("GPUSparseMatrix::axpy size mismatch");
     if (&dest == &w1) throw UGError("GPUSparseMatrix::axpy: dest must not alias the multiplied vector");
-    UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), a1, v1.data(), b1, w1.data(),
-                                         dest.data(), 0, lanes_, context().stream)));
+    if (sell_valid_)
+        UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
+                                              sell_val_.get(), a1, v1.data(), b1, w1.data(), dest.data(), 0,
+                                              rowptr_.get(), context().stream)));
+    else
+        UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), a1, v1.data(), b1,
+                                             w1.data(), dest.data(), 0, lanes_, context().stream)));
 }
 
 void GPUSparseMatrix::apply_ignore_zero_rows(GPUVector& dest, double beta, const GPUVector& w) const {
-    UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), 0.0, dest.data(), beta,
-                                         w.data(), dest.data(), 1, lanes_, context().stream)));
+    if (sell_valid_)
+        UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
+                                              sell_val_.get(), 0.0, dest.data(), beta, w.data(), dest.data(), 1,
+                                              rowptr_.get(), context().stream)));
+    else
+        UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), 0.0, dest.data(), beta,
+                                             w.data(), dest.data(), 1, lanes_, context().stream)));
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +458,29 @@ void DomainDiscretization::set_theta(const double* theta_host) {
     has_theta_ = true;
 }
 
+void DomainDiscretization::set_theta_device(const double* theta_dev) {
+    const int64_t nf = (int64_t)1 << refs_;
+    const int64_t n = kind_ == 0 ? nf * nf * nf : nf * nf;
+    theta_.resize(n);
+    UGX_HIP(hipMemcpyAsync(theta_.get(), theta_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, context().stream));
+    has_theta_ = true;
+}
+
+void DomainDiscretization::adjust_defect(GPUVector& d) {
+    LevelGrid& L = *levels_.back();
+    if (d.size() != L.N) throw UGError("adjust_defect: size mismatch");
+    UGX_LAUNCH(zero_range(L.n_inner, L.N, d.data(), context().stream));
+}
+
+void DomainDiscretization::adjoint_gradient(const GPUVector& u, const GPUVector& lambda, double* grad_dev) {
+    if (kind_ != 0) throw UGError("adjoint_gradient: implemented for the Q1 hexahedral discretisation");
+    LevelGrid& L = *levels_.back();
+    if (u.size() != L.N || lambda.size() != L.N) throw UGError("adjoint_gradient: size mismatch");
+    if (!has_theta_) throw UGError("adjoint_gradient: theta not set");
+    UGX_LAUNCH(adjoint_grad_q1_hex(L.n, 1.0 / L.n, theta_.get(), L.dof_of_lat.get(), L.n_inner, u.data(),
+                                   lambda.data(), grad_dev, context().stream));
+}
+
 void DomainDiscretization::set_source_dofs(const double* f_host) {
     const int64_t n = levels_.back()->N;
     f_dof_.resize(n);
@@ -463,6 +520,7 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     }
     UGX_LAUNCH(dirichlet_csr<double>(L.N, L.n_inner, A.rowptr(), A.col(), A.val(), b.data(), c.stream));
     sync();
+    A.finalize();
     t_assemble_ = wall() - t0;
 }
 
@@ -477,7 +535,7 @@ void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia
         const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = L.dim == 3 ? d / 9 - 1 : 0;
         offsets[d] = (int32_t)(dx + dy * W + dz * W * W);
     }
-    dia.resize((int64_t)nd * L.N);
+    dia.resize(dia_tiled_size(L.N, nd));
     dia.zero();
     const double hsz = 1.0 / L.n;
     const int nfine = 1 << refs_;
@@ -579,10 +637,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             // Jacobi: diagInv = 1/(a_ii * (1/omega)) from the FP64 centre diagonal
             DeviceBuffer<double> dinv64(l.N);
             UGX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
-            UGX_LAUNCH((dia_jacobi_diag<double, double>(l.N, A64.get() + (int64_t)(l.nd / 2) * l.N, omega,
-                                                        dinv64.get(), c.flag, c.stream)));
+            UGX_LAUNCH((dia_jacobi_diag<double, double>(l.N, l.nd, A64.get(), omega, dinv64.get(), c.flag, c.stream)));
             if (lev == base) {
-                std::vector<double> h((size_t)l.nd * l.N);
+                std::vector<double> h((size_t)dia_tiled_size(l.N, l.nd));
                 A64.download(h.data(), (int64_t)h.size());
                 lu_n = (int)l.N;
                 if (lu_n > 4096) throw UGError("GMG: base level too large for the dense LU");
@@ -590,7 +647,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                 for (int64_t i = 0; i < l.N; ++i)
                     for (int d = 0; d < l.nd; ++d) {
                         const int64_t j = i + l.off[d];
-                        const double v = h[(size_t)d * l.N + i];
+                        const double v = h[(size_t)dia_tiled_index(d, i, l.nd)];
                         if (j < 0 || j >= l.N) continue;
                         dense[(size_t)i * lu_n + j] += v;
                     }
@@ -636,8 +693,8 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                 l.A = std::move(A64);
                 UGX_HIP(hipMemcpyAsync(l.dinv, dinv64.get(), sizeof(double) * l.N, hipMemcpyDeviceToDevice, c.stream));
             } else {
-                l.A.resize((int64_t)l.nd * l.N);
-                UGX_LAUNCH((convert<double, T>((int64_t)l.nd * l.N, A64.get(), l.A.get(), c.stream)));
+                l.A.resize(A64.size());
+                UGX_LAUNCH((convert<double, T>(A64.size(), A64.get(), l.A.get(), c.stream)));
                 UGX_LAUNCH((convert<double, T>(l.N, dinv64.get(), l.dinv, c.stream)));
             }
             sync();
@@ -711,11 +768,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     }
 
     void download_level_dia(int lev, double* out) const override {
+        // un-tiled: out[d*N + i]
         const GmgLevel<T>& l = L[lev];
-        const int64_t n = (int64_t)l.nd * l.N;
-        std::vector<T> h(n);
-        l.A.download(h.data(), n);
-        for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i];
+        std::vector<T> h(l.A.size());
+        l.A.download(h.data(), (int64_t)h.size());
+        for (int d = 0; d < l.nd; ++d)
+            for (int64_t i = 0; i < l.N; ++i) out[(int64_t)d * l.N + i] = (double)h[dia_tiled_index(d, i, l.nd)];
     }
     int num_diags(int lev) const override { return L[lev].nd; }
     void smooth(int lev, int nsteps) override {

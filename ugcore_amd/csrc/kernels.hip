@@ -38,6 +38,14 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
     return lb;   // may exceed nblocks-1 for the last XCD: callers bound-check rows
 }
 
+// Level operators are stored as ND diagonals in wavefront tiles: the 64
+// values of diagonal d for rows 64t..64t+63 are contiguous, and the ND
+// diagonals of one tile follow each other, so a wavefront streams one
+// contiguous ND*64*sizeof(T) block.
+__host__ __device__ __forceinline__ int64_t dia_index(int d, int64_t i, int nd) {
+    return ((i >> 6) * nd + d) * 64 + (i & 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum_group(T v, int G) {
     for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -108,6 +116,95 @@ static int launch_spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* 
         default: hipLaunchKernelGGL((k_spmv_csr<TM, TV, 64>), dim3(grid), dim3(256), 0, s, nrows, rowptr, col, val, alpha, v1, beta, w, dest, mode, nblocks); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Sliced ELL (64-row slices): the SpMV layout of GPUSparseMatrix. Slice t
+// holds rows 64t..64t+63; entry k of its rows is stored at
+// slice_ptr[t] + 64*k + lane (columns int32, values TM). Rows shorter than the
+// slice width are padded with value 0 and a valid column, so every lane runs
+// the same trip count and every load is one coalesced 64-lane access.
+// ---------------------------------------------------------------------------
+__global__ void k_sell_widths(int64_t nrows, const int32_t* __restrict__ rowptr, int32_t* __restrict__ width) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nslices = (nrows + 63) / 64;
+    if (t >= nslices) return;
+    int32_t w = 0;
+    const int64_t r1 = (t + 1) * 64 < nrows ? (t + 1) * 64 : nrows;
+    for (int64_t r = t * 64; r < r1; ++r) {
+        const int32_t len = rowptr[r + 1] - rowptr[r];
+        w = len > w ? len : w;
+    }
+    width[t] = w;
+}
+
+template <typename TM>
+__global__ void k_sell_fill(int64_t nrows, int64_t ncols, const int32_t* __restrict__ rowptr,
+                            const int32_t* __restrict__ col, const TM* __restrict__ val,
+                            const int64_t* __restrict__ slice_ptr, const int32_t* __restrict__ width,
+                            int32_t* __restrict__ scol, TM* __restrict__ sval) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t t = r >> 6;
+    if (t >= (nrows + 63) / 64) return;
+    const int lane = (int)(r & 63);
+    const int32_t w = width[t];
+    const int64_t base = slice_ptr[t] + lane;
+    const int32_t rb = r < nrows ? rowptr[r] : 0;
+    const int32_t len = r < nrows ? rowptr[r + 1] - rb : 0;
+    const int32_t pad_col = (int32_t)(r < ncols ? r : 0);
+    for (int32_t k = 0; k < w; ++k) {
+        const bool real = k < len;
+        scol[base + (int64_t)64 * k] = real ? col[rb + k] : pad_col;
+        sval[base + (int64_t)64 * k] = real ? val[rb + k] : (TM)0;
+    }
+}
+
+// dest = alpha*v1 + beta*A*w (mode 0) or dest = beta*A*w on rows with entries
+// (mode 1, needs rowptr)
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_sell_spmv(int64_t nrows, const int64_t* __restrict__ slice_ptr,
+                                                   const int32_t* __restrict__ width, const int32_t* __restrict__ scol,
+                                                   const TM* __restrict__ sval, double alpha, const TV* v1, double beta,
+                                                   const TV* __restrict__ w, TV* dest, int mode,
+                                                   const int32_t* __restrict__ rowptr, int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    if (lb >= nblocks) return;
+    const int64_t r = lb * 256 + threadIdx.x;
+    const int64_t t = r >> 6;
+    if (t >= (nrows + 63) / 64) return;
+    const int lane = (int)(r & 63);
+    const int32_t wd = width[t];
+    const int32_t* cp = scol + slice_ptr[t] + lane;
+    const TM* vp = sval + slice_ptr[t] + lane;
+    double s0 = 0.0, s1 = 0.0;
+    int32_t k = 0;
+    for (; k + 4 <= wd; k += 4) {
+        const int32_t c0 = __builtin_nontemporal_load(cp + 64 * k);
+        const int32_t c1 = __builtin_nontemporal_load(cp + 64 * (k + 1));
+        const int32_t c2 = __builtin_nontemporal_load(cp + 64 * (k + 2));
+        const int32_t c3 = __builtin_nontemporal_load(cp + 64 * (k + 3));
+        const double a0 = (double)__builtin_nontemporal_load(vp + 64 * k);
+        const double a1 = (double)__builtin_nontemporal_load(vp + 64 * (k + 1));
+        const double a2 = (double)__builtin_nontemporal_load(vp + 64 * (k + 2));
+        const double a3 = (double)__builtin_nontemporal_load(vp + 64 * (k + 3));
+        s0 = fma(a0, (double)w[c0], s0);
+        s1 = fma(a1, (double)w[c1], s1);
+        s0 = fma(a2, (double)w[c2], s0);
+        s1 = fma(a3, (double)w[c3], s1);
+    }
+    for (; k < wd; ++k) {
+        const int32_t c0 = __builtin_nontemporal_load(cp + 64 * k);
+        const double a0 = (double)__builtin_nontemporal_load(vp + 64 * k);
+        s0 = fma(a0, (double)w[c0], s0);
+    }
+    if (r >= nrows) return;
+    const double sum = s0 + s1;
+    if (mode == 1) {
+        if (rowptr[r + 1] > rowptr[r]) dest[r] = (TV)(beta * sum);
+    } else {
+        const double base = (alpha == 0.0) ? 0.0 : alpha * (double)v1[r];
+        dest[r] = (TV)(base + beta * sum);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -232,7 +329,7 @@ __global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, DiaOffsets o, 
     if (lb >= nblocks || i >= n) return;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -252,7 +349,7 @@ __global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, DiaOffsets o, const
     if (lb >= nblocks || i >= n) return;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -276,7 +373,7 @@ __global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, DiaOffsets o
     if (lb >= nblocks || i >= n) return;
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[(int64_t)d * n + i];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int64_t j = i + o.off[d];
@@ -302,9 +399,9 @@ __global__ void k_jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV*
 
 // dinv from the DIA centre diagonal
 template <typename TM, typename TV>
-__global__ void k_dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad) {
+__global__ void k_dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double m = (double)diag[i];
+        double m = (double)dia[dia_index(nd / 2, i, nd)];
         m *= 1.0 / omega;
         if (m == 0.0) { bad[0] = 1; dinv[i] = (TV)0; }
         else dinv[i] = (TV)(1.0 / m);
@@ -621,12 +718,56 @@ __global__ __launch_bounds__(128) void k_assemble_q1_hex(int n, int color, doubl
                 int bx, by, bz;
                 hex_corner(b, bx, by, bz);
                 const int d = ((bz - az + 1) * 3 + (by - ay + 1)) * 3 + (bx - ax + 1);
-                TM* p = dia + (int64_t)d * dia_n + lat[a];
+                TM* p = dia + dia_index(d, lat[a], 27);
                 *p = (TM)((double)*p + Ke[a][b]);
             }
             if (rhs) rhs[lat[a]] += fe[a];
         }
     }
+}
+
+// Adjoint element gradient on the finest level (one element per thread, no
+// scatter): g_e = -sum_{a interior} lambda_a (A_e u_e)_a. On the finest level
+// k = exp(theta_e) at every quadrature point of e, so dA_e/dtheta_e = A_e; A_e
+// is recomputed here rather than stored. Dirichlet rows (DoF >= n_inner) do not
+// depend on theta.
+__global__ __launch_bounds__(128) void k_adjoint_grad_q1_hex(int n, double h, const double* __restrict__ theta,
+                                                             const int32_t* __restrict__ dof_of_lat, int64_t n_inner,
+                                                             const double* __restrict__ u,
+                                                             const double* __restrict__ lambda,
+                                                             double* __restrict__ grad) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n * n) return;
+    const int ex = (int)(e % n), ey = (int)((e / n) % n), ez = (int)(e / ((int64_t)n * n));
+    const int W = n + 1;
+    double X[8][3];
+    int32_t dof[8];
+    for (int a = 0; a < 8; ++a) {
+        int dx, dy, dz;
+        hex_corner(a, dx, dy, dz);
+        X[a][0] = (ex + dx) * h;
+        X[a][1] = (ey + dy) * h;
+        X[a][2] = (ez + dz) * h;
+        dof[a] = dof_of_lat[((int64_t)(ez + dz) * W + (ey + dy)) * W + (ex + dx)];
+    }
+    double Ke[8][8], fe[8];
+    q1_hex_element(X, theta, n, 0.0, Ke, fe);
+    double ue[8];
+    for (int b = 0; b < 8; ++b) ue[b] = u[dof[b]];
+    double acc = 0.0;
+    for (int a = 0; a < 8; ++a) {
+        if (dof[a] >= n_inner) continue;
+        double row = 0.0;
+        for (int b = 0; b < 8; ++b) row += Ke[a][b] * ue[b];
+        acc += lambda[dof[a]] * row;
+    }
+    grad[e] = -acc;
+}
+
+// zero the Dirichlet entries (DoF >= n_inner) of a top-level vector
+__global__ void k_zero_range(int64_t lo, int64_t hi, double* x) {
+    for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
 }
 
 // FV1 on P1 triangles (box method), lattice squares split along (1,1).
@@ -710,7 +851,7 @@ __global__ __launch_bounds__(128) void k_assemble_fv1_tri(int n, int color, doub
             for (int b = 0; b < 3; ++b) {
                 const int dx = (int)((lat[b] % W) - (lat[a] % W)), dy = (int)((lat[b] / W) - (lat[a] / W));
                 const int d = (dy + 1) * 3 + (dx + 1);
-                TM* p = dia + (int64_t)d * dia_n + lat[a];
+                TM* p = dia + dia_index(d, lat[a], 9);
                 *p = (TM)((double)*p + Ke[a][b]);
             }
             if (rhs && f_at_dof) rhs[lat[a]] += f_at_dof[dof_of_lat[lat[a]]] * (area / 3.0);
@@ -734,7 +875,7 @@ template <typename TM>
 __global__ void k_dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (!dir[i]) continue;
-        for (int d = 0; d < nd; ++d) dia[(int64_t)d * n + i] = (TM)(d == centre ? 1.0 : 0.0);
+        for (int d = 0; d < nd; ++d) dia[dia_index(d, i, nd)] = (TM)(d == centre ? 1.0 : 0.0);
     }
 }
 
@@ -900,13 +1041,12 @@ template int jacobi_accumulate<double>(int64_t, const double*, const double*, do
 template int jacobi_accumulate<float>(int64_t, const float*, const float*, float*, hipStream_t);
 
 template <typename TM, typename TV>
-int dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad, hipStream_t s) {
+int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_dia_jacobi_diag<TM, TV>), dim3(vec_grid(n)), dim3(256), 0, s, n, diag, omega, dinv, bad);
+    hipLaunchKernelGGL((k_dia_jacobi_diag<TM, TV>), dim3(vec_grid(n)), dim3(256), 0, s, n, nd, dia, omega, dinv, bad);
     UGX_CHECK_LAUNCH();
 }
-template int dia_jacobi_diag<double, double>(int64_t, const double*, double, double*, int*, hipStream_t);
-template int dia_jacobi_diag<float, float>(int64_t, const float*, double, float*, int*, hipStream_t);
+template int dia_jacobi_diag<double, double>(int64_t, int, const double*, double, double*, int*, hipStream_t);
 
 template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
@@ -1003,6 +1143,48 @@ template int convert<double, float>(int64_t, const double*, float*, hipStream_t)
 
 int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s) {
     hipLaunchKernelGGL(k_boundary_flags, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, dir);
+    UGX_CHECK_LAUNCH();
+}
+
+int sell_widths(int64_t nrows, const int32_t* rowptr, int32_t* width, hipStream_t s) {
+    const int64_t nsl = (nrows + 63) / 64;
+    if (nsl <= 0) return 0;
+    hipLaunchKernelGGL(k_sell_widths, dim3((nsl + 255) / 256), dim3(256), 0, s, nrows, rowptr, width);
+    UGX_CHECK_LAUNCH();
+}
+
+template <typename TM>
+int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
+              const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval, hipStream_t s) {
+    const int64_t nth = ((nrows + 63) / 64) * 64;
+    if (nth <= 0) return 0;
+    hipLaunchKernelGGL(k_sell_fill<TM>, dim3((nth + 255) / 256), dim3(256), 0, s, nrows, ncols, rowptr, col, val, slice_ptr, width, scol, sval);
+    UGX_CHECK_LAUNCH();
+}
+template int sell_fill<double>(int64_t, int64_t, const int32_t*, const int32_t*, const double*, const int64_t*, const int32_t*, int32_t*, double*, hipStream_t);
+
+template <typename TM, typename TV>
+int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,
+              double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode, const int32_t* rowptr,
+              hipStream_t s) {
+    if (nrows <= 0) return 0;
+    const int64_t nb = (((nrows + 63) / 64) * 64 + 255) / 256;
+    hipLaunchKernelGGL((k_sell_spmv<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, nrows, slice_ptr, width, scol, sval, alpha, v1, beta, w, dest, mode, rowptr, nb);
+    UGX_CHECK_LAUNCH();
+}
+template int sell_spmv<double, double>(int64_t, const int64_t*, const int32_t*, const int32_t*, const double*, double, const double*, double, const double*, double*, int, const int32_t*, hipStream_t);
+
+int adjoint_grad_q1_hex(int n, double h, const double* theta, const int32_t* dof_of_lat, int64_t n_inner,
+                        const double* u, const double* lambda, double* grad, hipStream_t s) {
+    const int64_t ne = (int64_t)n * n * n;
+    if (ne <= 0) return 0;
+    hipLaunchKernelGGL(k_adjoint_grad_q1_hex, dim3((ne + 127) / 128), dim3(128), 0, s, n, h, theta, dof_of_lat, n_inner, u, lambda, grad);
+    UGX_CHECK_LAUNCH();
+}
+
+int zero_range(int64_t lo, int64_t hi, double* x, hipStream_t s) {
+    if (hi <= lo) return 0;
+    hipLaunchKernelGGL(k_zero_range, dim3(vec_grid(hi - lo)), dim3(256), 0, s, lo, hi, x);
     UGX_CHECK_LAUNCH();
 }
 

@@ -11,6 +11,16 @@ template <typename TM, typename TV>
 int spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* col, const TM* val, double alpha, const TV* v1,
              double beta, const TV* w, TV* dest, int mode, int lanes, hipStream_t s);
 
+// sliced ELL (64-row slices) SpMV layout
+int sell_widths(int64_t nrows, const int32_t* rowptr, int32_t* width, hipStream_t s);
+template <typename TM>
+int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
+              const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval, hipStream_t s);
+template <typename TM, typename TV>
+int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,
+              double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode, const int32_t* rowptr,
+              hipStream_t s);
+
 template <typename T>
 int vec_set(int64_t n, T* x, double v, hipStream_t s);
 template <typename T>
@@ -40,7 +50,11 @@ int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
 template <typename TM, typename TV>
-int dia_jacobi_diag(int64_t n, const TM* diag, double omega, TV* dinv, int* bad, hipStream_t s);
+int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s);
+
+// tiled diagonal storage of level operators (see kernels.hip)
+inline int64_t dia_tiled_index(int d, int64_t i, int nd) { return ((i >> 6) * nd + d) * 64 + (i & 63); }
+inline int64_t dia_tiled_size(int64_t n, int nd) { return ((n + 63) / 64) * 64 * nd; }
 template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
                 hipStream_t s);
@@ -71,5 +85,8 @@ int dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia, hi
 template <typename TA, typename TB>
 int convert(int64_t n, const TA* a, TB* b, hipStream_t s);
 int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s);
+int adjoint_grad_q1_hex(int n, double h, const double* theta, const int32_t* dof_of_lat, int64_t n_inner,
+                        const double* u, const double* lambda, double* grad, hipStream_t s);
+int zero_range(int64_t lo, int64_t hi, double* x, hipStream_t s);
 
 }  // namespace ugamd
