@@ -194,8 +194,23 @@ def main():
 
     g32 = solvers["f32"][0]
     nd = L.ugx_gmg_level_num_diags(g32.h, refs)
+    nu_stored = nd // 2 + 1              # symmetric storage: centre + upper diagonals
     t_smooth = U.time_launches(lambda: g32.smooth_level(refs, 1), args.kernel_reps)
-    smooth_bytes = N * 4 * (nd + 5)      # 27 value streams + sd in/out + dinv + sc read/write
+    smooth_bytes = N * 4 * (nu_stored + 5)   # stored diagonals + sd in/out + dinv + sc read/write
+
+    # the operator the CG iterates with: surface operator in lattice order,
+    # FP64 symmetric storage
+    H = L.ugx_gmg_lattice_halo(g32.h)
+    xl = U.DeviceArray(np.concatenate([np.zeros(H), np.random.default_rng(1).normal(size=N), np.zeros(H)]))
+    yl = U.DeviceArray(np.zeros(N))
+    xl_ptr = xl.ptr + 8 * H
+
+    def lattice_spmv():
+        assert L.ugx_gmg_surface_apply_lattice(g32.h, yl.ptr, xl_ptr) == 0
+
+    t_lat = U.time_launches(lattice_spmv, args.kernel_reps)
+    lat_bytes = N * (8 * nu_stored + 8 + 8)
+    log(f"[bench] lattice surface SpMV f64 (symmetric DIA): {t_lat*1e3:.3f} ms  {lat_bytes/t_lat/1e9:.0f} GB/s")
     smooth_gbs = smooth_bytes / t_smooth / 1e9
     log(f"[bench] SpMV f64 ({layout}): {t_spmv*1e3:.3f} ms  {spmv_gbs:.0f} GB/s; "
         f"DIA Jacobi step f32: {t_smooth*1e3:.3f} ms  {smooth_gbs:.0f} GB/s")
@@ -208,7 +223,7 @@ def main():
     spmv_launches = iters["f32"] + 1
     dominant = "smoother" if smooth_launches * t_smooth >= spmv_launches * t_spmv else "spmv"
     if dominant == "smoother":
-        roof = {"kernel": "k_dia_jacobi_step<float> (top level)", "bound": "hbm",
+        roof = {"kernel": "k_dias_jacobi_step<float,float,27> (top level)", "bound": "hbm",
                 "achieved": smooth_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": smooth_gbs / HBM_PEAK_GBS, "traffic": None,
                 "algorithmic_bytes_per_launch": smooth_bytes, "launch_ms": t_smooth * 1e3}
@@ -263,6 +278,10 @@ def main():
             "spmv_csr_lanes_sweep_ms": {str(k): v * 1e3 for k, v in sweep.items()},
             "spmv_f64_GBps": spmv_gbs,
             "spmv_f64_frac_of_8TBps": spmv_gbs / HBM_PEAK_GBS,
+            "cg_lattice_path": {p: bool(solvers[p][1].used_lattice_path()) for p in solvers},
+            "lattice_spmv_f64_sym_dia_ms": t_lat * 1e3,
+            "lattice_spmv_f64_sym_dia_GBps": lat_bytes / t_lat / 1e9,
+            "lattice_spmv_algorithmic_bytes": lat_bytes,
             "smoother_dia_f32_ms": t_smooth * 1e3,
             "smoother_dia_f32_GBps": smooth_gbs,
         },

@@ -131,6 +131,11 @@ class GPUSparseMatrix {
     // assembly and uploads call it); until then SpMV runs on the CSR directly
     void finalize();
     bool uses_sliced_ell() const { return sell_valid_; }
+    // provenance: the discretisation that assembled the values and a counter
+    // bumped on every assembly/upload (used to validate derived operators)
+    const void* origin() const { return origin_; }
+    uint64_t version() const { return version_; }
+    void set_origin(const void* o) { origin_ = o; ++version_; }
     void set_use_sliced_ell(bool on) { sell_enabled_ = on; if (!on) sell_valid_ = false; }
     int lanes() const { return lanes_; }
     void set_lanes(int g) {
@@ -144,6 +149,8 @@ class GPUSparseMatrix {
     DeviceBuffer<int32_t> rowptr_, col_;
     DeviceBuffer<double> val_;
     bool sell_enabled_ = true, sell_valid_ = false;
+    const void* origin_ = nullptr;
+    uint64_t version_ = 0;
     DeviceBuffer<int32_t> sell_width_, sell_col_;
     DeviceBuffer<int64_t> sell_ptr_;
     DeviceBuffer<double> sell_val_;
@@ -239,6 +246,8 @@ class Jacobi : public ILinearIterator {
     DeviceBuffer<double> dinv_;
 };
 
+class AssembledMultiGridCycle;
+
 class ILinearOperatorInverse {
    public:
     virtual ~ILinearOperatorInverse() = default;
@@ -261,6 +270,14 @@ class CG : public ILinearOperatorInverse {
    public:
     const char* name() const override { return "CG"; }
     bool apply_return_defect(GPUVector& x, GPUVector& b) override;
+    // when on (default) and the preconditioner is a GMG initialised with this
+    // operator, the iteration runs on lattice-ordered copies of the vectors
+    void set_lattice_fast_path(bool on) { fast_ = on; }
+    bool last_used_lattice_path() const { return used_fast_; }
+
+   private:
+    bool apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg);
+    bool fast_ = true, used_fast_ = false;
 };
 
 class BiCGStab : public ILinearOperatorInverse {
@@ -346,6 +363,21 @@ class AssembledMultiGridCycle : public ILinearIterator {
     int level_num_diags(int lev) const;
     // nsteps fused smoothing steps on level `lev` (timing / inspection)
     void smooth_level(int lev, int nsteps);
+    // Lattice-order interface used by the CG fast path. Valid while A is the
+    // matrix this cycle was initialised with (same assembly). Lattice vectors
+    // are FP64 buffers of N entries with `lattice_halo()` zero entries on
+    // each side (allocated by the caller).
+    bool lattice_ready(const GPUSparseMatrix& A) const;
+    int64_t lattice_size() const;
+    int64_t lattice_halo() const;
+    const int32_t* dof_of_lattice() const;
+    const int32_t* lattice_of_dof() const;
+    int64_t num_inner_dofs() const;
+    // c = B d for a defect d in lattice order that vanishes on Dirichlet DoFs
+    void apply_lattice(double* c_lat, const double* d_lat);
+    // y = alpha*v + beta*A*x with the surface operator in lattice order
+    // (x must vanish on Dirichlet DoFs and carry the halo)
+    void surface_axpy_lattice(double* y, double alpha, const double* v, double beta, const double* x);
     struct Impl;
 
    private:
@@ -354,6 +386,8 @@ class AssembledMultiGridCycle : public ILinearIterator {
     double omega_ = 0.66;
     Precision prec_ = Precision::F64;
     Impl* impl_ = nullptr;
+    const GPUSparseMatrix* mat_ = nullptr;
+    uint64_t mat_version_ = 0;
 };
 
 }  // namespace gpu

@@ -155,6 +155,11 @@ int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
  * (st = dinv.*sd; sd -= A*st; sc += st), as in presmooth_and_restriction
  * (mg_solver_impl.hpp:1702-1730); used to time the smoother in isolation */
 int ugx_gmg_smooth_level(void* g, int level, int nsteps);
+/* y = A*x with the cycle's surface operator in lattice order (FP64, device
+ * pointers to (n+1)^dim entries; x needs ugx_gmg_lattice_halo zero entries on
+ * each side and must vanish on Dirichlet DoFs) */
+int ugx_gmg_surface_apply_lattice(void* g, double* y_dev, const double* x_dev);
+int64_t ugx_gmg_lattice_halo(void* g);
 int ugx_gmg_level_num_diags(void* g, int level);
 int ugx_gmg_level_dia(void* g, int level, double* out_host);  /* ndiag x (n+1)^dim values, lattice numbering */
 
@@ -172,6 +177,11 @@ int ugx_solver_init(void* s, void* matrix);
 /* returns 1 if converged, 0 if not, <0 on error */
 int ugx_solver_apply(void* s, void* x, void* b);
 int ugx_solver_steps(void* s);
+/* CG only: run on lattice-ordered copies of the vectors when the
+ * preconditioner is a GMG built on the same assembly (1 = on, default) */
+int ugx_solver_set_lattice_path(void* s, int on);
+/* 1 if the last CG apply ran on the lattice path */
+int ugx_solver_used_lattice_path(void* s);
 int ugx_solver_defects(void* s, double* out_host, int max_n);
 
 #ifdef __cplusplus

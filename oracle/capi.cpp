@@ -247,7 +247,7 @@ int orc_gmg_apply(void* h, int base_lev, int nu1, int nu2, int cycle, double ome
 
 // solve A x = rhs on the top level (rhs given; Dirichlet entries zeroed as
 // DirichletBoundary::adjust_defect does), GMG-CG; returns steps (<0 failure)
-int orc_solve_rhs(void* h, const double* rhs, double* x_out, int base_lev, double rel_red) {
+int orc_solve_rhs(void* h, const double* rhs, double* x_out, int base_lev, double rel_red, int zero_dirichlet) {
     auto* p = (Problem*)h;
     try {
         const int top = (int)p->mg.levels.size() - 1;
@@ -261,8 +261,9 @@ int orc_solve_rhs(void* h, const double* rhs, double* x_out, int base_lev, doubl
         const int64_t n = p->A[top].n_rows;
         auto dm = dirichlet_mask(p->mg, top);
         std::vector<double> b(rhs, rhs + n), x(n, 0.0);
-        for (int64_t i = 0; i < n; ++i)
-            if (dm[i]) b[i] = 0.0;
+        if (zero_dirichlet)
+            for (int64_t i = 0; i < n; ++i)
+                if (dm[i]) b[i] = 0.0;
         ConvCheck cc;
         cc.max_steps = 200;
         cc.rel_reduction = rel_red;

@@ -53,7 +53,7 @@ def lib():
             "orc_solution_get": (None, [P, F64P]),
             "orc_gmg_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, F64P, F64P]),
             "orc_defects_get": (C.c_int, [P, C.c_void_p, C.c_int]),
-            "orc_solve_rhs": (C.c_int, [P, F64P, F64P, C.c_int, C.c_double]),
+            "orc_solve_rhs": (C.c_int, [P, F64P, F64P, C.c_int, C.c_double, C.c_int]),
             "orc_adjoint_gradient": (None, [P, F64P, F64P, F64P]),
             "orc_timings": (None, [P, F64P]),
             "orc_builder_create": (P, [I64, I64]),
@@ -223,13 +223,14 @@ class Problem:
             raise RuntimeError(lib().orc_last_error().decode())
         return c
 
-    def solve_rhs(self, rhs, base_lev=0, rel_red=1e-12):
+    def solve_rhs(self, rhs, base_lev=0, rel_red=1e-12, zero_dirichlet=True, return_steps=False):
         top = self.num_levels - 1
         x = np.zeros(self.num_dofs(top), np.float64)
-        r = lib().orc_solve_rhs(self.h, np.ascontiguousarray(rhs, np.float64), x, base_lev, rel_red)
+        r = lib().orc_solve_rhs(self.h, np.ascontiguousarray(rhs, np.float64), x, base_lev, rel_red,
+                                1 if zero_dirichlet else 0)
         if r < 0:
             raise RuntimeError("oracle solve failed")
-        return x
+        return (x, r) if return_steps else x
 
     def adjoint_gradient(self, u, lam):
         g = np.zeros((2 ** self.num_refs) ** 3, np.float64)

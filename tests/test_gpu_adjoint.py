@@ -19,20 +19,25 @@ def say(*a):
     sys.stdout.flush()
 
 
+# theta, u and the gradient live in host tensors; the library moves them to
+# and from the device itself
+DEV = "cpu"
+
+
 def setup(refs, seed=7, seed_true=11, n_obs=64, noise=1e-4):
     from ugcore_amd import adjoint as ADJ
     import torch
     n = 2 ** refs
-    theta = torch.tensor(O.make_theta(n, seed, 3), dtype=torch.float64, device="cuda")
+    theta = torch.tensor(O.make_theta(n, seed, 3), dtype=torch.float64, device=DEV)
     prob = ADJ.DiffusionProblem(refs, reduction=1e-14, max_steps=200)
     obs_lat = ADJ.observation_points(n, n_obs, 5)
     dol = prob.disc.dof_of_lattice(refs)
-    obs = torch.tensor(dol[obs_lat], dtype=torch.int64, device="cuda")
+    obs = torch.tensor(dol[obs_lat], dtype=torch.int64, device=DEV)
     # data from a second field theta* plus N(0, noise^2)
     with torch.no_grad():
-        u_true = ADJ.solve(torch.tensor(O.make_theta(n, seed_true, 3), dtype=torch.float64, device="cuda"), prob)
+        u_true = ADJ.solve(torch.tensor(O.make_theta(n, seed_true, 3), dtype=torch.float64, device=DEV), prob)
     rng = np.random.default_rng(3)
-    d = u_true[obs] + torch.tensor(noise * rng.normal(size=n_obs), device="cuda")
+    d = u_true[obs] + torch.tensor(noise * rng.normal(size=n_obs), device=DEV)
     return ADJ, torch, prob, theta, obs, d
 
 
@@ -76,7 +81,7 @@ def test_gradient_vs_central_differences():
     worst = 0.0
     with torch.no_grad():
         for k in range(16):
-            v = torch.tensor(rng.normal(size=theta.numel()), device="cuda")
+            v = torch.tensor(rng.normal(size=theta.numel()), device=DEV)
             jp = loss(ADJ, prob, theta + h * v, obs, d).item()
             jm = loss(ADJ, prob, theta - h * v, obs, d).item()
             fd = (jp - jm) / (2 * h)
@@ -115,8 +120,8 @@ def test_gradient_without_stored_operator():
     from ugcore_amd import adjoint as ADJ
     import torch
     n = 2 ** refs
-    theta = torch.tensor(O.make_theta(n, 7, 3), device="cuda")
-    obs = torch.arange(0, 200, 7, device="cuda")
+    theta = torch.tensor(O.make_theta(n, 7, 3), device=DEV)
+    obs = torch.arange(0, 200, 7, device=DEV)
     grads = []
     for keep in (True, False):
         prob = ADJ.DiffusionProblem(refs, reduction=1e-14, keep_operator=keep)

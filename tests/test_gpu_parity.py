@@ -101,9 +101,11 @@ def test_surface_matrix_and_rhs(kind, refs):
 
 @pytest.mark.parametrize("kind,refs", [(0, 3), (1, 5)])
 def test_level_operators(kind, refs):
-    """Every GMG level operator (stored internally as diagonals in lattice
-    order) equals the oracle's level matrix (reference DoF order) entry by
-    entry; couplings outside the oracle pattern are exactly zero."""
+    """Every GMG level operator (stored internally in lattice order as the
+    symmetric, Dirichlet-eliminated diagonals) equals the oracle's level matrix
+    (reference DoF order) entry by entry, with the couplings of interior rows
+    to Dirichlet columns eliminated (level vectors vanish on Dirichlet DoFs,
+    so the products are the same); couplings outside the pattern are 0."""
     say(f"case levels kind={kind} refs={refs}")
     u = U()
     p, theta, f = oracle_problem(kind, refs)
@@ -118,12 +120,15 @@ def test_level_operators(kind, refs):
         lat = p.dof_lattice(lev) // (2 ** (refs - lev))
         li = (lat[:, 2] * W + lat[:, 1]) * W + lat[:, 0] if dim == 3 else lat[:, 1] * W + lat[:, 0]
         M = p.A(lev)
+        dm = p.dirichlet(lev)
         seen = np.zeros_like(dia, dtype=bool)
         for r in range(M.n_rows):
             cols = M.col[M.rowptr[r]:M.rowptr[r + 1]]
             vals = M.val[M.rowptr[r]:M.rowptr[r + 1]]
             scale = np.abs(vals).max()
             for c, v in zip(cols, vals):
+                if not dm[r] and dm[c]:
+                    v = 0.0
                 dl = lat[c] - lat[r]
                 if dim == 3:
                     k = (dl[2] + 1) * 9 + (dl[1] + 1) * 3 + (dl[0] + 1)
@@ -246,6 +251,8 @@ def test_gmg_krylov_solve(kind, refs, solver):
     S.init(A)
     x = u.Vector(d.num_dofs())
     assert S.apply(x, b)
+    if solver == "cg":
+        assert S.used_lattice_path()
     assert abs(S.steps() - r_orc["steps"]) <= 1
     xo = r_orc["x"]
     assert np.linalg.norm(x.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
@@ -365,3 +372,55 @@ def test_sliced_ell_and_csr_layouts_agree():
             expect = np.zeros(rows)
             O.lib().orc_csr_op(0, rows, cols, M.rowptr, M.col, M.val, alpha, v1.ctypes.data, beta, w, expect)
             assert np.allclose(dest.numpy(), expect, rtol=1e-13, atol=1e-13), layout
+
+
+@pytest.mark.parametrize("kind,refs", [(0, 4), (1, 6)])
+def test_cg_generic_and_lattice_paths_agree(kind, refs):
+    """CG on the reference-ordered vectors (generic path) and on the
+    lattice-ordered copies (fast path) give the same iterates."""
+    say(f"case cg paths kind={kind} refs={refs}")
+    u = U()
+    p, theta, f = oracle_problem(kind, refs)
+    d, A, b = gpu_problem(kind, refs, theta, f)
+    res = {}
+    for fast in (True, False):
+        g = u.GeometricMultiGrid(d, base_level=0, precision="f64")
+        S = u.CG(g, max_steps=200, reduction=1e-10)
+        S.set_lattice_path(fast)
+        S.init(A)
+        x = u.Vector(d.num_dofs())
+        assert S.apply(x, b)
+        assert S.used_lattice_path() == fast
+        res[fast] = (S.steps(), x.numpy(), S.defects())
+    assert res[True][0] == res[False][0]
+    assert np.linalg.norm(res[True][1] - res[False][1]) <= 1e-12 * np.linalg.norm(res[False][1])
+    assert np.allclose(res[True][2], res[False][2], rtol=1e-10)
+
+
+@pytest.mark.parametrize("kind,refs", [(0, 4), (1, 6)])
+def test_defect_with_dirichlet_entries(kind, refs):
+    """A right-hand side with non-zero Dirichlet entries (solution not
+    adjusted): the GMG switches to the full level storage and CG to the
+    generic path; results still equal the oracle."""
+    say(f"case dirichlet defect kind={kind} refs={refs}")
+    u = U()
+    p, theta, f = oracle_problem(kind, refs)
+    d, A, b = gpu_problem(kind, refs, theta, f)
+    rng = np.random.default_rng(21)
+    rhs = rng.normal(size=p.num_dofs(refs))
+    # one application of the preconditioner
+    g = u.GeometricMultiGrid(d, base_level=0, precision="f64")
+    g.init(A)
+    c = u.Vector(len(rhs))
+    g.apply(c, u.Vector(values=rhs))
+    c_orc = p.gmg_apply(rhs, base_lev=0)
+    assert np.linalg.norm(c.numpy() - c_orc) <= 1e-12 * np.linalg.norm(c_orc)
+    # a full solve
+    xo, steps = p.solve_rhs(rhs, base_lev=0, rel_red=1e-10, zero_dirichlet=False, return_steps=True)
+    S = u.CG(g, max_steps=200, reduction=1e-10)
+    S.init(A)
+    x = u.Vector(len(rhs))
+    assert S.apply(x, u.Vector(values=rhs))
+    assert not S.used_lattice_path()
+    assert abs(S.steps() - steps) <= 1
+    assert np.linalg.norm(x.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)

@@ -87,6 +87,8 @@ _SIGNATURES = {
     "ugx_gmg_init": (_I, [_P, _P]),
     "ugx_gmg_apply": (_I, [_P, _P, _P]),
     "ugx_gmg_smooth_level": (_I, [_P, _I, _I]),
+    "ugx_gmg_surface_apply_lattice": (_I, [_P, _P, _P]),
+    "ugx_gmg_lattice_halo": (_I64, [_P]),
     "ugx_gmg_level_num_diags": (_I, [_P, _I]),
     "ugx_gmg_level_dia": (_I, [_P, _I, _P]),
     "ugx_jacobi_create": (_P, [_D]),
@@ -99,6 +101,8 @@ _SIGNATURES = {
     "ugx_solver_apply": (_I, [_P, _P, _P]),
     "ugx_solver_steps": (_I, [_P]),
     "ugx_solver_defects": (_I, [_P, _P, _I]),
+    "ugx_solver_set_lattice_path": (_I, [_P, _I]),
+    "ugx_solver_used_lattice_path": (_I, [_P]),
 }
 
 
@@ -446,6 +450,12 @@ class _Solver:
 
 class CG(_Solver):
     _type = 0
+
+    def set_lattice_path(self, on):
+        _check(lib().ugx_solver_set_lattice_path(self.h, 1 if on else 0), "set_lattice_path")
+
+    def used_lattice_path(self):
+        return lib().ugx_solver_used_lattice_path(self.h) == 1
 
 
 class BiCGStab(_Solver):

@@ -454,6 +454,13 @@ int ugx_gmg_smooth_level(void* g, int level, int nsteps) {
         return 0;
     });
 }
+int ugx_gmg_surface_apply_lattice(void* g, double* y, const double* x) {
+    return guard([&] {
+        ((GmgH*)g)->g->surface_axpy_lattice(y, 0.0, y, 1.0, x);
+        return 0;
+    });
+}
+int64_t ugx_gmg_lattice_halo(void* g) { return ((GmgH*)g)->g->lattice_halo(); }
 int ugx_gmg_level_num_diags(void* g, int level) { return ((GmgH*)g)->g->level_num_diags(level); }
 int ugx_gmg_level_dia(void* g, int level, double* out) {
     return guard([&] {
@@ -505,6 +512,19 @@ int ugx_solver_apply(void* s, void* x, void* b) {
     return guard([&] { return ((SolverH*)s)->s->apply(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });
 }
 int ugx_solver_steps(void* s) { return ((SolverH*)s)->s->step(); }
+int ugx_solver_set_lattice_path(void* s, int on) {
+    auto* cg = dynamic_cast<CG*>(((SolverH*)s)->s.get());
+    if (!cg) {
+        g_err = "ugx_solver_set_lattice_path: CG only";
+        return -1;
+    }
+    cg->set_lattice_fast_path(on != 0);
+    return 0;
+}
+int ugx_solver_used_lattice_path(void* s) {
+    auto* cg = dynamic_cast<CG*>(((SolverH*)s)->s.get());
+    return cg && cg->last_used_lattice_path() ? 1 : 0;
+}
 int ugx_solver_defects(void* s, double* out, int max_n) {
     const auto& d = ((SolverH*)s)->s->convergence_check().defects();
     const int n = (int)d.size();

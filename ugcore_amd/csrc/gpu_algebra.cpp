@@ -203,6 +203,7 @@ void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
 void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
                                         const double* val) {
     allocate(rows, cols, rowptr[rows]);
+    set_origin(nullptr);
     rowptr_.upload(rowptr, rows + 1);
     col_.upload(col, nnz_);
     val_.upload(val, nnz_);
@@ -298,10 +299,66 @@ bool ILinearOperatorInverse::apply(GPUVector& x, const GPUVector& b) {
     return apply_return_defect(x, r);
 }
 
+static double dev_dot(int64_t n, const double* a, const double* b, bool take_sqrt) {
+    auto& c = context();
+    UGX_LAUNCH(dot<double>(n, a, b, c.red_partial, c.red_out, take_sqrt, c.stream));
+    return reduce_to_host(0);
+}
+
+bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg) {
+    // The iteration of cg.h:103-242 on lattice-ordered copies of x and r with
+    // the surface operator and the preconditioner applied in lattice order.
+    // Valid because p and z vanish on Dirichlet DoFs (r does on entry).
+    auto& c = context();
+    const int64_t N = gmg.lattice_size(), H = gmg.lattice_halo();
+    DeviceBuffer<double> bx(N + 2 * H), br(N + 2 * H), bp(N + 2 * H), bq(N + 2 * H), bz(N + 2 * H);
+    for (auto* b : {&bx, &br, &bp, &bq, &bz}) b->zero();
+    double *xl = bx.get() + H, *rl = br.get() + H, *pl = bp.get() + H, *ql = bq.get() + H, *zl = bz.get() + H;
+    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
+    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), r.data(), rl, c.stream)));
+    gmg.apply_lattice(zl, rl);
+    conv_->set_name(name());
+    conv_->start_defect(dev_dot(N, rl, rl, true));
+    UGX_HIP(hipMemcpyAsync(pl, zl, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+    double rho_old = dev_dot(N, zl, rl, false), rho;
+    bool ok = true;
+    while (!conv_->iteration_ended()) {
+        gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl);
+        const double lambda = dev_dot(N, ql, pl, false);
+        if (lambda == 0.0) { ok = false; break; }
+        const double alpha = rho_old / lambda;
+        UGX_LAUNCH(vec_scale_add<double>(N, xl, 1.0, xl, alpha, pl, 0.0, nullptr, c.stream));
+        UGX_LAUNCH(vec_scale_add<double>(N, rl, 1.0, rl, -alpha, ql, 0.0, nullptr, c.stream));
+        conv_->update_defect(dev_dot(N, rl, rl, true));
+        if (conv_->iteration_ended()) break;
+        gmg.apply_lattice(zl, rl);
+        rho = dev_dot(N, zl, rl, false);
+        const double beta = rho / rho_old;
+        UGX_LAUNCH(vec_scale_add<double>(N, pl, beta, pl, 1.0, zl, 0.0, nullptr, c.stream));
+        rho_old = rho;
+    }
+    UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), xl, x.data(), c.stream)));
+    UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, r.data(), c.stream)));
+    sync();
+    return ok && conv_->post();
+}
+
 bool CG::apply_return_defect(GPUVector& x, GPUVector& b) {
     // cg.h:103-242
     GPUVector& r = b;
     op_->apply_sub(r, x);
+    used_fast_ = false;
+    if (fast_) {
+        auto* gmg = dynamic_cast<AssembledMultiGridCycle*>(precond_.get());
+        if (gmg && gmg->lattice_ready(op_->get_matrix())) {
+            const int64_t ni = gmg->num_inner_dofs();
+            const double tail = r.size() > ni ? dev_dot(r.size() - ni, r.data() + ni, r.data() + ni, false) : 0.0;
+            if (tail == 0.0) {
+                used_fast_ = true;
+                return apply_lattice(x, r, *gmg);
+            }
+        }
+    }
     GPUVector q(r.size()), z(x.size()), p(x.size());
     if (precond_) {
         if (!precond_->apply(z, r)) return false;
@@ -520,6 +577,7 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     }
     UGX_LAUNCH(dirichlet_csr<double>(L.N, L.n_inner, A.rowptr(), A.col(), A.val(), b.data(), c.stream));
     sync();
+    A.set_origin(this);
     A.finalize();
     t_assemble_ = wall() - t0;
 }
@@ -559,6 +617,9 @@ struct AssembledMultiGridCycle::Impl {
     virtual void download_level_dia(int lev, double* out) const = 0;
     virtual int num_diags(int lev) const = 0;
     virtual void smooth(int lev, int nsteps) = 0;
+    virtual void apply_lattice(double* c_lat, const double* d_lat) = 0;
+    virtual void surface_axpy(double* y, double alpha, const double* v, double beta, const double* x) = 0;
+    virtual int64_t halo() const = 0;
 };
 
 namespace {
@@ -588,7 +649,8 @@ struct GmgLevel {
     int dim = 3, W = 0, nd = 0;
     int64_t N = 0, H = 0;
     std::vector<int32_t> off;
-    DeviceBuffer<T> A;
+    DeviceBuffer<T> A;       // symmetric Dirichlet-eliminated storage (nd/2+1 diagonals)
+    DeviceBuffer<T> Afull;   // all nd diagonals (defects with Dirichlet entries)
     DeviceBuffer<T> b_dinv, b_sc, b_sd, b_sd2, b_st;
     T *dinv = nullptr, *sc = nullptr, *sd = nullptr, *sd2 = nullptr, *st = nullptr;
     const uint8_t* dir = nullptr;
@@ -603,6 +665,22 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     DeviceBuffer<double> lu_linvT, lu_uinvT;
     DeviceBuffer<int32_t> lu_perm;
     int lu_n = 0;
+    DeviceBuffer<double> surf;   // top-level operator, FP64 symmetric storage
+    bool sym = true;             // level products through the symmetric storage
+
+    // level products: symmetric storage when every level vector vanishes on
+    // Dirichlet DoFs (the defect entering the cycle does), full otherwise
+    int op_apply_sub(GmgLevel<T>& l, const T* x, const T* d_in, T* d_out) {
+        auto& c = context();
+        return sym ? dias_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.A.get(), x, d_in, d_out, c.stream)
+                   : dia_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.Afull.get(), x, d_in, d_out, c.stream);
+    }
+    int op_jacobi_step(GmgLevel<T>& l) {
+        auto& c = context();
+        return sym ? dias_jacobi_step<T, T>(l.N, l.nd, l.off.data(), l.A.get(), l.dinv, l.sd, l.sd2, l.sc, c.stream)
+                   : dia_jacobi_step<T, T>(l.N, l.nd, l.off.data(), l.Afull.get(), l.dinv, l.sd, l.sd2, l.sc,
+                                           c.stream);
+    }
 
     GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w) {}
@@ -689,12 +767,26 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                 lu_perm.resize(m);
                 lu_perm.upload(perm.data(), m);
             }
+            // symmetric Dirichlet-eliminated storage of the level operator
+            const int nu = l.nd / 2 + 1;
+            DeviceBuffer<double> U64(dia_tiled_size(l.N, nu));
+            UGX_LAUNCH(dia_symmetrize<double>(l.N, l.nd, l.off.data(), l.dir, A64.get(), U64.get(), c.stream));
+            if (lev == top) {
+                // the surface operator in lattice order (FP64, symmetric
+                // storage) for the CG fast path
+                surf.resize(U64.size());
+                UGX_HIP(hipMemcpyAsync(surf.get(), U64.get(), sizeof(double) * U64.size(), hipMemcpyDeviceToDevice,
+                                       c.stream));
+            }
             if constexpr (std::is_same<T, double>::value) {
-                l.A = std::move(A64);
+                l.A = std::move(U64);
+                l.Afull = std::move(A64);
                 UGX_HIP(hipMemcpyAsync(l.dinv, dinv64.get(), sizeof(double) * l.N, hipMemcpyDeviceToDevice, c.stream));
             } else {
-                l.A.resize(A64.size());
-                UGX_LAUNCH((convert<double, T>(A64.size(), A64.get(), l.A.get(), c.stream)));
+                l.A.resize(U64.size());
+                UGX_LAUNCH((convert<double, T>(U64.size(), U64.get(), l.A.get(), c.stream)));
+                l.Afull.resize(A64.size());
+                UGX_LAUNCH((convert<double, T>(A64.size(), A64.get(), l.Afull.get(), c.stream)));
                 UGX_LAUNCH((convert<double, T>(l.N, dinv64.get(), l.dinv, c.stream)));
             }
             sync();
@@ -708,7 +800,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         auto& c = context();
         GmgLevel<T>& l = L[lev];
         UGX_LAUNCH(lu_solve<T>(lu_n, lu_linvT.get(), lu_uinvT.get(), lu_perm.get(), l.sd, l.sc, c.stream));
-        if (lev >= top) UGX_LAUNCH((dia_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.A.get(), l.sc, l.sd, l.sd, c.stream)));
+        if (lev >= top) UGX_LAUNCH(op_apply_sub(l, l.sc, l.sd, l.sd));
     }
 
     void presmooth_and_restriction(int lev) {
@@ -716,7 +808,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
         for (int nu = 0; nu < nu1; ++nu) {
-            UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+            UGX_LAUNCH(op_jacobi_step(f));
             std::swap(f.sd, f.sd2);
         }
         UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
@@ -729,14 +821,14 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         GmgLevel<T>& cl = L[lev - 1];
         UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
         if (nu2 == 0) {
-            if (lev >= top) UGX_LAUNCH((dia_apply_sub<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.st, f.sd, f.sd, c.stream)));
+            if (lev >= top) UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
             return;
         }
-        UGX_LAUNCH((dia_apply_sub<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.st, f.sd, f.sd, c.stream)));
+        UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
         for (int nu = 0; nu < nu2; ++nu) {
             const bool last = (nu == nu2 - 1);
             if (!last || lev >= top) {
-                UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+                UGX_LAUNCH(op_jacobi_step(f));
                 std::swap(f.sd, f.sd2);
             } else {
                 UGX_LAUNCH(jacobi_accumulate<T>(f.N, f.dinv, f.sd, f.sc, c.stream));
@@ -760,20 +852,63 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         LevelGrid& G = disc->level(top);
         GmgLevel<T>& t = L[top];
         if (d.size() != t.N || cvec.size() != t.N) throw UGError("GMG::apply: size mismatch");
+        // Dirichlet entries of the defect (DoFs n_inner..N-1): zero in the
+        // reference workflow (adjusted solution and rhs); otherwise the full
+        // level storage is used
+        double tail = 0.0;
+        if (G.N > G.n_inner) {
+            UGX_LAUNCH(dot<double>(G.N - G.n_inner, d.data() + G.n_inner, d.data() + G.n_inner, c.red_partial,
+                                   c.red_out, false, c.stream));
+            UGX_HIP(hipMemcpyAsync(c.red_host, c.red_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            sync();
+            tail = c.red_host[0];
+        }
+        sym = (tail == 0.0);
         UGX_LAUNCH((permute_to<double, T>(t.N, G.dof_of_lat.get(), d.data(), t.sd, c.stream)));
         cvec.set(0.0);
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
         lmgc(top);
         UGX_LAUNCH((permute_add_from<T, double>(t.N, G.lat_of_dof.get(), t.sc, cvec.data(), c.stream)));
+        sym = true;
     }
 
+    void apply_lattice(double* c_lat, const double* d_lat) override {
+        auto& c = context();
+        GmgLevel<T>& t = L[top];
+        sym = true;
+        UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
+        UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
+        lmgc(top);
+        UGX_LAUNCH((convert<T, double>(t.N, t.sc, c_lat, c.stream)));
+    }
+
+    void surface_axpy(double* y, double alpha, const double* v, double beta, const double* x) override {
+        GmgLevel<T>& t = L[top];
+        UGX_LAUNCH((dias_axpy<double, double>(t.N, t.nd, t.off.data(), surf.get(), alpha, v, beta, x, y,
+                                              context().stream)));
+    }
+
+    int64_t halo() const override { return L[top].H; }
+
     void download_level_dia(int lev, double* out) const override {
-        // un-tiled: out[d*N + i]
+        // the stored operator expanded to all nd diagonals (out[d*N + i]):
+        // the Dirichlet-eliminated level matrix, lower half read from the
+        // mirrored upper entries
         const GmgLevel<T>& l = L[lev];
+        const int nu = l.nd / 2 + 1;
         std::vector<T> h(l.A.size());
         l.A.download(h.data(), (int64_t)h.size());
         for (int d = 0; d < l.nd; ++d)
-            for (int64_t i = 0; i < l.N; ++i) out[(int64_t)d * l.N + i] = (double)h[dia_tiled_index(d, i, l.nd)];
+            for (int64_t i = 0; i < l.N; ++i) {
+                double v = 0.0;
+                if (d >= l.nd / 2) {
+                    v = (double)h[dia_tiled_index(d - l.nd / 2, i, nu)];
+                } else {
+                    const int64_t j = i + l.off[d];
+                    if (j >= 0 && j < l.N) v = (double)h[dia_tiled_index(l.nd - 1 - d - l.nd / 2, j, nu)];
+                }
+                out[(int64_t)d * l.N + i] = v;
+            }
     }
     int num_diags(int lev) const override { return L[lev].nd; }
     void smooth(int lev, int nsteps) override {
@@ -781,7 +916,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         auto& c = context();
         GmgLevel<T>& f = L[lev];
         for (int k = 0; k < nsteps; ++k) {
-            UGX_LAUNCH((dia_jacobi_step<T, T>(f.N, f.nd, f.off.data(), f.A.get(), f.dinv, f.sd, f.sd2, f.sc, c.stream)));
+            UGX_LAUNCH(op_jacobi_step(f));
             std::swap(f.sd, f.sd2);
         }
     }
@@ -792,8 +927,28 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
 AssembledMultiGridCycle::AssembledMultiGridCycle(std::shared_ptr<DomainDiscretization> disc) : disc_(std::move(disc)) {}
 AssembledMultiGridCycle::~AssembledMultiGridCycle() { delete impl_; }
 
+bool AssembledMultiGridCycle::lattice_ready(const GPUSparseMatrix& A) const {
+    return impl_ && mat_ == &A && A.version() == mat_version_ && A.origin() == disc_.get();
+}
+int64_t AssembledMultiGridCycle::lattice_size() const { return disc_->level(disc_->top_level()).N; }
+int64_t AssembledMultiGridCycle::lattice_halo() const { return impl_ ? impl_->halo() : 0; }
+const int32_t* AssembledMultiGridCycle::dof_of_lattice() const { return disc_->level(disc_->top_level()).dof_of_lat.get(); }
+const int32_t* AssembledMultiGridCycle::lattice_of_dof() const { return disc_->level(disc_->top_level()).lat_of_dof.get(); }
+int64_t AssembledMultiGridCycle::num_inner_dofs() const { return disc_->level(disc_->top_level()).n_inner; }
+void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat) {
+    if (!impl_) throw UGError("GMG: not initialised");
+    impl_->apply_lattice(c_lat, d_lat);
+}
+void AssembledMultiGridCycle::surface_axpy_lattice(double* y, double alpha, const double* v, double beta,
+                                                   const double* x) {
+    if (!impl_) throw UGError("GMG: not initialised");
+    impl_->surface_axpy(y, alpha, v, beta, x);
+}
+
 bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     op_ = J;
+    mat_ = &J->get_matrix();
+    mat_version_ = mat_->version();
     delete impl_;
     impl_ = nullptr;
     if (prec_ == Precision::F64) impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_);

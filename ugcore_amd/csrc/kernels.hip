@@ -265,6 +265,12 @@ __global__ void k_permute_to(int64_t n, const int32_t* __restrict__ dof_of_lat, 
         dst[l] = (TL)src[dof_of_lat[l]];
 }
 template <typename TL, typename TS>
+__global__ void k_permute_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
+                               TS* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (TS)src[lat_of_dof[i]];
+}
+template <typename TL, typename TS>
 __global__ void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
                                    TS* __restrict__ dst) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -374,6 +380,118 @@ __global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, DiaOffsets o
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const int64_t j = i + o.off[d];
+        dv[d] = (T)dinv[j];
+        sv[d] = (T)sd_in[j];
+    }
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], dv[d] * sv[d], s);
+    const T sdi = sv[ND / 2];
+    const T st = dv[ND / 2] * sdi;
+    sd_out[i] = (TV)(sdi - s);
+    sc[i] = (TV)((T)sc[i] + st);
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric storage of the level operators. Every level vector of the cycle
+// vanishes on Dirichlet DoFs (the defect enters with zero Dirichlet entries,
+// Dirichlet rows of A, P and R keep them zero), so A may be replaced by its
+// Dirichlet-eliminated form (Dirichlet columns of interior rows zeroed),
+// which gives identical products on those vectors and is symmetric. Only the
+// centre and the ND/2 "upper" diagonals (offset >= 0) are stored (NU =
+// ND/2+1, tiled like the full storage); the coupling of row i to the lower
+// neighbour j = i + off_d (d < ND/2) is the upper entry of row j at the mirrored
+// offset, U[ND-1-d-ND/2][j].
+// ---------------------------------------------------------------------------
+template <typename TM>
+__global__ void k_dia_symmetrize(int64_t n, int nd, DiaOffsets o, const uint8_t* __restrict__ dir,
+                                 const TM* __restrict__ full, TM* __restrict__ up) {
+    const int nu = nd / 2 + 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int u = 0; u < nu; ++u) {
+            const int d = nd / 2 + u;
+            const int64_t j = i + o.off[d];
+            TM v = full[dia_index(d, i, nd)];
+            if (u != 0 && j >= 0 && j < n && dir[j]) v = (TM)0;   // eliminate Dirichlet columns
+            up[dia_index(u, i, nu)] = v;
+        }
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ int64_t clamp_row(int64_t j, int64_t n) {
+    return j < 0 ? 0 : (j >= n ? n - 1 : j);
+}
+
+// d_out = d_in - A*x with symmetric storage (x must vanish on Dirichlet DoFs)
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dias_apply_sub(int64_t n, DiaOffsets o, const TM* __restrict__ U,
+                                                        const TV* __restrict__ x, const TV* d_in, TV* d_out,
+                                                        int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
+    constexpr int NU = ND / 2 + 1;
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    T a[ND], xv[ND];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+#pragma unroll
+    for (int d = 0; d < ND / 2; ++d)
+        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], xv[d], s);
+    d_out[i] = (TV)((T)d_in[i] - s);
+}
+
+// y = alpha*v + beta*A*x with symmetric storage (x vanishing on Dirichlet DoFs)
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dias_axpy(int64_t n, DiaOffsets o, const TM* __restrict__ U, double alpha,
+                                                   const TV* v, double beta, const TV* __restrict__ x, TV* y,
+                                                   int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
+    constexpr int NU = ND / 2 + 1;
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    T a[ND], xv[ND];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+#pragma unroll
+    for (int d = 0; d < ND / 2; ++d)
+        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s = fma(a[d], xv[d], s);
+    y[i] = (TV)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * (double)s);
+}
+
+// fused damped-Jacobi step with symmetric storage:
+//   st = dinv .* sd_in ;  sd_out = sd_in - A*st ;  sc += st
+template <typename TM, typename TV, int ND>
+__global__ __launch_bounds__(256) void k_dias_jacobi_step(int64_t n, DiaOffsets o, const TM* __restrict__ U,
+                                                          const TV* __restrict__ dinv, const TV* __restrict__ sd_in,
+                                                          TV* __restrict__ sd_out, TV* __restrict__ sc,
+                                                          int64_t nblocks) {
+    typedef typename Acc<TV>::type T;
+    constexpr int NU = ND / 2 + 1;
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t i = lb * 256 + threadIdx.x;
+    if (lb >= nblocks || i >= n) return;
+    T a[ND], dv[ND], sv[ND];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+#pragma unroll
+    for (int d = 0; d < ND / 2; ++d)
+        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int64_t j = i + o.off[d];
@@ -1001,6 +1119,51 @@ static inline int64_t remapped_grid(int64_t nblocks) { return nblocks < 64 ? nbl
             return -1;                                                                                    \
     } while (0)
 
+template <typename TM>
+int dia_symmetrize(int64_t n, int nd, const int32_t* off, const uint8_t* dir, const TM* full, TM* up, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_dia_symmetrize<TM>, dim3(vec_grid(n)), dim3(256), 0, s, n, nd, make_offsets(off, nd), dir, full, up);
+    UGX_CHECK_LAUNCH();
+}
+template int dia_symmetrize<double>(int64_t, int, const int32_t*, const uint8_t*, const double*, double*, hipStream_t);
+
+template <typename TM, typename TV>
+int dias_apply_sub(int64_t n, int nd, const int32_t* off, const TM* U, const TV* x, const TV* d_in, TV* d_out,
+                   hipStream_t s) {
+    if (n <= 0) return 0;
+    UGX_DIA_DISPATCH(k_dias_apply_sub, n, make_offsets(off, nd), U, x, d_in, d_out);
+    UGX_CHECK_LAUNCH();
+}
+template int dias_apply_sub<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, hipStream_t);
+template int dias_apply_sub<float, float>(int64_t, int, const int32_t*, const float*, const float*, const float*, float*, hipStream_t);
+
+template <typename TM, typename TV>
+int dias_axpy(int64_t n, int nd, const int32_t* off, const TM* U, double alpha, const TV* v, double beta, const TV* x,
+              TV* y, hipStream_t s) {
+    if (n <= 0) return 0;
+    UGX_DIA_DISPATCH(k_dias_axpy, n, make_offsets(off, nd), U, alpha, v, beta, x, y);
+    UGX_CHECK_LAUNCH();
+}
+template int dias_axpy<double, double>(int64_t, int, const int32_t*, const double*, double, const double*, double, const double*, double*, hipStream_t);
+
+template <typename TL, typename TS>
+int permute_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_permute_from<TL, TS>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    UGX_CHECK_LAUNCH();
+}
+template int permute_from<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
+
+template <typename TM, typename TV>
+int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,
+                     TV* sc, hipStream_t s) {
+    if (n <= 0) return 0;
+    UGX_DIA_DISPATCH(k_dias_jacobi_step, n, make_offsets(off, nd), U, dinv, sd_in, sd_out, sc);
+    UGX_CHECK_LAUNCH();
+}
+template int dias_jacobi_step<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, double*, hipStream_t);
+template int dias_jacobi_step<float, float>(int64_t, int, const int32_t*, const float*, const float*, const float*, float*, float*, hipStream_t);
+
 template <typename TM, typename TV>
 int dia_apply_sub(int64_t n, int nd, const int32_t* off, const TM* A, const TV* x, const TV* d_in, TV* d_out,
                   hipStream_t s) {
@@ -1140,6 +1303,8 @@ int convert(int64_t n, const TA* a, TB* b, hipStream_t s) {
     UGX_CHECK_LAUNCH();
 }
 template int convert<double, float>(int64_t, const double*, float*, hipStream_t);
+template int convert<float, double>(int64_t, const float*, double*, hipStream_t);
+template int convert<double, double>(int64_t, const double*, double*, hipStream_t);
 
 int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s) {
     hipLaunchKernelGGL(k_boundary_flags, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, dir);

@@ -49,6 +49,21 @@ int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV
                     TV* sc, hipStream_t s);
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
+// symmetric (Dirichlet-eliminated) storage: nd/2+1 tiled upper diagonals
+template <typename TM>
+int dia_symmetrize(int64_t n, int nd, const int32_t* off, const uint8_t* dir, const TM* full, TM* up, hipStream_t s);
+template <typename TM, typename TV>
+int dias_apply_sub(int64_t n, int nd, const int32_t* off, const TM* U, const TV* x, const TV* d_in, TV* d_out,
+                   hipStream_t s);
+template <typename TM, typename TV>
+int dias_axpy(int64_t n, int nd, const int32_t* off, const TM* U, double alpha, const TV* v, double beta, const TV* x,
+              TV* y, hipStream_t s);
+// dst_dof[i] = src_lat[lat_of_dof[i]]
+template <typename TL, typename TS>
+int permute_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s);
+template <typename TM, typename TV>
+int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,
+                     TV* sc, hipStream_t s);
 template <typename TM, typename TV>
 int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s);
 
