@@ -38,6 +38,40 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
     return lb;   // may exceed nblocks-1 for the last XCD: callers bound-check rows
 }
 
+// Row schedule of the lattice (DIA) kernels. Flat mode: 256 consecutive rows
+// per block, blocks remapped by xcd_block. Banded mode (3D lattices): the W
+// lines of every plane are split into 8 bands, XCD c (blockIdx % 8) owns band
+// c of every plane and walks its blocks plane by plane, so the rows one plane
+// below (offsets -W^2 +- W +- 1, whose coefficients and vector entries a row
+// reads again) were streamed through the same L2 one band-plane earlier.
+struct RowMap {
+    int64_t nblocks;   // flat mode: logical blocks
+    int64_t plane;     // banded mode: rows per plane (0 = flat mode)
+    int64_t nplanes;
+    int32_t lo[8];     // first row of band c within a plane
+    int32_t len[8];    // rows of band c within a plane
+    int32_t cps[8];    // 256-row chunks of band c per plane
+};
+
+// first row of this thread and the end of its block's row range; false when
+// the block has no rows
+__device__ __forceinline__ bool map_row(const RowMap& m, int64_t n, int64_t& i) {
+    if (m.plane == 0) {
+        const int64_t lb = xcd_block(m.nblocks);
+        i = lb * 256 + threadIdx.x;
+        return lb < m.nblocks && i < n;
+    }
+    const int c = blockIdx.x & 7;
+    const int64_t q = blockIdx.x >> 3;
+    const int64_t cps = m.cps[c];
+    const int64_t k = q / cps;
+    if (k >= m.nplanes) return false;
+    const int64_t ch = q - k * cps;
+    const int64_t r = ch * 256 + threadIdx.x;
+    i = k * m.plane + m.lo[c] + r;
+    return r < m.len[c];
+}
+
 // Level operators are stored as ND diagonals in wavefront tiles: the 64
 // values of diagonal d for rows 64t..64t+63 are contiguous, and the ND
 // diagonals of one tile follow each other, so a wavefront streams one
@@ -328,11 +362,10 @@ struct Acc {
 template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, DiaOffsets o, const TM* __restrict__ A,
                                                        const TV* __restrict__ x, const TV* d_in, TV* d_out,
-                                                       int64_t nblocks) {
+                                                       RowMap rm) {
     typedef typename Acc<TV>::type T;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
@@ -348,11 +381,10 @@ __global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, DiaOffsets o, 
 template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, DiaOffsets o, const TM* __restrict__ A,
                                                   double alpha, const TV* v, double beta, const TV* __restrict__ x,
-                                                  TV* y, int64_t nblocks) {
+                                                  TV* y, RowMap rm) {
     typedef typename Acc<TV>::type T;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
@@ -372,11 +404,10 @@ template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, DiaOffsets o, const TM* __restrict__ A,
                                                          const TV* __restrict__ dinv, const TV* __restrict__ sd_in,
                                                          TV* __restrict__ sd_out, TV* __restrict__ sc,
-                                                         int64_t nblocks) {
+                                                         RowMap rm) {
     typedef typename Acc<TV>::type T;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
@@ -430,12 +461,11 @@ __device__ __forceinline__ int64_t clamp_row(int64_t j, int64_t n) {
 template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dias_apply_sub(int64_t n, DiaOffsets o, const TM* __restrict__ U,
                                                         const TV* __restrict__ x, const TV* d_in, TV* d_out,
-                                                        int64_t nblocks) {
+                                                        RowMap rm) {
     typedef typename Acc<TV>::type T;
     constexpr int NU = ND / 2 + 1;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
     for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
@@ -454,12 +484,11 @@ __global__ __launch_bounds__(256) void k_dias_apply_sub(int64_t n, DiaOffsets o,
 template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dias_axpy(int64_t n, DiaOffsets o, const TM* __restrict__ U, double alpha,
                                                    const TV* v, double beta, const TV* __restrict__ x, TV* y,
-                                                   int64_t nblocks) {
+                                                   RowMap rm) {
     typedef typename Acc<TV>::type T;
     constexpr int NU = ND / 2 + 1;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
     for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
@@ -480,12 +509,11 @@ template <typename TM, typename TV, int ND>
 __global__ __launch_bounds__(256) void k_dias_jacobi_step(int64_t n, DiaOffsets o, const TM* __restrict__ U,
                                                           const TV* __restrict__ dinv, const TV* __restrict__ sd_in,
                                                           TV* __restrict__ sd_out, TV* __restrict__ sc,
-                                                          int64_t nblocks) {
+                                                          RowMap rm) {
     typedef typename Acc<TV>::type T;
     constexpr int NU = ND / 2 + 1;
-    const int64_t lb = xcd_block(nblocks);
-    const int64_t i = lb * 256 + threadIdx.x;
-    if (lb >= nblocks || i >= n) return;
+    int64_t i;
+    if (!map_row(rm, n, i)) return;
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
     for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
@@ -505,6 +533,161 @@ __global__ __launch_bounds__(256) void k_dias_jacobi_step(int64_t n, DiaOffsets 
     const T st = dv[ND / 2] * sdi;
     sd_out[i] = (TV)(sdi - s);
     sc[i] = (TV)((T)sc[i] + st);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged symmetric 27-point kernels for full 3D lattices (n = W^3).
+// A workgroup owns TY consecutive lines [j0, j0+TY) of one plane k (rows
+// k*W^2 + j0*W ... contiguous) and first stages, for dz = -1, 0, 1, the
+// contiguous vector range covering lines j0-1 .. j0+TY of plane k+dz plus one
+// entry on each side. The 27 neighbour values of a row are then LDS reads;
+// global loads per row drop from 14 + 13 + 27*(vectors) to
+// 14 + 13 + 3*(TY+2)/TY*(vectors). Operand values and the order of the
+// multiply-adds are those of the unstaged kernels, so results are identical.
+// Workgroups are grouped per XCD as in the banded row schedule: XCD c owns a
+// contiguous group of bands in every plane and walks them plane by plane.
+// ---------------------------------------------------------------------------
+struct StagedMap {
+    int32_t W, TY, maxg;
+    int32_t glo[8];   // first band of XCD c's group
+    int32_t gsz[8];   // bands in XCD c's group
+};
+
+template <typename T>
+__device__ __forceinline__ T* lds_base() {
+    extern __shared__ __align__(16) unsigned char ugx_lds[];
+    return reinterpret_cast<T*>(ugx_lds);
+}
+
+// (plane k, first line j0, rows R of this workgroup); false if the block is idle
+__device__ __forceinline__ bool staged_block(const StagedMap& m, int64_t& k, int64_t& j0, int64_t& R) {
+    const int c = blockIdx.x & 7;
+    const int64_t q = blockIdx.x >> 3;
+    const int64_t g = m.gsz[c];
+    if (g == 0) return false;
+    k = q / g;
+    if (k >= m.W) return false;
+    j0 = (int64_t)(m.glo[c] + (q - k * g)) * m.TY;
+    const int64_t j1 = j0 + m.TY < m.W ? j0 + m.TY : m.W;
+    R = (j1 - j0) * m.W;
+    return true;
+}
+
+// stage f(j) for j in [base_dz - 1, base_dz - 1 + S) of the three planes;
+// S = R + 2W + 2
+template <typename T, typename F>
+__device__ __forceinline__ void stage3(T* lds, int64_t S, int64_t W2, int64_t first, F f) {
+    for (int dz = 0; dz < 3; ++dz) {
+        const int64_t b = first + (dz - 1) * W2;
+        for (int64_t t = threadIdx.x; t < S; t += blockDim.x) lds[dz * S + t] = f(b + t);
+    }
+}
+
+// LDS index of neighbour d (dx,dy,dz = d%3-1, d/3%3-1, d/9-1) of local row r
+__device__ __forceinline__ int64_t staged_idx(int d, int64_t r, int64_t W, int64_t S) {
+    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9;
+    return dz * S + r + W + 1 + dy * W + dx;
+}
+
+template <typename TM, typename T>
+__device__ __forceinline__ void load_sym27(const TM* __restrict__ U, const DiaOffsets& o, int64_t i, int64_t n,
+                                           T (&a)[27]) {
+#pragma unroll
+    for (int u = 0; u < 14; ++u) a[13 + u] = (T)U[dia_index(u, i, 14)];
+#pragma unroll
+    for (int d = 0; d < 13; ++d) a[d] = (T)U[dia_index(26 - d - 13, clamp_row<27>(i + o.off[d], n), 14)];
+}
+
+// d_out = d_in - A*x
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dias27_staged_apply_sub(int64_t n, DiaOffsets o, StagedMap m,
+                                                                 const TM* __restrict__ U, const TV* __restrict__ x,
+                                                                 const TV* d_in, TV* d_out) {
+    typedef typename Acc<TV>::type T;
+    int64_t k, j0, R;
+    if (!staged_block(m, k, j0, R)) return;
+    const int64_t W = m.W, W2 = W * W, S = R + 2 * W + 2;
+    const int64_t base = k * W2 + j0 * W;
+    T* lds = lds_base<T>();
+    stage3(lds, S, W2, base - W - 1, [&](int64_t j) { return (T)x[j]; });
+    __syncthreads();
+    // lanes start on a 64-row storage tile boundary so that the coefficient
+    // loads of a wavefront are aligned
+    const int64_t a0 = base & 63;
+    for (int64_t rr = threadIdx.x; rr < R + a0; rr += blockDim.x) {
+        const int64_t r = rr - a0;
+        if (r < 0) continue;
+        const int64_t i = base + r;
+        T a[27];
+        load_sym27(U, o, i, n, a);
+        T s = (T)0;
+#pragma unroll
+        for (int d = 0; d < 27; ++d) s = fma(a[d], lds[staged_idx(d, r, W, S)], s);
+        d_out[i] = (TV)((T)d_in[i] - s);
+    }
+}
+
+// y = alpha*v + beta*A*x
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dias27_staged_axpy(int64_t n, DiaOffsets o, StagedMap m,
+                                                            const TM* __restrict__ U, double alpha, const TV* v,
+                                                            double beta, const TV* __restrict__ x, TV* y) {
+    typedef typename Acc<TV>::type T;
+    int64_t k, j0, R;
+    if (!staged_block(m, k, j0, R)) return;
+    const int64_t W = m.W, W2 = W * W, S = R + 2 * W + 2;
+    const int64_t base = k * W2 + j0 * W;
+    T* lds = lds_base<T>();
+    stage3(lds, S, W2, base - W - 1, [&](int64_t j) { return (T)x[j]; });
+    __syncthreads();
+    // lanes start on a 64-row storage tile boundary so that the coefficient
+    // loads of a wavefront are aligned
+    const int64_t a0 = base & 63;
+    for (int64_t rr = threadIdx.x; rr < R + a0; rr += blockDim.x) {
+        const int64_t r = rr - a0;
+        if (r < 0) continue;
+        const int64_t i = base + r;
+        T a[27];
+        load_sym27(U, o, i, n, a);
+        T s = (T)0;
+#pragma unroll
+        for (int d = 0; d < 27; ++d) s = fma(a[d], lds[staged_idx(d, r, W, S)], s);
+        y[i] = (TV)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * (double)s);
+    }
+}
+
+// st = dinv .* sd_in (staged) ;  sd_out = sd_in - A*st ;  sc += st
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_dias27_staged_jacobi_step(int64_t n, DiaOffsets o, StagedMap m,
+                                                                   const TM* __restrict__ U,
+                                                                   const TV* __restrict__ dinv,
+                                                                   const TV* __restrict__ sd_in,
+                                                                   TV* __restrict__ sd_out, TV* __restrict__ sc) {
+    typedef typename Acc<TV>::type T;
+    int64_t k, j0, R;
+    if (!staged_block(m, k, j0, R)) return;
+    const int64_t W = m.W, W2 = W * W, S = R + 2 * W + 2;
+    const int64_t base = k * W2 + j0 * W;
+    T* lds = lds_base<T>();
+    stage3(lds, S, W2, base - W - 1, [&](int64_t j) { return (T)dinv[j] * (T)sd_in[j]; });
+    __syncthreads();
+    // lanes start on a 64-row storage tile boundary so that the coefficient
+    // loads of a wavefront are aligned
+    const int64_t a0 = base & 63;
+    for (int64_t rr = threadIdx.x; rr < R + a0; rr += blockDim.x) {
+        const int64_t r = rr - a0;
+        if (r < 0) continue;
+        const int64_t i = base + r;
+        T a[27];
+        load_sym27(U, o, i, n, a);
+        T s = (T)0;
+#pragma unroll
+        for (int d = 0; d < 27; ++d) s = fma(a[d], lds[staged_idx(d, r, W, S)], s);
+        const T sdi = (T)sd_in[i];
+        const T st = lds[staged_idx(13, r, W, S)];
+        sd_out[i] = (TV)(sdi - s);
+        sc[i] = (TV)((T)sc[i] + st);
+    }
 }
 
 // sc += dinv .* sd  (last post-smoothing step on levels below the top, where
@@ -1108,13 +1291,63 @@ static DiaOffsets make_offsets(const int32_t* off, int nd) {
 
 static inline int64_t remapped_grid(int64_t nblocks) { return nblocks < 64 ? nblocks : ((nblocks + 7) / 8) * 8; }
 
+// Row schedule for a DIA launch over n rows with offsets `off`: banded when
+// the operator is a full 3D lattice (27 diagonals, n = W^3) wide enough for
+// the bands to matter, flat otherwise. Returns the grid size.
+static int64_t make_row_map(int64_t n, int nd, const int32_t* off, RowMap& m) {
+    m = RowMap{};
+    m.nblocks = (n + 255) / 256;
+    if (nd == 27) {
+        const int64_t W = (int64_t)off[25] - off[22];
+        const int64_t plane = off[22];
+        if (W >= 128 && plane == W * W && n == W * W * W) {
+            int64_t maxcps = 0;
+            for (int c = 0; c < 8; ++c) {
+                const int64_t j0 = c * W / 8, j1 = (c + 1) * W / 8;
+                m.lo[c] = (int32_t)(j0 * W);
+                m.len[c] = (int32_t)((j1 - j0) * W);
+                m.cps[c] = (int32_t)((m.len[c] + 255) / 256);
+                maxcps = m.cps[c] > maxcps ? m.cps[c] : maxcps;
+            }
+            m.plane = plane;
+            m.nplanes = W;
+            return 8 * maxcps * W;
+        }
+    }
+    return remapped_grid(m.nblocks);
+}
+
+// LDS-staged launch plan for symmetric 27-point operators on full 3D lattices:
+// bands of TY lines, grouped per XCD. Returns the grid size, 0 if not eligible.
+static int64_t make_staged_map(int64_t n, int nd, const int32_t* off, size_t elem, StagedMap& m, size_t& lds) {
+    if (nd != 27) return 0;
+    const int64_t W = (int64_t)off[25] - off[22];
+    if (W < 128 || off[22] != W * W || n != W * W * W) return 0;
+    m = StagedMap{};
+    m.W = (int32_t)W;
+    m.TY = 4;
+    const int64_t nb = (W + m.TY - 1) / m.TY;
+    int32_t maxg = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int64_t b0 = c * nb / 8, b1 = (c + 1) * nb / 8;
+        m.glo[c] = (int32_t)b0;
+        m.gsz[c] = (int32_t)(b1 - b0);
+        maxg = m.gsz[c] > maxg ? m.gsz[c] : maxg;
+    }
+    m.maxg = maxg;
+    lds = (size_t)3 * (size_t)(m.TY * W + 2 * W + 2) * elem;
+    if (lds > 64 * 1024) return 0;
+    return 8 * (int64_t)maxg * W;
+}
+
 #define UGX_DIA_DISPATCH(KERNEL, ...)                                                                      \
     do {                                                                                                  \
-        const int64_t nb = (n + 255) / 256;                                                               \
+        RowMap rm;                                                                                        \
+        const int64_t grid = make_row_map(n, nd, off, rm);                                                \
         if (nd == 27)                                                                                     \
-            hipLaunchKernelGGL((KERNEL<TM, TV, 27>), dim3(remapped_grid(nb)), dim3(256), 0, s, __VA_ARGS__, nb); \
+            hipLaunchKernelGGL((KERNEL<TM, TV, 27>), dim3(grid), dim3(256), 0, s, __VA_ARGS__, rm);       \
         else if (nd == 9)                                                                                 \
-            hipLaunchKernelGGL((KERNEL<TM, TV, 9>), dim3(remapped_grid(nb)), dim3(256), 0, s, __VA_ARGS__, nb);  \
+            hipLaunchKernelGGL((KERNEL<TM, TV, 9>), dim3(grid), dim3(256), 0, s, __VA_ARGS__, rm);        \
         else                                                                                              \
             return -1;                                                                                    \
     } while (0)
@@ -1131,6 +1364,13 @@ template <typename TM, typename TV>
 int dias_apply_sub(int64_t n, int nd, const int32_t* off, const TM* U, const TV* x, const TV* d_in, TV* d_out,
                    hipStream_t s) {
     if (n <= 0) return 0;
+    StagedMap sm;
+    size_t lds = 0;
+    typedef typename Acc<TV>::type T;
+    if (const int64_t grid = make_staged_map(n, nd, off, sizeof(T), sm, lds)) {
+        hipLaunchKernelGGL((k_dias27_staged_apply_sub<TM, TV>), dim3(grid), dim3(256), lds, s, n, make_offsets(off, nd), sm, U, x, d_in, d_out);
+        UGX_CHECK_LAUNCH();
+    }
     UGX_DIA_DISPATCH(k_dias_apply_sub, n, make_offsets(off, nd), U, x, d_in, d_out);
     UGX_CHECK_LAUNCH();
 }
@@ -1141,6 +1381,13 @@ template <typename TM, typename TV>
 int dias_axpy(int64_t n, int nd, const int32_t* off, const TM* U, double alpha, const TV* v, double beta, const TV* x,
               TV* y, hipStream_t s) {
     if (n <= 0) return 0;
+    StagedMap sm;
+    size_t lds = 0;
+    typedef typename Acc<TV>::type T;
+    if (const int64_t grid = make_staged_map(n, nd, off, sizeof(T), sm, lds)) {
+        hipLaunchKernelGGL((k_dias27_staged_axpy<TM, TV>), dim3(grid), dim3(256), lds, s, n, make_offsets(off, nd), sm, U, alpha, v, beta, x, y);
+        UGX_CHECK_LAUNCH();
+    }
     UGX_DIA_DISPATCH(k_dias_axpy, n, make_offsets(off, nd), U, alpha, v, beta, x, y);
     UGX_CHECK_LAUNCH();
 }
@@ -1158,6 +1405,13 @@ template <typename TM, typename TV>
 int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,
                      TV* sc, hipStream_t s) {
     if (n <= 0) return 0;
+    StagedMap sm;
+    size_t lds = 0;
+    typedef typename Acc<TV>::type T;
+    if (const int64_t grid = make_staged_map(n, nd, off, sizeof(T), sm, lds)) {
+        hipLaunchKernelGGL((k_dias27_staged_jacobi_step<TM, TV>), dim3(grid), dim3(256), lds, s, n, make_offsets(off, nd), sm, U, dinv, sd_in, sd_out, sc);
+        UGX_CHECK_LAUNCH();
+    }
     UGX_DIA_DISPATCH(k_dias_jacobi_step, n, make_offsets(off, nd), U, dinv, sd_in, sd_out, sc);
     UGX_CHECK_LAUNCH();
 }
