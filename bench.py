@@ -83,6 +83,157 @@ def measured_traffic(kernel):
     return None
 
 
+def _timed(U, fn, reps, warmup=1):
+    """mean wall time of fn() over reps runs, device synchronised around each"""
+    out = []
+    for it in range(warmup + reps):
+        U.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        U.synchronize()
+        if it >= warmup:
+            out.append(time.perf_counter() - t0)
+    return float(np.mean(out)), r
+
+
+def bench_poisson_2d(U, seed, reps):
+    """2D FV1 Poisson: 2 triangles refined 7 times (16,641 DoFs), f =
+    2 pi^2 sin(pi x) sin(pi y) + N(0, 0.1^2) per vertex, CG + GMG V(2,2) Jacobi
+    0.66, LU on level 0, FP64, tol 1e-10"""
+    from ugcore_amd.synthetic import poisson_source_lattice
+    refs = 7
+    n = 2 ** refs
+    d = U.DomainDiscretization(1, refs)
+    dol = d.dof_of_lattice(refs)
+    f = np.empty(d.num_dofs())
+    f[dol] = poisson_source_lattice(n, seed)
+    d.set_source(f)
+    A = U.SparseMatrix()
+    b = U.Vector(d.num_dofs())
+    d.assemble_linear(A, b)
+    g = U.GeometricMultiGrid(d, base_level=0, omega=0.66, precision="f64")
+    S = U.CG(g, max_steps=200, reduction=1e-10)
+    S.init(A)
+    x = U.Vector(d.num_dofs())
+
+    def solve():
+        x.set(0.0)
+        if not S.apply(x, b):
+            raise RuntimeError("2D solve did not converge")
+    t, _ = _timed(U, solve, reps)
+    return {"workload": "2D FV1 Poisson, 2 triangles x 7 refinements, CG + GMG V(2,2), FP64, tol 1e-10",
+            "dofs": d.num_dofs(), "nnz": A.nnz(), "solve_s": t, "cg_steps": S.steps(),
+            "assembly_s": d.assemble_seconds()}
+
+
+def bench_adjoint(U, refs, seed, reps, precision="f32"):
+    """adjoint gradient of J = 1/2 |u(obs) - d|^2 for the Q1 log-diffusion
+    field: forward GMG-CG solve, adjoint GMG-CG solve (same operator, Dirichlet
+    entries of dJ/du zeroed), element gradient kernel; 4096 observations"""
+    from ugcore_amd.adjoint import observation_points
+    from ugcore_amd.synthetic import make_theta
+    n = 2 ** refs
+    d = U.DomainDiscretization(0, refs)
+    d.set_theta(make_theta(n, seed, 3))
+    A = U.SparseMatrix()
+    N = d.num_dofs()
+    b = U.Vector(N)
+    t_asm, _ = _timed(U, lambda: d.assemble_linear(A, b), 1, warmup=0)
+    g = U.GeometricMultiGrid(d, base_level=min(2, refs - 1), omega=0.66, precision=precision)
+    S = U.CG(g, max_steps=200, reduction=1e-10)
+    S.init(A)
+    obs_lat = observation_points(n, 4096, seed)
+    obs = d.dof_of_lattice(refs)[obs_lat]
+    rng = np.random.default_rng(seed)
+    dobs = rng.normal(size=obs.size) * 1e-2
+    u = U.Vector(N)
+    lam = U.Vector(N)
+    rhs = U.Vector(N)
+    grad = U.DeviceArray(np.zeros(n ** 3))
+
+    def forward():
+        u.set(0.0)
+        if not S.apply(u, b):
+            raise RuntimeError("forward solve did not converge")
+        return S.steps()
+
+    def backward():
+        uv = u.numpy()
+        r = np.zeros(N)
+        r[obs] = uv[obs] - dobs
+        rhs.upload(r)
+        d.adjust_defect(rhs)
+        lam.set(0.0)
+        if not S.apply(lam, rhs):
+            raise RuntimeError("adjoint solve did not converge")
+        d.adjoint_gradient(u, lam, grad_ptr=grad.ptr)
+        return S.steps()
+
+    t_f, st_f = _timed(U, forward, reps)
+    t_b, st_b = _timed(U, backward, reps)
+    t_k = U.time_launches(lambda: d.adjoint_gradient(u, lam, grad_ptr=grad.ptr), 10)
+    return {"workload": f"adjoint gradient, {n}^3 Q1 log-diffusion, 4096 observations, GMG({precision})-CG tol 1e-10",
+            "parameters": n ** 3, "forward_solve_s": t_f, "backward_s": t_b, "gradient_kernel_ms": t_k * 1e3,
+            "forward_cg_steps": st_f, "adjoint_cg_steps": st_b, "assembly_s": t_asm,
+            "gradient_total_s_excl_assembly": t_f + t_b}
+
+
+def bench_tet(U, refs, seed, reps):
+    """96^3 x 6 tetrahedra, FV1 full upwind, v = 20*(1, 0.5, 0.25), jitter
+    +-0.2h, BiCGStab + GMG V(2,2) Jacobi 0.6 (FP32 and FP64 levels), tol 1e-10;
+    adjoint solve with the transposed operator and transposed GMG"""
+    from ugcore_amd.synthetic import make_theta
+    d = U.DomainDiscretization(2, refs)
+    d.set_theta(make_theta(d.n_fine, seed, 3))
+    d.set_velocity((20.0, 10.0, 5.0))
+    d.set_jitter(11, 0.2)
+    d.set_source(1.0)
+    N = d.num_dofs()
+    A = U.SparseMatrix()
+    b = U.Vector(N)
+    d.assemble_linear(A, b)
+    res = {"workload": f"{d.n_fine}^3 x 6 tetrahedra FV1 convection-diffusion (full upwind, Pe 20, jitter 0.2h), "
+                       "BiCGStab + GMG V(2,2) Jacobi 0.6, tol 1e-10",
+           "dofs": N, "nnz": A.nnz(), "assembly_s": d.assemble_seconds()}
+    xs = {}
+    for prec in ("f32", "f64"):
+        g = U.GeometricMultiGrid(d, base_level=0, omega=0.6, precision=prec)
+        S = U.BiCGStab(g, max_steps=500, reduction=1e-10)
+        S.init(A)
+        x = U.Vector(N)
+
+        def solve():
+            x.set(0.0)
+            if not S.apply(x, b):
+                raise RuntimeError("tetrahedral solve did not converge")
+        t, _ = _timed(U, solve, reps)
+        res[f"{prec}_precond_s"] = t
+        res[f"{prec}_bicgstab_steps"] = S.steps()
+        xs[prec] = x.numpy()
+    res["rel_l2_diff_fp32_vs_fp64_precond_solution"] = float(
+        np.linalg.norm(xs["f32"] - xs["f64"]) / np.linalg.norm(xs["f64"]))
+    AT = d.adjoint_operator(A)
+    gT = U.GeometricMultiGrid(d, base_level=0, omega=0.6, precision="f32", transposed=True)
+    ST = U.BiCGStab(gT, max_steps=500, reduction=1e-10)
+    ST.init(AT)
+    rng = np.random.default_rng(seed)
+    r = U.Vector(values=rng.normal(size=N))
+    d.adjust_defect(r)
+    lam = U.Vector(N)
+
+    def adj():
+        lam.set(0.0)
+        if not ST.apply(lam, r):
+            raise RuntimeError("transposed solve did not converge")
+    t_adj, _ = _timed(U, adj, reps)
+    grad = U.DeviceArray(np.zeros(d.n_fine ** 3))
+    u = U.Vector(values=xs["f64"])
+    t_k = U.time_launches(lambda: d.adjoint_gradient(u, lam, grad_ptr=grad.ptr), 10)
+    res.update({"adjoint_transposed_f32_precond_s": t_adj, "adjoint_bicgstab_steps": ST.steps(),
+                "gradient_kernel_ms": t_k * 1e3})
+    return res
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -105,6 +256,8 @@ def main():
     ap.add_argument("--cpu-refs", type=int, default=7)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--skip-extra", action="store_true",
+                    help="skip the secondary configurations (2D FV1, adjoint, tetrahedra)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="issue one 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -269,6 +422,17 @@ def main():
                 "launches_per_solve": op_launches}
     roof["traffic"] = measured_traffic(roof["kernel"])
 
+    secondary = None
+    if not args.skip_extra:
+        secondary = {}
+        log("[bench] secondary: 2D FV1 Poisson")
+        secondary["poisson_2d_fv1"] = bench_poisson_2d(U, args.seed, args.steps)
+        for r_adj in (6, refs):
+            log(f"[bench] secondary: adjoint gradient {2 ** r_adj}^3")
+            secondary[f"adjoint_q1_{2 ** r_adj}"] = bench_adjoint(U, r_adj, args.seed, max(1, args.steps // 2))
+        log("[bench] secondary: tetrahedra 96^3 x 6")
+        secondary["tet_fv1_upwind_96"] = bench_tet(U, 5, args.seed, max(1, args.steps // 2))
+
     cpu = None
     if not args.skip_cpu:
         log(f"[bench] CPU baseline (oracle, 1 core) on {2**args.cpu_refs}^3")
@@ -320,6 +484,7 @@ def main():
             "lattice_spmv_algorithmic_bytes": lat_bytes,
             "smoother_dia_f32_ms": t_smooth * 1e3,
             "smoother_dia_f32_GBps": smooth_gbs,
+            "secondary": secondary,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -297,10 +297,16 @@ struct LevelGrid {
     int64_t n_inner = 0;  // DoFs [0, n_inner) are not Dirichlet
     DeviceBuffer<int32_t> dof_of_lat, lat_of_dof;
     DeviceBuffer<uint8_t> dir;   // Dirichlet flag per lattice point
+    // kind 2: 6 packed tetrahedra per cell; parent-edge code per lattice
+    // point (levels >= 1)
+    DeviceBuffer<uint16_t> cell_tets;
+    DeviceBuffer<uint8_t> pcode;
 };
 
 // Domain of lattice topology: kind 0 = unit cube from one hexahedron (Q1 FE,
-// -div(k grad u) = f), kind 1 = unit square from two triangles (FV1).
+// -div(k grad u) = f), kind 1 = unit square from two triangles (FV1),
+// kind 2 = unit cube as 3x3x3 hexahedra x 6 tetrahedra, regularly refined
+// (FV1 convection-diffusion -div(k grad u) + div(v u) = f, full upwind).
 class DomainDiscretization {
    public:
     // ordering 0: the reference's LEVEL numbering (lib_grid creation order,
@@ -309,7 +315,11 @@ class DomainDiscretization {
     // lib_disc/ordering_strategies)
     DomainDiscretization(int kind, int num_refs, int ordering = 0);
     int kind() const { return kind_; }
-    int dim() const { return kind_ == 0 ? 3 : 2; }
+    int dim() const { return kind_ == 1 ? 2 : 3; }
+    // level operators symmetric on vectors vanishing on Dirichlet DoFs
+    bool symmetric() const { return kind_ != 2; }
+    // fine elements (kind 0/1) or fine lattice cells (kind 2) per side
+    int fine_cells() const { return (kind_ == 2 ? 3 : 1) << refs_; }
     int num_levels() const { return (int)levels_.size(); }
     int top_level() const { return num_levels() - 1; }
     LevelGrid& level(int l) { return *levels_[l]; }
@@ -330,8 +340,22 @@ class DomainDiscretization {
     // top-level vector (lagrange_dirichlet_boundary_impl.h:920-1035)
     void adjust_defect(GPUVector& d);
     // adjoint gradient g_e = -lambda_e^T (dA_e/dtheta_e) u_e per fine element
-    // (kind 0); grad_dev holds n_fine^3 doubles
+    // (kind 0) or fine cell (kind 2); grad_dev holds fine_cells()^3 doubles
     void adjoint_gradient(const GPUVector& u, const GPUVector& lambda, double* grad_dev);
+    // kind 2: convection velocity; jitter of the top-level interior vertices
+    // by amp * h * (2U - 1) per component (U uniform from the seeded counter
+    // generator, counter 3 * lattice index + component)
+    void set_velocity(double vx, double vy, double vz);
+    void set_jitter(uint64_t seed, double amp);
+    // operator of the adjoint solve: AT = (A with the Dirichlet columns of its
+    // interior rows zeroed)^T, identity Dirichlet rows, on A's pattern (which
+    // must be structurally symmetric); A from assemble_linear
+    void assemble_adjoint_operator(const GPUSparseMatrix& A, GPUSparseMatrix& AT);
+    const double* velocity() const { return vel_; }
+    uint64_t jitter_seed() const { return jseed_; }
+    double jitter_amp() const { return jamp_; }
+    double source_constant() const { return f_const_; }
+    int num_refs() const { return refs_; }
 
    private:
     int kind_;
@@ -340,6 +364,9 @@ class DomainDiscretization {
     DeviceBuffer<double> theta_, f_dof_;
     bool has_theta_ = false, has_f_ = false;
     double f_const_ = 1.0;
+    double vel_[3] = {0.0, 0.0, 0.0};
+    uint64_t jseed_ = 0;
+    double jamp_ = 0.0;
     double t_assemble_ = 0.0;
 };
 
@@ -356,6 +383,10 @@ class AssembledMultiGridCycle : public ILinearIterator {
     void set_cycle_type(int c) { cycle_ = c; }
     void set_smoother_damp(double w) { omega_ = w; }
     void set_precision(Precision p) { prec_ = p; }
+    // precondition the adjoint operator: level operators A_l,elim^T (the
+    // transposed Dirichlet-eliminated level matrices), transfer unchanged
+    void set_transposed(bool t) { transposed_ = t; }
+    bool transposed() const { return transposed_; }
     bool init(std::shared_ptr<MatrixOperator> J) override;
     bool apply(GPUVector& c, const GPUVector& d) override;
     // testing access: level matrix (FP64 DIA values in lattice numbering)
@@ -385,6 +416,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     int base_lev_ = 0, nu1_ = 2, nu2_ = 2, cycle_ = 1;
     double omega_ = 0.66;
     Precision prec_ = Precision::F64;
+    bool transposed_ = false;
     Impl* impl_ = nullptr;
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;

@@ -83,15 +83,31 @@ int ugx_jacobi_apply_f64(int64_t n, double* c, const double* dinv, const double*
 /* Host-only (no device use): LEVEL DoF numbering of a structured hierarchy as
  * produced by lib_grid's GlobalMultiGridRefiner + DoFDistribution
  * (global_multi_grid_refiner.cpp:135-455, dof_distribution.cpp:1401-1421).
- * out_host receives (2^level+1)^dim entries: lattice point (x fastest) -> DoF.
+ * kind 0: unit cube from one hexahedron, kind 1: unit square from two
+ * triangles, kind 2: unit cube as 3x3x3 hexahedra x 6 tetrahedra (regular
+ * tetrahedron rule, tetrahedron_rules.cpp:487-609).
+ * out_host receives W^dim entries, W = 2^level+1 (kind 2: 3*2^level+1):
+ * lattice point (x fastest) -> DoF.
  * n_inner receives the number of DoFs of subset "Inner" (the rest are Dirichlet). */
 int ugx_hierarchy_numbering(int kind, int num_refs, int level, int32_t* out_host, int64_t* n_inner);
+/* Host-only: the tetrahedra of level `level` of the kind-2 hierarchy, per
+ * lattice cell (x fastest) 6 entries in creation order, each packing the 4
+ * cell-corner indices (bit 0 x, bit 1 y, bit 2 z) of the tetrahedron's local
+ * vertex order as c0 | c1<<3 | c2<<6 | c3<<9; parent_edge (level >= 1, may be
+ * NULL) receives per lattice vertex 0 (copy of a coarse vertex) or the
+ * direction code 1..13 of the coarse edge it halves. No reference counterpart:
+ * it exposes the refinement result the reference keeps inside lib_grid. */
+int ugx_hierarchy_tets(int num_refs, int level, uint16_t* cell_tets, uint8_t* parent_edge);
 
 /* structured domain + DoF distribution + DomainDiscretization
  * (lib_disc/spatial_disc/domain_disc_impl.h:469-948 for the assembly; the DoF
  * numbering follows lib_disc/dof_manager/dof_distribution.cpp:1401-1421).
  * kind 0: unit cube from one hexahedron, Q1 FE for -div(k grad u) = f;
- * kind 1: unit square from two triangles, FV1 box method. */
+ * kind 1: unit square from two triangles, FV1 box method;
+ * kind 2: unit cube as 3x3x3 hexahedra x 6 tetrahedra, FV1 for
+ * -div(k grad u) + div(v u) = f with full upwind convection
+ * (ConvectionDiffusion FV1 with ConvectionShapesFullUpwind, conv_shape.h:278-330),
+ * theta per fine lattice cell. */
 void* ugx_disc_create(int kind, int num_refs);
 /* ordering 0: the reference's numbering (as ugx_disc_create); 1: lexicographic
  * lattice order with Dirichlet DoFs last (OrderLex-style renumbering) */
@@ -112,6 +128,19 @@ int ugx_disc_adjust_defect(void* disc, void* vec);
  * grad_dev[e] = -lambda_e^T (dA_e/dtheta_e) u_e for every fine element e
  * (lattice order, n_fine^3 doubles, device memory). Q1 hexahedra (kind 0). */
 int ugx_disc_adjoint_gradient(void* disc, void* u, void* lambda, double* grad_dev);
+/* kind 2: ConvectionDiffusion::set_velocity with a constant vector */
+int ugx_disc_set_velocity(void* disc, double vx, double vy, double vz);
+/* kind 2: jitter of the top-level interior vertices, amp * h * (2U-1) per
+ * component, U uniform from the seeded counter generator (counter 3 * lattice
+ * index + component); amp in [0, 0.5). No counterpart in the reference (a
+ * grid modification a script would make before assembling). */
+int ugx_disc_set_jitter(void* disc, uint64_t seed, double amp);
+/* operator of the adjoint solve (no counterpart in the reference; replaces
+ * SparseMatrix::set_as_transpose_of, sparsematrix_impl.h:148-183, applied to
+ * the Dirichlet-eliminated matrix): at = (A with Dirichlet columns of interior
+ * rows zeroed)^T with identity Dirichlet rows, on A's structurally symmetric
+ * pattern; a from ugx_disc_assemble_linear */
+int ugx_disc_adjoint_operator(void* disc, void* a, void* at);
 /* DomainDiscretization::assemble_linear(A, b) (domain_disc_impl.h:835-948) incl. Dirichlet rows */
 int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
 double ugx_disc_assemble_seconds(void* disc);
@@ -149,6 +178,9 @@ int ugx_vector_set(void* v, double value);
  * precision: 0 = FP64 level operators, 1 = FP32 level operators (base LU FP64). */
 void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision);
 void ugx_gmg_destroy(void* g);
+/* before ugx_gmg_init: 1 = precondition the adjoint operator (level operators
+ * A_l,elim^T, transfer unchanged); no counterpart in the reference */
+int ugx_gmg_set_transposed(void* g, int transposed);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

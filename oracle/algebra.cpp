@@ -181,6 +181,10 @@ double gauss_from_counter(uint64_t seed, uint64_t counter) {
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
+double uniform_from_counter(uint64_t seed, uint64_t counter) {
+    return unit_open(splitmix64(splitmix64(seed) ^ counter));
+}
+
 std::vector<double> make_theta(int n, uint64_t seed, int passes) {
     const int64_t N = (int64_t)n * n * n;
     std::vector<double> t(N), s(N);

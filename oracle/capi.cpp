@@ -21,6 +21,7 @@ struct Problem {
     std::vector<CSR> A, P, R;
     std::vector<std::vector<double>> b;
     std::vector<double> x;
+    TetParams tp;                       // kind 2: velocity, jitter
     SolveResult last;
     double t_assemble = 0, t_setup = 0, t_solve = 0;
     std::string err;
@@ -51,7 +52,7 @@ void* orc_problem_create(int kind, int num_refs) {
         auto* p = new Problem();
         p->kind = kind;
         p->mg = build_hierarchy(kind, num_refs);
-        p->n_fine = 1 << num_refs;
+        p->n_fine = (kind == 2 ? 3 : 1) << num_refs;
         p->A.resize(num_refs + 1);
         p->P.resize(num_refs + 1);
         p->R.resize(num_refs + 1);
@@ -96,8 +97,9 @@ void orc_elements(void* h, int lev, int32_t* out) {
     auto* p = (Problem*)h;
     const Level& L = p->mg.levels[lev];
     if (p->mg.dim == 3) {
+        const int nv = p->kind == 2 ? 4 : 8;
         for (size_t e = 0; e < L.vols.size(); ++e)
-            for (int i = 0; i < 8; ++i) out[8 * e + i] = L.dof[L.vols[e][i]];
+            for (int i = 0; i < nv; ++i) out[nv * e + i] = L.dof[L.vols[e][i]];
     } else {
         for (size_t e = 0; e < L.faces.size(); ++e)
             for (int i = 0; i < 3; ++i) out[3 * e + i] = L.dof[L.faces[e][i]];
@@ -119,6 +121,23 @@ void orc_set_theta(void* h, const double* theta) {
     p->theta.assign(theta, theta + n);
 }
 
+// kind 2: convection velocity and top-level jitter (amplitude in units of h)
+void orc_set_tet_params(void* h, double vx, double vy, double vz, uint64_t jitter_seed, double jitter_amp) {
+    auto* p = (Problem*)h;
+    p->tp.vel[0] = vx;
+    p->tp.vel[1] = vy;
+    p->tp.vel[2] = vz;
+    p->tp.jitter_seed = jitter_seed;
+    p->tp.jitter_amp = jitter_amp;
+}
+
+// positions (jittered on the top level) per DoF of level lev, 3 doubles each
+void orc_dof_positions(void* h, int lev, double* out) {
+    auto* p = (Problem*)h;
+    const Level& L = p->mg.levels[lev];
+    for (int d = 0; d < L.n_dof; ++d) tet_vertex_position(p->mg, lev, L.vtx_of_dof[d], p->tp, out + 3 * d);
+}
+
 void orc_set_source_top(void* h, const double* f) {
     auto* p = (Problem*)h;
     const int top = (int)p->mg.levels.size() - 1;
@@ -137,6 +156,8 @@ int orc_assemble(void* h, int lev, double f_const) {
         double t0 = now();
         if (p->kind == 0) {
             assemble_q1_hex(p->mg, lev, k, f_const, p->A[lev], p->b[lev]);
+        } else if (p->kind == 2) {
+            assemble_fv1_tet(p->mg, lev, p->theta, p->tp, f_const, p->A[lev], p->b[lev]);
         } else {
             const int top = (int)p->mg.levels.size() - 1;
             std::vector<double> f(p->mg.levels[lev].n_dof, 0.0);
@@ -285,8 +306,60 @@ void orc_adjoint_gradient(void* h, const double* u, const double* lambda, double
     const int top = (int)p->mg.levels.size() - 1;
     const int64_t n = p->mg.levels[top].n_dof;
     std::vector<double> uu(u, u + n), ll(lambda, lambda + n), g;
-    adjoint_gradient_q1_hex(p->mg, k, uu, ll, g);
+    if (p->kind == 2)
+        adjoint_gradient_fv1_tet(p->mg, p->theta, p->tp, uu, ll, g);
+    else
+        adjoint_gradient_q1_hex(p->mg, k, uu, ll, g);
     std::memcpy(g_out, g.data(), sizeof(double) * g.size());
+}
+
+// adjoint solve A^T lambda = rhs on the top level for an operator with
+// identity Dirichlet rows: lambda on the interior solves A_II^T lambda_I =
+// rhs_I, i.e. the transpose of the Dirichlet-eliminated operator (Dirichlet
+// columns of interior rows zeroed), with rhs Dirichlet entries zeroed.
+// GMG V(nu1,nu2) on the transposed eliminated level operators (P, R
+// unchanged), BiCGStab (solver 1) or CG (solver 0). Returns steps (<0 failure).
+int orc_solve_adjoint(void* h, int solver, const double* rhs, double* lambda_out, int base_lev, int nu1, int nu2,
+                      double omega, double rel_red, int max_steps) {
+    auto* p = (Problem*)h;
+    try {
+        const int top = (int)p->mg.levels.size() - 1;
+        GMG g;
+        g.base_lev = base_lev;
+        g.top_lev = top;
+        g.nu1 = nu1;
+        g.nu2 = nu2;
+        g.omega = omega;
+        g.A.resize(p->A.size());
+        for (int l = base_lev; l <= top; ++l) {
+            CSR E = p->A[l];
+            auto dm = dirichlet_mask(p->mg, l);
+            for (int64_t r = 0; r < E.n_rows; ++r) {
+                if (dm[r]) continue;
+                for (int64_t q = E.rowptr[r]; q < E.rowptr[r + 1]; ++q)
+                    if (dm[E.col[q]]) E.val[q] = 0.0;
+            }
+            g.A[l] = transpose_keep_zeros(E);
+        }
+        g.P = p->P;
+        g.R = p->R;
+        g.init();
+        const int64_t n = p->A[top].n_rows;
+        auto dm = dirichlet_mask(p->mg, top);
+        std::vector<double> b(rhs, rhs + n), x(n, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            if (dm[i]) b[i] = 0.0;
+        ConvCheck cc;
+        cc.max_steps = max_steps;
+        cc.rel_reduction = rel_red;
+        auto r = solver == 0 ? cg_solve(g.A[top], &g, x, b, cc) : bicgstab_solve(g.A[top], &g, x, b, cc);
+        std::memcpy(lambda_out, x.data(), sizeof(double) * n);
+        p->last = r;
+        return r.ok ? r.steps : -1 - r.steps;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100000;
+    }
 }
 
 void orc_solution_get(void* h, double* out) {

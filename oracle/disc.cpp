@@ -234,6 +234,216 @@ void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
         }
 }
 
+// ---------------------------------------------------------------------------
+// FV1 on tetrahedra (fv1_geom.cpp:258-463 with the ReferenceTetrahedron tables
+// of reference_element.cpp:547-760): one scvf per edge i with corners
+// [edge midpoint, centre of face id(1,i,2,0), element centre, centre of face
+// id(1,i,2,1)] (fv1_geom.cpp:118-150), normal = ElementNormal of that
+// quadrilateral, 0.5*(c2-c0)x(c3-c1) (geometry_util.h:579-586), oriented from
+// edge vertex 0 ("from") to vertex 1 ("to"); integration point = average of
+// the four corners. scv of corner i: hexahedron [corner, mid(edge id(0,i,1,1)),
+// face id(0,i,2,0), mid(edge id(0,i,1,0)), mid(edge id(0,i,1,2)), face
+// id(0,i,2,2), centre, face id(0,i,2,1)] (fv1_geom.cpp:156-198), volume by
+// ElementSize<ReferenceHexahedron> (geometry_util.h, two prisms).
+// Diffusion flux k grad(phi_j).n subtracted from `from`, added to `to`;
+// full upwind convection (conv_shape.h:278-330): up = flux >= 0 ? from : to,
+// shape(up) = v.n, added to `from`, subtracted from `to`.
+// ---------------------------------------------------------------------------
+namespace {
+
+const int RT_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+const int RT_FACE[4][3] = {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}};
+const int RT_EDGE_FACE[6][2] = {{0, 3}, {0, 1}, {0, 2}, {3, 2}, {1, 3}, {2, 1}};
+const int RT_POINT_EDGE[4][3] = {{2, 0, 3}, {0, 1, 4}, {1, 2, 5}, {3, 4, 5}};
+const int RT_POINT_FACE[4][3] = {{0, 2, 3}, {0, 3, 1}, {0, 1, 2}, {3, 2, 1}};
+
+void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double hex_volume(const double (*x)[3]) {
+    double x50[3], x14[3], x10[3], x20[3], x60[3], x24[3], x30[3], x70[3], m1[3], n1[3], m2[3], n2[3];
+    for (int d = 0; d < 3; ++d) {
+        x50[d] = x[5][d] - x[0][d];
+        x14[d] = x[1][d] - x[4][d];
+        x10[d] = x[1][d] - x[0][d];
+        x20[d] = x[2][d] - x[0][d];
+        x60[d] = x[6][d] - x[0][d];
+        x24[d] = x[2][d] - x[4][d];
+        x30[d] = x[3][d] - x[0][d];
+        x70[d] = x[7][d] - x[0][d];
+    }
+    cross3(x50, x14, m1);
+    cross3(x10, x20, n1);
+    cross3(x60, x24, m2);
+    cross3(x20, x30, n2);
+    double s1 = 0, s2 = 0;
+    for (int d = 0; d < 3; ++d) {
+        s1 += (n1[d] + m1[d]) * x60[d];
+        s2 += (n2[d] + m2[d]) * x70[d];
+    }
+    return (1.0 / 6.0) * (s1 + s2);
+}
+
+// element matrix (diffusion with kip[scvf] + convection vel; conv=false drops
+// the convection) and scv volumes of a tetrahedron with corners X
+void fv1_tet_element(const double (*X)[3], const double* kip, const double* vel, bool conv, double Ke[4][4],
+                     double vol[4]) {
+    double M[6][3], F[4][3], C[3];
+    for (int e = 0; e < 6; ++e)
+        for (int d = 0; d < 3; ++d) M[e][d] = (X[RT_EDGE[e][0]][d] + X[RT_EDGE[e][1]][d]) * 0.5;
+    for (int f = 0; f < 4; ++f)
+        for (int d = 0; d < 3; ++d) F[f][d] = (X[RT_FACE[f][0]][d] + X[RT_FACE[f][1]][d] + X[RT_FACE[f][2]][d]) / 3.0;
+    for (int d = 0; d < 3; ++d) C[d] = (X[0][d] + X[1][d] + X[2][d] + X[3][d]) * 0.25;
+    // P1 gradients: grad phi_c = row c-1 of J^{-1} (c = 1..3), grad phi_0 = -sum
+    double J[3][3], Ji[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J[r][c] = X[c + 1][r] - X[0][r];
+    inv3(J, Ji);
+    double grad[4][3];
+    for (int c = 1; c < 4; ++c)
+        for (int d = 0; d < 3; ++d) grad[c][d] = Ji[c - 1][d];
+    for (int d = 0; d < 3; ++d) grad[0][d] = -(grad[1][d] + grad[2][d] + grad[3][d]);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Ke[i][j] = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        const double* c0 = M[i];
+        const double* c1 = F[RT_EDGE_FACE[i][0]];
+        const double* c3 = F[RT_EDGE_FACE[i][1]];
+        double a[3], b[3], n[3];
+        for (int d = 0; d < 3; ++d) {
+            a[d] = C[d] - c0[d];
+            b[d] = c3[d] - c1[d];
+        }
+        cross3(a, b, n);
+        for (int d = 0; d < 3; ++d) n[d] *= 0.5;
+        const int from = RT_EDGE[i][0], to = RT_EDGE[i][1];
+        for (int sh = 0; sh < 4; ++sh) {
+            const double flux = kip[i] * (grad[sh][0] * n[0] + grad[sh][1] * n[1] + grad[sh][2] * n[2]);
+            Ke[from][sh] -= flux;
+            Ke[to][sh] += flux;
+        }
+        if (conv) {
+            const double q = vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2];
+            const int up = q >= 0.0 ? from : to;
+            Ke[from][up] += q;
+            Ke[to][up] -= q;
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        double H[8][3];
+        const int* pe = RT_POINT_EDGE[i];
+        const int* pf = RT_POINT_FACE[i];
+        const double* src[8] = {X[i], M[pe[1]], F[pf[0]], M[pe[0]], M[pe[2]], F[pf[2]], C, F[pf[1]]};
+        for (int q = 0; q < 8; ++q)
+            for (int d = 0; d < 3; ++d) H[q][d] = src[q][d];
+        vol[i] = hex_volume(H);
+    }
+}
+
+// fine cell containing the integration point of scvf i of tetrahedron t on
+// level L (lattice positions, exact integer arithmetic: 48 * ip)
+int64_t tet_ip_cell(const Level& L, size_t t, int i, int64_t n_cells) {
+    int64_t c[3];
+    for (int d = 0; d < 3; ++d) {
+        auto P = [&](int q) { return L.vpos[L.vols[t][q]][d]; };
+        const int64_t sall = P(0) + P(1) + P(2) + P(3);
+        const int* f0 = RT_FACE[RT_EDGE_FACE[i][0]];
+        const int* f1 = RT_FACE[RT_EDGE_FACE[i][1]];
+        const int64_t s48 = 6 * (P(RT_EDGE[i][0]) + P(RT_EDGE[i][1])) + 4 * (P(f0[0]) + P(f0[1]) + P(f0[2])) +
+                            3 * sall + 4 * (P(f1[0]) + P(f1[1]) + P(f1[2]));
+        int64_t cc = s48 / 48;
+        if (cc >= n_cells) cc = n_cells - 1;
+        c[d] = cc;
+    }
+    return (c[2] * n_cells + c[1]) * n_cells + c[0];
+}
+
+}  // namespace
+
+void tet_vertex_position(const MultiGrid& mg, int lev, int v, const TetParams& tp, double x[3]) {
+    const Level& L = mg.levels[lev];
+    const double scale = 1.0 / (double)mg.finest_scale;
+    for (int d = 0; d < 3; ++d) x[d] = (double)L.vpos[v][d] * scale;
+    const int top = (int)mg.levels.size() - 1;
+    if (lev != top || tp.jitter_amp == 0.0 || L.vsub[v] != 0) return;
+    const int64_t W = mg.finest_scale + 1;
+    const int64_t l = (L.vpos[v][2] * W + L.vpos[v][1]) * W + L.vpos[v][0];
+    for (int d = 0; d < 3; ++d) {
+        const double u = uniform_from_counter(tp.jitter_seed, (uint64_t)(3 * l + d));
+        x[d] += tp.jitter_amp * scale * (2.0 * u - 1.0);
+    }
+}
+
+void assemble_fv1_tet(const MultiGrid& mg, int lev, const std::vector<double>& theta_cells, const TetParams& tp,
+                      double f_const, CSR& A, std::vector<double>& b) {
+    const Level& L = mg.levels[lev];
+    const int64_t n_cells = mg.finest_scale;
+    RowBuilder B;
+    B.resize_and_clear(L.n_dof, L.n_dof);
+    b.assign(L.n_dof, 0.0);
+    for (size_t t = 0; t < L.vols.size(); ++t) {
+        if (L.volsub[t] != 0 || L.volnv[t] != 4) continue;
+        double X[4][3], kip[6], Ke[4][4], vol[4];
+        int ind[4];
+        for (int q = 0; q < 4; ++q) {
+            tet_vertex_position(mg, lev, L.vols[t][q], tp, X[q]);
+            ind[q] = L.dof[L.vols[t][q]];
+        }
+        for (int i = 0; i < 6; ++i)
+            kip[i] = theta_cells.empty() ? 1.0 : std::exp(theta_cells[tet_ip_cell(L, t, i, n_cells)]);
+        fv1_tet_element(X, kip, tp.vel, true, Ke, vol);
+        for (int i = 0; i < 4; ++i) {
+            for (int j = 0; j < 4; ++j) B.at(ind[i], ind[j]) += Ke[i][j];
+            b[ind[i]] += f_const * vol[i];
+        }
+    }
+    A = defragment(B);
+    auto dm = dirichlet_mask(mg, lev);
+    for (int64_t r = 0; r < A.n_rows; ++r)
+        if (dm[r]) {
+            set_row(A, r, 0.0);
+            double* d = find_entry(A, r, r);
+            if (!d) throw std::runtime_error("oracle: Dirichlet row without diagonal");
+            *d = 1.0;
+            b[r] = 0.0;
+        }
+}
+
+void adjoint_gradient_fv1_tet(const MultiGrid& mg, const std::vector<double>& theta_cells, const TetParams& tp,
+                              const std::vector<double>& u, const std::vector<double>& lambda,
+                              std::vector<double>& g) {
+    // on the top level every scvf integration point of a tetrahedron lies in
+    // its own fine cell, so dA_e/dtheta_cell = the diffusion part of A_e
+    const int top = (int)mg.levels.size() - 1;
+    const Level& L = mg.levels[top];
+    const int64_t n = mg.finest_scale;
+    g.assign(n * n * n, 0.0);
+    auto dm = dirichlet_mask(mg, top);
+    for (size_t t = 0; t < L.vols.size(); ++t) {
+        double X[4][3], kip[6], Ke[4][4], vol[4];
+        int ind[4];
+        for (int q = 0; q < 4; ++q) {
+            tet_vertex_position(mg, top, L.vols[t][q], tp, X[q]);
+            ind[q] = L.dof[L.vols[t][q]];
+        }
+        const int64_t cell = tet_ip_cell(L, t, 0, n);
+        const double k = theta_cells.empty() ? 1.0 : std::exp(theta_cells[cell]);
+        for (int i = 0; i < 6; ++i) kip[i] = k;
+        fv1_tet_element(X, kip, tp.vel, false, Ke, vol);
+        double acc = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            if (dm[ind[a]]) continue;
+            double row = 0.0;
+            for (int b2 = 0; b2 < 4; ++b2) row += Ke[a][b2] * u[ind[b2]];
+            acc += lambda[ind[a]] * row;
+        }
+        g[cell] -= acc;
+    }
+}
+
 CSR prolongation_p1(const MultiGrid& mg, int fine_lev) {
     const Level& F = mg.levels[fine_lev];
     const Level& C = mg.levels[fine_lev - 1];

@@ -45,6 +45,10 @@ const int HEX_EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
 const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
                             {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
 
+// tetrahedron_rules.h: EDGE_VRT_INDS, FACE_VRT_INDS
+const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+const int TET_FACE[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+
 uint64_t edge_key(int a, int b) {
     if (a > b) std::swap(a, b);
     return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
@@ -128,6 +132,34 @@ struct Builder {
         L.vol_edges.push_back(ve);
         return id;
     }
+    // Grid::register_volume with VOLOPT_AUTOGENERATE_FACES (tetrahedron):
+    // missing faces in local face order, created as Tetrahedron::create_face
+    // does (vertices f0, f2, f1; grid_objects_3d.cpp:361-368)
+    int register_tet(const int* t, int si) {
+        int id = (int)L.vols.size();
+        std::array<int, 8> tv{t[0], t[1], t[2], t[3], -1, -1, -1, -1};
+        L.vols.push_back(tv);
+        L.volnv.push_back(4);
+        L.volsub.push_back(si);
+        std::array<int, 6> vf{-1, -1, -1, -1, -1, -1};
+        for (int i = 0; i < 4; ++i) {
+            const int* f = TET_FACE[i];
+            int fd[3] = {t[f[0]], t[f[2]], t[f[1]]};
+            int fid = find_face(fd, 3);
+            if (fid < 0) fid = register_face(fd, 3, si);
+            vf[i] = fid;
+        }
+        L.vol_faces.push_back(vf);
+        std::array<int, 12> ve;
+        ve.fill(-1);
+        for (int i = 0; i < 6; ++i) {
+            int e = find_edge(t[TET_EDGE[i][0]], t[TET_EDGE[i][1]]);
+            if (e < 0) e = add_edge(t[TET_EDGE[i][0]], t[TET_EDGE[i][1]], si);
+            ve[i] = e;
+        }
+        L.vol_edges.push_back(ve);
+        return id;
+    }
 };
 
 std::array<int64_t, 3> avg(const Level& L, const int* v, int n) {
@@ -139,6 +171,57 @@ std::array<int64_t, 3> avg(const Level& L, const int* v, int n) {
         s[d] /= n;
     }
     return s;
+}
+
+// Regular refinement of a tetrahedron, rule STANDARD
+// (tetrahedron_rules.cpp:487-609): four corner children, then the inner
+// octahedron split along the shortest of the diagonals between the midpoints
+// of opposite edges (0-5, 1-3, 2-4), 2-4 unless another is strictly shorter
+// (1-3 checked first, then 0-5). Children are registered in rule order. The
+// corner positions are the refiner's projector geometry
+// (global_multi_grid_refiner.cpp:388-396); lattice coordinates make the
+// squared distances exact.
+void refine_tet(const Level& C, Builder& B, const std::vector<int>& cv, const std::vector<int>& me, int v) {
+    const int NV = 4;
+    int all[10];
+    for (int j = 0; j < 4; ++j) all[j] = cv[C.vols[v][j]];
+    for (int j = 0; j < 6; ++j) all[NV + j] = me[C.vol_edges[v][j]];
+    const int si = C.volsub[v];
+    auto P = [&](int j) { return C.vpos[C.vols[v][j]]; };
+    // squared distance between midpoints of (a,b) and (c,d), times 4
+    auto d2 = [&](int a, int b, int c, int d) {
+        int64_t s = 0;
+        for (int k = 0; k < 3; ++k) {
+            const int64_t x = (P(a)[k] + P(b)[k]) - (P(c)[k] + P(d)[k]);
+            s += x * x;
+        }
+        return s;
+    };
+    const int64_t d05 = d2(0, 1, 2, 3), d13 = d2(1, 2, 0, 3);
+    int64_t d = d2(0, 2, 1, 3);
+    int best = 2;
+    if (d13 < d) {
+        best = 1;
+        d = d13;
+    }
+    if (d05 < d) best = 0;
+    const int E = NV;
+    const int corner[4][4] = {{0, E + 0, E + 2, E + 3},
+                              {1, E + 1, E + 0, E + 4},
+                              {2, E + 2, E + 1, E + 5},
+                              {E + 3, E + 4, E + 5, 3}};
+    const int inner[3][4][4] = {
+        {{E + 0, E + 1, E + 2, E + 5}, {E + 1, E + 4, E + 5, E + 0}, {E + 2, E + 5, E + 3, E + 0}, {E + 0, E + 3, E + 4, E + 5}},
+        {{E + 0, E + 1, E + 2, E + 3}, {E + 1, E + 4, E + 5, E + 3}, {E + 2, E + 5, E + 3, E + 1}, {E + 0, E + 3, E + 4, E + 1}},
+        {{E + 0, E + 2, E + 3, E + 4}, {E + 1, E + 2, E + 0, E + 4}, {E + 2, E + 3, E + 4, E + 5}, {E + 4, E + 1, E + 2, E + 5}}};
+    for (auto& c : corner) {
+        int t[4] = {all[c[0]], all[c[1]], all[c[2]], all[c[3]]};
+        B.register_tet(t, si);
+    }
+    for (auto& c : inner[best]) {
+        int t[4] = {all[c[0]], all[c[1]], all[c[2]], all[c[3]]};
+        B.register_tet(t, si);
+    }
 }
 
 Level refine(const Level& C) {
@@ -184,7 +267,11 @@ Level refine(const Level& C) {
     }
     // 4. volumes (hexahedra, regular rule)
     for (size_t v = 0; v < C.vols.size(); ++v) {
-        if (C.volnv[v] != 8) throw std::runtime_error("oracle grid: only hexahedra supported");
+        if (C.volnv[v] == 4) {
+            refine_tet(C, B, cv, me, (int)v);
+            continue;
+        }
+        if (C.volnv[v] != 8) throw std::runtime_error("oracle grid: unsupported volume");
         int all[27];
         for (int j = 0; j < 8; ++j) all[j] = cv[C.vols[v][j]];
         for (int j = 0; j < 12; ++j) all[8 + j] = me[C.vol_edges[v][j]];
@@ -256,6 +343,58 @@ MultiGrid build_hierarchy(int kind, int num_refs) {
             bool diag = (std::min(a, b) == 0 && std::max(a, b) == 2);
             L0.esub[e] = diag ? 0 : 1;
         }
+    } else if (kind == 2) {
+        // unit cube as 3x3x3 hexahedra, each split into 6 Kuhn tetrahedra
+        // around its main diagonal; vertices in lexicographic order (x
+        // fastest), cells likewise, per cell the tetrahedra
+        // p0, p0+e_a, p0+e_a+e_b, p0+(1,1,1) for (a,b,c) in lexicographic
+        // permutation order. Odd permutations are stored as (P0,P3,P2,P1) so
+        // that every tetrahedron is positively oriented while its two face
+        // diagonals stay the local edges (0,2) and (1,3).
+        mg.dim = L0.dim = 3;
+        const int M = 3;
+        const int64_t s = S;
+        mg.finest_scale = M * S;
+        auto vid = [&](int i, int j, int k) { return (k * (M + 1) + j) * (M + 1) + i; };
+        for (int k = 0; k <= M; ++k)
+            for (int j = 0; j <= M; ++j)
+                for (int i = 0; i <= M; ++i) {
+                    const bool bnd = i == 0 || j == 0 || k == 0 || i == M || j == M || k == M;
+                    B.add_vertex({i * s, j * s, k * s}, bnd ? 1 : 0, -1, -1);
+                }
+        const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        const bool odd[6] = {false, true, true, false, false, true};
+        for (int k = 0; k < M; ++k)
+            for (int j = 0; j < M; ++j)
+                for (int i = 0; i < M; ++i)
+                    for (int p = 0; p < 6; ++p) {
+                        int c[3] = {i, j, k};
+                        int P[4];
+                        P[0] = vid(c[0], c[1], c[2]);
+                        for (int q = 0; q < 3; ++q) {
+                            c[perms[p][q]] += 1;
+                            P[q + 1] = vid(c[0], c[1], c[2]);
+                        }
+                        int t[4] = {P[0], P[1], P[2], P[3]};
+                        if (odd[p]) {
+                            t[1] = P[3];
+                            t[3] = P[1];
+                        }
+                        B.register_tet(t, 0);
+                    }
+        // boundary faces/edges: all vertices on one boundary plane
+        auto on_plane = [&](const int* v, int n) {
+            for (int ax = 0; ax < 3; ++ax)
+                for (int side = 0; side < 2; ++side) {
+                    const int64_t val = side ? M * s : 0;
+                    bool all = true;
+                    for (int q = 0; q < n; ++q) all = all && L0.vpos[v[q]][ax] == val;
+                    if (all) return true;
+                }
+            return false;
+        };
+        for (size_t f = 0; f < L0.faces.size(); ++f) L0.fsub[f] = on_plane(L0.faces[f].data(), 3) ? 1 : 0;
+        for (size_t e = 0; e < L0.edges.size(); ++e) L0.esub[e] = on_plane(L0.edges[e].data(), 2) ? 1 : 0;
     } else {
         throw std::runtime_error("oracle grid: unknown base grid kind");
     }

@@ -67,11 +67,12 @@ struct Level {
 struct MultiGrid {
     int dim = 3;
     int num_subsets = 2;           // 0 = "Inner", 1 = "Boundary"
-    int64_t finest_scale = 1;      // lattice units per base-element edge at the top level
+    int64_t finest_scale = 1;      // lattice units per unit length at the top level
     std::vector<Level> levels;
 };
 
-// kind: 0 = unit cube as 1 hexahedron; 1 = unit square as 2 triangles
+// kind: 0 = unit cube as 1 hexahedron; 1 = unit square as 2 triangles;
+// 2 = unit cube as 3x3x3 hexahedra x 6 Kuhn tetrahedra
 MultiGrid build_hierarchy(int kind, int num_refs);
 void number_dofs(Level& L, int num_subsets);
 
@@ -121,6 +122,8 @@ double norm(const std::vector<double>& a);                                // vec
 // ---------------------------------------------------------------------------
 uint64_t splitmix64(uint64_t x);
 double gauss_from_counter(uint64_t seed, uint64_t counter);
+// uniform in (0,1) from the same counter generator
+double uniform_from_counter(uint64_t seed, uint64_t counter);
 // theta per fine element of an n^3 element lattice, i.i.d. N(0,1) then `passes`
 // rounds of the 27-neighbour box average (clipped at the domain boundary)
 std::vector<double> make_theta(int n, uint64_t seed, int passes);
@@ -148,6 +151,27 @@ void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double 
 void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
                       const std::vector<double>& f_at_dof, CSR& A, std::vector<double>& b);
 std::vector<uint8_t> dirichlet_mask(const MultiGrid& mg, int lev);
+
+// FV1 convection-diffusion on the tetrahedral hierarchy (base grid kind 2):
+// -div(k grad u) + div(v u) = f, k = exp(theta[fine cell containing the
+// scvf integration point]) (the 6 tetrahedra of a fine cell share theta),
+// full upwind convection, interior vertices of the top level jittered by
+// jitter_amp * h * (2U - 1) per component (U from the seeded counter
+// generator, counter 3*lattice_index + component).
+struct TetParams {
+    double vel[3] = {0.0, 0.0, 0.0};
+    uint64_t jitter_seed = 0;
+    double jitter_amp = 0.0;
+};
+void assemble_fv1_tet(const MultiGrid& mg, int lev, const std::vector<double>& theta_cells, const TetParams& tp,
+                      double f_const, CSR& A, std::vector<double>& b);
+// g[cell] = -sum over the tetrahedra of the cell of lambda_e^T (dA_e/dtheta) u_e
+// (diffusion part, interior rows), fine cells in lattice order
+void adjoint_gradient_fv1_tet(const MultiGrid& mg, const std::vector<double>& theta_cells, const TetParams& tp,
+                              const std::vector<double>& u, const std::vector<double>& lambda,
+                              std::vector<double>& g);
+// position of vertex v of level lev (jittered on the top level)
+void tet_vertex_position(const MultiGrid& mg, int lev, int v, const TetParams& tp, double x[3]);
 // adjoint element gradient on the top level: g_e = -lambda_e^T (dA_e/dtheta_e) u_e
 // (interior rows only), g in fine-element lattice order
 void adjoint_gradient_q1_hex(const MultiGrid& mg, const Coefficient& k, const std::vector<double>& u,

@@ -55,6 +55,10 @@ def lib():
             "orc_defects_get": (C.c_int, [P, C.c_void_p, C.c_int]),
             "orc_solve_rhs": (C.c_int, [P, F64P, F64P, C.c_int, C.c_double, C.c_int]),
             "orc_adjoint_gradient": (None, [P, F64P, F64P, F64P]),
+            "orc_set_tet_params": (None, [P, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_double]),
+            "orc_dof_positions": (None, [P, C.c_int, F64P]),
+            "orc_solve_adjoint": (C.c_int, [P, C.c_int, F64P, F64P, C.c_int, C.c_int, C.c_int, C.c_double,
+                                            C.c_double, C.c_int]),
             "orc_timings": (None, [P, F64P]),
             "orc_builder_create": (P, [I64, I64]),
             "orc_builder_set": (None, [P, I64, I64, C.c_double]),
@@ -119,7 +123,8 @@ class CSR:
 
 class Problem:
     """Oracle problem: mesh hierarchy + level operators + solver (kind 0: unit
-    cube from one hexahedron, Q1 FE; kind 1: unit square from two triangles, FV1)."""
+    cube from one hexahedron, Q1 FE; kind 1: unit square from two triangles, FV1;
+    kind 2: unit cube from 3x3x3 hexahedra x 6 tetrahedra, FV1 convection-diffusion)."""
 
     def __init__(self, kind, num_refs):
         L = lib()
@@ -128,7 +133,9 @@ class Problem:
             raise RuntimeError(L.orc_last_error().decode())
         self.kind = kind
         self.num_refs = num_refs
-        self.dim = 3 if kind == 0 else 2
+        self.dim = 2 if kind == 1 else 3
+        # fine cells (elements for kind 0/1) per axis
+        self.n_fine = (3 if kind == 2 else 1) << num_refs
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -155,7 +162,7 @@ class Problem:
 
     def elements(self, lev):
         ne = lib().orc_num_elements(self.h, lev)
-        nv = 8 if self.kind == 0 else 3
+        nv = {0: 8, 1: 3, 2: 4}[self.kind]
         out = np.zeros(ne * nv, np.int32)
         lib().orc_elements(self.h, lev, out)
         return out.reshape(ne, nv)
@@ -233,10 +240,30 @@ class Problem:
         return (x, r) if return_steps else x
 
     def adjoint_gradient(self, u, lam):
-        g = np.zeros((2 ** self.num_refs) ** 3, np.float64)
+        g = np.zeros(self.n_fine ** 3, np.float64)
         lib().orc_adjoint_gradient(self.h, np.ascontiguousarray(u, np.float64),
                                    np.ascontiguousarray(lam, np.float64), g)
         return g
+
+    def set_tet_params(self, vel=(0.0, 0.0, 0.0), jitter_seed=0, jitter_amp=0.0):
+        lib().orc_set_tet_params(self.h, float(vel[0]), float(vel[1]), float(vel[2]), int(jitter_seed),
+                                 float(jitter_amp))
+
+    def dof_positions(self, lev):
+        n = self.num_dofs(lev)
+        out = np.zeros(3 * n, np.float64)
+        lib().orc_dof_positions(self.h, lev, out)
+        return out.reshape(n, 3)
+
+    def solve_adjoint(self, rhs, solver="bicgstab", base_lev=0, nu1=2, nu2=2, omega=0.6, rel_red=1e-12,
+                      max_steps=200):
+        top = self.num_levels - 1
+        lam = np.zeros(self.num_dofs(top), np.float64)
+        r = lib().orc_solve_adjoint(self.h, 0 if solver == "cg" else 1, np.ascontiguousarray(rhs, np.float64),
+                                    lam, base_lev, nu1, nu2, omega, rel_red, max_steps)
+        if r < 0:
+            raise RuntimeError("oracle adjoint solve failed")
+        return lam, r
 
     def timings(self):
         out = np.zeros(3, np.float64)

@@ -332,3 +332,62 @@ def test_theta_generators_agree():
     from ugcore_amd.synthetic import make_theta
     for n in (4, 8, 16):
         assert np.abs(make_theta(n, 9, 3) - O.make_theta(n, 9, 3)).max() < 1e-13
+
+
+def _tet_cells_from_oracle(p, lev):
+    """oracle tetrahedra of a level as per-cell lists of packed corner codes"""
+    refs = p.num_refs
+    step = 2 ** (refs - lev)
+    n = 3 * 2 ** lev
+    lat = p.dof_lattice(lev) // step
+    E = p.elements(lev)
+    cells = [[] for _ in range(n ** 3)]
+    for t in E:
+        q = lat[t]
+        mn = q.min(axis=0)
+        c = q - mn
+        code = 0
+        for j in range(4):
+            code |= int(c[j, 0] | (c[j, 1] << 1) | (c[j, 2] << 2)) << (3 * j)
+        cells[(mn[2] * n + mn[1]) * n + mn[0]].append(code)
+    return cells
+
+
+@pytest.mark.parametrize("refs", [1, 2, 3])
+def test_product_tet_hierarchy_matches_oracle(refs):
+    import ugcore_amd as U
+    p = O.Problem(2, refs)
+    p.assemble_all(1.0)
+    for lev in range(refs + 1):
+        W = 3 * 2 ** lev + 1
+        lat = p.dof_lattice(lev) // (2 ** (refs - lev))
+        expect = np.full(W ** 3, -1, np.int64)
+        expect[lattice_index(lat, W, 3)] = np.arange(len(lat))
+        got, n_inner = U.hierarchy_numbering(2, refs, lev)
+        assert np.array_equal(got, expect)
+        assert n_inner == int((p.dirichlet(lev) == 0).sum())
+        cells, par = U.hierarchy_tets(refs, lev)
+        ocells = _tet_cells_from_oracle(p, lev)
+        assert all(len(c) == 6 for c in ocells)
+        assert np.array_equal(cells, np.array(ocells, np.uint16))
+        if lev > 0:
+            # parent codes: lattice points with all-even coordinates are copies
+            lw = np.arange(W ** 3)
+            x, y, z = lw % W, (lw // W) % W, lw // (W * W)
+            copy = (x % 2 == 0) & (y % 2 == 0) & (z % 2 == 0)
+            assert np.all(par[copy] == 0) and np.all(par[~copy] > 0) and par.max() <= 13
+            # interior edge children: the code is the direction of the two
+            # coarse parents in the oracle's P
+            P = p.P(lev).to_scipy().tocsr()
+            latc = p.dof_lattice(lev - 1) // (2 ** (refs - lev + 1))
+            dm = p.dirichlet(lev).astype(bool)
+            for r in np.where(~dm)[0]:
+                cols = P.indices[P.indptr[r]:P.indptr[r + 1]]
+                li = lattice_index(lat[r:r + 1], W, 3)[0]
+                if len(cols) == 1:
+                    assert par[li] == 0
+                    continue
+                d = latc[cols[1]] - latc[cols[0]]
+                c = (d[0] + 1) + 3 * (d[1] + 1) + 9 * (d[2] + 1)
+                c = c if c > 13 else 26 - c
+                assert par[li] == c - 13

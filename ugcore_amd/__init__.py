@@ -49,6 +49,11 @@ _SIGNATURES = {
     "ugx_jacobi_diag_inv_f64": (_I, [_I64, _P, _P, _P, _D, _P, _P]),
     "ugx_jacobi_apply_f64": (_I, [_I64, _P, _P, _P, _P]),
     "ugx_hierarchy_numbering": (_I, [_I, _I, _I, _P, _P]),
+    "ugx_hierarchy_tets": (_I, [_I, _I, _P, _P]),
+    "ugx_disc_set_velocity": (_I, [_P, _D, _D, _D]),
+    "ugx_disc_set_jitter": (_I, [_P, C.c_uint64, _D]),
+    "ugx_disc_adjoint_operator": (_I, [_P, _P, _P]),
+    "ugx_gmg_set_transposed": (_I, [_P, _I]),
     "ugx_disc_create": (_P, [_I, _I]),
     "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
     "ugx_disc_destroy": (None, [_P]),
@@ -195,11 +200,24 @@ def time_launches(fn, reps, stream=None):
 def hierarchy_numbering(kind, num_refs, level):
     """LEVEL DoF numbering (lattice point -> DoF) computed on the host; returns
     (dof_of_lattice, n_inner)."""
-    dim = 3 if kind == 0 else 2
-    out = np.empty((2 ** level + 1) ** dim, np.int32)
+    dim = 2 if kind == 1 else 3
+    W = (3 if kind == 2 else 1) * 2 ** level + 1
+    out = np.empty(W ** dim, np.int32)
     ninner = C.c_int64(0)
     _check(lib().ugx_hierarchy_numbering(kind, num_refs, level, _ptr(out), C.byref(ninner)), "hierarchy_numbering")
     return out, ninner.value
+
+
+def hierarchy_tets(num_refs, level):
+    """Tetrahedra of level `level` of the kind-2 hierarchy (host): per lattice
+    cell 6 packed corner codes (creation order), and per lattice vertex the
+    parent-edge direction code (None on level 0)."""
+    n = 3 * 2 ** level
+    cells = np.empty(n ** 3 * 6, np.uint16)
+    par = np.empty((n + 1) ** 3, np.uint8) if level > 0 else None
+    _check(lib().ugx_hierarchy_tets(num_refs, level, _ptr(cells), _ptr(par) if par is not None else None),
+           "hierarchy_tets")
+    return cells.reshape(-1, 6), par
 
 
 class Vector:
@@ -298,13 +316,17 @@ class SparseMatrix:
 
 
 class DomainDiscretization:
-    """Structured domain (kind 0: unit cube from one hexahedron, Q1 FE;
-    kind 1: unit square from two triangles, FV1) with device assembly."""
+    """Structured domain with device assembly. kind 0: unit cube from one
+    hexahedron, Q1 FE; kind 1: unit square from two triangles, FV1; kind 2:
+    unit cube as 3x3x3 hexahedra x 6 tetrahedra, FV1 convection-diffusion with
+    full upwind (theta per fine lattice cell)."""
 
     def __init__(self, kind, num_refs, ordering="reference"):
         self.kind = kind
         self.num_refs = num_refs
-        self.dim = 3 if kind == 0 else 2
+        self.dim = 2 if kind == 1 else 3
+        # fine elements (kind 0/1) or fine lattice cells (kind 2) per side
+        self.n_fine = (3 if kind == 2 else 1) << num_refs
         o = {"reference": 0, "lex": 1}[ordering]
         self.h = _handle(lib().ugx_disc_create_ordered(int(kind), int(num_refs), o), "DomainDiscretization")
 
@@ -321,16 +343,32 @@ class DomainDiscretization:
             level = self.num_levels() - 1
         return lib().ugx_disc_num_dofs(self.h, level)
 
+    def lattice_width(self, level):
+        return (3 if self.kind == 2 else 1) * 2 ** level + 1
+
     def dof_of_lattice(self, level):
-        n = (2 ** level + 1) ** self.dim
+        n = self.lattice_width(level) ** self.dim
         out = np.empty(n, np.int32)
         _check(lib().ugx_disc_dof_of_lattice(self.h, level, _ptr(out)), "dof_of_lattice")
         return out
 
     def set_theta(self, theta):
         theta = np.ascontiguousarray(theta, np.float64)
-        assert theta.size == (2 ** self.num_refs) ** self.dim
+        if theta.size != self.n_fine ** self.dim:
+            raise ValueError("theta needs one value per fine element / cell")
         _check(lib().ugx_disc_set_theta(self.h, _ptr(theta)), "set_theta")
+
+    def set_velocity(self, v):
+        _check(lib().ugx_disc_set_velocity(self.h, float(v[0]), float(v[1]), float(v[2])), "set_velocity")
+
+    def set_jitter(self, seed, amp):
+        _check(lib().ugx_disc_set_jitter(self.h, int(seed), float(amp)), "set_jitter")
+
+    def adjoint_operator(self, A):
+        """matrix of the adjoint solve: (A, Dirichlet-eliminated)^T"""
+        AT = SparseMatrix()
+        _check(lib().ugx_disc_adjoint_operator(self.h, A.h, AT.h), "adjoint_operator")
+        return AT
 
     def set_source(self, f):
         if np.isscalar(f):
@@ -354,7 +392,7 @@ class DomainDiscretization:
     def adjoint_gradient(self, u, lam, grad_ptr=None):
         """g_e = -lambda_e^T (dA_e/dtheta_e) u_e per fine element; returns a
         numpy array unless a device pointer for the result is given."""
-        ne = (2 ** self.num_refs) ** 3
+        ne = self.n_fine ** 3
         if grad_ptr is not None:
             _check(lib().ugx_disc_adjoint_gradient(self.h, u.h, lam.h, grad_ptr), "adjoint_gradient")
             return None
@@ -368,11 +406,13 @@ class GeometricMultiGrid:
     Jacobi smoother, P1 transfer, dense FP64 LU on the base level.
     precision 'f64' or 'f32' for the level operators."""
 
-    def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64"):
+    def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64", transposed=False):
         self.disc = disc
         p = 1 if precision in ("f32", "float32", 1) else 0
         self.h = _handle(lib().ugx_gmg_create(disc.h, base_level, nu1, nu2, cycle, float(omega), p),
                          "GeometricMultiGrid")
+        if transposed:
+            _check(lib().ugx_gmg_set_transposed(self.h, 1), "GMG.set_transposed")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -390,7 +430,7 @@ class GeometricMultiGrid:
 
     def level_dia(self, level):
         nd = lib().ugx_gmg_level_num_diags(self.h, level)
-        n = (2 ** level + 1) ** self.disc.dim
+        n = self.disc.lattice_width(level) ** self.disc.dim
         out = np.empty(nd * n, np.float64)
         _check(lib().ugx_gmg_level_dia(self.h, level, _ptr(out)), "GMG.level_dia")
         return out.reshape(nd, n)

@@ -254,12 +254,24 @@ int ugx_jacobi_apply_f64(int64_t n, double* c, const double* dinv, const double*
 
 int ugx_hierarchy_numbering(int kind, int num_refs, int level, int32_t* out, int64_t* n_inner) {
     return guard([&] {
-        if (num_refs < 0 || num_refs > 10 || level < 0 || level > num_refs)
+        if (num_refs < 0 || num_refs > 10 || level < 0 || level > num_refs || kind < 0 || kind > 2)
             throw UGError("ugx_hierarchy_numbering: level out of range");
         auto nums = ugamd::build_level_numbering(kind, level);
         const auto& L = nums.back();
         std::copy(L.dof_of_lattice.begin(), L.dof_of_lattice.end(), out);
         if (n_inner) *n_inner = L.n_inner;
+        return 0;
+    });
+}
+
+int ugx_hierarchy_tets(int num_refs, int level, uint16_t* cell_tets, uint8_t* parent_edge) {
+    return guard([&] {
+        if (num_refs < 0 || num_refs > 7 || level < 0 || level > num_refs)
+            throw UGError("ugx_hierarchy_tets: level out of range");
+        auto nums = ugamd::build_level_numbering(2, level);
+        const auto& L = nums.back();
+        if (cell_tets) std::copy(L.cell_tets.begin(), L.cell_tets.end(), cell_tets);
+        if (parent_edge && level > 0) std::copy(L.parent_edge.begin(), L.parent_edge.end(), parent_edge);
         return 0;
     });
 }
@@ -314,6 +326,24 @@ int ugx_disc_adjoint_gradient(void* disc, void* u, void* lambda, double* grad) {
     return guard([&] {
         if (!grad) throw UGError("ugx_disc_adjoint_gradient: null output");
         ((DiscH*)disc)->d->adjoint_gradient(((VectorH*)u)->v, ((VectorH*)lambda)->v, grad);
+        return 0;
+    });
+}
+int ugx_disc_set_velocity(void* disc, double vx, double vy, double vz) {
+    return guard([&] {
+        ((DiscH*)disc)->d->set_velocity(vx, vy, vz);
+        return 0;
+    });
+}
+int ugx_disc_set_jitter(void* disc, uint64_t seed, double amp) {
+    return guard([&] {
+        ((DiscH*)disc)->d->set_jitter(seed, amp);
+        return 0;
+    });
+}
+int ugx_disc_adjoint_operator(void* disc, void* a, void* at) {
+    return guard([&] {
+        ((DiscH*)disc)->d->assemble_adjoint_operator(*((MatrixH*)a)->A, *((MatrixH*)at)->A);
         return 0;
     });
 }
@@ -428,6 +458,12 @@ int ugx_vector_set(void* v, double value) {
 
 // ---- GMG / Jacobi / solvers ---------------------------------------------------------------
 
+int ugx_gmg_set_transposed(void* g, int transposed) {
+    return guard([&] {
+        ((GmgH*)g)->g->set_transposed(transposed != 0);
+        return 0;
+    });
+}
 void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision) {
     return guard_ptr([&] {
         auto* h = new GmgH();

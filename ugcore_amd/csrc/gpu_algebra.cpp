@@ -94,6 +94,8 @@ template class DeviceBuffer<double>;
 template class DeviceBuffer<float>;
 template class DeviceBuffer<int32_t>;
 template class DeviceBuffer<uint8_t>;
+template class DeviceBuffer<uint16_t>;
+template class DeviceBuffer<uint32_t>;
 template class DeviceBuffer<int64_t>;
 
 // ---------------------------------------------------------------------------
@@ -460,8 +462,9 @@ bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
 // DomainDiscretization (structured)
 // ---------------------------------------------------------------------------
 DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering) : kind_(kind), refs_(num_refs) {
-    if (kind != 0 && kind != 1) throw UGError("DomainDiscretization: unknown domain kind");
-    if (num_refs < 0 || num_refs > (kind == 0 ? 9 : 13)) throw UGError("DomainDiscretization: refinement out of range");
+    if (kind < 0 || kind > 2) throw UGError("DomainDiscretization: unknown domain kind");
+    if (num_refs < 0 || num_refs > (kind == 0 ? 9 : kind == 1 ? 13 : 6))
+        throw UGError("DomainDiscretization: refinement out of range");
     auto nums = build_level_numbering(kind, num_refs);
     if (ordering == 1) {
         // lexicographic lattice order, inner DoFs first then Dirichlet DoFs
@@ -502,22 +505,79 @@ DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering)
         UGX_LAUNCH(invert_map(L->N, L->dof_of_lat.get(), L->lat_of_dof.get(), c.stream));
         L->dir.resize(L->N);
         UGX_LAUNCH(boundary_flags(L->dim, L->W, L->N, L->dir.get(), c.stream));
+        if (kind == 2) {
+            const int64_t cells = (int64_t)L->n * L->n * L->n;
+            if ((int64_t)ln.cell_tets.size() != cells * 6) throw UGError("DomainDiscretization: tetrahedra missing");
+            L->cell_tets.resize(cells * 6);
+            L->cell_tets.upload(ln.cell_tets.data(), cells * 6);
+            if (!ln.parent_edge.empty()) {
+                // every parent edge must join two coarse lattice points: the
+                // odd coordinates of the child are exactly the non-zero
+                // components of the direction
+                const int64_t W = L->W;
+                for (int64_t l = 0; l < L->N; ++l) {
+                    const int code = ln.parent_edge[l];
+                    const int64_t x = l % W, y = (l / W) % W, z = l / (W * W);
+                    if (code == 0) {
+                        if ((x | y | z) & 1) throw UGError("DomainDiscretization: inconsistent parent code");
+                        continue;
+                    }
+                    if (code > 13) throw UGError("DomainDiscretization: inconsistent parent code");
+                    const int d = code + 13;
+                    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+                    if ((x & 1) != (dx != 0) || (y & 1) != (dy != 0) || (z & 1) != (dz != 0))
+                        throw UGError("DomainDiscretization: inconsistent parent code");
+                }
+                L->pcode.resize(L->N);
+                L->pcode.upload(ln.parent_edge.data(), L->N);
+            }
+        }
         levels_.push_back(std::move(L));
     }
     sync();
 }
 
+void DomainDiscretization::set_velocity(double vx, double vy, double vz) {
+    vel_[0] = vx;
+    vel_[1] = vy;
+    vel_[2] = vz;
+}
+
+void DomainDiscretization::set_jitter(uint64_t seed, double amp) {
+    if (amp < 0.0 || amp >= 0.5) throw UGError("set_jitter: amplitude must lie in [0, 0.5)");
+    jseed_ = seed;
+    jamp_ = amp;
+}
+
+namespace {
+// kernel parameters of level `lev` of a tetrahedral discretisation
+ugamd::TetDisc tet_disc(const DomainDiscretization& D, const LevelGrid& L, int lev, const double* theta) {
+    ugamd::TetDisc P;
+    P.n = L.n;
+    P.shift = D.top_level() - lev;
+    P.n_fine = D.fine_cells();
+    P.scale = 1.0 / (double)P.n_fine;
+    P.theta = theta;
+    P.jseed = D.jitter_seed();
+    P.jamp = D.jitter_amp();
+    P.jitter = (lev == D.top_level() && D.jitter_amp() != 0.0) ? 1 : 0;
+    for (int d = 0; d < 3; ++d) P.vel[d] = D.velocity()[d];
+    P.f = D.source_constant();
+    return P;
+}
+}  // namespace
+
 void DomainDiscretization::set_theta(const double* theta_host) {
-    const int64_t nf = (int64_t)1 << refs_;
-    const int64_t n = kind_ == 0 ? nf * nf * nf : nf * nf;
+    const int64_t nf = fine_cells();
+    const int64_t n = kind_ == 1 ? nf * nf : nf * nf * nf;
     theta_.resize(n);
     theta_.upload(theta_host, n);
     has_theta_ = true;
 }
 
 void DomainDiscretization::set_theta_device(const double* theta_dev) {
-    const int64_t nf = (int64_t)1 << refs_;
-    const int64_t n = kind_ == 0 ? nf * nf * nf : nf * nf;
+    const int64_t nf = fine_cells();
+    const int64_t n = kind_ == 1 ? nf * nf : nf * nf * nf;
     theta_.resize(n);
     UGX_HIP(hipMemcpyAsync(theta_.get(), theta_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, context().stream));
     has_theta_ = true;
@@ -530,10 +590,16 @@ void DomainDiscretization::adjust_defect(GPUVector& d) {
 }
 
 void DomainDiscretization::adjoint_gradient(const GPUVector& u, const GPUVector& lambda, double* grad_dev) {
-    if (kind_ != 0) throw UGError("adjoint_gradient: implemented for the Q1 hexahedral discretisation");
+    if (kind_ == 1) throw UGError("adjoint_gradient: not implemented for the triangle discretisation");
     LevelGrid& L = *levels_.back();
     if (u.size() != L.N || lambda.size() != L.N) throw UGError("adjoint_gradient: size mismatch");
     if (!has_theta_) throw UGError("adjoint_gradient: theta not set");
+    if (kind_ == 2) {
+        UGX_LAUNCH(adjoint_grad_fv1_tet(tet_disc(*this, L, top_level(), theta_.get()), L.cell_tets.get(),
+                                        L.dof_of_lat.get(), L.n_inner, u.data(), lambda.data(), grad_dev,
+                                        context().stream));
+        return;
+    }
     UGX_LAUNCH(adjoint_grad_q1_hex(L.n, 1.0 / L.n, theta_.get(), L.dof_of_lat.get(), L.n_inner, u.data(),
                                    lambda.data(), grad_dev, context().stream));
 }
@@ -550,7 +616,15 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     const double t0 = wall();
     LevelGrid& L = *levels_.back();
     DeviceBuffer<int32_t> cnt(L.N);
-    UGX_LAUNCH(pattern_count(L.dim, L.W, L.N, L.lat_of_dof.get(), cnt.get(), c.stream));
+    DeviceBuffer<uint32_t> mask;
+    if (kind_ == 2) {
+        mask.resize(L.N);
+        mask.zero();
+        UGX_LAUNCH(tet_mask(L.n, L.cell_tets.get(), mask.get(), c.stream));
+        UGX_LAUNCH(pattern_count_mask(L.N, L.lat_of_dof.get(), mask.get(), cnt.get(), c.stream));
+    } else {
+        UGX_LAUNCH(pattern_count(L.dim, L.W, L.N, L.lat_of_dof.get(), cnt.get(), c.stream));
+    }
     std::vector<int32_t> h(L.N + 1, 0);
     cnt.download(h.data() + 1, L.N);
     int64_t acc = 0;
@@ -561,13 +635,22 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     }
     A.allocate(L.N, L.N, acc);
     UGX_HIP(hipMemcpyAsync(A.rowptr(), h.data(), sizeof(int32_t) * (L.N + 1), hipMemcpyHostToDevice, c.stream));
-    UGX_LAUNCH(pattern_fill(L.dim, L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), A.rowptr(), A.col(), c.stream));
+    if (kind_ == 2)
+        UGX_LAUNCH(pattern_fill_mask(L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), mask.get(), A.rowptr(), A.col(),
+                                     c.stream));
+    else
+        UGX_LAUNCH(pattern_fill(L.dim, L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), A.rowptr(), A.col(),
+                                c.stream));
     UGX_HIP(hipMemsetAsync(A.val(), 0, sizeof(double) * acc, c.stream));
     b.resize(L.N);
     b.set(0.0);
     const double hsz = 1.0 / L.n;
     const int nfine = 1 << refs_;
-    if (kind_ == 0) {
+    if (kind_ == 2) {
+        UGX_LAUNCH(assemble_fv1_tet<double>(tet_disc(*this, L, top_level(), theta_device()), L.cell_tets.get(),
+                                            L.dof_of_lat.get(), A.rowptr(), A.col(), A.val(), nullptr, b.data(),
+                                            c.stream));
+    } else if (kind_ == 0) {
         UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), A.rowptr(),
                                            A.col(), A.val(), nullptr, 0, b.data(), c.stream));
     } else {
@@ -580,6 +663,26 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     A.set_origin(this);
     A.finalize();
     t_assemble_ = wall() - t0;
+}
+
+void DomainDiscretization::assemble_adjoint_operator(const GPUSparseMatrix& A, GPUSparseMatrix& AT) {
+    auto& c = context();
+    LevelGrid& L = *levels_.back();
+    if (A.num_rows() != L.N || A.num_cols() != L.N) throw UGError("assemble_adjoint_operator: size mismatch");
+    const int64_t nnz = A.total_num_connections();
+    AT.allocate(L.N, L.N, nnz);
+    UGX_HIP(hipMemcpyAsync(AT.rowptr(), A.rowptr(), sizeof(int32_t) * (L.N + 1), hipMemcpyDeviceToDevice, c.stream));
+    UGX_HIP(hipMemcpyAsync(AT.col(), A.col(), sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, c.stream));
+    DeviceBuffer<int32_t> tp(nnz);
+    UGX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    UGX_LAUNCH(csr_transpose_map(L.N, A.rowptr(), A.col(), tp.get(), c.flag, c.stream));
+    UGX_LAUNCH(csr_adjoint_values(L.N, L.n_inner, A.rowptr(), A.col(), tp.get(), A.val(), AT.val(), c.stream));
+    sync();
+    int bad = 0;
+    UGX_HIP(hipMemcpy(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) throw UGError("assemble_adjoint_operator: pattern is not structurally symmetric");
+    AT.set_origin(nullptr);
+    AT.finalize();
 }
 
 void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd,
@@ -597,7 +700,11 @@ void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia
     dia.zero();
     const double hsz = 1.0 / L.n;
     const int nfine = 1 << refs_;
-    if (kind_ == 0) {
+    if (kind_ == 2) {
+        UGX_LAUNCH(assemble_fv1_tet<double>(tet_disc(*this, L, lev, theta_device()), L.cell_tets.get(),
+                                            L.dof_of_lat.get(), nullptr, nullptr, nullptr, dia.get(), nullptr,
+                                            c.stream));
+    } else if (kind_ == 0) {
         UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), nullptr,
                                            nullptr, nullptr, dia.get(), L.N, nullptr, c.stream));
     } else {
@@ -666,6 +773,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     DeviceBuffer<int32_t> lu_perm;
     int lu_n = 0;
     DeviceBuffer<double> surf;   // top-level operator, FP64 symmetric storage
+    bool sym_ok = true;          // the discretisation's level operators admit symmetric storage
+    bool transposed = false;     // level operators A_l,elim^T (adjoint)
+    bool tets = false;           // edge-based transfer of the tetrahedral hierarchy
     bool sym = true;             // level products through the symmetric storage
 
     // level products: symmetric storage when every level vector vanishes on
@@ -682,8 +792,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                                            c.stream);
     }
 
-    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w)
-        : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w) {}
+    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr)
+        : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
+          sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2), sym(d->symmetric()) {}
 
     void alloc_vec(DeviceBuffer<T>& buf, T*& p, int64_t N, int64_t H) {
         buf.resize(N + 2 * H);
@@ -707,6 +818,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             l.H = G.dim == 3 ? ((W * W + W + 1 + 63) / 64) * 64 : ((W + 1 + 63) / 64) * 64;
             DeviceBuffer<double> A64;
             disc->assemble_level_dia(lev, A64, l.nd, l.off);
+            if (transposed) {
+                if (l.nd != 27) throw UGError("GMG: transposed level operators need a 3D lattice");
+                DeviceBuffer<double> AT(A64.size());
+                UGX_LAUNCH(dia_transpose_elim<double>(l.W, l.N, l.dir, A64.get(), AT.get(), c.stream));
+                A64 = std::move(AT);
+            }
             alloc_vec(l.b_dinv, l.dinv, l.N, l.H);
             alloc_vec(l.b_sc, l.sc, l.N, l.H);
             alloc_vec(l.b_sd, l.sd, l.N, l.H);
@@ -767,6 +884,23 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                 lu_perm.resize(m);
                 lu_perm.upload(perm.data(), m);
             }
+            if (!sym_ok) {
+                // non-symmetric operator: full storage only
+                if constexpr (std::is_same<T, double>::value) {
+                    l.Afull = std::move(A64);
+                    UGX_HIP(hipMemcpyAsync(l.dinv, dinv64.get(), sizeof(double) * l.N, hipMemcpyDeviceToDevice,
+                                           c.stream));
+                } else {
+                    l.Afull.resize(A64.size());
+                    UGX_LAUNCH((convert<double, T>(A64.size(), A64.get(), l.Afull.get(), c.stream)));
+                    UGX_LAUNCH((convert<double, T>(l.N, dinv64.get(), l.dinv, c.stream)));
+                }
+                sync();
+                int bad = 0;
+                UGX_HIP(hipMemcpy(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost));
+                if (bad && lev > base) throw UGError("GMG: Jacobi found a zero diagonal");
+                continue;
+            }
             // symmetric Dirichlet-eliminated storage of the level operator
             const int nu = l.nd / 2 + 1;
             DeviceBuffer<double> U64(dia_tiled_size(l.N, nu));
@@ -812,14 +946,22 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             std::swap(f.sd, f.sd2);
         }
         UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
-        UGX_LAUNCH(restrict_defect<T>(f.dim, cl.W, f.W, cl.N, f.sd, cl.dir, f.dir, cl.sd, c.stream));
+        if (tets)
+            UGX_LAUNCH(restrict_tet<T>(cl.W, f.W, cl.N, f.sd, disc->level(lev).pcode.get(), cl.dir, f.dir, cl.sd,
+                                       c.stream));
+        else
+            UGX_LAUNCH(restrict_defect<T>(f.dim, cl.W, f.W, cl.N, f.sd, cl.dir, f.dir, cl.sd, c.stream));
     }
 
     void prolongation_and_postsmooth(int lev) {
         auto& c = context();
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
-        UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
+        if (tets)
+            UGX_LAUNCH(prolong_add_tet<T>(cl.W, f.W, f.N, cl.sc, disc->level(lev).pcode.get(), f.dir, f.st, f.sc,
+                                          c.stream));
+        else
+            UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
         if (nu2 == 0) {
             if (lev >= top) UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
             return;
@@ -863,18 +1005,19 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             sync();
             tail = c.red_host[0];
         }
-        sym = (tail == 0.0);
+        sym = sym_ok && (tail == 0.0);
         UGX_LAUNCH((permute_to<double, T>(t.N, G.dof_of_lat.get(), d.data(), t.sd, c.stream)));
         cvec.set(0.0);
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
         lmgc(top);
         UGX_LAUNCH((permute_add_from<T, double>(t.N, G.lat_of_dof.get(), t.sc, cvec.data(), c.stream)));
-        sym = true;
+        sym = sym_ok;
     }
 
     void apply_lattice(double* c_lat, const double* d_lat) override {
         auto& c = context();
         GmgLevel<T>& t = L[top];
+        if (!sym_ok) throw UGError("GMG::apply_lattice: needs a symmetric discretisation");
         sym = true;
         UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
@@ -895,6 +1038,13 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         // the Dirichlet-eliminated level matrix, lower half read from the
         // mirrored upper entries
         const GmgLevel<T>& l = L[lev];
+        if (!sym_ok) {
+            std::vector<T> h(l.Afull.size());
+            l.Afull.download(h.data(), (int64_t)h.size());
+            for (int d = 0; d < l.nd; ++d)
+                for (int64_t i = 0; i < l.N; ++i) out[(int64_t)d * l.N + i] = (double)h[dia_tiled_index(d, i, l.nd)];
+            return;
+        }
         const int nu = l.nd / 2 + 1;
         std::vector<T> h(l.A.size());
         l.A.download(h.data(), (int64_t)h.size());
@@ -928,7 +1078,8 @@ AssembledMultiGridCycle::AssembledMultiGridCycle(std::shared_ptr<DomainDiscretiz
 AssembledMultiGridCycle::~AssembledMultiGridCycle() { delete impl_; }
 
 bool AssembledMultiGridCycle::lattice_ready(const GPUSparseMatrix& A) const {
-    return impl_ && mat_ == &A && A.version() == mat_version_ && A.origin() == disc_.get();
+    return impl_ && !transposed_ && disc_->symmetric() && mat_ == &A && A.version() == mat_version_ &&
+           A.origin() == disc_.get();
 }
 int64_t AssembledMultiGridCycle::lattice_size() const { return disc_->level(disc_->top_level()).N; }
 int64_t AssembledMultiGridCycle::lattice_halo() const { return impl_ ? impl_->halo() : 0; }
@@ -951,8 +1102,10 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     mat_version_ = mat_->version();
     delete impl_;
     impl_ = nullptr;
-    if (prec_ == Precision::F64) impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_);
-    else impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_);
+    if (prec_ == Precision::F64)
+        impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_);
+    else
+        impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_);
     impl_->init();
     return true;
 }

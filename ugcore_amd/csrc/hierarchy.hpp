@@ -11,9 +11,26 @@ struct LevelNumbering {
     int64_t n_dof = 0;
     int64_t n_inner = 0;                    // DoFs [0, n_inner) are subset "Inner"
     std::vector<int32_t> dof_of_lattice;    // lattice vertex (x fastest) -> DoF
+    // kind 2 (tetrahedra): per lattice cell (x fastest) its 6 tetrahedra in
+    // creation order, each as 4 cell-corner indices (bit 0 x, bit 1 y, bit 2 z)
+    // in the tetrahedron's local vertex order, packed c0 | c1<<3 | c2<<6 | c3<<9
+    std::vector<uint16_t> cell_tets;
+    // kind 2, levels >= 1: per lattice vertex 0 = copy of a coarse vertex,
+    // otherwise 1..13 = direction code of the coarse edge it is the midpoint
+    // of (edge_direction_code)
+    std::vector<uint8_t> parent_edge;
 };
 
-// kind 0: unit cube from one hexahedron; kind 1: unit square from two triangles
+// direction code of a lattice edge vector d (components in {-1,0,1}, d != 0):
+// 1..13, equal for d and -d
+inline int edge_direction_code(int dx, int dy, int dz) {
+    int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    if (c < 13) c = 26 - c;
+    return c - 13;
+}
+
+// kind 0: unit cube from one hexahedron; kind 1: unit square from two triangles;
+// kind 2: unit cube as 3x3x3 hexahedra, each split into 6 tetrahedra
 std::vector<LevelNumbering> build_level_numbering(int kind, int num_refs);
 
 }  // namespace ugamd

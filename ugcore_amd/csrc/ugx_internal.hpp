@@ -104,4 +104,38 @@ int adjoint_grad_q1_hex(int n, double h, const double* theta, const int32_t* dof
                         const double* u, const double* lambda, double* grad, hipStream_t s);
 int zero_range(int64_t lo, int64_t hi, double* x, hipStream_t s);
 
+// ---- tetrahedral hierarchy (tet_kernels.hip) ----
+struct TetDisc {
+    int n = 0;                    // cells per side of the level
+    int shift = 0;                // fine lattice units per level lattice unit: 1 << shift
+    int n_fine = 0;               // fine cells per side
+    double scale = 0.0;           // coordinate per fine lattice unit (1 / n_fine)
+    const double* theta = nullptr;   // per fine cell (lattice order); null: k = 1
+    uint64_t jseed = 0;
+    double jamp = 0.0;            // jitter amplitude in units of the fine h
+    int jitter = 0;               // jitter the interior vertices (top level only)
+    double vel[3] = {0.0, 0.0, 0.0};
+    double f = 0.0;               // constant source
+};
+int tet_mask(int n, const uint16_t* cell_tets, uint32_t* mask, hipStream_t s);
+int pattern_count_mask(int64_t N, const int32_t* lat_of_dof, const uint32_t* mask, int32_t* cnt, hipStream_t s);
+int pattern_fill_mask(int W, int64_t N, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const uint32_t* mask,
+                      const int32_t* rowptr, int32_t* col, hipStream_t s);
+template <typename TM>
+int assemble_fv1_tet(const TetDisc& P, const uint16_t* cell_tets, const int32_t* dof_of_lat, const int32_t* rowptr,
+                     const int32_t* col, TM* csr_val, TM* dia, double* rhs, hipStream_t s);
+int adjoint_grad_fv1_tet(const TetDisc& P, const uint16_t* cell_tets, const int32_t* dof_of_lat, int64_t n_inner,
+                         const double* u, const double* lambda, double* grad, hipStream_t s);
+template <typename TV>
+int prolong_add_tet(int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* pcode, const uint8_t* dir_f, TV* st,
+                    TV* sc, hipStream_t s);
+template <typename TV>
+int restrict_tet(int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* pcode, const uint8_t* dir_c,
+                 const uint8_t* dir_f, TV* sdc, hipStream_t s);
+template <typename T>
+int dia_transpose_elim(int W, int64_t n, const uint8_t* dir, const T* A, T* AT, hipStream_t s);
+int csr_transpose_map(int64_t n, const int32_t* rowptr, const int32_t* col, int32_t* tp, int* bad, hipStream_t s);
+int csr_adjoint_values(int64_t n, int64_t n_inner, const int32_t* rowptr, const int32_t* col, const int32_t* tp,
+                       const double* val, double* valT, hipStream_t s);
+
 }  // namespace ugamd
