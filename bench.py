@@ -179,20 +179,20 @@ def bench_adjoint(U, refs, seed, reps, precision="f32"):
 
 
 def bench_tet(U, refs, seed, reps):
-    """96^3 x 6 tetrahedra, FV1 full upwind, v = 20*(1, 0.5, 0.25), jitter
-    +-0.2h, BiCGStab + GMG V(2,2) Jacobi 0.6 (FP32 and FP64 levels), tol 1e-10;
+    """96^3 x 6 tetrahedra, FV1 full upwind, v = 20*(1, 0.5, 0.25), interior
+    vertices displaced by at most 0.2h (+-0.2h/sqrt(3) per component), BiCGStab + GMG V(2,2) Jacobi 0.6 (FP32 and FP64 levels), tol 1e-10;
     adjoint solve with the transposed operator and transposed GMG"""
     from ugcore_amd.synthetic import make_theta
     d = U.DomainDiscretization(2, refs)
     d.set_theta(make_theta(d.n_fine, seed, 3))
     d.set_velocity((20.0, 10.0, 5.0))
-    d.set_jitter(11, 0.2)
+    d.set_jitter(11, 0.2 / np.sqrt(3.0))      # displacement length <= 0.2h
     d.set_source(1.0)
     N = d.num_dofs()
     A = U.SparseMatrix()
     b = U.Vector(N)
     d.assemble_linear(A, b)
-    res = {"workload": f"{d.n_fine}^3 x 6 tetrahedra FV1 convection-diffusion (full upwind, Pe 20, jitter 0.2h), "
+    res = {"workload": f"{d.n_fine}^3 x 6 tetrahedra FV1 convection-diffusion (full upwind, Pe 20, jitter |d| <= 0.2h), "
                        "BiCGStab + GMG V(2,2) Jacobi 0.6, tol 1e-10",
            "dofs": N, "nnz": A.nnz(), "assembly_s": d.assemble_seconds()}
     xs = {}
@@ -377,7 +377,8 @@ def main():
     nd = L.ugx_gmg_level_num_diags(g32.h, refs)
     nu_stored = nd // 2 + 1              # symmetric storage: centre + upper diagonals
     t_smooth = U.time_launches(lambda: g32.smooth_level(refs, 1), args.kernel_reps)
-    smooth_bytes = N * 4 * (nu_stored + 5)   # stored diagonals + sd in/out + dinv + sc read/write
+    # matrix-free top level: per-element coefficient + dinv + sd in/out + sc read/write
+    smooth_bytes = (n ** 3 + N * 5) * 4
 
     # the operator the CG iterates with: surface operator in lattice order,
     # FP64 symmetric storage
@@ -390,11 +391,28 @@ def main():
         assert L.ugx_gmg_surface_apply_lattice(g32.h, yl.ptr, xl_ptr) == 0
 
     t_lat = U.time_launches(lattice_spmv, args.kernel_reps)
-    lat_bytes = N * (8 * nu_stored + 8 + 8)
+    lat_bytes = n ** 3 * 8 + N * (8 + 8)   # matrix-free: per-element coefficient, x, y
     log(f"[bench] lattice surface SpMV f64 (symmetric DIA): {t_lat*1e3:.3f} ms  {lat_bytes/t_lat/1e9:.0f} GB/s")
     smooth_gbs = smooth_bytes / t_smooth / 1e9
     log(f"[bench] SpMV f64 ({layout}): {t_spmv*1e3:.3f} ms  {spmv_gbs:.0f} GB/s; "
         f"DIA Jacobi step f32: {t_smooth*1e3:.3f} ms  {smooth_gbs:.0f} GB/s")
+
+    # the same FP32-preconditioned solve with the stored top-level operator
+    # (LDS-staged symmetric diagonals) instead of the matrix-free one
+    g_st = U.GeometricMultiGrid(disc, base_level=args.base_level, nu1=2, nu2=2, cycle=1, omega=0.66,
+                                precision="f32", matrix_free=False)
+    S_st = U.CG(g_st, max_steps=100, reduction=1e-10)
+    S_st.init(A)
+    x_st = U.Vector(N)
+
+    def solve_stored():
+        x_st.set(0.0)
+        if not S_st.apply(x_st, b):
+            raise RuntimeError("stored-operator solve did not converge")
+    t32_stored, _ = _timed(U, solve_stored, args.steps, warmup=args.warmup)
+    t_smooth_stored = U.time_launches(lambda: g_st.smooth_level(refs, 1), args.kernel_reps)
+    del g_st, S_st, x_st
+    log(f"[bench] stored top-level operator: solve {t32_stored*1e3:.2f} ms, smoother {t_smooth_stored*1e3:.3f} ms")
 
     t32 = float(np.mean(times["f32"]))
     t64 = float(np.mean(times["f64"]))
@@ -406,10 +424,10 @@ def main():
     smooth_launches = 4 * (iters["f32"] + 1)
     op_launches = iters["f32"]
     t_op, op_bytes = (t_lat, lat_bytes) if lattice else (t_spmv, spmv_bytes)
-    op_kernel = "k_dias27_staged_axpy<double,double>" if lattice else (
+    op_kernel = "k_q1mf_z<double,2>" if lattice else (
         "k_sell_spmv<double,double>" if layout == "sell" else f"k_spmv_csr<double,double,{best_lanes}>")
     if smooth_launches * t_smooth >= op_launches * t_op:
-        roof = {"kernel": "k_dias27_staged_jacobi_step<float,float>", "bound": "hbm",
+        roof = {"kernel": "k_q1mf_z<float,0>", "bound": "hbm",
                 "achieved": smooth_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": smooth_gbs / HBM_PEAK_GBS,
                 "algorithmic_bytes_per_launch": smooth_bytes, "launch_ms": t_smooth * 1e3,
@@ -479,11 +497,14 @@ def main():
             "spmv_f64_GBps": spmv_gbs,
             "spmv_f64_frac_of_8TBps": spmv_gbs / HBM_PEAK_GBS,
             "cg_lattice_path": {p: bool(solvers[p][1].used_lattice_path()) for p in solvers},
-            "lattice_spmv_f64_sym_dia_ms": t_lat * 1e3,
-            "lattice_spmv_f64_sym_dia_GBps": lat_bytes / t_lat / 1e9,
+            "lattice_spmv_f64_matrix_free_ms": t_lat * 1e3,
+            "lattice_spmv_f64_matrix_free_GBps": lat_bytes / t_lat / 1e9,
             "lattice_spmv_algorithmic_bytes": lat_bytes,
-            "smoother_dia_f32_ms": t_smooth * 1e3,
-            "smoother_dia_f32_GBps": smooth_gbs,
+            "smoother_matrix_free_f32_ms": t_smooth * 1e3,
+            "smoother_matrix_free_f32_GBps": smooth_gbs,
+            "fp32_precond_stored_top_operator_s": t32_stored,
+            "smoother_stored_sym_dia_f32_ms": t_smooth_stored * 1e3,
+            "smoother_stored_sym_dia_f32_GBps": N * 4 * (nu_stored + 5) / t_smooth_stored / 1e9,
             "secondary": secondary,
         },
         "roofline": roof,

@@ -387,6 +387,10 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // transposed Dirichlet-eliminated level matrices), transfer unchanged
     void set_transposed(bool t) { transposed_ = t; }
     bool transposed() const { return transposed_; }
+    // apply the top-level operator of the hexahedral discretisation from its
+    // per-element coefficients (matrix-free) instead of the stored diagonals
+    // (default on; the stored level matrices are kept either way)
+    void set_matrix_free(bool on) { matrix_free_ = on; }
     bool init(std::shared_ptr<MatrixOperator> J) override;
     bool apply(GPUVector& c, const GPUVector& d) override;
     // testing access: level matrix (FP64 DIA values in lattice numbering)
@@ -417,6 +421,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     double omega_ = 0.66;
     Precision prec_ = Precision::F64;
     bool transposed_ = false;
+    bool matrix_free_ = true;
     Impl* impl_ = nullptr;
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;

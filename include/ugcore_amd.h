@@ -181,6 +181,11 @@ void ugx_gmg_destroy(void* g);
 /* before ugx_gmg_init: 1 = precondition the adjoint operator (level operators
  * A_l,elim^T, transfer unchanged); no counterpart in the reference */
 int ugx_gmg_set_transposed(void* g, int transposed);
+/* before ugx_gmg_init: 1 (default) = apply the top-level operator of the
+ * hexahedral discretisation from its per-element coefficients (matrix-free,
+ * same operator up to rounding), 0 = from the stored level matrix. No
+ * counterpart in the reference (a storage choice behind apply/apply_sub). */
+int ugx_gmg_set_matrix_free(void* g, int on);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

@@ -424,3 +424,33 @@ def test_defect_with_dirichlet_entries(kind, refs):
     assert not S.used_lattice_path()
     assert abs(S.steps() - steps) <= 1
     assert np.linalg.norm(x.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_matrix_free_top_level_matches_stored(precision):
+    """the matrix-free top-level operator (per-element coefficients) gives the
+    same cycle and the same CG solve as the stored level matrix"""
+    refs = 4
+    say(f"case matrix-free vs stored precision={precision}")
+    u = U()
+    p, theta, f = oracle_problem(0, refs)
+    d, A, b = gpu_problem(0, refs, theta, f)
+    rng = np.random.default_rng(11)
+    dv = rng.normal(size=d.num_dofs())
+    dv[p.dirichlet(refs) == 1] = 0.0
+    out = {}
+    for mf in (True, False):
+        g = u.GeometricMultiGrid(d, base_level=0, precision=precision, matrix_free=mf)
+        g.init(A)
+        c = u.Vector(len(dv))
+        g.apply(c, u.Vector(values=dv))
+        S = u.CG(g, max_steps=200, reduction=1e-10)
+        S.init(A)
+        x = u.Vector(d.num_dofs())
+        assert S.apply(x, b)
+        assert S.used_lattice_path()
+        out[mf] = (c.numpy(), S.steps(), x.numpy())
+    tol = 1e-13 if precision == "f64" else 1e-5
+    assert np.linalg.norm(out[True][0] - out[False][0]) <= tol * np.linalg.norm(out[False][0])
+    assert out[True][1] == out[False][1]
+    assert np.linalg.norm(out[True][2] - out[False][2]) <= 1e-11 * np.linalg.norm(out[False][2])

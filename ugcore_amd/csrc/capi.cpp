@@ -458,6 +458,12 @@ int ugx_vector_set(void* v, double value) {
 
 // ---- GMG / Jacobi / solvers ---------------------------------------------------------------
 
+int ugx_gmg_set_matrix_free(void* g, int on) {
+    return guard([&] {
+        ((GmgH*)g)->g->set_matrix_free(on != 0);
+        return 0;
+    });
+}
 int ugx_gmg_set_transposed(void* g, int transposed) {
     return guard([&] {
         ((GmgH*)g)->g->set_transposed(transposed != 0);

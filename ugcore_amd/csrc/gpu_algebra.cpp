@@ -776,25 +776,32 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     bool sym_ok = true;          // the discretisation's level operators admit symmetric storage
     bool transposed = false;     // level operators A_l,elim^T (adjoint)
     bool tets = false;           // edge-based transfer of the tetrahedral hierarchy
+    bool mf = false;             // matrix-free top-level operator (hexahedral kind)
+    DeviceBuffer<T> cell_c;      // per-element coefficients h k_e of the top level
+    DeviceBuffer<double> cell_c64;
     bool sym = true;             // level products through the symmetric storage
 
     // level products: symmetric storage when every level vector vanishes on
     // Dirichlet DoFs (the defect entering the cycle does), full otherwise
     int op_apply_sub(GmgLevel<T>& l, const T* x, const T* d_in, T* d_out) {
         auto& c = context();
+        if (sym && mf && &l == &L[top]) return q1mf_apply_sub<T>(l.W, cell_c.get(), x, d_in, d_out, c.stream);
         return sym ? dias_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.A.get(), x, d_in, d_out, c.stream)
                    : dia_apply_sub<T, T>(l.N, l.nd, l.off.data(), l.Afull.get(), x, d_in, d_out, c.stream);
     }
     int op_jacobi_step(GmgLevel<T>& l) {
         auto& c = context();
+        if (sym && mf && &l == &L[top])
+            return q1mf_jacobi_step<T>(l.W, cell_c.get(), l.dinv, l.sd, l.sd2, l.sc, c.stream);
         return sym ? dias_jacobi_step<T, T>(l.N, l.nd, l.off.data(), l.A.get(), l.dinv, l.sd, l.sd2, l.sc, c.stream)
                    : dia_jacobi_step<T, T>(l.N, l.nd, l.off.data(), l.Afull.get(), l.dinv, l.sd, l.sd2, l.sc,
                                            c.stream);
     }
 
-    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr)
+    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
-          sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2), sym(d->symmetric()) {}
+          sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2),
+          mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()) {}
 
     void alloc_vec(DeviceBuffer<T>& buf, T*& p, int64_t N, int64_t H) {
         buf.resize(N + 2 * H);
@@ -807,6 +814,16 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         if (base < 0 || base > top) throw UGError("GMG: base level out of range");
         L.clear();
         L.resize(top + 1);
+        if (mf) {
+            // top level: every quadrature point of element e sees k_e, so the
+            // element matrix is h k_e times the unit-cube Q1 stiffness
+            LevelGrid& G = disc->level(top);
+            const int64_t ne = (int64_t)G.n * G.n * G.n;
+            cell_c.resize(ne);
+            UGX_LAUNCH(q1mf_coeff<T>(ne, 1.0 / G.n, disc->theta_device(), cell_c.get(), c.stream));
+            cell_c64.resize(ne);
+            UGX_LAUNCH(q1mf_coeff<double>(ne, 1.0 / G.n, disc->theta_device(), cell_c64.get(), c.stream));
+        }
         for (int lev = base; lev <= top; ++lev) {
             LevelGrid& G = disc->level(lev);
             GmgLevel<T>& l = L[lev];
@@ -1027,6 +1044,10 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
 
     void surface_axpy(double* y, double alpha, const double* v, double beta, const double* x) override {
         GmgLevel<T>& t = L[top];
+        if (mf) {
+            UGX_LAUNCH(q1mf_axpy<double>(t.W, cell_c64.get(), alpha, v, beta, x, y, context().stream));
+            return;
+        }
         UGX_LAUNCH((dias_axpy<double, double>(t.N, t.nd, t.off.data(), surf.get(), alpha, v, beta, x, y,
                                               context().stream)));
     }
@@ -1103,9 +1124,9 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     delete impl_;
     impl_ = nullptr;
     if (prec_ == Precision::F64)
-        impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_);
+        impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_);
     else
-        impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_);
+        impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_);
     impl_->init();
     return true;
 }

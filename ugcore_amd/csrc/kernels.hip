@@ -577,9 +577,25 @@ __device__ __forceinline__ bool staged_block(const StagedMap& m, int64_t& k, int
 // S = R + 2W + 2
 template <typename T, typename F>
 __device__ __forceinline__ void stage3(T* lds, int64_t S, int64_t W2, int64_t first, F f) {
-    for (int dz = 0; dz < 3; ++dz) {
-        const int64_t b = first + (dz - 1) * W2;
-        for (int64_t t = threadIdx.x; t < S; t += blockDim.x) lds[dz * S + t] = f(b + t);
+    // the three planes are one flat index space of 3S entries; each thread
+    // issues its loads in batches of 8 independent ones before the stores
+    const int n3 = 3 * (int)S, Si = (int)S, step = blockDim.x;
+    constexpr int B = 8;
+    for (int t0 = threadIdx.x; t0 < n3; t0 += B * step) {
+        T v[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int t = t0 + q * step;
+            if (t < n3) {
+                const int dz = t >= 2 * Si ? 2 : (t >= Si ? 1 : 0);
+                v[q] = f(first + (int64_t)(dz - 1) * W2 + (t - dz * Si));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int t = t0 + q * step;
+            if (t < n3) lds[t] = v[q];
+        }
     }
 }
 
@@ -688,6 +704,329 @@ __global__ __launch_bounds__(256) void k_dias27_staged_jacobi_step(int64_t n, Di
         sd_out[i] = (TV)(sdi - s);
         sc[i] = (TV)((T)sc[i] + st);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free Q1 operator of a uniformly refined cube with a coefficient that
+// is constant per element (the top level of the hexahedral hierarchy):
+// A = sum_e c_e K_ref with c_e = h k_e and the unit-cube Q1 stiffness K_ref
+// (diagonal 1/3, face diagonals -1/12, body diagonal -1/12, couplings along
+// element edges 0), so for an interior row
+//   (A x)_i = sum_{8 cells e at i} c_e (x_i/3 - (x_f1 + x_f2 + x_f3 + x_b)/12)
+// with f1..f3 the face-diagonal and b the body-diagonal corner of e seen from
+// i. Boundary (Dirichlet) rows are identity rows. The products equal those of
+// the Dirichlet-eliminated stored operator on vectors vanishing on Dirichlet
+// DoFs, up to rounding (the assembled entries carry quadrature round-off).
+// Staging as in the symmetric kernels; the block's cells (planes k-1, k,
+// lines j0-1 .. j0+TY-1) are staged too.
+// ---------------------------------------------------------------------------
+struct MfMap {
+    int32_t W, TY;
+    int32_t glo[8], gsz[8];
+};
+
+template <typename T, typename F, typename O>
+__device__ __forceinline__ void q1mf_block(const MfMap& m, const T* __restrict__ cell, F stage_value, O out) {
+    StagedMap sm;
+    sm.W = m.W;
+    sm.TY = m.TY;
+    for (int c = 0; c < 8; ++c) {
+        sm.glo[c] = m.glo[c];
+        sm.gsz[c] = m.gsz[c];
+    }
+    int64_t k, j0, R64;
+    if (!staged_block(sm, k, j0, R64)) return;
+    const int W = m.W, n = W - 1, TY = m.TY, R = (int)R64;
+    const int64_t W2 = (int64_t)W * W;
+    const int S = R + 2 * W + 2;
+    const int64_t base = k * W2 + j0 * W;
+    T* lds = lds_base<T>();
+    T* cl = lds + 3 * S;                       // cells: [2][TY+1][n+2], x shifted by one
+    const int CX = n + 2;
+    stage3(lds, (int64_t)S, W2, base - W - 1, stage_value);
+    {
+        // cells: 2 (TY+1) lines of CX entries, loads batched as in stage3
+        const int nc = 2 * (TY + 1) * CX, step = blockDim.x;
+        constexpr int B = 8;
+        for (int t0 = threadIdx.x; t0 < nc; t0 += B * step) {
+            T v[B];
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int t = t0 + q * step;
+                v[q] = (T)0;
+                if (t < nc) {
+                    const int line = t / CX, cx = t - line * CX;
+                    const int64_t cz = k - 1 + line / (TY + 1), cy = j0 - 1 + line % (TY + 1);
+                    if (cz >= 0 && cz < n && cy >= 0 && cy < n && cx >= 1 && cx <= n)
+                        v[q] = cell[(cz * n + cy) * n + cx - 1];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int t = t0 + q * step;
+                if (t < nc) cl[t] = v[q];
+            }
+        }
+    }
+    __syncthreads();
+    const int a0 = (int)(base & 63);
+    const bool plane_bnd = (k == 0 || k == W - 1);
+    for (int rr = threadIdx.x; rr < R + a0; rr += blockDim.x) {
+        const int r = rr - a0;
+        if (r < 0) continue;
+        const int jl = (int)((unsigned)r / (unsigned)W), x = r - jl * W;
+        const int y = (int)j0 + jl;
+        // neighbour (dx,dy,dz) of local row r in the staged planes
+        const int c0 = r + W + 1;
+        auto X = [&](int dx, int dy, int dz) { return lds[(dz + 1) * S + c0 + dy * W + dx]; };
+        const T xi = X(0, 0, 0);
+        T s;
+        if (plane_bnd || x == 0 || x == W - 1 || y == 0 || y == W - 1) {
+            s = xi;
+        } else {
+            // cell (a,b,c) = element (x-1+a, y-1+b, k-1+c)
+            auto Cc = [&](int a, int b, int c) { return cl[(c * (TY + 1) + jl + b) * CX + x + a]; };
+            T ce[2][2][2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) ce[c][b][a] = Cc(a, b, c);
+            T csum = (T)0, off = (T)0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        csum += ce[c][b][a];
+                        off = fma(ce[c][b][a], X(2 * a - 1, 2 * b - 1, 2 * c - 1), off);   // body diagonal
+                    }
+            // face diagonals: each shared by the two cells of the element face
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    off = fma(ce[0][b][a] + ce[1][b][a], X(2 * a - 1, 2 * b - 1, 0), off);
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    off = fma(ce[c][0][a] + ce[c][1][a], X(2 * a - 1, 0, 2 * c - 1), off);
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    off = fma(ce[c][b][0] + ce[c][b][1], X(0, 2 * b - 1, 2 * c - 1), off);
+            s = csum * xi * (T)(1.0 / 3.0) - off * (T)(1.0 / 12.0);
+        }
+        out(base + r, xi, s);
+    }
+}
+
+// z-marching variant: a block owns TY lines of a chunk of ZC planes and walks
+// it plane by plane. The staged values of 4 planes live in an LDS ring (3 in
+// use, 1 being filled), the block's cells of 3 cell planes in a second ring;
+// the loads of plane k+2 are issued into registers before plane k is
+// computed and stored after it, so every plane is read once per block and
+// its load latency overlaps the arithmetic.
+struct MfZMap {
+    int32_t W, TY, ZC, nchunks;
+    int32_t glo[8], gsz[8];
+};
+constexpr int kMfPrefetch = 8;   // staged entries per thread and plane (P <= 8 * 256)
+
+constexpr int kMfRows = 5;       // rows per thread and plane (R + 63 <= 5 * 256)
+
+// MODE 0: fused Jacobi step (staged st = dinv.*sd; sd_out = sd - A st; sc += st)
+// MODE 1: d_out = d_in - A x;  MODE 2: y = alpha v + beta A x
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
+                                                const T* __restrict__ b_in, const T* c_in, T* out0, T* out1,
+                                                double alpha, double beta) {
+    // c_in and out0 may alias (d_out = d_in); every row is read and then
+    // written by the same thread
+    // MODE 0: a_in = dinv, b_in = sd_in, out0 = sd_out, out1 = sc
+    // MODE 1: a_in = x, c_in = d_in, out0 = d_out
+    // MODE 2: a_in = x, c_in = v, out0 = y
+    const int xc = blockIdx.x & 7, q = blockIdx.x >> 3, g = m.gsz[xc];
+    if (g == 0) return;
+    const int chunk = q / g;
+    if (chunk >= m.nchunks) return;
+    const int band = m.glo[xc] + (q - chunk * g);
+    const int W = m.W, n = W - 1, TY = m.TY;
+    const int j0 = band * TY, j1 = j0 + TY < W ? j0 + TY : W, R = (j1 - j0) * W;
+    const int k0 = chunk * m.ZC, k1 = k0 + m.ZC < W ? k0 + m.ZC : W;
+    const int P = R + 2 * W + 2;
+    const int CX = n + 2, CL = (TY + 1) * CX;
+    const int64_t W2 = (int64_t)W * W;
+    T* ring = lds_base<T>();
+    T* cring = ring + 4 * P;
+    const int tid = threadIdx.x, step = blockDim.x;
+    auto plane_first = [&](int p) { return (int64_t)p * W2 + (int64_t)(j0 - 1) * W - 1; };
+    auto cell_value = [&](int cz, int t) -> T {
+        const int line = t / CX, cx = t - line * CX;
+        const int cy = j0 - 1 + line;
+        if (cz < 0 || cz >= n || cy < 0 || cy >= n || cx < 1 || cx > n) return (T)0;
+        return cell[((int64_t)cz * n + cy) * n + cx - 1];
+    };
+    auto staged = [&](int64_t j) -> T { return MODE == 0 ? a_in[j] * b_in[j] : a_in[j]; };
+    for (int p = k0 - 1; p <= k0 + 1; ++p) {
+        T* dst = ring + (p & 3) * P;
+        const int64_t f0 = plane_first(p);
+        for (int t = tid; t < P; t += step) dst[t] = staged(f0 + t);
+    }
+    for (int cz = k0 - 1; cz <= k0; ++cz) {
+        T* dst = cring + ((cz + 3) % 3) * CL;
+        for (int t = tid; t < CL; t += step) dst[t] = cell_value(cz, t);
+    }
+    __syncthreads();
+    for (int k = k0; k < k1; ++k) {
+        const int64_t base = (int64_t)k * W2 + (int64_t)j0 * W;
+        const int a0 = (int)(base & 63);
+        // 1. issue every global load of this step: plane k+2 (raw), cell plane
+        //    k+1, and the row inputs of plane k
+        T pa[kMfPrefetch], pb[kMfPrefetch], cv[kMfPrefetch];
+        const int64_t f2 = plane_first(k + 2);
+        const bool more = (k + 1 < k1);
+#pragma unroll
+        for (int u = 0; u < kMfPrefetch; ++u) {
+            const int t = tid + u * step;
+            const bool ok = more && t < P;
+            pa[u] = ok ? a_in[f2 + t] : (T)0;
+            if (MODE == 0) pb[u] = ok ? b_in[f2 + t] : (T)0;
+            cv[u] = (more && t < CL) ? cell_value(k + 1, t) : (T)0;
+        }
+        T ra[kMfRows], rb[kMfRows];
+#pragma unroll
+        for (int u = 0; u < kMfRows; ++u) {
+            const int r = tid + u * step - a0;
+            const bool ok = r >= 0 && r < R;
+            if (MODE == 0) {
+                ra[u] = ok ? b_in[base + r] : (T)0;   // sd_in
+                rb[u] = ok ? out1[base + r] : (T)0;   // sc
+            } else if (MODE == 1) {
+                ra[u] = ok ? c_in[base + r] : (T)0;   // d_in
+            } else {
+                ra[u] = (ok && alpha != 0.0) ? c_in[base + r] : (T)0;   // v
+            }
+        }
+        // 2. the operator on this thread's rows of plane k
+        const T* sm = ring + ((k - 1) & 3) * P;
+        const T* s0 = ring + (k & 3) * P;
+        const T* sp = ring + ((k + 1) & 3) * P;
+        const T* cm = cring + ((k + 2) % 3) * CL;
+        const T* c0 = cring + (k % 3) * CL;
+        const bool plane_bnd = (k == 0 || k == W - 1);
+#pragma unroll
+        for (int u = 0; u < kMfRows; ++u) {
+            const int r = tid + u * step - a0;
+            if (r < 0 || r >= R) continue;
+            const int jl = (int)((unsigned)r / (unsigned)W), x = r - jl * W;
+            const int y = j0 + jl;
+            const int cc0 = r + W + 1;
+            auto X = [&](int dx, int dy, int dz) {
+                const T* pl = dz < 0 ? sm : (dz > 0 ? sp : s0);
+                return pl[cc0 + dy * W + dx];
+            };
+            const T xi = s0[cc0];
+            T sv;
+            if (plane_bnd || x == 0 || x == W - 1 || y == 0 || y == W - 1) {
+                sv = xi;
+            } else {
+                T ce[2][2][2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                        for (int aa = 0; aa < 2; ++aa) ce[c][bb][aa] = (c ? c0 : cm)[(jl + bb) * CX + x + aa];
+                T csum = (T)0, off = (T)0;
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                        for (int aa = 0; aa < 2; ++aa) {
+                            csum += ce[c][bb][aa];
+                            off = fma(ce[c][bb][aa], X(2 * aa - 1, 2 * bb - 1, 2 * c - 1), off);
+                        }
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                    for (int aa = 0; aa < 2; ++aa)
+                        off = fma(ce[0][bb][aa] + ce[1][bb][aa], X(2 * aa - 1, 2 * bb - 1, 0), off);
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int aa = 0; aa < 2; ++aa)
+                        off = fma(ce[c][0][aa] + ce[c][1][aa], X(2 * aa - 1, 0, 2 * c - 1), off);
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb)
+                        off = fma(ce[c][bb][0] + ce[c][bb][1], X(0, 2 * bb - 1, 2 * c - 1), off);
+                sv = csum * xi * (T)(1.0 / 3.0) - off * (T)(1.0 / 12.0);
+            }
+            // 3. outputs
+            const int64_t i = base + r;
+            if (MODE == 0) {
+                out0[i] = ra[u] - sv;
+                out1[i] = rb[u] + xi;
+            } else if (MODE == 1) {
+                out0[i] = ra[u] - sv;
+            } else {
+                out0[i] = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
+            }
+        }
+        // 4. the prefetched plane k+2 and cell plane k+1 into the rings
+        if (more) {
+            T* dst = ring + ((k + 2) & 3) * P;
+            T* cdst = cring + ((k + 1) % 3) * CL;
+#pragma unroll
+            for (int u = 0; u < kMfPrefetch; ++u) {
+                const int t = tid + u * step;
+                if (t < P) dst[t] = MODE == 0 ? pa[u] * pb[u] : pa[u];
+                if (t < CL) cdst[t] = cv[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_q1mf_jacobi_step(MfMap m, const T* __restrict__ cell,
+                                                          const T* __restrict__ dinv, const T* __restrict__ sd_in,
+                                                          T* __restrict__ sd_out, T* __restrict__ sc) {
+    q1mf_block<T>(
+        m, cell, [&](int64_t j) { return dinv[j] * sd_in[j]; },
+        [&](int64_t i, T st, T s) {
+            sd_out[i] = sd_in[i] - s;
+            sc[i] = sc[i] + st;
+        });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_q1mf_apply_sub(MfMap m, const T* __restrict__ cell, const T* __restrict__ x,
+                                                        const T* d_in, T* d_out) {
+    q1mf_block<T>(
+        m, cell, [&](int64_t j) { return x[j]; }, [&](int64_t i, T, T s) { d_out[i] = d_in[i] - s; });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_q1mf_axpy(MfMap m, const T* __restrict__ cell, double alpha, const T* v,
+                                                   double beta, const T* __restrict__ x, T* y) {
+    q1mf_block<T>(
+        m, cell, [&](int64_t j) { return x[j]; },
+        [&](int64_t i, T, T s) { y[i] = (T)((alpha == 0.0 ? 0.0 : alpha * (double)v[i]) + beta * (double)s); });
+}
+
+// c_e = h exp(theta_e) per element (lattice order of elements); theta null: k = 1
+template <typename T>
+__global__ void k_q1mf_coeff(int64_t ne, double h, const double* __restrict__ theta, T* __restrict__ c) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x)
+        c[e] = (T)(h * (theta ? exp(theta[e]) : 1.0));
 }
 
 // sc += dinv .* sd  (last post-smoothing step on levels below the top, where
@@ -1340,6 +1679,48 @@ static int64_t make_staged_map(int64_t n, int nd, const int32_t* off, size_t ele
     return 8 * (int64_t)maxg * W;
 }
 
+// z-marching plan: bands of TY lines grouped per XCD, chunks of ZC planes;
+// returns the grid (0 when the prefetch registers cannot hold a plane)
+static int64_t make_mf_zmap(int W, size_t elem, MfZMap& m, size_t& lds) {
+    m = MfZMap{};
+    m.W = W;
+    m.TY = elem == 8 ? 2 : 4;
+    m.ZC = 8;
+    m.nchunks = (W + m.ZC - 1) / m.ZC;
+    const int64_t P = (int64_t)m.TY * W + 2 * W + 2, CL = (int64_t)(m.TY + 1) * (W + 1);
+    if (P > (int64_t)kMfPrefetch * 256 || CL > (int64_t)kMfPrefetch * 256 || W < 8) return 0;
+    if ((int64_t)m.TY * W + 63 > (int64_t)kMfRows * 256) return 0;
+    const int64_t nb = (W + m.TY - 1) / m.TY;
+    int32_t maxg = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int64_t b0 = c * nb / 8, b1 = (c + 1) * nb / 8;
+        m.glo[c] = (int32_t)b0;
+        m.gsz[c] = (int32_t)(b1 - b0);
+        maxg = m.gsz[c] > maxg ? m.gsz[c] : maxg;
+    }
+    lds = (size_t)(4 * P + 3 * CL) * elem;
+    if (lds > 64 * 1024) return 0;
+    return 8 * (int64_t)maxg * m.nchunks;
+}
+
+// matrix-free launch plan: bands of 4 lines grouped per XCD (any W >= 3)
+static int64_t make_mf_map(int W, size_t elem, MfMap& m, size_t& lds) {
+    m = MfMap{};
+    m.W = W;
+    m.TY = 4;
+    const int64_t nb = (W + m.TY - 1) / m.TY;
+    int32_t maxg = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int64_t b0 = c * nb / 8, b1 = (c + 1) * nb / 8;
+        m.glo[c] = (int32_t)b0;
+        m.gsz[c] = (int32_t)(b1 - b0);
+        maxg = m.gsz[c] > maxg ? m.gsz[c] : maxg;
+    }
+    const int64_t S = (int64_t)m.TY * W + 2 * W + 2;
+    lds = (size_t)(3 * S + 2 * (int64_t)(m.TY + 1) * (W + 1)) * elem;
+    return 8 * (int64_t)maxg * W;
+}
+
 #define UGX_DIA_DISPATCH(KERNEL, ...)                                                                      \
     do {                                                                                                  \
         RowMap rm;                                                                                        \
@@ -1417,6 +1798,70 @@ int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const T
 }
 template int dias_jacobi_step<double, double>(int64_t, int, const int32_t*, const double*, const double*, const double*, double*, double*, hipStream_t);
 template int dias_jacobi_step<float, float>(int64_t, int, const int32_t*, const float*, const float*, const float*, float*, float*, hipStream_t);
+
+template <typename T>
+int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s) {
+    MfZMap zm;
+    size_t zlds;
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+        hipLaunchKernelGGL((k_q1mf_z<T, 0>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
+                           sd_out, sc, 0.0, 0.0);
+        UGX_CHECK_LAUNCH();
+    }
+    MfMap m;
+    size_t lds;
+    const int64_t grid = make_mf_map(W, sizeof(T), m, lds);
+    if (lds > 64 * 1024) return -1;
+    hipLaunchKernelGGL(k_q1mf_jacobi_step<T>, dim3(grid), dim3(256), lds, s, m, cell, dinv, sd_in, sd_out, sc);
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_jacobi_step<float>(int, const float*, const float*, const float*, float*, float*, hipStream_t);
+template int q1mf_jacobi_step<double>(int, const double*, const double*, const double*, double*, double*, hipStream_t);
+
+template <typename T>
+int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s) {
+    MfZMap zm;
+    size_t zlds;
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+        hipLaunchKernelGGL((k_q1mf_z<T, 1>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, d_in,
+                           d_out, (T*)nullptr, 0.0, 0.0);
+        UGX_CHECK_LAUNCH();
+    }
+    MfMap m;
+    size_t lds;
+    const int64_t grid = make_mf_map(W, sizeof(T), m, lds);
+    if (lds > 64 * 1024) return -1;
+    hipLaunchKernelGGL(k_q1mf_apply_sub<T>, dim3(grid), dim3(256), lds, s, m, cell, x, d_in, d_out);
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_apply_sub<float>(int, const float*, const float*, const float*, float*, hipStream_t);
+template int q1mf_apply_sub<double>(int, const double*, const double*, const double*, double*, hipStream_t);
+
+template <typename T>
+int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s) {
+    MfZMap zm;
+    size_t zlds;
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+        hipLaunchKernelGGL((k_q1mf_z<T, 2>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, v, y,
+                           (T*)nullptr, alpha, beta);
+        UGX_CHECK_LAUNCH();
+    }
+    MfMap m;
+    size_t lds;
+    const int64_t grid = make_mf_map(W, sizeof(T), m, lds);
+    if (lds > 64 * 1024) return -1;
+    hipLaunchKernelGGL(k_q1mf_axpy<T>, dim3(grid), dim3(256), lds, s, m, cell, alpha, v, beta, x, y);
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_axpy<double>(int, const double*, double, const double*, double, const double*, double*, hipStream_t);
+
+template <typename T>
+int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_q1mf_coeff<T>, dim3(vec_grid(ne)), dim3(256), 0, s, ne, h, theta, c);
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_coeff<float>(int64_t, double, const double*, float*, hipStream_t);
+template int q1mf_coeff<double>(int64_t, double, const double*, double*, hipStream_t);
 
 template <typename TM, typename TV>
 int dia_apply_sub(int64_t n, int nd, const int32_t* off, const TM* A, const TV* x, const TV* d_in, TV* d_out,

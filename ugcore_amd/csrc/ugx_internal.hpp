@@ -64,6 +64,16 @@ int permute_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, h
 template <typename TM, typename TV>
 int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,
                      TV* sc, hipStream_t s);
+// matrix-free Q1 operator on a W^3 lattice with per-element coefficients
+// c_e = h k_e ((W-1)^3, element lattice order); vectors carry the lattice halo
+template <typename T>
+int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s);
+template <typename T>
+int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s);
+template <typename T>
+int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s);
+template <typename T>
+int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s);
 template <typename TM, typename TV>
 int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s);
 
