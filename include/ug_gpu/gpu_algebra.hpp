@@ -278,6 +278,9 @@ class CG : public ILinearOperatorInverse {
    private:
     bool apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg);
     bool fast_ = true, used_fast_ = false;
+    // lattice-path work space, kept across solves (halos stay zero)
+    int64_t ws_n_ = -1, ws_h_ = -1;
+    DeviceBuffer<double> ws_x_, ws_r_, ws_p_, ws_q_, ws_z_, ws_partial_, ws_sc_;
 };
 
 class BiCGStab : public ILinearOperatorInverse {
@@ -408,11 +411,17 @@ class AssembledMultiGridCycle : public ILinearIterator {
     const int32_t* dof_of_lattice() const;
     const int32_t* lattice_of_dof() const;
     int64_t num_inner_dofs() const;
-    // c = B d for a defect d in lattice order that vanishes on Dirichlet DoFs
-    void apply_lattice(double* c_lat, const double* d_lat);
+    // c = B d for a defect d in lattice order that vanishes on Dirichlet DoFs;
+    // with cd_out (device), also (c, d) into cd_out[0] (partial: device
+    // scratch of at least 1024 doubles)
+    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out = nullptr, double* partial = nullptr);
     // y = alpha*v + beta*A*x with the surface operator in lattice order
-    // (x must vanish on Dirichlet DoFs and carry the halo)
-    void surface_axpy_lattice(double* y, double alpha, const double* v, double beta, const double* x);
+    // (x must vanish on Dirichlet DoFs and carry the halo); with yx_out
+    // (device) also (y, x) into yx_out[0] (partial: device scratch of
+    // max_partials() doubles), false if that fusion is not available
+    bool surface_axpy_lattice(double* y, double alpha, const double* v, double beta, const double* x,
+                              double* yx_out = nullptr, double* partial = nullptr);
+    static constexpr int max_partials() { return 65536; }
     struct Impl;
 
    private:

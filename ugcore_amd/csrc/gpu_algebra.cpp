@@ -311,33 +311,54 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
     // The iteration of cg.h:103-242 on lattice-ordered copies of x and r with
     // the surface operator and the preconditioner applied in lattice order.
     // Valid because p and z vanish on Dirichlet DoFs (r does on entry).
+    // The scalars rho, lambda, alpha, beta stay in device memory; the host
+    // reads |r| once per step (the convergence check) together with the
+    // breakdown flag. (q, p) is summed inside the operator kernel, (z, r)
+    // inside the cycle's final conversion, and x, r, |r|^2 are updated by one
+    // kernel.
     auto& c = context();
     const int64_t N = gmg.lattice_size(), H = gmg.lattice_halo();
-    DeviceBuffer<double> bx(N + 2 * H), br(N + 2 * H), bp(N + 2 * H), bq(N + 2 * H), bz(N + 2 * H);
-    for (auto* b : {&bx, &br, &bp, &bq, &bz}) b->zero();
-    double *xl = bx.get() + H, *rl = br.get() + H, *pl = bp.get() + H, *ql = bq.get() + H, *zl = bz.get() + H;
+    if (ws_n_ != N || ws_h_ != H) {
+        for (auto* b : {&ws_x_, &ws_r_, &ws_p_, &ws_q_, &ws_z_}) {
+            b->resize(N + 2 * H);
+            b->zero();   // the halos; the kernels write only [0, N)
+        }
+        ws_partial_.resize(AssembledMultiGridCycle::max_partials());
+        ws_sc_.resize(8);
+        ws_n_ = N;
+        ws_h_ = H;
+    }
+    double *xl = ws_x_.get() + H, *rl = ws_r_.get() + H, *pl = ws_p_.get() + H, *ql = ws_q_.get() + H,
+           *zl = ws_z_.get() + H;
+    double* sc = ws_sc_.get();
+    double* part = ws_partial_.get();
+    ws_sc_.zero();
     UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
     UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), r.data(), rl, c.stream)));
-    gmg.apply_lattice(zl, rl);
+    gmg.apply_lattice(zl, rl, sc + 0, part);   // z = B r, rho = (z, r)
     conv_->set_name(name());
     conv_->start_defect(dev_dot(N, rl, rl, true));
     UGX_HIP(hipMemcpyAsync(pl, zl, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
-    double rho_old = dev_dot(N, zl, rl, false), rho;
     bool ok = true;
     while (!conv_->iteration_ended()) {
-        gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl);
-        const double lambda = dev_dot(N, ql, pl, false);
-        if (lambda == 0.0) { ok = false; break; }
-        const double alpha = rho_old / lambda;
-        UGX_LAUNCH(vec_scale_add<double>(N, xl, 1.0, xl, alpha, pl, 0.0, nullptr, c.stream));
-        UGX_LAUNCH(vec_scale_add<double>(N, rl, 1.0, rl, -alpha, ql, 0.0, nullptr, c.stream));
-        conv_->update_defect(dev_dot(N, rl, rl, true));
+        // q = A p, lambda = (q, p)
+        if (!gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl, sc + 1, part)) {
+            gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl);
+            UGX_LAUNCH(dot<double>(N, ql, pl, c.red_partial, sc + 1, false, c.stream));
+        }
+        UGX_LAUNCH(cg_alpha(sc, c.stream));
+        UGX_LAUNCH(cg_update(N, sc, pl, ql, xl, rl, part, c.stream));
+        UGX_HIP(hipMemcpyAsync(c.red_host, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        sync();
+        if (c.red_host[1] != 0.0) {   // lambda == 0: breakdown (x, r unchanged)
+            ok = false;
+            break;
+        }
+        conv_->update_defect(std::sqrt(c.red_host[0]));
         if (conv_->iteration_ended()) break;
-        gmg.apply_lattice(zl, rl);
-        rho = dev_dot(N, zl, rl, false);
-        const double beta = rho / rho_old;
-        UGX_LAUNCH(vec_scale_add<double>(N, pl, beta, pl, 1.0, zl, 0.0, nullptr, c.stream));
-        rho_old = rho;
+        gmg.apply_lattice(zl, rl, sc + 4, part);   // z = B r, rho_new = (z, r)
+        UGX_LAUNCH(cg_beta(sc, c.stream));          // beta = rho_new / rho, rho = rho_new
+        UGX_LAUNCH(cg_direction(N, sc, pl, zl, c.stream));
     }
     UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), xl, x.data(), c.stream)));
     UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, r.data(), c.stream)));
@@ -724,8 +745,9 @@ struct AssembledMultiGridCycle::Impl {
     virtual void download_level_dia(int lev, double* out) const = 0;
     virtual int num_diags(int lev) const = 0;
     virtual void smooth(int lev, int nsteps) = 0;
-    virtual void apply_lattice(double* c_lat, const double* d_lat) = 0;
-    virtual void surface_axpy(double* y, double alpha, const double* v, double beta, const double* x) = 0;
+    virtual void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) = 0;
+    virtual bool surface_axpy(double* y, double alpha, const double* v, double beta, const double* x, double* yx_out,
+                              double* partial) = 0;
     virtual int64_t halo() const = 0;
 };
 
@@ -871,7 +893,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                 for (int i = 0; i < lu_n; ++i)
                     if (i < piv[i]) std::swap(perm[i], perm[piv[i]]);
                 // inverses of the unit lower and the upper triangular factor,
-                // stored column-major: LinvT[k*n+i] = Linv(i,k)
+                // stored row-major: linvT[i*n+k] = Linv(i,k)
                 const int m = lu_n;
                 std::vector<double> linvT((size_t)m * m, 0.0), uinvT((size_t)m * m, 0.0);
                 auto LU = [&](int r, int c) { return dense[(size_t)r * m + c]; };
@@ -884,7 +906,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                         for (int j = k; j < i; ++j) acc += LU(i, j) * y[j];
                         y[i] = -acc;
                     }
-                    for (int i = 0; i < m; ++i) linvT[(size_t)k * m + i] = y[i];
+                    for (int i = 0; i < m; ++i) linvT[(size_t)i * m + k] = y[i];
                     // column k of U^{-1}: solve U z = e_k
                     std::vector<double> z(m, 0.0);
                     for (int i = k; i >= 0; --i) {
@@ -892,7 +914,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                         for (int j = i + 1; j <= k; ++j) acc -= LU(i, j) * z[j];
                         z[i] = acc / LU(i, i);
                     }
-                    for (int i = 0; i < m; ++i) uinvT[(size_t)k * m + i] = z[i];
+                    for (int i = 0; i < m; ++i) uinvT[(size_t)i * m + k] = z[i];
                 }
                 lu_linvT.resize((int64_t)m * m);
                 lu_linvT.upload(linvT.data(), (int64_t)m * m);
@@ -1031,7 +1053,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         sym = sym_ok;
     }
 
-    void apply_lattice(double* c_lat, const double* d_lat) override {
+    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) override {
         auto& c = context();
         GmgLevel<T>& t = L[top];
         if (!sym_ok) throw UGError("GMG::apply_lattice: needs a symmetric discretisation");
@@ -1039,17 +1061,30 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
         lmgc(top);
-        UGX_LAUNCH((convert<T, double>(t.N, t.sc, c_lat, c.stream)));
+        if (cd_out)
+            UGX_LAUNCH(convert_dot<T>(t.N, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
+        else
+            UGX_LAUNCH((convert<T, double>(t.N, t.sc, c_lat, c.stream)));
     }
 
-    void surface_axpy(double* y, double alpha, const double* v, double beta, const double* x) override {
+    bool surface_axpy(double* y, double alpha, const double* v, double beta, const double* x, double* yx_out,
+                      double* partial) override {
         GmgLevel<T>& t = L[top];
+        auto& c = context();
         if (mf) {
-            UGX_LAUNCH(q1mf_axpy<double>(t.W, cell_c64.get(), alpha, v, beta, x, y, context().stream));
-            return;
+            if (yx_out) {
+                int np = AssembledMultiGridCycle::max_partials();
+                if (q1mf_axpy<double>(t.W, cell_c64.get(), alpha, v, beta, x, y, c.stream, partial, &np) != 0)
+                    return false;
+                UGX_LAUNCH(sum_partials(np, partial, yx_out, c.stream));
+                return true;
+            }
+            UGX_LAUNCH(q1mf_axpy<double>(t.W, cell_c64.get(), alpha, v, beta, x, y, c.stream));
+            return true;
         }
-        UGX_LAUNCH((dias_axpy<double, double>(t.N, t.nd, t.off.data(), surf.get(), alpha, v, beta, x, y,
-                                              context().stream)));
+        if (yx_out) return false;
+        UGX_LAUNCH((dias_axpy<double, double>(t.N, t.nd, t.off.data(), surf.get(), alpha, v, beta, x, y, c.stream)));
+        return true;
     }
 
     int64_t halo() const override { return L[top].H; }
@@ -1107,14 +1142,14 @@ int64_t AssembledMultiGridCycle::lattice_halo() const { return impl_ ? impl_->ha
 const int32_t* AssembledMultiGridCycle::dof_of_lattice() const { return disc_->level(disc_->top_level()).dof_of_lat.get(); }
 const int32_t* AssembledMultiGridCycle::lattice_of_dof() const { return disc_->level(disc_->top_level()).lat_of_dof.get(); }
 int64_t AssembledMultiGridCycle::num_inner_dofs() const { return disc_->level(disc_->top_level()).n_inner; }
-void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat) {
+void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) {
     if (!impl_) throw UGError("GMG: not initialised");
-    impl_->apply_lattice(c_lat, d_lat);
+    impl_->apply_lattice(c_lat, d_lat, cd_out, partial);
 }
-void AssembledMultiGridCycle::surface_axpy_lattice(double* y, double alpha, const double* v, double beta,
-                                                   const double* x) {
+bool AssembledMultiGridCycle::surface_axpy_lattice(double* y, double alpha, const double* v, double beta,
+                                                   const double* x, double* yx_out, double* partial) {
     if (!impl_) throw UGError("GMG: not initialised");
-    impl_->surface_axpy(y, alpha, v, beta, x);
+    return impl_->surface_axpy(y, alpha, v, beta, x, yx_out, partial);
 }
 
 bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {

@@ -38,6 +38,25 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
     return lb;   // may exceed nblocks-1 for the last XCD: callers bound-check rows
 }
 
+// Division by a run-time constant d >= 1 of 32-bit unsigned n (round-up
+// multiplier method, Granlund & Montgomery 1994): q = (umulhi(n, m) + n) >> s
+// with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+struct FastDiv {
+    uint32_t d, m, s;
+};
+static inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{d, 0u, 0u};
+    if (d <= 1) return f;
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    f.s = s;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1);
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.s);
+}
+
 // Row schedule of the lattice (DIA) kernels. Flat mode: 256 consecutive rows
 // per block, blocks remapped by xcd_block. Banded mode (3D lattices): the W
 // lines of every plane are split into 8 bands, XCD c (blockIdx % 8) owns band
@@ -340,6 +359,73 @@ __global__ __launch_bounds__(256) void k_sum_partials(int np, const double* part
         double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
         out[0] = sqrt_flag ? sqrt(r) : r;
     }
+}
+
+// ---------------------------------------------------------------------------
+// CG building blocks with the scalars kept in device memory
+// (sc[0] rho, sc[1] lambda, sc[2] alpha, sc[3] beta, sc[4] rho_new, sc[5] |r|^2,
+//  sc[6] = 1 after a breakdown lambda == 0)
+// ---------------------------------------------------------------------------
+__global__ void k_cg_alpha(double* sc) {
+    if (threadIdx.x != 0) return;
+    const double lambda = sc[1];
+    if (lambda == 0.0) {
+        sc[6] = 1.0;
+        sc[2] = 0.0;
+    } else {
+        sc[2] = sc[0] / lambda;
+    }
+}
+
+__global__ void k_cg_beta(double* sc) {
+    if (threadIdx.x != 0) return;
+    sc[3] = sc[4] / sc[0];
+    sc[0] = sc[4];
+}
+
+// x += alpha p ; r -= alpha q ; partial sums of r.r
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __restrict__ sc,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ partial) {
+    const double alpha = sc[2];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        x[i] = 1.0 * x[i] + alpha * p[i];
+        const double ri = 1.0 * r[i] + (-alpha) * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// p = beta p + z
+__global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const double* __restrict__ sc, double* __restrict__ p,
+                                                      const double* __restrict__ z) {
+    const double beta = sc[3];
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = beta * p[i] + 1.0 * z[i];
+}
+
+// z = (double) c ; partial sums of z.r
+template <typename T>
+__global__ __launch_bounds__(256) void k_convert_dot(int64_t n, const T* __restrict__ c, double* __restrict__ z,
+                                                     const double* __restrict__ r, double* __restrict__ partial) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double zi = (double)c[i];
+        z[i] = zi;
+        acc += zi * r[i];
+    }
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -835,7 +921,21 @@ struct MfZMap {
     int32_t W, TY, ZC, nchunks;
     int32_t glo[8], gsz[8];
 };
-constexpr int kMfPrefetch = 8;   // staged entries per thread and plane (P <= 8 * 256)
+// tile shape of the z-marching kernels: lines per block (FP32 / FP64) and
+// planes per block
+#ifndef UGX_MF_TY32
+#define UGX_MF_TY32 4
+#endif
+#ifndef UGX_MF_TY64
+#define UGX_MF_TY64 2
+#endif
+#ifndef UGX_MF_ZC
+#define UGX_MF_ZC 8
+#endif
+#ifndef UGX_MF_PF
+#define UGX_MF_PF 7
+#endif
+constexpr int kMfPrefetch = UGX_MF_PF;   // staged entries per thread and plane (P <= kMfPrefetch * 256)
 
 constexpr int kMfRows = 5;       // rows per thread and plane (R + 63 <= 5 * 256)
 
@@ -844,16 +944,29 @@ constexpr int kMfRows = 5;       // rows per thread and plane (R + 63 <= 5 * 256
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                                                 const T* __restrict__ b_in, const T* c_in, T* out0, T* out1,
-                                                double alpha, double beta) {
+                                                double alpha, double beta, double* dot_partial) {
     // c_in and out0 may alias (d_out = d_in); every row is read and then
-    // written by the same thread
+    // written by the same thread. MODE 2 with dot_partial: per-block partial
+    // sums of y_i * x_i (fixed order) in dot_partial[blockIdx.x]
+    double dot_acc = 0.0;
+    __shared__ double dot_sh[4];
+    auto finish_dot = [&]() {
+        if (MODE != 2 || !dot_partial) return;
+        double v = dot_acc;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0) dot_sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) dot_partial[blockIdx.x] = (dot_sh[0] + dot_sh[1]) + (dot_sh[2] + dot_sh[3]);
+    };
     // MODE 0: a_in = dinv, b_in = sd_in, out0 = sd_out, out1 = sc
     // MODE 1: a_in = x, c_in = d_in, out0 = d_out
     // MODE 2: a_in = x, c_in = v, out0 = y
     const int xc = blockIdx.x & 7, q = blockIdx.x >> 3, g = m.gsz[xc];
-    if (g == 0) return;
-    const int chunk = q / g;
-    if (chunk >= m.nchunks) return;
+    const int chunk = g == 0 ? 0 : q / g;
+    if (g == 0 || chunk >= m.nchunks) {
+        finish_dot();   // idle block: partial 0
+        return;
+    }
     const int band = m.glo[xc] + (q - chunk * g);
     const int W = m.W, n = W - 1, TY = m.TY;
     const int j0 = band * TY, j1 = j0 + TY < W ? j0 + TY : W, R = (j1 - j0) * W;
@@ -977,7 +1090,9 @@ __global__ __launch_bounds__(256) void k_q1mf_z(MfZMap m, const T* __restrict__ 
             } else if (MODE == 1) {
                 out0[i] = ra[u] - sv;
             } else {
-                out0[i] = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
+                const T yv = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
+                out0[i] = yv;
+                dot_acc += (double)yv * (double)xi;
             }
         }
         // 4. the prefetched plane k+2 and cell plane k+1 into the rings
@@ -993,6 +1108,7 @@ __global__ __launch_bounds__(256) void k_q1mf_z(MfZMap m, const T* __restrict__ 
         }
         __syncthreads();
     }
+    finish_dot();
 }
 
 template <typename T>
@@ -1058,18 +1174,22 @@ __global__ void k_dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega
 template <typename TV>
 __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* __restrict__ scc,
                                                      const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
-                                                     TV* __restrict__ sc) {
+                                                     TV* __restrict__ sc, FastDiv fw) {
     const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (i >= n_f) return;
-    const int x = (int)(i % Wf);
-    const int y = (int)((i / Wf) % Wf);
-    const int z = dim == 3 ? (int)(i / ((int64_t)Wf * Wf)) : 0;
+    const uint32_t q1 = fdiv((uint32_t)i, fw);
+    const int x = (int)((uint32_t)i - q1 * (uint32_t)Wf);
+    const uint32_t q2 = dim == 3 ? fdiv(q1, fw) : 0u;
+    const int y = (int)(q1 - q2 * (uint32_t)Wf);
+    const int z = (int)q2;
     const int x0 = x >> 1, y0 = y >> 1, z0 = z >> 1;
     const int ox = x & 1, oy = y & 1, oz = z & 1;
     double v = 0.0;
+    // Dirichlet DoFs lie on the lattice boundary: only there is the flag read
+    const bool bnd = x == 0 || y == 0 || x == Wf - 1 || y == Wf - 1 || (dim == 3 && (z == 0 || z == Wf - 1));
     if (dim == 3) {
         const int nodd = ox + oy + oz;
-        if (dir_f[i] && nodd != 0) {
+        if (bnd && dir_f[i] && nodd != 0) {
             v = 0.0;   // Dirichlet rows of P are zeroed except vertex copies
         } else {
             const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
@@ -1081,7 +1201,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
     } else {
         // triangle mesh: odd coordinates lie on axis edges or on the (1,1) diagonal
         const int nodd = ox + oy;
-        if (dir_f[i] && nodd != 0) {
+        if (bnd && dir_f[i] && nodd != 0) {
             v = 0.0;
         } else if (nodd == 0) {
             v = (double)scc[(int64_t)y0 * Wc + x0];
@@ -1100,14 +1220,33 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
 template <typename TV>
 __global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64_t n_c, const TV* __restrict__ sdf,
                                                   const uint8_t* __restrict__ dir_c, const uint8_t* __restrict__ dir_f,
-                                                  TV* __restrict__ sdc) {
+                                                  TV* __restrict__ sdc, FastDiv fw) {
     const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (c >= n_c) return;
-    const int x = (int)(c % Wc);
-    const int y = (int)((c / Wc) % Wc);
-    const int z = dim == 3 ? (int)(c / ((int64_t)Wc * Wc)) : 0;
+    const uint32_t q1 = fdiv((uint32_t)c, fw);
+    const int x = (int)((uint32_t)c - q1 * (uint32_t)Wc);
+    const uint32_t q2 = dim == 3 ? fdiv(q1, fw) : 0u;
+    const int y = (int)(q1 - q2 * (uint32_t)Wc);
+    const int z = (int)q2;
     const int fx = 2 * x, fy = 2 * y, fz = 2 * z;
     const int nf = Wf - 1;
+    // Dirichlet DoFs lie on the lattice boundary: away from it no flag is read
+    const bool inner = x >= 2 && y >= 2 && x <= Wc - 3 && y <= Wc - 3 && (dim == 2 || (z >= 2 && z <= Wc - 3));
+    if (dim == 3 && inner) {
+        double s = 0.0;
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int nodd = (dx != 0) + (dy != 0) + (dz != 0);
+                    const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+                    s += w * (double)sdf[((int64_t)(fz + dz) * Wf + (fy + dy)) * Wf + (fx + dx)];
+                }
+        sdc[c] = (TV)s;
+        return;
+    }
     if (dir_c[c]) {
         const int64_t f = dim == 3 ? ((int64_t)fz * Wf + fy) * Wf + fx : (int64_t)fy * Wf + fx;
         sdc[c] = sdf[f];
@@ -1152,24 +1291,32 @@ __global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64
 // the threads of a block read consecutive entries. One workgroup.
 // ---------------------------------------------------------------------------
 template <typename TV>
-__global__ __launch_bounds__(256) void k_lu_solve(int n, const double* __restrict__ LinvT, const double* __restrict__ UinvT,
-                                                  const int32_t* __restrict__ perm, const TV* __restrict__ b,
-                                                  TV* __restrict__ x) {
+__global__ __launch_bounds__(1024) void k_lu_solve(int n, const double* __restrict__ Linv, const double* __restrict__ Uinv,
+                                                   const int32_t* __restrict__ perm, const TV* __restrict__ b,
+                                                   TV* __restrict__ x) {
+    // Linv, Uinv row-major; 8 lanes per row, partial sums combined by shuffles
     extern __shared__ double sh[];
     double* pb = sh;        // permuted rhs
     double* t = sh + n;     // L^{-1} P b
     for (int i = threadIdx.x; i < n; i += blockDim.x) pb[i] = (double)b[perm[i]];
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int g = threadIdx.x & 7, rows = blockDim.x >> 3;
+    for (int i = threadIdx.x >> 3; i < n; i += rows) {
         double s = 0.0;
-        for (int k = 0; k <= i; ++k) s += LinvT[(int64_t)k * n + i] * pb[k];
-        t[i] = s;
+        for (int k = g; k <= i; k += 8) s += Linv[(int64_t)i * n + k] * pb[k];
+        s += __shfl_xor(s, 1, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 4, 8);
+        if (g == 0) t[i] = s;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x >> 3; i < n; i += rows) {
         double s = 0.0;
-        for (int k = i; k < n; ++k) s += UinvT[(int64_t)k * n + i] * t[k];
-        x[i] = (TV)s;
+        for (int k = i + g; k < n; k += 8) s += Uinv[(int64_t)i * n + k] * t[k];
+        s += __shfl_xor(s, 1, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 4, 8);
+        if (g == 0) x[i] = (TV)s;
     }
 }
 
@@ -1602,6 +1749,37 @@ int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool ta
     UGX_CHECK_LAUNCH();
 }
 template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
+
+int sum_partials(int np, const double* partial, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, np, partial, out, 0);
+    UGX_CHECK_LAUNCH();
+}
+int cg_alpha(double* sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(64), 0, s, sc);
+    UGX_CHECK_LAUNCH();
+}
+int cg_beta(double* sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, sc);
+    UGX_CHECK_LAUNCH();
+}
+int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
+              hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update, dim3(kRedBlocks), dim3(256), 0, s, n, sc, p, q, x, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, kRedBlocks, partial, (double*)(sc + 5), 0);
+    UGX_CHECK_LAUNCH();
+}
+int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_direction, dim3(vec_grid(n)), dim3(256), 0, s, n, sc, p, z);
+    UGX_CHECK_LAUNCH();
+}
+template <typename T>
+int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_convert_dot<T>, dim3(kRedBlocks), dim3(256), 0, s, n, c, z, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, kRedBlocks, partial, out, 0);
+    UGX_CHECK_LAUNCH();
+}
+template int convert_dot<float>(int64_t, const float*, double*, const double*, double*, double*, hipStream_t);
+template int convert_dot<double>(int64_t, const double*, double*, const double*, double*, double*, hipStream_t);
 template int dot<float>(int64_t, const float*, const float*, double*, double*, bool, hipStream_t);
 
 template <typename TS, typename TL>
@@ -1684,8 +1862,8 @@ static int64_t make_staged_map(int64_t n, int nd, const int32_t* off, size_t ele
 static int64_t make_mf_zmap(int W, size_t elem, MfZMap& m, size_t& lds) {
     m = MfZMap{};
     m.W = W;
-    m.TY = elem == 8 ? 2 : 4;
-    m.ZC = 8;
+    m.TY = elem == 8 ? UGX_MF_TY64 : UGX_MF_TY32;
+    m.ZC = UGX_MF_ZC;
     m.nchunks = (W + m.ZC - 1) / m.ZC;
     const int64_t P = (int64_t)m.TY * W + 2 * W + 2, CL = (int64_t)(m.TY + 1) * (W + 1);
     if (P > (int64_t)kMfPrefetch * 256 || CL > (int64_t)kMfPrefetch * 256 || W < 8) return 0;
@@ -1805,7 +1983,7 @@ int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 0>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
-                           sd_out, sc, 0.0, 0.0);
+                           sd_out, sc, 0.0, 0.0, (double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -1824,7 +2002,7 @@ int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hi
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 1>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, d_in,
-                           d_out, (T*)nullptr, 0.0, 0.0);
+                           d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -1838,14 +2016,18 @@ template int q1mf_apply_sub<float>(int, const float*, const float*, const float*
 template int q1mf_apply_sub<double>(int, const double*, const double*, const double*, double*, hipStream_t);
 
 template <typename T>
-int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s) {
+int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s,
+              double* dot_partial, int* n_partial) {
     MfZMap zm;
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+        if (dot_partial && n_partial && zgrid > *n_partial) return -1;
         hipLaunchKernelGGL((k_q1mf_z<T, 2>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, v, y,
-                           (T*)nullptr, alpha, beta);
+                           (T*)nullptr, alpha, beta, dot_partial);
+        if (n_partial) *n_partial = (int)zgrid;
         UGX_CHECK_LAUNCH();
     }
+    if (dot_partial) return -1;   // the fused dot needs the z-marching kernel
     MfMap m;
     size_t lds;
     const int64_t grid = make_mf_map(W, sizeof(T), m, lds);
@@ -1853,7 +2035,8 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
     hipLaunchKernelGGL(k_q1mf_axpy<T>, dim3(grid), dim3(256), lds, s, m, cell, alpha, v, beta, x, y);
     UGX_CHECK_LAUNCH();
 }
-template int q1mf_axpy<double>(int, const double*, double, const double*, double, const double*, double*, hipStream_t);
+template int q1mf_axpy<double>(int, const double*, double, const double*, double, const double*, double*, hipStream_t,
+                               double*, int*);
 
 template <typename T>
 int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s) {
@@ -1914,7 +2097,9 @@ template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
                 hipStream_t s) {
     if (n_f <= 0) return 0;
-    hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, scc, dir_f, st, sc);
+    if (n_f >= ((int64_t)1 << 32)) return -1;
+    hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, scc, dir_f, st, sc,
+                       make_fastdiv((uint32_t)Wf));
     UGX_CHECK_LAUNCH();
 }
 template int prolong_add<double>(int, int, int, int64_t, const double*, const uint8_t*, double*, double*, hipStream_t);
@@ -1924,7 +2109,9 @@ template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
                     TV* sdc, hipStream_t s) {
     if (n_c <= 0) return 0;
-    hipLaunchKernelGGL(k_restrict<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_c, sdf, dir_c, dir_f, sdc);
+    if (n_c >= ((int64_t)1 << 32)) return -1;
+    hipLaunchKernelGGL(k_restrict<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_c, sdf, dir_c, dir_f, sdc,
+                       make_fastdiv((uint32_t)Wc));
     UGX_CHECK_LAUNCH();
 }
 template int restrict_defect<double>(int, int, int, int64_t, const double*, const uint8_t*, const uint8_t*, double*, hipStream_t);
@@ -1933,7 +2120,7 @@ template int restrict_defect<float>(int, int, int, int64_t, const float*, const 
 template <typename TV>
 int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s) {
     if (n <= 0 || n > 4096) return -1;
-    hipLaunchKernelGGL(k_lu_solve<TV>, dim3(1), dim3(256), 2 * n * sizeof(double), s, n, LinvT, UinvT, perm, b, x);
+    hipLaunchKernelGGL(k_lu_solve<TV>, dim3(1), dim3(1024), 2 * n * sizeof(double), s, n, LinvT, UinvT, perm, b, x);
     UGX_CHECK_LAUNCH();
 }
 template int lu_solve<double>(int, const double*, const double*, const int32_t*, const double*, double*, hipStream_t);

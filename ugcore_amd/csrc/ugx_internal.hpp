@@ -33,6 +33,16 @@ int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const 
                     int* bad, hipStream_t s);
 template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s);
+// CG with device scalars sc[0..6] = rho, lambda, alpha, beta, rho_new, |r|^2,
+// breakdown flag
+int sum_partials(int np, const double* partial, double* out, hipStream_t s);
+int cg_alpha(double* sc, hipStream_t s);
+int cg_beta(double* sc, hipStream_t s);
+int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
+              hipStream_t s);
+int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s);
+template <typename T>
+int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s);
 template <typename TS, typename TL>
 int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hipStream_t s);
 template <typename TL, typename TS>
@@ -70,8 +80,11 @@ template <typename T>
 int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s);
 template <typename T>
 int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s);
+// dot_partial (optional): per-block partial sums of y.x (n_partial in: capacity,
+// out: number written); needs the z-marching kernel
 template <typename T>
-int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s);
+int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s,
+              double* dot_partial = nullptr, int* n_partial = nullptr);
 template <typename T>
 int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s);
 template <typename TM, typename TV>
@@ -86,7 +99,7 @@ int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8
 template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
                     TV* sdc, hipStream_t s);
-// x = U^{-1} L^{-1} P b with column-major triangular inverses; perm[i] = row of b
+// x = U^{-1} L^{-1} P b with row-major triangular inverses; perm[i] = row of b
 template <typename TV>
 int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s);
 
