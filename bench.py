@@ -74,10 +74,15 @@ def measured_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "gmg_cg_256_pmc_traffic.md")
     try:
         with open(path) as f:
+            cols = None
             for line in f:
+                if not line.startswith("|"):
+                    continue
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) >= 6 and cells[0] == f"`{kernel}`":
-                    return float(cells[4].replace(",", "")) + float(cells[5].replace(",", ""))
+                if "HBM fetch B/launch" in cells:
+                    cols = (cells.index("HBM fetch B/launch"), cells.index("HBM write B/launch"))
+                elif cols and cells[0] == f"`{kernel}`":
+                    return float(cells[cols[0]].replace(",", "")) + float(cells[cols[1]].replace(",", ""))
     except OSError:
         pass
     return None

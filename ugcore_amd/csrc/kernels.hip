@@ -1217,6 +1217,53 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
 
 // restriction: sdc_c = sum_f R(c,f) sd_f with R = P^T (Dirichlet coarse rows:
 // injection from the vertex copy)
+// 3D prolongation driven by the coarse points: thread c loads its (up to 8)
+// coarse corners once and writes its 2x2x2 fine children (fine x pairs of
+// neighbouring threads are contiguous). Weights and summation order as in
+// k_prolong_add, so the values are identical.
+template <typename TV>
+__global__ __launch_bounds__(256) void k_prolong_add3(int Wc, int Wf, int64_t n_c, const TV* __restrict__ scc,
+                                                      const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
+                                                      TV* __restrict__ sc, FastDiv fc) {
+    const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (c >= n_c) return;
+    const uint32_t q1 = fdiv((uint32_t)c, fc);
+    const int x = (int)((uint32_t)c - q1 * (uint32_t)Wc);
+    const uint32_t q2 = fdiv(q1, fc);
+    const int y = (int)(q1 - q2 * (uint32_t)Wc), z = (int)q2;
+    double v[2][2][2];
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const bool in = x + dx < Wc && y + dy < Wc && z + dz < Wc;
+                v[dz][dy][dx] = in ? (double)scc[((int64_t)(z + dz) * Wc + (y + dy)) * Wc + (x + dx)] : 0.0;
+            }
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+            for (int ox = 0; ox < 2; ++ox) {
+                const int fx = 2 * x + ox, fy = 2 * y + oy, fz = 2 * z + oz;
+                if (fx >= Wf || fy >= Wf || fz >= Wf) continue;
+                const int64_t f = ((int64_t)fz * Wf + fy) * Wf + fx;
+                const int nodd = ox + oy + oz;
+                const bool bnd = fx == 0 || fy == 0 || fz == 0 || fx == Wf - 1 || fy == Wf - 1 || fz == Wf - 1;
+                double val = 0.0;
+                if (!(bnd && nodd != 0 && dir_f[f])) {
+                    const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+                    for (int dz = 0; dz <= oz; ++dz)
+                        for (int dy = 0; dy <= oy; ++dy)
+                            for (int dx = 0; dx <= ox; ++dx) val += w * v[dz][dy][dx];
+                }
+                st[f] = (TV)val;
+                sc[f] = (TV)((double)sc[f] + val);
+            }
+}
+
 template <typename TV>
 __global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64_t n_c, const TV* __restrict__ sdf,
                                                   const uint8_t* __restrict__ dir_c, const uint8_t* __restrict__ dir_f,
@@ -2098,6 +2145,12 @@ int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8
                 hipStream_t s) {
     if (n_f <= 0) return 0;
     if (n_f >= ((int64_t)1 << 32)) return -1;
+    if (dim == 3 && Wf == 2 * Wc - 1) {
+        const int64_t n_c = (int64_t)Wc * Wc * Wc;
+        hipLaunchKernelGGL(k_prolong_add3<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, Wc, Wf, n_c, scc, dir_f, st,
+                           sc, make_fastdiv((uint32_t)Wc));
+        UGX_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, scc, dir_f, st, sc,
                        make_fastdiv((uint32_t)Wf));
     UGX_CHECK_LAUNCH();
