@@ -317,11 +317,16 @@ def main():
     A.set_spmv_lanes(best_lanes)
     t_csr = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
     A.set_spmv_layout("sell")
+    assert A.has_row_order()
     t_sell = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
+    # the same slices filled in DoF order instead of the lattice walk
+    A.set_row_order(None)
+    t_sell_dof_order = U.time_launches(lambda: A.apply(ys, xs), args.kernel_reps)
+    A.set_row_order(disc.dof_of_lattice(refs))
     layout = "sell" if t_sell <= t_csr else "csr"
     A.set_spmv_layout(layout)
     log(f"[bench] CSR SpMV lanes sweep (ms): " + ", ".join(f"{k}: {v*1e3:.3f}" for k, v in sweep.items())
-        + f"; sliced ELL {t_sell*1e3:.3f} ms -> {layout}")
+        + f"; sliced ELL {t_sell*1e3:.3f} ms (slices in DoF order {t_sell_dof_order*1e3:.3f} ms) -> {layout}")
 
     solvers = {}
     setup_s = {}
@@ -495,6 +500,7 @@ def main():
             "spmv_f64_ms": t_spmv * 1e3,
             "spmv_csr_direct_ms": t_csr * 1e3,
             "spmv_sliced_ell_ms": t_sell * 1e3,
+            "spmv_sliced_ell_slices_in_dof_order_ms": t_sell_dof_order * 1e3,
             "spmv_csr_lanes": best_lanes,
             "spmv_f64_lex_ordering_ms": t_spmv_lex * 1e3,
             "spmv_f64_lex_ordering_GBps": spmv_bytes / t_spmv_lex / 1e9,

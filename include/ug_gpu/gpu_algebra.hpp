@@ -137,6 +137,15 @@ class GPUSparseMatrix {
     uint64_t version() const { return version_; }
     void set_origin(const void* o) { origin_ = o; ++version_; }
     void set_use_sliced_ell(bool on) { sell_enabled_ = on; if (!on) sell_valid_ = false; }
+    // the row permutation that fills the sliced-ELL slices (rows order[64t..
+    // 64t+63] form slice t); it groups rows whose columns lie close together so
+    // that the x gathers of a wavefront stay local. Results do not depend on
+    // it. Host array, checked to be a permutation of [0, rows); nullptr clears.
+    // Assembly sets the lattice walk of the discretisation.
+    void set_row_order(const int32_t* order, int64_t n);
+    // the same from device memory that is a permutation by construction
+    void set_row_order_device(const int32_t* order, int64_t n);
+    bool has_row_order() const { return rows_ > 0 && row_order_.size() == rows_; }
     int lanes() const { return lanes_; }
     void set_lanes(int g) {
         if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64) throw UGError("set_lanes: 4, 8, 16, 32 or 64");
@@ -151,7 +160,7 @@ class GPUSparseMatrix {
     bool sell_enabled_ = true, sell_valid_ = false;
     const void* origin_ = nullptr;
     uint64_t version_ = 0;
-    DeviceBuffer<int32_t> sell_width_, sell_col_;
+    DeviceBuffer<int32_t> sell_width_, sell_col_, row_order_;
     DeviceBuffer<int64_t> sell_ptr_;
     DeviceBuffer<double> sell_val_;
 };

@@ -161,6 +161,14 @@ int ugx_matrix_set_spmv_lanes(void* m, int lanes);
  * 0 = CSR directly. Call ugx_matrix_finalize after changing values through
  * the raw CSR pointers of ugx_matrix_device_csr. */
 int ugx_matrix_set_spmv_layout(void* m, int sliced_ell);
+/* Row permutation that fills the sliced-ELL slices (host array of n = rows
+ * entries, checked to be a permutation; order = NULL clears it). Rows whose
+ * columns lie close together should be neighbours in it, so that the column
+ * gathers of one 64-row slice stay local; SpMV results do not depend on it.
+ * Assembly by a discretization sets its lattice walk. No counterpart in the
+ * reference (SparseMatrix has one storage, sparsematrix.h). */
+int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n);
+int ugx_matrix_has_row_order(void* m);
 int ugx_matrix_finalize(void* m);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
 

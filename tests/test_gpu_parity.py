@@ -362,16 +362,31 @@ def test_sliced_ell_and_csr_layouts_agree():
     A.upload_csr(M.rowptr.astype(np.int32), M.col, M.val, n_cols=cols)
     w = rng.normal(size=cols)
     v1 = rng.normal(size=rows)
-    for layout in ("sell", "csr"):
-        A.set_spmv_layout(layout)
+    order = rng.permutation(rows).astype(np.int32)
+    got = {}
+    for layout in ("sell", "sell-ordered", "csr"):
+        A.set_spmv_layout("csr" if layout == "csr" else "sell")
+        A.set_row_order(order if layout == "sell-ordered" else None)
+        assert A.has_row_order() == (layout == "sell-ordered")
         for alpha, beta in ((0.0, 1.0), (1.0, -1.0), (0.7, 1.3)):
-            dest = u.Vector(values=rng.normal(size=rows))
+            d0 = np.random.default_rng(int(alpha * 10)).normal(size=rows)
+            dest = u.Vector(values=d0)
             vv = u.Vector(values=v1)
             ww = u.Vector(values=w)
             A.axpy(dest, alpha, vv, beta, ww)
             expect = np.zeros(rows)
             O.lib().orc_csr_op(0, rows, cols, M.rowptr, M.col, M.val, alpha, v1.ctypes.data, beta, w, expect)
             assert np.allclose(dest.numpy(), expect, rtol=1e-13, atol=1e-13), layout
+            got[(layout, alpha)] = dest.numpy()
+    # the slice order changes which lanes hold a row, not its sum
+    for alpha in (0.0, 1.0, 0.7):
+        assert np.array_equal(got[("sell", alpha)], got[("sell-ordered", alpha)])
+    bad = order.copy()
+    bad[3] = bad[4]
+    with pytest.raises(u.UGError):
+        A.set_row_order(bad)
+    with pytest.raises(u.UGError):
+        A.set_row_order(order[:-1])
 
 
 @pytest.mark.parametrize("kind,refs", [(0, 4), (1, 6)])

@@ -79,6 +79,8 @@ _SIGNATURES = {
     "ugx_matrix_axpy": (_I, [_P, _P, _D, _P, _D, _P]),
     "ugx_matrix_set_spmv_lanes": (_I, [_P, _I]),
     "ugx_matrix_set_spmv_layout": (_I, [_P, _I]),
+    "ugx_matrix_set_row_order": (_I, [_P, _P, _I64]),
+    "ugx_matrix_has_row_order": (_I, [_P]),
     "ugx_matrix_finalize": (_I, [_P]),
     "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
     "ugx_vector_create": (_P, [_I64]),
@@ -308,6 +310,18 @@ class SparseMatrix:
     def set_spmv_layout(self, layout):
         """'sell' (sliced ELL, default) or 'csr'"""
         _check(lib().ugx_matrix_set_spmv_layout(self.h, 1 if layout == "sell" else 0), "set_spmv_layout")
+
+    def set_row_order(self, order):
+        """Row permutation that fills the sliced-ELL slices (None clears it);
+        SpMV results do not depend on it."""
+        if order is None:
+            _check(lib().ugx_matrix_set_row_order(self.h, None, 0), "set_row_order")
+            return
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        _check(lib().ugx_matrix_set_row_order(self.h, o.ctypes.data, o.size), "set_row_order")
+
+    def has_row_order(self):
+        return bool(lib().ugx_matrix_has_row_order(self.h))
 
     def finalize(self):
         _check(lib().ugx_matrix_finalize(self.h), "SparseMatrix.finalize")

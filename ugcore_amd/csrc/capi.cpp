@@ -413,6 +413,13 @@ int ugx_matrix_set_spmv_layout(void* m, int sliced_ell) {
         return 0;
     });
 }
+int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n) {
+    return guard([&] {
+        ((MatrixH*)m)->A->set_row_order(order, n);
+        return 0;
+    });
+}
+int ugx_matrix_has_row_order(void* m) { return ((MatrixH*)m)->A->has_row_order() ? 1 : 0; }
 int ugx_matrix_finalize(void* m) {
     return guard([&] {
         ((MatrixH*)m)->A->finalize();

@@ -175,7 +175,8 @@ void GPUSparseMatrix::finalize() {
     auto& c = context();
     const int64_t nsl = (rows_ + 63) / 64;
     sell_width_.resize(nsl);
-    UGX_LAUNCH(sell_widths(rows_, rowptr_.get(), sell_width_.get(), c.stream));
+    const int32_t* order = has_row_order() ? row_order_.get() : nullptr;
+    UGX_LAUNCH(sell_widths(rows_, rowptr_.get(), order, sell_width_.get(), c.stream));
     std::vector<int32_t> w(nsl);
     sell_width_.download(w.data(), nsl);
     std::vector<int64_t> ptr(nsl + 1, 0);
@@ -184,7 +185,7 @@ void GPUSparseMatrix::finalize() {
     sell_ptr_.upload(ptr.data(), nsl + 1);
     sell_col_.resize(ptr[nsl]);
     sell_val_.resize(ptr[nsl]);
-    UGX_LAUNCH(sell_fill<double>(rows_, cols_, rowptr_.get(), col_.get(), val_.get(), sell_ptr_.get(),
+    UGX_LAUNCH(sell_fill<double>(rows_, cols_, rowptr_.get(), col_.get(), val_.get(), order, sell_ptr_.get(),
                                  sell_width_.get(), sell_col_.get(), sell_val_.get(), c.stream));
     sync();
     sell_valid_ = true;
@@ -192,6 +193,7 @@ void GPUSparseMatrix::finalize() {
 
 void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
     sell_valid_ = false;
+    row_order_.resize(0);
     if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix: nnz exceeds int32 range");
     rows_ = rows;
     cols_ = cols;
@@ -200,6 +202,32 @@ void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
     col_.resize(nnz);
     val_.resize(nnz);
     lanes_ = lanes_for(rows, nnz);
+}
+
+void GPUSparseMatrix::set_row_order(const int32_t* order, int64_t n) {
+    if (!order) {
+        row_order_.resize(0);
+        finalize();
+        return;
+    }
+    if (n != rows_) throw UGError("set_row_order: the order must list every row once");
+    std::vector<uint8_t> seen(rows_, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t r = order[i];
+        if (r < 0 || r >= rows_ || seen[r]) throw UGError("set_row_order: not a permutation of the rows");
+        seen[r] = 1;
+    }
+    row_order_.resize(n);
+    row_order_.upload(order, n);
+    finalize();
+}
+
+void GPUSparseMatrix::set_row_order_device(const int32_t* order, int64_t n) {
+    if (n != rows_) throw UGError("set_row_order_device: the order must list every row once");
+    row_order_.resize(n);
+    UGX_HIP(hipMemcpyAsync(row_order_.get(), order, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                           context().stream));
+    sync();
 }
 
 void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
@@ -224,7 +252,8 @@ void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, doub
     if (sell_valid_)
         UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
                                               sell_val_.get(), a1, v1.data(), b1, w1.data(), dest.data(), 0,
-                                              rowptr_.get(), context().stream)));
+                                              rowptr_.get(), has_row_order() ? row_order_.get() : nullptr,
+                                              context().stream)));
     else
         UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), a1, v1.data(), b1,
                                              w1.data(), dest.data(), 0, lanes_, context().stream)));
@@ -234,7 +263,8 @@ void GPUSparseMatrix::apply_ignore_zero_rows(GPUVector& dest, double beta, const
     if (sell_valid_)
         UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
                                               sell_val_.get(), 0.0, dest.data(), beta, w.data(), dest.data(), 1,
-                                              rowptr_.get(), context().stream)));
+                                              rowptr_.get(), has_row_order() ? row_order_.get() : nullptr,
+                                              context().stream)));
     else
         UGX_LAUNCH((spmv_csr<double, double>(rows_, rowptr_.get(), col_.get(), val_.get(), 0.0, dest.data(), beta,
                                              w.data(), dest.data(), 1, lanes_, context().stream)));
@@ -682,6 +712,9 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     UGX_LAUNCH(dirichlet_csr<double>(L.N, L.n_inner, A.rowptr(), A.col(), A.val(), b.data(), c.stream));
     sync();
     A.set_origin(this);
+    int64_t lat_points = 1;
+    for (int d = 0; d < L.dim; ++d) lat_points *= L.W;
+    if (lat_points == L.N) A.set_row_order_device(L.dof_of_lat.get(), L.N);
     A.finalize();
     t_assemble_ = wall() - t0;
 }
@@ -703,6 +736,7 @@ void DomainDiscretization::assemble_adjoint_operator(const GPUSparseMatrix& A, G
     UGX_HIP(hipMemcpy(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) throw UGError("assemble_adjoint_operator: pattern is not structurally symmetric");
     AT.set_origin(nullptr);
+    if (A.has_row_order()) AT.set_row_order_device(L.dof_of_lat.get(), L.N);
     AT.finalize();
 }
 

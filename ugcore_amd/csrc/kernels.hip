@@ -172,19 +172,27 @@ static int launch_spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// Sliced ELL (64-row slices): the SpMV layout of GPUSparseMatrix. Slice t
-// holds rows 64t..64t+63; entry k of its rows is stored at
-// slice_ptr[t] + 64*k + lane (columns int32, values TM). Rows shorter than the
-// slice width are padded with value 0 and a valid column, so every lane runs
-// the same trip count and every load is one coalesced 64-lane access.
+// Sliced ELL (64-row slices): the SpMV layout of GPUSparseMatrix. Slice
+// position p = 64t + lane holds row order[p] (row p without an order); entry k
+// of that row is stored at slice_ptr[t] + 64*k + lane (columns int32, values
+// TM). Rows shorter than the slice width are padded with value 0 and a valid
+// column, so every lane runs the same trip count and every load is one
+// coalesced 64-lane access. The order (a permutation of the rows, e.g. the
+// lattice walk of the assembling discretization) only decides which rows share
+// a slice: it keeps the column gathers of neighbouring lanes close in memory;
+// each row's entries and summation order are unchanged.
 // ---------------------------------------------------------------------------
-__global__ void k_sell_widths(int64_t nrows, const int32_t* __restrict__ rowptr, int32_t* __restrict__ width) {
+__device__ __forceinline__ int64_t sell_row(const int32_t* order, int64_t p) { return order ? order[p] : p; }
+
+__global__ void k_sell_widths(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ order,
+                              int32_t* __restrict__ width) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t nslices = (nrows + 63) / 64;
     if (t >= nslices) return;
     int32_t w = 0;
-    const int64_t r1 = (t + 1) * 64 < nrows ? (t + 1) * 64 : nrows;
-    for (int64_t r = t * 64; r < r1; ++r) {
+    const int64_t p1 = (t + 1) * 64 < nrows ? (t + 1) * 64 : nrows;
+    for (int64_t p = t * 64; p < p1; ++p) {
+        const int64_t r = sell_row(order, p);
         const int32_t len = rowptr[r + 1] - rowptr[r];
         w = len > w ? len : w;
     }
@@ -194,14 +202,15 @@ __global__ void k_sell_widths(int64_t nrows, const int32_t* __restrict__ rowptr,
 template <typename TM>
 __global__ void k_sell_fill(int64_t nrows, int64_t ncols, const int32_t* __restrict__ rowptr,
                             const int32_t* __restrict__ col, const TM* __restrict__ val,
-                            const int64_t* __restrict__ slice_ptr, const int32_t* __restrict__ width,
-                            int32_t* __restrict__ scol, TM* __restrict__ sval) {
-    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t t = r >> 6;
+                            const int32_t* __restrict__ order, const int64_t* __restrict__ slice_ptr,
+                            const int32_t* __restrict__ width, int32_t* __restrict__ scol, TM* __restrict__ sval) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t t = p >> 6;
     if (t >= (nrows + 63) / 64) return;
-    const int lane = (int)(r & 63);
+    const int lane = (int)(p & 63);
     const int32_t w = width[t];
     const int64_t base = slice_ptr[t] + lane;
+    const int64_t r = p < nrows ? sell_row(order, p) : p;
     const int32_t rb = r < nrows ? rowptr[r] : 0;
     const int32_t len = r < nrows ? rowptr[r + 1] - rb : 0;
     const int32_t pad_col = (int32_t)(r < ncols ? r : 0);
@@ -219,13 +228,14 @@ __global__ __launch_bounds__(256) void k_sell_spmv(int64_t nrows, const int64_t*
                                                    const int32_t* __restrict__ width, const int32_t* __restrict__ scol,
                                                    const TM* __restrict__ sval, double alpha, const TV* v1, double beta,
                                                    const TV* __restrict__ w, TV* dest, int mode,
-                                                   const int32_t* __restrict__ rowptr, int64_t nblocks) {
+                                                   const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ order, int64_t nblocks) {
     const int64_t lb = xcd_block(nblocks);
     if (lb >= nblocks) return;
-    const int64_t r = lb * 256 + threadIdx.x;
-    const int64_t t = r >> 6;
+    const int64_t p = lb * 256 + threadIdx.x;
+    const int64_t t = p >> 6;
     if (t >= (nrows + 63) / 64) return;
-    const int lane = (int)(r & 63);
+    const int lane = (int)(p & 63);
     const int32_t wd = width[t];
     const int32_t* cp = scol + slice_ptr[t] + lane;
     const TM* vp = sval + slice_ptr[t] + lane;
@@ -250,7 +260,8 @@ __global__ __launch_bounds__(256) void k_sell_spmv(int64_t nrows, const int64_t*
         const double a0 = (double)__builtin_nontemporal_load(vp + 64 * k);
         s0 = fma(a0, (double)w[c0], s0);
     }
-    if (r >= nrows) return;
+    if (p >= nrows) return;
+    const int64_t r = sell_row(order, p);
     const double sum = s0 + s1;
     if (mode == 1) {
         if (rowptr[r + 1] > rowptr[r]) dest[r] = (TV)(beta * sum);
@@ -2250,33 +2261,39 @@ int boundary_flags(int dim, int W, int64_t n, uint8_t* dir, hipStream_t s) {
     UGX_CHECK_LAUNCH();
 }
 
-int sell_widths(int64_t nrows, const int32_t* rowptr, int32_t* width, hipStream_t s) {
+int sell_widths(int64_t nrows, const int32_t* rowptr, const int32_t* order, int32_t* width, hipStream_t s) {
     const int64_t nsl = (nrows + 63) / 64;
     if (nsl <= 0) return 0;
-    hipLaunchKernelGGL(k_sell_widths, dim3((nsl + 255) / 256), dim3(256), 0, s, nrows, rowptr, width);
+    hipLaunchKernelGGL(k_sell_widths, dim3((nsl + 255) / 256), dim3(256), 0, s, nrows, rowptr, order, width);
     UGX_CHECK_LAUNCH();
 }
 
 template <typename TM>
 int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
-              const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval, hipStream_t s) {
+              const int32_t* order, const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval,
+              hipStream_t s) {
     const int64_t nth = ((nrows + 63) / 64) * 64;
     if (nth <= 0) return 0;
-    hipLaunchKernelGGL(k_sell_fill<TM>, dim3((nth + 255) / 256), dim3(256), 0, s, nrows, ncols, rowptr, col, val, slice_ptr, width, scol, sval);
+    hipLaunchKernelGGL(k_sell_fill<TM>, dim3((nth + 255) / 256), dim3(256), 0, s, nrows, ncols, rowptr, col, val,
+                       order, slice_ptr, width, scol, sval);
     UGX_CHECK_LAUNCH();
 }
-template int sell_fill<double>(int64_t, int64_t, const int32_t*, const int32_t*, const double*, const int64_t*, const int32_t*, int32_t*, double*, hipStream_t);
+template int sell_fill<double>(int64_t, int64_t, const int32_t*, const int32_t*, const double*, const int32_t*,
+                               const int64_t*, const int32_t*, int32_t*, double*, hipStream_t);
 
 template <typename TM, typename TV>
 int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,
               double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode, const int32_t* rowptr,
-              hipStream_t s) {
+              const int32_t* order, hipStream_t s) {
     if (nrows <= 0) return 0;
     const int64_t nb = (((nrows + 63) / 64) * 64 + 255) / 256;
-    hipLaunchKernelGGL((k_sell_spmv<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, nrows, slice_ptr, width, scol, sval, alpha, v1, beta, w, dest, mode, rowptr, nb);
+    hipLaunchKernelGGL((k_sell_spmv<TM, TV>), dim3(remapped_grid(nb)), dim3(256), 0, s, nrows, slice_ptr, width,
+                       scol, sval, alpha, v1, beta, w, dest, mode, rowptr, order, nb);
     UGX_CHECK_LAUNCH();
 }
-template int sell_spmv<double, double>(int64_t, const int64_t*, const int32_t*, const int32_t*, const double*, double, const double*, double, const double*, double*, int, const int32_t*, hipStream_t);
+template int sell_spmv<double, double>(int64_t, const int64_t*, const int32_t*, const int32_t*, const double*, double,
+                                       const double*, double, const double*, double*, int, const int32_t*,
+                                       const int32_t*, hipStream_t);
 
 int adjoint_grad_q1_hex(int n, double h, const double* theta, const int32_t* dof_of_lat, int64_t n_inner,
                         const double* u, const double* lambda, double* grad, hipStream_t s) {

@@ -11,15 +11,17 @@ template <typename TM, typename TV>
 int spmv_csr(int64_t nrows, const int32_t* rowptr, const int32_t* col, const TM* val, double alpha, const TV* v1,
              double beta, const TV* w, TV* dest, int mode, int lanes, hipStream_t s);
 
-// sliced ELL (64-row slices) SpMV layout
-int sell_widths(int64_t nrows, const int32_t* rowptr, int32_t* width, hipStream_t s);
+// sliced ELL (64-row slices) SpMV layout; order (nullable) is the row
+// permutation that fills the slices
+int sell_widths(int64_t nrows, const int32_t* rowptr, const int32_t* order, int32_t* width, hipStream_t s);
 template <typename TM>
 int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
-              const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval, hipStream_t s);
+              const int32_t* order, const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval,
+              hipStream_t s);
 template <typename TM, typename TV>
 int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,
               double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode, const int32_t* rowptr,
-              hipStream_t s);
+              const int32_t* order, hipStream_t s);
 
 template <typename T>
 int vec_set(int64_t n, T* x, double v, hipStream_t s);
