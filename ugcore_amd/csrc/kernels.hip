@@ -1059,6 +1059,9 @@ struct MfZMap {
 #endif
 constexpr int kMfPrefetch = UGX_MF_PF;   // staged entries per thread and plane (P <= kMfPrefetch * 256)
 
+#ifndef UGX_MF_WAVES32
+#define UGX_MF_WAVES32 6
+#endif
 #ifndef UGX_MF_ROWS
 #define UGX_MF_ROWS 3
 #endif
@@ -1067,7 +1070,8 @@ constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= k
 // MODE 0: fused Jacobi step (staged st = dinv.*sd; sd_out = sd - A st; sc += st)
 // MODE 1: d_out = d_in - A x;  MODE 2: y = alpha v + beta A x
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? UGX_MF_WAVES32 : 1)))
+void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                                                 const T* __restrict__ b_in, const T* c_in, T* out0, T* out1,
                                                 double alpha, double beta, double* dot_partial) {
     // c_in and out0 may alias (d_out = d_in); every row is read and then
