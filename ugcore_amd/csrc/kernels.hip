@@ -433,15 +433,27 @@ __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const T* __restr
     block_partial(acc, partial);
 }
 
-__global__ __launch_bounds__(256) void k_sum_partials(int np, const double* partial, double* out, int sqrt_flag) {
+// one workgroup of 1024: thread t sums partial[t], partial[t + 1024], ... in
+// that order (eight loads in flight), then the waves and the 16 wave sums are
+// combined in a fixed order, so the result does not depend on timing
+__global__ __launch_bounds__(1024) void k_sum_partials(int np, const double* partial, double* out, int sqrt_flag) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += 256) s += partial[i];
-    __shared__ double sh[4];
+    int i = threadIdx.x;
+    for (; i + 7 * 1024 < np; i += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[i + u * 1024];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; i < np; i += 1024) s += partial[i];
+    __shared__ double sh[16];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        double r = 0.0;
+        for (int w = 0; w < 16; ++w) r += sh[w];
         out[0] = sqrt_flag ? sqrt(r) : r;
     }
 }
@@ -1939,13 +1951,13 @@ template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s) {
     const int g = red_grid(n);
     hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(256), 0, s, n, a, b, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, out, take_sqrt ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, out, take_sqrt ? 1 : 0);
     UGX_CHECK_LAUNCH();
 }
 template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
 
 int sum_partials(int np, const double* partial, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, np, partial, out, 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, np, partial, out, 0);
     UGX_CHECK_LAUNCH();
 }
 int cg_alpha(double* sc, hipStream_t s) {
@@ -1960,7 +1972,7 @@ int cg_update(int64_t n, const double* sc, const double* p, const double* q, dou
               hipStream_t s) {
     const int g = red_grid(n);
     hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, s, n, sc, p, q, x, r, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, (double*)(sc + 5), 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, (double*)(sc + 5), 0);
     UGX_CHECK_LAUNCH();
 }
 int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s) {
@@ -1971,7 +1983,7 @@ template <typename T>
 int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s) {
     const int g = red_grid(n);
     hipLaunchKernelGGL(k_convert_dot<T>, dim3(g), dim3(256), 0, s, n, c, z, r, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, g, partial, out, 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, out, 0);
     UGX_CHECK_LAUNCH();
 }
 template int convert_dot<float>(int64_t, const float*, double*, const double*, double*, double*, hipStream_t);
