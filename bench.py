@@ -131,6 +131,37 @@ def bench_poisson_2d(U, seed, reps):
             "assembly_s": d.assemble_seconds()}
 
 
+def bench_galerkin(U, disc, A, b, base_level, reps):
+    """the headline 256^3 Q1 problem with Galerkin coarse operators
+    (set_rap: A_{l-1} = R_l A_l P_l from the assembled top level) instead of
+    the assembled level matrices: set-up (R A P on every level, FP64) and
+    solve, both preconditioner precisions"""
+    N = disc.num_dofs()
+    out = {"workload": "256^3 Q1 FE as the headline solve, GMG with Galerkin coarse operators (set_rap), "
+                       "CG tol 1e-10"}
+    for prec in ("f32", "f64"):
+        g = U.GeometricMultiGrid(disc, base_level=base_level, nu1=2, nu2=2, cycle=1, omega=0.66, precision=prec,
+                                 rap=True)
+        S = U.CG(g, max_steps=100, reduction=1e-10)
+        U.synchronize()
+        t0 = time.perf_counter()
+        S.init(A)
+        U.synchronize()
+        setup = time.perf_counter() - t0
+        x = U.Vector(N)
+
+        def solve():
+            x.set(0.0)
+            if not S.apply(x, b):
+                raise RuntimeError("Galerkin solve did not converge")
+        t, _ = _timed(U, solve, reps)
+        out[f"{prec}_precond_solve_s"] = t
+        out[f"{prec}_precond_cg_steps"] = S.steps()
+        out[f"{prec}_setup_s"] = setup
+        del S, g
+    return out
+
+
 def bench_adjoint(U, refs, seed, reps, precision="f32"):
     """adjoint gradient of J = 1/2 |u(obs) - d|^2 for the Q1 log-diffusion
     field: forward GMG-CG solve, adjoint GMG-CG solve (same operator, Dirichlet
@@ -478,6 +509,8 @@ def main():
             secondary[f"adjoint_q1_{2 ** r_adj}"] = bench_adjoint(U, r_adj, args.seed, max(1, args.steps // 2))
         log("[bench] secondary: tetrahedra 96^3 x 6")
         secondary["tet_fv1_upwind_96"] = bench_tet(U, 5, args.seed, max(1, args.steps // 2))
+        log(f"[bench] secondary: Galerkin coarse operators at {n}^3")
+        secondary[f"galerkin_q1_{n}"] = bench_galerkin(U, disc, A, b, args.base_level, args.steps)
 
     cpu = None
     if not args.skip_cpu:

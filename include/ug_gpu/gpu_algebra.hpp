@@ -345,6 +345,8 @@ class DomainDiscretization {
     // assemble a level operator in lattice numbering as DIA values (27 or 9
     // diagonals of length N, FP64), Dirichlet rows set
     void assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd, std::vector<int32_t>& offsets);
+    // the diagonal count and lattice offsets of a level's DIA storage
+    void level_dia_layout(int lev, int& nd, std::vector<int32_t>& offsets) const;
     double assemble_seconds() const { return t_assemble_; }
     // theta from device memory (fine-element lattice order)
     void set_theta_device(const double* theta_dev);
@@ -403,6 +405,11 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // per-element coefficients (matrix-free) instead of the stored diagonals
     // (default on; the stored level matrices are kept either way)
     void set_matrix_free(bool on) { matrix_free_ = on; }
+    // Galerkin coarse operators A_{l-1} = R_l A_l P_l from the assembled top
+    // level instead of the assembled level matrices (set_rap, mg_solver.h:149;
+    // init_rap_operator, mg_solver_impl.hpp:829-1013)
+    void set_rap(bool on) { rap_ = on; }
+    bool rap() const { return rap_; }
     bool init(std::shared_ptr<MatrixOperator> J) override;
     bool apply(GPUVector& c, const GPUVector& d) override;
     // testing access: level matrix (FP64 DIA values in lattice numbering)
@@ -440,6 +447,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     Precision prec_ = Precision::F64;
     bool transposed_ = false;
     bool matrix_free_ = true;
+    bool rap_ = false;
     Impl* impl_ = nullptr;
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;

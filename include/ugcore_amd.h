@@ -194,6 +194,10 @@ int ugx_gmg_set_transposed(void* g, int transposed);
  * same operator up to rounding), 0 = from the stored level matrix. No
  * counterpart in the reference (a storage choice behind apply/apply_sub). */
 int ugx_gmg_set_matrix_free(void* g, int on);
+/* before ugx_gmg_init: 1 = Galerkin coarse operators A_{l-1} = R_l A_l P_l
+ * from the assembled top level (hexahedral and triangular hierarchies);
+ * replaces AssembledMultiGridCycle::set_rap (mg_solver.h:149) */
+int ugx_gmg_set_rap(void* g, int on);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

@@ -93,6 +93,47 @@ CSR transpose_drop_zeros(const CSR& A) {
     return T;
 }
 
+CSR multiply_of(const CSR& A, const CSR& B, const CSR& C) {
+    if (A.n_cols != B.n_rows || B.n_cols != C.n_rows) throw std::runtime_error("multiply_of: size mismatch");
+    CSR M;
+    M.n_rows = A.n_rows;
+    M.n_cols = C.n_cols;
+    M.rowptr.assign(M.n_rows + 1, 0);
+    std::vector<double> row(C.n_cols, 0.0);
+    std::vector<char> used(C.n_cols, 0);
+    std::vector<int> touched;
+    for (int64_t i = 0; i < A.n_rows; ++i) {
+        touched.clear();
+        for (int64_t p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) {
+            if (A.val[p] == 0.0) continue;
+            const int k = A.col[p];
+            for (int64_t q = B.rowptr[k]; q < B.rowptr[k + 1]; ++q) {
+                if (B.val[q] == 0.0) continue;
+                const double ab = A.val[p] * B.val[q];
+                const int l = B.col[q];
+                for (int64_t t = C.rowptr[l]; t < C.rowptr[l + 1]; ++t) {
+                    if (C.val[t] == 0.0) continue;
+                    const int j = C.col[t];
+                    if (!used[j]) {
+                        used[j] = 1;
+                        touched.push_back(j);
+                    }
+                    row[j] += ab * C.val[t];
+                }
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int j : touched) {
+            M.col.push_back(j);
+            M.val.push_back(row[j]);
+            row[j] = 0.0;
+            used[j] = 0;
+        }
+        M.rowptr[i + 1] = (int64_t)M.col.size();
+    }
+    return M;
+}
+
 void set_row(CSR& A, int64_t r, double v) {
     for (int64_t k = A.rowptr[r]; k < A.rowptr[r + 1]; ++k) A.val[k] = v;
 }

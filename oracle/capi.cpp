@@ -176,6 +176,21 @@ int orc_assemble(void* h, int lev, double f_const) {
     }
 }
 
+// Galerkin coarse operators A_{l-1} = R_l A_l P_l from the top level down
+// (AssembledMultiGridCycle::init_rap_operator, mg_solver_impl.hpp:829-1013);
+// call after every level is assembled
+int orc_set_rap(void* h) {
+    auto* p = (Problem*)h;
+    try {
+        const int top = (int)p->mg.levels.size() - 1;
+        for (int lev = top; lev >= 1; --lev) p->A[lev - 1] = multiply_of(p->R[lev], p->A[lev], p->P[lev]);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 int64_t orc_matrix_nnz(void* h, int which, int lev) {
     auto* p = (Problem*)h;
     const CSR& A = which == 0 ? p->A[lev] : which == 1 ? p->P[lev] : p->R[lev];

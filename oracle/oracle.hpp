@@ -102,6 +102,9 @@ CSR transpose_keep_zeros(const CSR& A);                    // set_as_transpose_o
 CSR transpose_drop_zeros(const CSR& A);                    // set_as_transpose_of2, sparsematrix_impl.h:185-254
 void set_row(CSR& A, int64_t r, double v);                 // SetRow, sparsematrix_util.h:759-765
 double* find_entry(CSR& A, int64_t r, int64_t c);
+// M = A*B*C with entries created for nonzero products only, rows sorted
+// (AddMultiplyOf into a cleared matrix, sparsematrix_util.h:167-240)
+CSR multiply_of(const CSR& A, const CSR& B, const CSR& C);
 
 // y = alpha1*v1 + beta1*A*w1 (SparseMatrix::axpy, sparsematrix_impl.h:291-339)
 void axpy(const CSR& A, std::vector<double>& dest, double alpha1, const std::vector<double>& v1,

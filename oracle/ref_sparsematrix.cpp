@@ -8,6 +8,7 @@
 
 #include "lib_algebra/cpu_algebra/sparsematrix_impl.h"
 #include "lib_algebra/cpu_algebra/vector_impl.h"
+#include "lib_algebra/algebra_common/sparsematrix_util.h"
 
 using Mat = ug::SparseMatrix<double>;
 using Vec = ug::Vector<double>;
@@ -31,6 +32,17 @@ void* ref_sm_derive(void* h, int mode) {
     else if (mode == 1) b->set_as_transpose_of(*a);
     else b->set_as_transpose_of2(*a);
     return b;
+}
+
+// M = A*B*C through AddMultiplyOf into a cleared matrix (the Galerkin
+// product of AssembledMultiGridCycle::init_rap_operator)
+void* ref_sm_multiply3(void* a, void* b, void* c) {
+    const Mat& A = *(const Mat*)a;
+    const Mat& C = *(const Mat*)c;
+    Mat* m = new Mat();
+    m->resize_and_clear(A.num_rows(), C.num_cols());
+    ug::AddMultiplyOf(*m, A, *(const Mat*)b, C);
+    return m;
 }
 
 int64_t ref_sm_rows(void* h) { return (int64_t)((Mat*)h)->num_rows(); }

@@ -60,6 +60,7 @@ def lib():
             "orc_solve_adjoint": (C.c_int, [P, C.c_int, F64P, F64P, C.c_int, C.c_int, C.c_int, C.c_double,
                                             C.c_double, C.c_int]),
             "orc_timings": (None, [P, F64P]),
+            "orc_set_rap": (C.c_int, [P]),
             "orc_builder_create": (P, [I64, I64]),
             "orc_builder_set": (None, [P, I64, I64, C.c_double]),
             "orc_builder_add": (None, [P, I64, I64, C.c_double]),
@@ -93,6 +94,7 @@ def ref_lib():
             "ref_sm_nnz": (I64, [P]),
             "ref_sm_csr": (None, [P, I64P, I32P, F64P]),
             "ref_sm_op": (None, [P, C.c_int, C.c_double, C.c_void_p, C.c_double, F64P, F64P]),
+            "ref_sm_multiply3": (P, [P, P, P]),
             "ref_vec_dot": (C.c_double, [I64, F64P, F64P]),
             "ref_vec_norm": (C.c_double, [I64, F64P]),
         }
@@ -244,6 +246,12 @@ class Problem:
         lib().orc_adjoint_gradient(self.h, np.ascontiguousarray(u, np.float64),
                                    np.ascontiguousarray(lam, np.float64), g)
         return g
+
+    def set_rap(self):
+        """replace the assembled coarse matrices by Galerkin products R A P
+        (after assemble_all)"""
+        if lib().orc_set_rap(self.h) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
 
     def set_tet_params(self, vel=(0.0, 0.0, 0.0), jitter_seed=0, jitter_amp=0.0):
         lib().orc_set_tet_params(self.h, float(vel[0]), float(vel[1]), float(vel[2]), int(jitter_seed),

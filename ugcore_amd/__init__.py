@@ -55,6 +55,7 @@ _SIGNATURES = {
     "ugx_disc_adjoint_operator": (_I, [_P, _P, _P]),
     "ugx_gmg_set_transposed": (_I, [_P, _I]),
     "ugx_gmg_set_matrix_free": (_I, [_P, _I]),
+    "ugx_gmg_set_rap": (_I, [_P, _I]),
     "ugx_disc_create": (_P, [_I, _I]),
     "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
     "ugx_disc_destroy": (None, [_P]),
@@ -419,10 +420,11 @@ class DomainDiscretization:
 class GeometricMultiGrid:
     """AssembledMultiGridCycle: V(nu1,nu2) (cycle=1) or W (cycle=2), damped
     Jacobi smoother, P1 transfer, dense FP64 LU on the base level.
-    precision 'f64' or 'f32' for the level operators."""
+    precision 'f64' or 'f32' for the level operators; rap=True builds the
+    coarse levels as Galerkin products R A P (set_rap)."""
 
     def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64", transposed=False,
-                 matrix_free=True):
+                 matrix_free=True, rap=False):
         self.disc = disc
         p = 1 if precision in ("f32", "float32", 1) else 0
         self.h = _handle(lib().ugx_gmg_create(disc.h, base_level, nu1, nu2, cycle, float(omega), p),
@@ -431,6 +433,8 @@ class GeometricMultiGrid:
             _check(lib().ugx_gmg_set_transposed(self.h, 1), "GMG.set_transposed")
         if not matrix_free:
             _check(lib().ugx_gmg_set_matrix_free(self.h, 0), "GMG.set_matrix_free")
+        if rap:
+            _check(lib().ugx_gmg_set_rap(self.h, 1), "GMG.set_rap")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

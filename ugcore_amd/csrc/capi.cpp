@@ -471,6 +471,12 @@ int ugx_gmg_set_matrix_free(void* g, int on) {
         return 0;
     });
 }
+int ugx_gmg_set_rap(void* g, int on) {
+    return guard([&] {
+        ((GmgH*)g)->g->set_rap(on != 0);
+        return 0;
+    });
+}
 int ugx_gmg_set_transposed(void* g, int transposed) {
     return guard([&] {
         ((GmgH*)g)->g->set_transposed(transposed != 0);

@@ -740,10 +740,8 @@ void DomainDiscretization::assemble_adjoint_operator(const GPUSparseMatrix& A, G
     AT.finalize();
 }
 
-void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd,
-                                              std::vector<int32_t>& offsets) {
-    auto& c = context();
-    LevelGrid& L = *levels_[lev];
+void DomainDiscretization::level_dia_layout(int lev, int& nd, std::vector<int32_t>& offsets) const {
+    const LevelGrid& L = *levels_[lev];
     nd = L.dim == 3 ? 27 : 9;
     offsets.assign(nd, 0);
     const int64_t W = L.W;
@@ -751,6 +749,13 @@ void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia
         const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = L.dim == 3 ? d / 9 - 1 : 0;
         offsets[d] = (int32_t)(dx + dy * W + dz * W * W);
     }
+}
+
+void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd,
+                                              std::vector<int32_t>& offsets) {
+    auto& c = context();
+    LevelGrid& L = *levels_[lev];
+    level_dia_layout(lev, nd, offsets);
     dia.resize(dia_tiled_size(L.N, nd));
     dia.zero();
     const double hsz = 1.0 / L.n;
@@ -836,6 +841,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     DeviceBuffer<T> cell_c;      // per-element coefficients h k_e of the top level
     DeviceBuffer<double> cell_c64;
     bool sym = true;             // level products through the symmetric storage
+    bool rap = false;            // Galerkin coarse operators R A P below the top level
 
     // level products: symmetric storage when every level vector vanishes on
     // Dirichlet DoFs (the defect entering the cycle does), full otherwise
@@ -854,10 +860,14 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                                            c.stream);
     }
 
-    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free)
+    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free, bool galerkin)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
           sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2),
-          mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()) {}
+          mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()), rap(galerkin) {
+        if (rap && d->kind() == 2)
+            throw UGError("GMG: Galerkin coarse operators need the P1 lattice transfer (hexahedral or triangular "
+                          "hierarchy)");
+    }
 
     void alloc_vec(DeviceBuffer<T>& buf, T*& p, int64_t N, int64_t H) {
         buf.resize(N + 2 * H);
@@ -880,6 +890,23 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             cell_c64.resize(ne);
             UGX_LAUNCH(q1mf_coeff<double>(ne, 1.0 / G.n, disc->theta_device(), cell_c64.get(), c.stream));
         }
+        // Galerkin hierarchy (AssembledMultiGridCycle::init_rap_operator,
+        // mg_solver_impl.hpp:829-1013): the assembled top-level operator, then
+        // A_{l-1} = R_l A_l P_l from the top down
+        std::vector<DeviceBuffer<double>> galerkin(rap ? top + 1 : 0);
+        std::vector<int> galerkin_nd(rap ? top + 1 : 0);
+        std::vector<std::vector<int32_t>> galerkin_off(rap ? top + 1 : 0);
+        if (rap) {
+            disc->assemble_level_dia(top, galerkin[top], galerkin_nd[top], galerkin_off[top]);
+            for (int lev = top; lev > base; --lev) {
+                LevelGrid& Gf = disc->level(lev);
+                LevelGrid& Gc = disc->level(lev - 1);
+                disc->level_dia_layout(lev - 1, galerkin_nd[lev - 1], galerkin_off[lev - 1]);
+                galerkin[lev - 1].resize(dia_tiled_size(Gc.N, galerkin_nd[lev - 1]));
+                UGX_LAUNCH(rap_dia(Gf.dim, Gf.W, Gc.W, Gf.dir.get(), Gc.dir.get(), galerkin[lev].get(),
+                                   galerkin[lev - 1].get(), c.stream));
+            }
+        }
         for (int lev = base; lev <= top; ++lev) {
             LevelGrid& G = disc->level(lev);
             GmgLevel<T>& l = L[lev];
@@ -890,7 +917,13 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             const int64_t W = G.W;
             l.H = G.dim == 3 ? ((W * W + W + 1 + 63) / 64) * 64 : ((W + 1 + 63) / 64) * 64;
             DeviceBuffer<double> A64;
-            disc->assemble_level_dia(lev, A64, l.nd, l.off);
+            if (rap) {
+                A64 = std::move(galerkin[lev]);
+                l.nd = galerkin_nd[lev];
+                l.off = galerkin_off[lev];
+            } else {
+                disc->assemble_level_dia(lev, A64, l.nd, l.off);
+            }
             if (transposed) {
                 if (l.nd != 27) throw UGError("GMG: transposed level operators need a 3D lattice");
                 DeviceBuffer<double> AT(A64.size());
@@ -1193,9 +1226,11 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     delete impl_;
     impl_ = nullptr;
     if (prec_ == Precision::F64)
-        impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_);
+        impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
+                                    rap_);
     else
-        impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_);
+        impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
+                                   rap_);
     impl_->init();
     return true;
 }

@@ -157,6 +157,10 @@ int prolong_add_tet(int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* p
 template <typename TV>
 int restrict_tet(int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* pcode, const uint8_t* dir_c,
                  const uint8_t* dir_f, TV* sdc, hipStream_t s);
+// Galerkin coarse operator R A_f P of the P1 lattice transfer (dim 3: 27
+// diagonals, dim 2: 9), tiled DIA storage, FP64; Wf = 2 Wc - 1
+int rap_dia(int dim, int Wf, int Wc, const uint8_t* dir_f, const uint8_t* dir_c, const double* Af, double* Ac,
+            hipStream_t s);
 template <typename T>
 int dia_transpose_elim(int W, int64_t n, const uint8_t* dir, const T* A, T* AT, hipStream_t s);
 int csr_transpose_map(int64_t n, const int32_t* rowptr, const int32_t* col, int32_t* tp, int* bad, hipStream_t s);
