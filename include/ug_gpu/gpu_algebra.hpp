@@ -345,6 +345,10 @@ class DomainDiscretization {
     // assemble a level operator in lattice numbering as DIA values (27 or 9
     // diagonals of length N, FP64), Dirichlet rows set
     void assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd, std::vector<int32_t>& offsets);
+    // coordinates of the level's DoFs in DoF order (2 or 3 per DoF), as the
+    // grid function's approximation space would give them to
+    // SaveMatrixForConnectionViewer (grid_function_util.h)
+    void dof_positions(int lev, double* host_out);
     // the diagonal count and lattice offsets of a level's DIA storage
     void level_dia_layout(int lev, int& nd, std::vector<int32_t>& offsets) const;
     double assemble_seconds() const { return t_assemble_; }
@@ -452,6 +456,20 @@ class AssembledMultiGridCycle : public ILinearIterator {
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;
 };
+
+// ---------------------------------------------------------------------------
+// file formats (io.cpp): ConnectionViewer text output (connection_viewer_
+// output.h:74-141, 402-422) and the serial SaveToFile / ReadFromFile vector
+// file (restart_bridge.cpp:49-75, 146-158; vector_impl.h:347-361)
+// ---------------------------------------------------------------------------
+void cv_write_matrix(const std::string& path, int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                     const double* positions, int dim);
+void cv_write_vector(const std::string& path, int64_t n, const double* values, const double* positions, int dim);
+void vector_file_write(const std::string& path, int64_t n, const double* values);
+int64_t vector_file_length(const std::string& path);
+void vector_file_read(const std::string& path, int64_t n, double* values);
+void SaveToFile(const GPUVector& v, const std::string& filename);
+void ReadFromFile(GPUVector& v, const std::string& filename);
 
 }  // namespace gpu
 }  // namespace ug

@@ -116,6 +116,10 @@ void ugx_disc_destroy(void* disc);
 int ugx_disc_num_levels(void* disc);
 int64_t ugx_disc_num_dofs(void* disc, int level);
 int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out_host);   /* (n+1)^dim entries */
+/* DoF coordinates of a level in DoF order, dim (2 or 3) per DoF: what the
+ * approximation space hands to SaveMatrixForConnectionViewer
+ * (grid_function_util.h); tetrahedra: the jittered top-level positions */
+int ugx_disc_dof_positions(void* disc, int level, double* out_host);
 int ugx_disc_set_theta(void* disc, const double* theta_host);            /* k = exp(theta) per fine element */
 int ugx_disc_set_source_const(void* disc, double f);
 int ugx_disc_set_source_dofs(void* disc, const double* f_host);          /* per top-level DoF (FV1) */
@@ -232,6 +236,25 @@ int ugx_solver_set_lattice_path(void* s, int on);
 /* 1 if the last CG apply ran on the lattice path */
 int ugx_solver_used_lattice_path(void* s);
 int ugx_solver_defects(void* s, double* out_host, int max_n);
+
+/* ---- file formats (host only, no GPU work) -------------------------------
+ * ConnectionViewer text files: ConnectionViewer::WriteMatrix / WriteVector
+ * (lib_algebra/common/connection_viewer_output.h:74-141, 402-422) with one
+ * position (dim coordinates) per row; rowptr/col/val a host CSR. */
+int ugx_cv_write_matrix(const char* path, int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                        const double* positions, int dim);
+int ugx_cv_write_vector(const char* path, int64_t n, const double* values, const double* positions, int dim);
+/* the serial SaveToFile / ReadFromFile vector file (bridge/algebra_bridges/
+ * restart_bridge.cpp:49-75, 146-158; lib_algebra/cpu_algebra/vector_impl.h:
+ * 347-361): host arrays; ugx_vector_file_length returns the stored length or
+ * -1 (message in ugx_last_error) */
+int ugx_vector_file_write(const char* path, int64_t n, const double* values);
+int64_t ugx_vector_file_length(const char* path);
+int ugx_vector_file_read(const char* path, int64_t n, double* values);
+/* SaveToFile(v, filename) / ReadFromFile(v, filename) on a GPUVector handle
+ * (restart_bridge.cpp:146-158); ReadFromFile resizes v */
+int ugx_vector_save(void* v, const char* path);
+int ugx_vector_read(void* v, const char* path);
 
 #ifdef __cplusplus
 }

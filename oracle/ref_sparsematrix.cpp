@@ -9,6 +9,8 @@
 #include "lib_algebra/cpu_algebra/sparsematrix_impl.h"
 #include "lib_algebra/cpu_algebra/vector_impl.h"
 #include "lib_algebra/algebra_common/sparsematrix_util.h"
+#include "common/serialization.h"
+#include "common/util/binary_buffer.h"
 
 using Mat = ug::SparseMatrix<double>;
 using Vec = ug::Vector<double>;
@@ -76,6 +78,19 @@ void ref_sm_op(void* h, int op, double alpha, const double* v1, double beta, con
     else if (op == 1) a->apply_ignore_zero_rows(d, beta, ww);
     else a->axpy_transposed(d, alpha, vv, beta, ww);
     for (size_t i = 0; i < nd; ++i) dest[i] = d[i];
+}
+
+// the bytes Serialize(BinaryBuffer, Vector<double>) produces (the payload of
+// SaveToFile, restart_bridge.cpp:146-151); returns their count, copies at most
+// cap of them to out
+int64_t ref_vec_serialize(int64_t n, const double* a, char* out, int64_t cap) {
+    Vec x(n);
+    for (int64_t i = 0; i < n; ++i) x[i] = a[i];
+    ug::BinaryBuffer b;
+    ug::Serialize(b, x);
+    const int64_t len = (int64_t)b.write_pos();
+    if (out) std::memcpy(out, b.buffer(), (size_t)(len < cap ? len : cap));
+    return len;
 }
 
 double ref_vec_dot(int64_t n, const double* a, const double* b) {

@@ -95,6 +95,7 @@ def ref_lib():
             "ref_sm_csr": (None, [P, I64P, I32P, F64P]),
             "ref_sm_op": (None, [P, C.c_int, C.c_double, C.c_void_p, C.c_double, F64P, F64P]),
             "ref_sm_multiply3": (P, [P, P, P]),
+            "ref_vec_serialize": (I64, [I64, F64P, C.c_char_p, I64]),
             "ref_vec_dot": (C.c_double, [I64, F64P, F64P]),
             "ref_vec_norm": (C.c_double, [I64, F64P]),
         }

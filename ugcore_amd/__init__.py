@@ -62,6 +62,14 @@ _SIGNATURES = {
     "ugx_disc_num_levels": (_I, [_P]),
     "ugx_disc_num_dofs": (_I64, [_P, _I]),
     "ugx_disc_dof_of_lattice": (_I, [_P, _I, _P]),
+    "ugx_disc_dof_positions": (_I, [_P, _I, _P]),
+    "ugx_cv_write_matrix": (_I, [C.c_char_p, _I64, _P, _P, _P, _P, _I]),
+    "ugx_cv_write_vector": (_I, [C.c_char_p, _I64, _P, _P, _I]),
+    "ugx_vector_file_write": (_I, [C.c_char_p, _I64, _P]),
+    "ugx_vector_file_length": (_I64, [C.c_char_p]),
+    "ugx_vector_file_read": (_I, [C.c_char_p, _I64, _P]),
+    "ugx_vector_save": (_I, [_P, C.c_char_p]),
+    "ugx_vector_read": (_I, [_P, C.c_char_p]),
     "ugx_disc_set_theta": (_I, [_P, _P]),
     "ugx_disc_set_theta_device": (_I, [_P, _P]),
     "ugx_disc_adjust_defect": (_I, [_P, _P]),
@@ -362,6 +370,14 @@ class DomainDiscretization:
     def lattice_width(self, level):
         return (3 if self.kind == 2 else 1) * 2 ** level + 1
 
+    def dof_positions(self, level=None):
+        """coordinates of the level's DoFs in DoF order, shape (N, dim)"""
+        level = self.num_refs if level is None else level
+        n = lib().ugx_disc_num_dofs(self.h, level)
+        out = np.empty((n, self.dim), np.float64)
+        _check(lib().ugx_disc_dof_positions(self.h, level, _ptr(out)), "dof_positions")
+        return out
+
     def dof_of_lattice(self, level):
         n = self.lattice_width(level) ** self.dim
         out = np.empty(n, np.int32)
@@ -522,3 +538,72 @@ class CG(_Solver):
 
 class BiCGStab(_Solver):
     _type = 1
+
+
+# ---- file formats ------------------------------------------------------------------------------
+
+def _path(filename):
+    return os.fsencode(os.fspath(filename))
+
+
+def SaveToFile(v, filename):
+    """binary vector file of a serial build (restart_bridge.cpp:146-158)"""
+    _check(lib().ugx_vector_save(v.h, _path(filename)), "SaveToFile")
+
+
+def ReadFromFile(v, filename):
+    """reads a SaveToFile vector into v (resized to the stored length)"""
+    _check(lib().ugx_vector_read(v.h, _path(filename)), "ReadFromFile")
+
+
+def write_vector_file(filename, values):
+    """host-array form of SaveToFile"""
+    a = np.ascontiguousarray(values, np.float64)
+    _check(lib().ugx_vector_file_write(_path(filename), a.size, _ptr(a)), "write_vector_file")
+
+
+def read_vector_file(filename):
+    """host-array form of ReadFromFile"""
+    n = lib().ugx_vector_file_length(_path(filename))
+    if n < 0:
+        raise UGError(f"read_vector_file: {lib().ugx_last_error().decode()}")
+    out = np.empty(n, np.float64)
+    _check(lib().ugx_vector_file_read(_path(filename), n, _ptr(out)), "read_vector_file")
+    return out
+
+
+def write_connection_viewer_matrix(filename, rowptr, col, val, positions):
+    """ConnectionViewer::WriteMatrix (connection_viewer_output.h:74-141) for a
+    host CSR and one position per row (shape (n, dim))"""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    pos = np.ascontiguousarray(positions, np.float64)
+    n = rowptr.size - 1
+    if pos.shape[0] != n:
+        raise UGError("write_connection_viewer_matrix: one position per row is needed")
+    _check(lib().ugx_cv_write_matrix(_path(filename), n, _ptr(rowptr), _ptr(col), _ptr(val), _ptr(pos),
+                                     pos.shape[1]), "write_connection_viewer_matrix")
+
+
+def write_connection_viewer_vector(filename, values, positions):
+    """ConnectionViewer::WriteVector (connection_viewer_output.h:402-422)"""
+    a = np.ascontiguousarray(values, np.float64)
+    pos = np.ascontiguousarray(positions, np.float64)
+    if pos.shape[0] != a.size:
+        raise UGError("write_connection_viewer_vector: one position per value is needed")
+    _check(lib().ugx_cv_write_vector(_path(filename), a.size, _ptr(a), _ptr(pos), pos.shape[1]),
+           "write_connection_viewer_vector")
+
+
+def SaveMatrixForConnectionViewer(disc, A, filename):
+    """the surface matrix with the top-level DoF positions
+    (SaveMatrixForConnectionViewer, grid_function_util.h)"""
+    rp, col, val = A.download_csr()
+    write_connection_viewer_matrix(filename, rp, col, val, disc.dof_positions())
+
+
+def SaveVectorForConnectionViewer(disc, v, filename):
+    """a surface vector with the top-level DoF positions
+    (SaveVectorForConnectionViewer, grid_function_util.h)"""
+    write_connection_viewer_vector(filename, v.numpy(), disc.dof_positions())

@@ -304,6 +304,12 @@ int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out) {
         return 0;
     });
 }
+int ugx_disc_dof_positions(void* disc, int level, double* out) {
+    return guard([&] {
+        ((DiscH*)disc)->d->dof_positions(level, out);
+        return 0;
+    });
+}
 int ugx_disc_set_theta(void* disc, const double* theta) {
     return guard([&] {
         ((DiscH*)disc)->d->set_theta(theta);
@@ -586,6 +592,61 @@ int ugx_solver_defects(void* s, double* out, int max_n) {
     if (out)
         for (int i = 0; i < n && i < max_n; ++i) out[i] = d[i];
     return n;
+}
+
+// ---- file formats -------------------------------------------------------------------------
+
+int ugx_cv_write_matrix(const char* path, int64_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                        const double* positions, int dim) {
+    return guard([&] {
+        if (!path || n < 0 || (n > 0 && (!rowptr || !positions))) throw UGError("ugx_cv_write_matrix: bad arguments");
+        cv_write_matrix(path, n, rowptr, col, val, positions, dim);
+        return 0;
+    });
+}
+int ugx_cv_write_vector(const char* path, int64_t n, const double* values, const double* positions, int dim) {
+    return guard([&] {
+        if (!path || n < 0 || (n > 0 && (!values || !positions))) throw UGError("ugx_cv_write_vector: bad arguments");
+        cv_write_vector(path, n, values, positions, dim);
+        return 0;
+    });
+}
+int ugx_vector_file_write(const char* path, int64_t n, const double* values) {
+    return guard([&] {
+        if (!path || n < 0 || (n > 0 && !values)) throw UGError("ugx_vector_file_write: bad arguments");
+        vector_file_write(path, n, values);
+        return 0;
+    });
+}
+int64_t ugx_vector_file_length(const char* path) {
+    try {
+        if (!path) throw UGError("ugx_vector_file_length: no path");
+        return vector_file_length(path);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+int ugx_vector_file_read(const char* path, int64_t n, double* values) {
+    return guard([&] {
+        if (!path || n < 0 || (n > 0 && !values)) throw UGError("ugx_vector_file_read: bad arguments");
+        vector_file_read(path, n, values);
+        return 0;
+    });
+}
+int ugx_vector_save(void* v, const char* path) {
+    return guard([&] {
+        if (!path) throw UGError("ugx_vector_save: no path");
+        SaveToFile(((VectorH*)v)->v, path);
+        return 0;
+    });
+}
+int ugx_vector_read(void* v, const char* path) {
+    return guard([&] {
+        if (!path) throw UGError("ugx_vector_read: no path");
+        ReadFromFile(((VectorH*)v)->v, path);
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -618,6 +618,16 @@ ugamd::TetDisc tet_disc(const DomainDiscretization& D, const LevelGrid& L, int l
 }
 }  // namespace
 
+void DomainDiscretization::dof_positions(int lev, double* host_out) {
+    if (lev < 0 || lev > top_level()) throw UGError("dof_positions: level out of range");
+    LevelGrid& L = *levels_[lev];
+    const int dim = kind_ == 1 ? 2 : 3;
+    DeviceBuffer<double> out(L.N * dim);
+    UGX_LAUNCH(ugamd::dof_positions(tet_disc(*this, L, lev, nullptr), kind_, dim, L.W, L.N, L.lat_of_dof.get(),
+                                    out.get(), context().stream));
+    out.download(host_out, L.N * dim);
+}
+
 void DomainDiscretization::set_theta(const double* theta_host) {
     const int64_t nf = fine_cells();
     const int64_t n = kind_ == 1 ? nf * nf : nf * nf * nf;

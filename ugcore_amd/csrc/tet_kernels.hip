@@ -574,4 +574,33 @@ int csr_adjoint_values(int64_t n, int64_t n_inner, const int32_t* rowptr, const 
     UGX_TET_CHECK();
 }
 
+// coordinates of the DoFs of a level, in DoF order (dim per DoF): the
+// lattice points of the unit square/cube, for the tetrahedral hierarchy the
+// vertex positions the assembly uses (jittered on the top level)
+__global__ void k_dof_positions(TetDisc P, int kind, int dim, int W, int64_t N, const int32_t* __restrict__ lat_of_dof,
+                                double* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t l = lat_of_dof[i];
+    const int x = (int)(l % W), y = (int)((l / W) % W), z = dim == 3 ? (int)(l / ((int64_t)W * W)) : 0;
+    if (kind == 2) {
+        double X[3];
+        vertex_position(P, x, y, z, X);
+        for (int d = 0; d < 3; ++d) out[3 * i + d] = X[d];
+        return;
+    }
+    const double h = 1.0 / (double)(W - 1);
+    out[dim * i] = x * h;
+    out[dim * i + 1] = y * h;
+    if (dim == 3) out[dim * i + 2] = z * h;
+}
+
+int dof_positions(const TetDisc& P, int kind, int dim, int W, int64_t N, const int32_t* lat_of_dof, double* out,
+                  hipStream_t s) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_dof_positions, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, P, kind, dim, W, N,
+                       lat_of_dof, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace ugamd

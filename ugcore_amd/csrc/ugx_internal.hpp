@@ -143,6 +143,10 @@ struct TetDisc {
     double f = 0.0;               // constant source
 };
 int tet_mask(int n, const uint16_t* cell_tets, uint32_t* mask, hipStream_t s);
+// DoF coordinates of a level in DoF order (dim per DoF); kind 2 uses the
+// (jittered) tetrahedral vertex positions of P
+int dof_positions(const TetDisc& P, int kind, int dim, int W, int64_t N, const int32_t* lat_of_dof, double* out,
+                  hipStream_t s);
 int pattern_count_mask(int64_t N, const int32_t* lat_of_dof, const uint32_t* mask, int32_t* cnt, hipStream_t s);
 int pattern_fill_mask(int W, int64_t N, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const uint32_t* mask,
                       const int32_t* rowptr, int32_t* col, hipStream_t s);
