@@ -414,6 +414,10 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // init_rap_operator, mg_solver_impl.hpp:829-1013)
     void set_rap(bool on) { rap_ = on; }
     bool rap() const { return rap_; }
+    // replay the lattice-order cycle (the CG fast path) from captured HIP
+    // graphs (default) or launch its kernels one by one; same kernels, same
+    // results. Before init.
+    void set_graphs(bool on) { graphs_ = on; }
     bool init(std::shared_ptr<MatrixOperator> J) override;
     bool apply(GPUVector& c, const GPUVector& d) override;
     // testing access: level matrix (FP64 DIA values in lattice numbering)
@@ -452,6 +456,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     bool transposed_ = false;
     bool matrix_free_ = true;
     bool rap_ = false;
+    bool graphs_ = true;
     Impl* impl_ = nullptr;
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;

@@ -202,6 +202,10 @@ int ugx_gmg_set_matrix_free(void* g, int on);
  * from the assembled top level (hexahedral and triangular hierarchies);
  * replaces AssembledMultiGridCycle::set_rap (mg_solver.h:149) */
 int ugx_gmg_set_rap(void* g, int on);
+/* before ugx_gmg_init: 1 (default) = the lattice-order cycle of the CG fast
+ * path replays captured HIP graphs, 0 = launches its kernels one by one (same
+ * kernels and results). No counterpart in the reference (launch mechanics). */
+int ugx_gmg_set_graphs(void* g, int on);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

@@ -56,6 +56,7 @@ _SIGNATURES = {
     "ugx_gmg_set_transposed": (_I, [_P, _I]),
     "ugx_gmg_set_matrix_free": (_I, [_P, _I]),
     "ugx_gmg_set_rap": (_I, [_P, _I]),
+    "ugx_gmg_set_graphs": (_I, [_P, _I]),
     "ugx_disc_create": (_P, [_I, _I]),
     "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
     "ugx_disc_destroy": (None, [_P]),
@@ -440,7 +441,7 @@ class GeometricMultiGrid:
     coarse levels as Galerkin products R A P (set_rap)."""
 
     def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64", transposed=False,
-                 matrix_free=True, rap=False):
+                 matrix_free=True, rap=False, graphs=True):
         self.disc = disc
         p = 1 if precision in ("f32", "float32", 1) else 0
         self.h = _handle(lib().ugx_gmg_create(disc.h, base_level, nu1, nu2, cycle, float(omega), p),
@@ -451,6 +452,8 @@ class GeometricMultiGrid:
             _check(lib().ugx_gmg_set_matrix_free(self.h, 0), "GMG.set_matrix_free")
         if rap:
             _check(lib().ugx_gmg_set_rap(self.h, 1), "GMG.set_rap")
+        if not graphs:
+            _check(lib().ugx_gmg_set_graphs(self.h, 0), "GMG.set_graphs")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -477,6 +477,12 @@ int ugx_gmg_set_matrix_free(void* g, int on) {
         return 0;
     });
 }
+int ugx_gmg_set_graphs(void* g, int on) {
+    return guard([&] {
+        ((GmgH*)g)->g->set_graphs(on != 0);
+        return 0;
+    });
+}
 int ugx_gmg_set_rap(void* g, int on) {
     return guard([&] {
         ((GmgH*)g)->g->set_rap(on != 0);

@@ -852,6 +852,17 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     DeviceBuffer<double> cell_c64;
     bool sym = true;             // level products through the symmetric storage
     bool rap = false;            // Galerkin coarse operators R A P below the top level
+    bool graphs = true;          // replay the lattice cycle from captured HIP graphs
+    struct CycleGraph {
+        const void* key[4];
+        hipGraphExec_t exec;
+    };
+    std::vector<CycleGraph> cycle_graphs;
+    void clear_graphs() {
+        for (auto& g : cycle_graphs) (void)hipGraphExecDestroy(g.exec);
+        cycle_graphs.clear();
+    }
+    ~GmgImpl() override { clear_graphs(); }
 
     // level products: symmetric storage when every level vector vanishes on
     // Dirichlet DoFs (the defect entering the cycle does), full otherwise
@@ -870,10 +881,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                                            c.stream);
     }
 
-    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free, bool galerkin)
+    GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free, bool galerkin,
+            bool use_graphs)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
           sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2),
-          mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()), rap(galerkin) {
+          mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()), rap(galerkin),
+          graphs(use_graphs) {
         if (rap && d->kind() == 2)
             throw UGError("GMG: Galerkin coarse operators need the P1 lattice transfer (hexahedral or triangular "
                           "hierarchy)");
@@ -887,6 +900,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
 
     void init() override {
         auto& c = context();
+        clear_graphs();
         if (base < 0 || base > top) throw UGError("GMG: base level out of range");
         L.clear();
         L.resize(top + 1);
@@ -1130,10 +1144,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         sym = sym_ok;
     }
 
-    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) override {
+    void run_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) {
         auto& c = context();
         GmgLevel<T>& t = L[top];
-        if (!sym_ok) throw UGError("GMG::apply_lattice: needs a symmetric discretisation");
         sym = true;
         UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
@@ -1142,6 +1155,47 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             UGX_LAUNCH(convert_dot<T>(t.N, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
         else
             UGX_LAUNCH((convert<T, double>(t.N, t.sc, c_lat, c.stream)));
+    }
+
+    // One cycle is about 60 launches, most of them on the small coarse levels
+    // where the launch costs more than the kernel. The cycle has no host
+    // decisions and every level vector is rewritten before it is read, so it
+    // is captured once per (c, d, cd_out, partial) and replayed as one graph
+    // (kernel nodes only: the zeroing is a kernel, not a memset).
+    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) override {
+        auto& c = context();
+        if (!sym_ok) throw UGError("GMG::apply_lattice: needs a symmetric discretisation");
+        if (!graphs) {
+            run_lattice(c_lat, d_lat, cd_out, partial);
+            return;
+        }
+        const void* key[4] = {c_lat, d_lat, cd_out, partial};
+        for (auto& g : cycle_graphs)
+            if (std::equal(key, key + 4, g.key)) {
+                UGX_HIP(hipGraphLaunch(g.exec, c.stream));
+                return;
+            }
+        if (cycle_graphs.size() >= 4) clear_graphs();
+        UGX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_lattice(c_lat, d_lat, cd_out, partial);
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(c.stream, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        UGX_HIP(hipStreamEndCapture(c.stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) throw UGError(std::string("GMG: graph instantiation failed: ") + hipGetErrorString(e));
+        CycleGraph g;
+        std::copy(key, key + 4, g.key);
+        g.exec = exec;
+        cycle_graphs.push_back(g);
+        UGX_HIP(hipGraphLaunch(exec, c.stream));
     }
 
     bool surface_axpy(double* y, double alpha, const double* v, double beta, const double* x, double* yx_out,
@@ -1237,10 +1291,10 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     impl_ = nullptr;
     if (prec_ == Precision::F64)
         impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                    rap_);
+                                    rap_, graphs_);
     else
         impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                   rap_);
+                                   rap_, graphs_);
     impl_->init();
     return true;
 }
