@@ -11,6 +11,11 @@ is solved with A itself. Only u (and theta) are kept for the backward pass;
 with keep_operator=False the operator is dropped after the forward solve and
 re-assembled in backward.
 
+Tensors are host (CPU) tensors; the library moves them to the device. torch
+ships its own HIP runtime while this library links the system one, and in one
+process only the runtime that initialises first sees the GPU, so device
+tensors are rejected with that explanation.
+
 The reference has no adjoint; this module has no counterpart there.
 """
 import numpy as np
@@ -21,29 +26,27 @@ U.lib()            # load the HIP library (and its runtime) before torch
 import torch       # noqa: E402
 
 
+_DEVICE_TENSORS = ("device tensors are not supported: torch ships its own HIP runtime and this library links "
+                   "the system one; in one process only the runtime that initialises first sees the GPU. "
+                   "Pass host (CPU) tensors: the library moves them to the device itself.")
+
+
+def _host_tensor(t):
+    if t.is_cuda:
+        raise U.UGError(_DEVICE_TENSORS)
+    return t
+
+
 def _to_vector(t, vec):
-    """copy a float64 torch tensor into a GPUVector"""
-    t = t.detach().contiguous()
+    """copy a float64 host tensor into a GPUVector"""
+    t = _host_tensor(t.detach()).contiguous()
     if t.dtype != torch.float64:
         raise TypeError("float64 tensors expected")
-    if t.is_cuda:
-        torch.cuda.synchronize()
-        rc = U.lib().ugx_memcpy_dtod(vec.device_ptr(), t.data_ptr(), t.numel() * 8)
-        if rc != 0:
-            raise U.UGError(U.lib().ugx_last_error().decode())
-    else:
-        vec.upload(t.numpy())
+    vec.upload(t.numpy())
 
 
 def _from_vector(vec, like):
-    n = vec.size()
-    if like.is_cuda:
-        out = torch.empty(n, dtype=torch.float64, device=like.device)
-        torch.cuda.synchronize()
-        rc = U.lib().ugx_memcpy_dtod(out.data_ptr(), vec.device_ptr(), n * 8)
-        if rc != 0:
-            raise U.UGError(U.lib().ugx_last_error().decode())
-        return out
+    _host_tensor(like)
     return torch.from_numpy(vec.numpy())
 
 
@@ -67,14 +70,10 @@ class DiffusionProblem:
         self.last_steps = {}
 
     def _set_theta(self, theta):
-        theta = theta.detach().contiguous()
+        theta = _host_tensor(theta.detach()).contiguous()
         if theta.dtype != torch.float64 or theta.numel() != self.n ** 3:
             raise ValueError("theta must be float64 with n_fine^3 entries")
-        if theta.is_cuda:
-            torch.cuda.synchronize()
-            self.disc.set_theta_device(theta.data_ptr())
-        else:
-            self.disc.set_theta(theta.numpy())
+        self.disc.set_theta(theta.numpy())
 
     def _assemble(self, theta):
         self._set_theta(theta)
@@ -129,13 +128,7 @@ class _Solve(torch.autograd.Function):
         lam = problem.backward(theta, ut, grad_u)
         u = U.Vector(problem.N)
         _to_vector(ut, u)
-        if theta.is_cuda:
-            g = torch.empty(problem.n ** 3, dtype=torch.float64, device=theta.device)
-            torch.cuda.synchronize()
-            problem.disc.adjoint_gradient(u, lam, grad_ptr=g.data_ptr())
-            U.synchronize()
-        else:
-            g = torch.from_numpy(problem.disc.adjoint_gradient(u, lam))
+        g = torch.from_numpy(problem.disc.adjoint_gradient(u, lam))
         return g, None
 
 
