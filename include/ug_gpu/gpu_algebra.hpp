@@ -296,6 +296,17 @@ class BiCGStab : public ILinearOperatorInverse {
    public:
     const char* name() const override { return "BiCGStab"; }
     bool apply_return_defect(GPUVector& x, GPUVector& b) override;
+    // restart every n steps (0: never) and whenever |(r0, r)| <=
+    // minOrtho * ||r|| ||r0|| (bicgstab.h:386-390)
+    void set_restart(int n) {
+        if (n < 0) throw UGError("BiCGStab::set_restart: n >= 0");
+        num_restarts_ = n;
+    }
+    void set_min_orthogonality(double m) { min_ortho_ = m; }
+
+   private:
+    int num_restarts_ = 0;
+    double min_ortho_ = 0.0;
 };
 
 // ---------------------------------------------------------------------------

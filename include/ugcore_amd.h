@@ -240,6 +240,11 @@ int ugx_solver_set_lattice_path(void* s, int on);
 /* 1 if the last CG apply ran on the lattice path */
 int ugx_solver_used_lattice_path(void* s);
 int ugx_solver_defects(void* s, double* out_host, int max_n);
+/* BiCGStab only: BiCGStab::set_restart (restart every n steps, 0 = never) and
+ * set_min_orthogonality (restart when |(r0,r)| <= m ||r|| ||r0||),
+ * bicgstab.h:386-390 */
+int ugx_solver_set_restart(void* s, int n);
+int ugx_solver_set_min_orthogonality(void* s, double m);
 
 /* ---- file formats (host only, no GPU work) -------------------------------
  * ConnectionViewer text files: ConnectionViewer::WriteMatrix / WriteVector

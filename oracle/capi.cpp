@@ -22,6 +22,7 @@ struct Problem {
     std::vector<std::vector<double>> b;
     std::vector<double> x;
     TetParams tp;                       // kind 2: velocity, jitter
+    BiCGStabOptions bicg;               // BiCGStab restarts
     SolveResult last;
     double t_assemble = 0, t_setup = 0, t_solve = 0;
     std::string err;
@@ -191,6 +192,12 @@ int orc_set_rap(void* h) {
     }
 }
 
+void orc_set_bicgstab_options(void* h, int num_restarts, double min_ortho) {
+    auto* p = (Problem*)h;
+    p->bicg.num_restarts = num_restarts;
+    p->bicg.min_ortho = min_ortho;
+}
+
 int64_t orc_matrix_nnz(void* h, int which, int lev) {
     auto* p = (Problem*)h;
     const CSR& A = which == 0 ? p->A[lev] : which == 1 ? p->P[lev] : p->R[lev];
@@ -243,7 +250,7 @@ int orc_solve(void* h, int solver, int precond, int base_lev, int nu1, int nu2, 
         cc.min_defect = min_def;
         cc.rel_reduction = rel_red;
         p->last = solver == 0 ? cg_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc)
-                              : bicgstab_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc);
+                              : bicgstab_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc, p->bicg);
         double t2 = now();
         p->t_setup = t1 - t0;
         p->t_solve = t2 - t1;
@@ -367,7 +374,7 @@ int orc_solve_adjoint(void* h, int solver, const double* rhs, double* lambda_out
         ConvCheck cc;
         cc.max_steps = max_steps;
         cc.rel_reduction = rel_red;
-        auto r = solver == 0 ? cg_solve(g.A[top], &g, x, b, cc) : bicgstab_solve(g.A[top], &g, x, b, cc);
+        auto r = solver == 0 ? cg_solve(g.A[top], &g, x, b, cc) : bicgstab_solve(g.A[top], &g, x, b, cc, p->bicg);
         std::memcpy(lambda_out, x.data(), sizeof(double) * n);
         p->last = r;
         return r.ok ? r.steps : -1 - r.steps;

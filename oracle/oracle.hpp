@@ -253,7 +253,12 @@ struct SolveResult {
 SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
                      ConvCheck cc);
 // BiCGStab::apply_return_defect (linear_solver/bicgstab.h:112-385), no restarts
+// BiCGStab::set_restart / set_min_orthogonality (bicgstab.h:386-390)
+struct BiCGStabOptions {
+    int num_restarts = 0;
+    double min_ortho = 0.0;
+};
 SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                           ConvCheck cc);
+                           ConvCheck cc, BiCGStabOptions opt = BiCGStabOptions());
 
 }  // namespace orc

@@ -199,10 +199,11 @@ SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const s
     return res;
 }
 
-// ---- BiCGStab (bicgstab.h:112-385), m_numRestarts = 0, m_minOrtho = 0 ---------------
+// ---- BiCGStab (bicgstab.h:112-385); restarts every opt.num_restarts steps and
+// when |(r0, r)| <= opt.min_ortho ||r|| ||r0|| (bicgstab.h:157-215) -------------------
 
 SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                           ConvCheck cc) {
+                           ConvCheck cc, BiCGStabOptions opt) {
     SolveResult res;
     const size_t n = b.size();
     std::vector<double> r = b;
@@ -213,6 +214,7 @@ SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, c
     bool restart = true;
     auto fail = [&]() { res.ok = false; res.steps = cc.step; res.defects = cc.defects; return res; };
     while (!cc.iteration_ended()) {
+        if (opt.num_restarts > 0 && cc.step % opt.num_restarts == 0) restart = true;
         if (restart) {
             r0 = r;
             std::fill(p.begin(), p.end(), 0.0);
@@ -226,7 +228,7 @@ SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, c
         const double rho_old = rho;
         rho = dot(r0, r);
         const double norm_r = cc.current;
-        if (std::fabs(rho) / (norm_r * norm_r0) <= 0.0) restart = true;
+        if (std::fabs(rho) / (norm_r * norm_r0) <= opt.min_ortho) restart = true;
         if (rho_old == 0.0) return fail();
         const double beta = (rho / rho_old) * (alpha / omega);
         for (size_t i = 0; i < n; ++i) p[i] = 1.0 * r[i] + beta * p[i] + (-beta * omega) * v[i];

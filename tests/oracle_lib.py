@@ -61,6 +61,7 @@ def lib():
                                             C.c_double, C.c_int]),
             "orc_timings": (None, [P, F64P]),
             "orc_set_rap": (C.c_int, [P]),
+            "orc_set_bicgstab_options": (None, [P, C.c_int, C.c_double]),
             "orc_builder_create": (P, [I64, I64]),
             "orc_builder_set": (None, [P, I64, I64, C.c_double]),
             "orc_builder_add": (None, [P, I64, I64, C.c_double]),
@@ -247,6 +248,9 @@ class Problem:
         lib().orc_adjoint_gradient(self.h, np.ascontiguousarray(u, np.float64),
                                    np.ascontiguousarray(lam, np.float64), g)
         return g
+
+    def set_bicgstab_options(self, num_restarts=0, min_ortho=0.0):
+        lib().orc_set_bicgstab_options(self.h, int(num_restarts), float(min_ortho))
 
     def set_rap(self):
         """replace the assembled coarse matrices by Galerkin products R A P

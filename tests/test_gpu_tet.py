@@ -296,3 +296,33 @@ def test_full_size_96_properties():
         sols[prec] = x.numpy()
         say(f"  {prec}: {S.steps()} steps")
     assert np.linalg.norm(sols["f32"] - sols["f64"]) <= 1e-8 * np.linalg.norm(sols["f64"])
+
+
+@pytest.mark.parametrize("num_restarts,min_ortho", [(3, 0.0), (0, 0.3), (4, 0.2)])
+def test_bicgstab_restarts_match_oracle(num_restarts, min_ortho):
+    """BiCGStab::set_restart / set_min_orthogonality (bicgstab.h:157-215):
+    the same restarted iteration on the GPU and in the oracle (FP64 levels)."""
+    say(f"case tet bicgstab restarts={num_restarts} min_ortho={min_ortho}")
+    u = U()
+    refs = 2
+    p, theta = oracle_problem(refs)
+    p.set_bicgstab_options(num_restarts=num_restarts, min_ortho=min_ortho)
+    try:
+        r_orc = p.solve(solver="bicgstab", base_lev=0, omega=OMEGA, rel_red=1e-10, max_steps=300)
+    finally:
+        p.set_bicgstab_options()
+    assert r_orc["ok"]
+    d, A, b = gpu_problem(refs, theta)
+    g = u.GeometricMultiGrid(d, base_level=0, omega=OMEGA, precision="f64")
+    S = u.BiCGStab(g, max_steps=300, reduction=1e-10)
+    S.set_restart(num_restarts)
+    S.set_min_orthogonality(min_ortho)
+    S.init(A)
+    x = u.Vector(d.num_dofs())
+    assert S.apply(x, b)
+    assert abs(S.steps() - r_orc["steps"]) <= 1, (S.steps(), r_orc["steps"])
+    xo = r_orc["x"]
+    assert np.linalg.norm(x.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+    dg, do = S.defects(), r_orc["defects"]
+    m = min(len(dg), len(do))
+    assert (np.abs(dg[:m] - do[:m]) / do[:m])[: m // 2].max() <= 1e-9

@@ -57,6 +57,8 @@ _SIGNATURES = {
     "ugx_gmg_set_matrix_free": (_I, [_P, _I]),
     "ugx_gmg_set_rap": (_I, [_P, _I]),
     "ugx_gmg_set_graphs": (_I, [_P, _I]),
+    "ugx_solver_set_restart": (_I, [_P, _I]),
+    "ugx_solver_set_min_orthogonality": (_I, [_P, _D]),
     "ugx_disc_create": (_P, [_I, _I]),
     "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
     "ugx_disc_destroy": (None, [_P]),
@@ -541,6 +543,14 @@ class CG(_Solver):
 
 class BiCGStab(_Solver):
     _type = 1
+
+    def set_restart(self, n):
+        """restart every n steps (0: never), BiCGStab::set_restart"""
+        _check(lib().ugx_solver_set_restart(self.h, int(n)), "set_restart")
+
+    def set_min_orthogonality(self, m):
+        """restart when |(r0, r)| <= m ||r|| ||r0||"""
+        _check(lib().ugx_solver_set_min_orthogonality(self.h, float(m)), "set_min_orthogonality")
 
 
 # ---- file formats ------------------------------------------------------------------------------

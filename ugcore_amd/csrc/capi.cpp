@@ -579,6 +579,22 @@ int ugx_solver_apply(void* s, void* x, void* b) {
     return guard([&] { return ((SolverH*)s)->s->apply(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });
 }
 int ugx_solver_steps(void* s) { return ((SolverH*)s)->s->step(); }
+int ugx_solver_set_restart(void* s, int n) {
+    return guard([&] {
+        auto* b = dynamic_cast<BiCGStab*>(((SolverH*)s)->s.get());
+        if (!b) throw UGError("ugx_solver_set_restart: BiCGStab only");
+        b->set_restart(n);
+        return 0;
+    });
+}
+int ugx_solver_set_min_orthogonality(void* s, double m) {
+    return guard([&] {
+        auto* b = dynamic_cast<BiCGStab*>(((SolverH*)s)->s.get());
+        if (!b) throw UGError("ugx_solver_set_min_orthogonality: BiCGStab only");
+        b->set_min_orthogonality(m);
+        return 0;
+    });
+}
 int ugx_solver_set_lattice_path(void* s, int on) {
     auto* cg = dynamic_cast<CG*>(((SolverH*)s)->s.get());
     if (!cg) {

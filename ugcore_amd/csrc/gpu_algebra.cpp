@@ -448,7 +448,8 @@ bool CG::apply_return_defect(GPUVector& x, GPUVector& b) {
 }
 
 bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
-    // bicgstab.h:112-385 with m_numRestarts = 0 and m_minOrtho = 0
+    // bicgstab.h:112-385 (restart every num_restarts_ steps and when the
+    // orthogonality of r to r0 drops to min_ortho_, bicgstab.h:157-215)
     op_->apply_sub(b, x);
     GPUVector& r = b;
     const int64_t n = r.size();
@@ -458,6 +459,7 @@ bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
     double rho = 1, alpha = 1, omega = 1, norm_r0 = 0.0;
     bool restart = true;
     while (!conv_->iteration_ended()) {
+        if (num_restarts_ > 0 && conv_->step() % num_restarts_ == 0) restart = true;
         if (restart) {
             r0 = r;
             p.set(0.0);
@@ -471,7 +473,7 @@ bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
         const double rho_old = rho;
         rho = r0.dotprod(r);
         const double norm_r = conv_->defect();
-        if (std::fabs(rho) / (norm_r * norm_r0) <= 0.0) restart = true;
+        if (std::fabs(rho) / (norm_r * norm_r0) <= min_ortho_) restart = true;
         if (rho_old == 0.0) return false;
         const double beta = (rho / rho_old) * (alpha / omega);
         VecScaleAdd(p, 1.0, r, beta, p, -beta * omega, v);
