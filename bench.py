@@ -332,6 +332,14 @@ def main():
     t0 = time.time()
     disc = U.DomainDiscretization(0, refs)
     t_mesh = time.time() - t0
+    # the same hierarchy refined and numbered on the host, then uploaded
+    t0 = time.time()
+    disc_host = U.DomainDiscretization(0, refs, setup="host")
+    t_mesh_host = time.time() - t0
+    same_numbering = bool(np.array_equal(disc.dof_of_lattice(refs), disc_host.dof_of_lattice(refs)))
+    del disc_host
+    log(f"[bench] hierarchy + numbering: device {t_mesh:.3f} s, host {t_mesh_host:.3f} s, "
+        f"identical={same_numbering}")
     theta = make_theta(n, args.seed, 3)
     disc.set_theta(theta)
     A = U.SparseMatrix()
@@ -555,6 +563,8 @@ def main():
             "assembly_elements_per_s": n ** 3 / t_asm,
             "gmg_setup_s": setup_s,
             "mesh_and_numbering_s": t_mesh,
+            "mesh_and_numbering_host_s": t_mesh_host,
+            "mesh_numbering_device_equals_host": same_numbering,
             "spmv_layout": layout,
             "spmv_f64_ms": t_spmv * 1e3,
             "spmv_csr_direct_ms": t_csr * 1e3,

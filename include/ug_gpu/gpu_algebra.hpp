@@ -326,6 +326,13 @@ struct LevelGrid {
     DeviceBuffer<uint8_t> pcode;
 };
 
+// Refines the lattice hierarchy of domain kind `kind` num_refs times on the
+// device and fills each level's size, DoF numbering (ordering 0: the
+// reference's LEVEL numbering, 1: lexicographic), Inner count and, for the
+// tetrahedral domain, the per-cell tetrahedra and parent-edge codes
+// (mesh_kernels.hip).
+void build_device_levels(int kind, int num_refs, int ordering, std::vector<std::unique_ptr<LevelGrid>>& levels);
+
 // Domain of lattice topology: kind 0 = unit cube from one hexahedron (Q1 FE,
 // -div(k grad u) = f), kind 1 = unit square from two triangles (FV1),
 // kind 2 = unit cube as 3x3x3 hexahedra x 6 tetrahedra, regularly refined
@@ -336,7 +343,10 @@ class DomainDiscretization {
     // subset-major); ordering 1: lexicographic lattice order with the
     // Dirichlet DoFs last (an OrderLex-style renumbering,
     // lib_disc/ordering_strategies)
-    DomainDiscretization(int kind, int num_refs, int ordering = 0);
+    // setup 0: the hierarchy and numbering are built on the device
+    // (mesh_kernels.hip); setup 1: on the host (hierarchy.cpp), then uploaded.
+    // Both give the same numbering.
+    DomainDiscretization(int kind, int num_refs, int ordering = 0, int setup = 0);
     int kind() const { return kind_; }
     int dim() const { return kind_ == 1 ? 2 : 3; }
     // level operators symmetric on vectors vanishing on Dirichlet DoFs
@@ -387,6 +397,7 @@ class DomainDiscretization {
     int num_refs() const { return refs_; }
 
    private:
+    void build_host_levels(int kind, int num_refs, int ordering);
     int kind_;
     int refs_;
     std::vector<std::unique_ptr<LevelGrid>> levels_;

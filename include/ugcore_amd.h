@@ -112,9 +112,24 @@ void* ugx_disc_create(int kind, int num_refs);
 /* ordering 0: the reference's numbering (as ugx_disc_create); 1: lexicographic
  * lattice order with Dirichlet DoFs last (OrderLex-style renumbering) */
 void* ugx_disc_create_ordered(int kind, int num_refs, int ordering);
+/* setup 0: the multigrid hierarchy and its DoF numbering are built on the
+ * device (GlobalMultiGridRefiner::perform_refinement,
+ * global_multi_grid_refiner.cpp:135-455, and DoFDistribution::reinit,
+ * dof_distribution.cpp:1401-1421, replayed in parallel); setup 1: built on the
+ * host and uploaded. Both give the same numbering; ugx_disc_create and
+ * ugx_disc_create_ordered use setup 0. */
+void* ugx_disc_create_setup(int kind, int num_refs, int ordering, int setup);
+/* kind 2: the level's tetrahedra per lattice cell (6 packed corner codes per
+ * cell, n^3 cells, creation order) and, for levels >= 1, the parent-edge
+ * direction code of every lattice point ((n+1)^3 entries, 0 for copies of
+ * coarse vertices); either pointer may be NULL */
+int ugx_disc_level_tets(void* disc, int level, uint16_t* cell_tets_host, uint8_t* parent_codes_host);
 void ugx_disc_destroy(void* disc);
 int ugx_disc_num_levels(void* disc);
 int64_t ugx_disc_num_dofs(void* disc, int level);
+/* DoFs [0, num_inner) of the level belong to subset "Inner", the rest are
+ * Dirichlet DoFs; -1 for a level out of range */
+int64_t ugx_disc_num_inner(void* disc, int level);
 int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out_host);   /* (n+1)^dim entries */
 /* DoF coordinates of a level in DoF order, dim (2 or 3) per DoF: what the
  * approximation space hands to SaveMatrixForConnectionViewer

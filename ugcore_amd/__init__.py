@@ -61,6 +61,9 @@ _SIGNATURES = {
     "ugx_solver_set_min_orthogonality": (_I, [_P, _D]),
     "ugx_disc_create": (_P, [_I, _I]),
     "ugx_disc_create_ordered": (_P, [_I, _I, _I]),
+    "ugx_disc_create_setup": (_P, [_I, _I, _I, _I]),
+    "ugx_disc_level_tets": (_I, [_P, _I, _P, _P]),
+    "ugx_disc_num_inner": (_I64, [_P, _I]),
     "ugx_disc_destroy": (None, [_P]),
     "ugx_disc_num_levels": (_I, [_P]),
     "ugx_disc_num_dofs": (_I64, [_P, _I]),
@@ -348,14 +351,17 @@ class DomainDiscretization:
     unit cube as 3x3x3 hexahedra x 6 tetrahedra, FV1 convection-diffusion with
     full upwind (theta per fine lattice cell)."""
 
-    def __init__(self, kind, num_refs, ordering="reference"):
+    def __init__(self, kind, num_refs, ordering="reference", setup="device"):
         self.kind = kind
         self.num_refs = num_refs
         self.dim = 2 if kind == 1 else 3
         # fine elements (kind 0/1) or fine lattice cells (kind 2) per side
         self.n_fine = (3 if kind == 2 else 1) << num_refs
         o = {"reference": 0, "lex": 1}[ordering]
-        self.h = _handle(lib().ugx_disc_create_ordered(int(kind), int(num_refs), o), "DomainDiscretization")
+        # "device": hierarchy and numbering refined on the device; "host":
+        # built on the host and uploaded (the same numbering)
+        st = {"device": 0, "host": 1}[setup]
+        self.h = _handle(lib().ugx_disc_create_setup(int(kind), int(num_refs), o, st), "DomainDiscretization")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -370,6 +376,11 @@ class DomainDiscretization:
             level = self.num_levels() - 1
         return lib().ugx_disc_num_dofs(self.h, level)
 
+    def num_inner(self, level=None):
+        """DoFs [0, num_inner) of the level are subset Inner, the rest Dirichlet"""
+        level = self.num_refs if level is None else level
+        return lib().ugx_disc_num_inner(self.h, level)
+
     def lattice_width(self, level):
         return (3 if self.kind == 2 else 1) * 2 ** level + 1
 
@@ -380,6 +391,16 @@ class DomainDiscretization:
         out = np.empty((n, self.dim), np.float64)
         _check(lib().ugx_disc_dof_positions(self.h, level, _ptr(out)), "dof_positions")
         return out
+
+    def level_tets(self, level):
+        """tetrahedral domain: (cell_tets (n^3, 6) uint16, parent codes
+        ((n+1)^3,) uint8 or None on level 0) of a level"""
+        n = 3 * 2 ** level
+        cells = np.empty((n ** 3, 6), np.uint16)
+        par = np.empty((n + 1) ** 3, np.uint8) if level > 0 else None
+        _check(lib().ugx_disc_level_tets(self.h, level, _ptr(cells), _ptr(par) if par is not None else None),
+               "level_tets")
+        return cells, par
 
     def dof_of_lattice(self, level):
         n = self.lattice_width(level) ** self.dim

@@ -290,12 +290,38 @@ void* ugx_disc_create_ordered(int kind, int num_refs, int ordering) {
         return (void*)h;
     });
 }
+void* ugx_disc_create_setup(int kind, int num_refs, int ordering, int setup) {
+    return guard_ptr([&] {
+        auto* h = new DiscH();
+        h->d = std::make_shared<DomainDiscretization>(kind, num_refs, ordering, setup);
+        return (void*)h;
+    });
+}
+int ugx_disc_level_tets(void* disc, int level, uint16_t* cell_tets, uint8_t* parent_codes) {
+    return guard([&] {
+        auto* h = (DiscH*)disc;
+        if (h->d->kind() != 2) throw UGError("level_tets: the domain has no tetrahedra");
+        if (level < 0 || level >= h->d->num_levels()) throw UGError("level_tets: level out of range");
+        const LevelGrid& L = h->d->level(level);
+        if (cell_tets) L.cell_tets.download(cell_tets, L.cell_tets.size());
+        if (parent_codes) {
+            if (level == 0) throw UGError("level_tets: level 0 has no parent codes");
+            L.pcode.download(parent_codes, L.N);
+        }
+        return 0;
+    });
+}
 void ugx_disc_destroy(void* disc) { delete (DiscH*)disc; }
 int ugx_disc_num_levels(void* disc) { return ((DiscH*)disc)->d->num_levels(); }
 int64_t ugx_disc_num_dofs(void* disc, int level) {
     auto* h = (DiscH*)disc;
     if (level < 0 || level >= h->d->num_levels()) return -1;
     return h->d->level(level).N;
+}
+int64_t ugx_disc_num_inner(void* disc, int level) {
+    auto* h = (DiscH*)disc;
+    if (level < 0 || level >= h->d->num_levels()) return -1;
+    return h->d->level(level).n_inner;
 }
 int ugx_disc_dof_of_lattice(void* disc, int level, int32_t* out) {
     return guard([&] {

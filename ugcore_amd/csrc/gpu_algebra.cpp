@@ -514,10 +514,30 @@ bool BiCGStab::apply_return_defect(GPUVector& x, GPUVector& b) {
 // ---------------------------------------------------------------------------
 // DomainDiscretization (structured)
 // ---------------------------------------------------------------------------
-DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering) : kind_(kind), refs_(num_refs) {
+DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering, int setup)
+    : kind_(kind), refs_(num_refs) {
     if (kind < 0 || kind > 2) throw UGError("DomainDiscretization: unknown domain kind");
     if (num_refs < 0 || num_refs > (kind == 0 ? 9 : kind == 1 ? 13 : 6))
         throw UGError("DomainDiscretization: refinement out of range");
+    if (ordering != 0 && ordering != 1) throw UGError("DomainDiscretization: unknown ordering");
+    if (setup == 0)
+        build_device_levels(kind, num_refs, ordering, levels_);
+    else if (setup == 1)
+        build_host_levels(kind, num_refs, ordering);
+    else
+        throw UGError("DomainDiscretization: unknown set-up (0 device, 1 host)");
+    auto& c = context();
+    for (auto& L : levels_) {
+        L->lat_of_dof.resize(L->N);
+        UGX_LAUNCH(invert_map(L->N, L->dof_of_lat.get(), L->lat_of_dof.get(), c.stream));
+        L->dir.resize(L->N);
+        UGX_LAUNCH(boundary_flags(L->dim, L->W, L->N, L->dir.get(), c.stream));
+    }
+    sync();
+}
+
+// the hierarchy and numbering built on the host (hierarchy.cpp), then uploaded
+void DomainDiscretization::build_host_levels(int kind, int num_refs, int ordering) {
     auto nums = build_level_numbering(kind, num_refs);
     if (ordering == 1) {
         // lexicographic lattice order, inner DoFs first then Dirichlet DoFs
@@ -532,10 +552,7 @@ DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering)
                     if (bnd == (pass == 1)) ln.dof_of_lattice[l] = k++;
                 }
         }
-    } else if (ordering != 0) {
-        throw UGError("DomainDiscretization: unknown ordering");
     }
-    auto& c = context();
     for (auto& ln : nums) {
         auto L = std::make_unique<LevelGrid>();
         L->dim = ln.dim;
@@ -554,10 +571,6 @@ DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering)
         }
         L->dof_of_lat.resize(L->N);
         L->dof_of_lat.upload(ln.dof_of_lattice.data(), L->N);
-        L->lat_of_dof.resize(L->N);
-        UGX_LAUNCH(invert_map(L->N, L->dof_of_lat.get(), L->lat_of_dof.get(), c.stream));
-        L->dir.resize(L->N);
-        UGX_LAUNCH(boundary_flags(L->dim, L->W, L->N, L->dir.get(), c.stream));
         if (kind == 2) {
             const int64_t cells = (int64_t)L->n * L->n * L->n;
             if ((int64_t)ln.cell_tets.size() != cells * 6) throw UGError("DomainDiscretization: tetrahedra missing");
@@ -587,7 +600,6 @@ DomainDiscretization::DomainDiscretization(int kind, int num_refs, int ordering)
         }
         levels_.push_back(std::move(L));
     }
-    sync();
 }
 
 void DomainDiscretization::set_velocity(double vx, double vy, double vz) {

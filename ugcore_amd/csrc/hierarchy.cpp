@@ -26,35 +26,13 @@
 #include <vector>
 
 #include "hierarchy.hpp"
+#include "mesh_tables.hpp"
 
 namespace ugamd {
 
 namespace {
 
-// local index tables of the reference hexahedron (hexahedron_rules.h)
-constexpr int kHexEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
-                                 {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
-constexpr int kHexFace[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
-                                {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
-// regular 1:8 rule in the index space [corners 0-7 | edge vertices 8-19 |
-// face vertices 20-25 | centre 26]
-constexpr int kHexChild[8][8] = {
-    {0, 8, 20, 11, 12, 21, 26, 24}, {8, 1, 9, 20, 21, 13, 22, 26},
-    {20, 9, 2, 10, 26, 22, 14, 23}, {11, 20, 10, 3, 24, 26, 23, 15},
-    {12, 21, 26, 24, 4, 16, 25, 19}, {21, 13, 22, 26, 16, 5, 17, 25},
-    {26, 22, 14, 23, 25, 17, 6, 18}, {24, 26, 23, 15, 19, 25, 18, 7}};
-
-// local index tables of the reference tetrahedron (tetrahedron_rules.h)
-constexpr int kTetEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-constexpr int kTetFace[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
-// regular rule STANDARD (tetrahedron_rules.cpp:487-609) in the index space
-// [corners 0-3 | edge vertices 4-9]: four corner children, then the inner
-// octahedron split along diagonal 0-5, 1-3 or 2-4 (edge vertices)
-constexpr int kTetCorner[4][4] = {{0, 4, 6, 7}, {1, 5, 4, 8}, {2, 6, 5, 9}, {7, 8, 9, 3}};
-constexpr int kTetInner[3][4][4] = {
-    {{4, 5, 6, 9}, {5, 8, 9, 4}, {6, 9, 7, 4}, {4, 7, 8, 9}},
-    {{4, 5, 6, 7}, {5, 8, 9, 7}, {6, 9, 7, 5}, {4, 7, 8, 5}},
-    {{4, 6, 7, 8}, {5, 6, 4, 8}, {6, 7, 8, 9}, {8, 5, 6, 9}}};
+using namespace mesh;
 
 struct P3 {
     int32_t x, y, z;
@@ -435,6 +413,36 @@ Lvl base_level(int kind) {
 }
 
 }  // namespace
+
+BaseMesh base_mesh(int kind) {
+    const Lvl L = base_level(kind);
+    BaseMesh B;
+    B.dim = L.dim;
+    B.n = L.n;
+    B.tets = L.tets;
+    B.face_nv = L.f_nv.empty() ? 4 : L.f_nv[0];
+    for (uint8_t k : L.f_nv)
+        if (k != B.face_nv) throw std::runtime_error("hierarchy: mixed face types");
+    for (const P3& p : L.vpos) {
+        B.vpos.push_back(p.x);
+        B.vpos.push_back(p.y);
+        B.vpos.push_back(p.z);
+    }
+    B.vsub = L.vsub;
+    B.e_v = L.e_v;
+    B.e_sub = L.e_sub;
+    B.f_v = L.f_v;
+    B.f_e = L.f_e;
+    B.f_sub = L.f_sub;
+    B.h_v = L.h_v;
+    B.h_f = L.h_f;
+    B.h_e = L.h_e;
+    B.h_sub = L.h_sub;
+    B.t_v = L.t_v;
+    B.t_e = L.t_e;
+    B.t_sub = L.t_sub;
+    return B;
+}
 
 std::vector<LevelNumbering> build_level_numbering(int kind, int num_refs) {
     std::vector<LevelNumbering> out;

@@ -33,4 +33,26 @@ inline int edge_direction_code(int dx, int dy, int dz) {
 // kind 2: unit cube as 3x3x3 hexahedra, each split into 6 tetrahedra
 std::vector<LevelNumbering> build_level_numbering(int kind, int num_refs);
 
+// The coarsest level of a hierarchy as flat entity lists in creation order,
+// the start of the device refinement. Vertex positions are lattice
+// coordinates (x, y, z per vertex; z = 0 in 2D); faces hold face_nv vertices
+// and sides padded to 4 with -1; subset 0 is "Inner", 1 "Boundary".
+struct BaseMesh {
+    int dim = 3;
+    int n = 1;            // cells per side
+    int face_nv = 4;      // 4: quadrilaterals, 3: triangles
+    bool tets = false;
+    std::vector<int32_t> vpos;
+    std::vector<uint8_t> vsub;
+    std::vector<int32_t> e_v;                  // 2 per edge
+    std::vector<uint8_t> e_sub;
+    std::vector<int32_t> f_v, f_e;             // 4 per face
+    std::vector<uint8_t> f_sub;
+    std::vector<int32_t> h_v, h_f, h_e;        // 8, 6, 12 per hexahedron
+    std::vector<uint8_t> h_sub;
+    std::vector<int32_t> t_v, t_e;             // 4, 6 per tetrahedron
+    std::vector<uint8_t> t_sub;
+};
+BaseMesh base_mesh(int kind);
+
 }  // namespace ugamd
