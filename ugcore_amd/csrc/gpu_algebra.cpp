@@ -1074,11 +1074,20 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         }
     }
 
+    // The reference updates the top-level defect once more after the last
+    // post-smoothing step (mg_solver_impl.hpp:1954, on m_LocalFullRefLevel).
+    // No caller reads that defect here: apply() returns only the correction,
+    // apply_update_defect() forms d -= A c itself, and the lattice entry
+    // converts the correction. The update is therefore skipped on every level,
+    // and the top level ends with the same correction update as the levels
+    // below it; the lattice entry fuses that update with its conversion
+    // (defer_top_final).
+    bool defer_top_final = false;
+
     void base_solve(int lev) {
         auto& c = context();
         GmgLevel<T>& l = L[lev];
         UGX_LAUNCH(lu_solve<T>(lu_n, lu_linvT.get(), lu_uinvT.get(), lu_perm.get(), l.sd, l.sc, c.stream));
-        if (lev >= top) UGX_LAUNCH(op_apply_sub(l, l.sc, l.sd, l.sd));
     }
 
     void presmooth_and_restriction(int lev) {
@@ -1106,17 +1115,13 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                                           c.stream));
         else
             UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
-        if (nu2 == 0) {
-            if (lev >= top) UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
-            return;
-        }
+        if (nu2 == 0) return;
         UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
         for (int nu = 0; nu < nu2; ++nu) {
-            const bool last = (nu == nu2 - 1);
-            if (!last || lev >= top) {
+            if (nu < nu2 - 1) {
                 UGX_LAUNCH(op_jacobi_step(f));
                 std::swap(f.sd, f.sd2);
-            } else {
+            } else if (!(defer_top_final && lev == top)) {
                 UGX_LAUNCH(jacobi_accumulate<T>(f.N, f.dinv, f.sd, f.sc, c.stream));
             }
         }
@@ -1164,11 +1169,23 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         sym = true;
         UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
-        lmgc(top);
-        if (cd_out)
+        // with post-smoothing below a coarser level the cycle ends with the
+        // top level's correction update, fused here with the conversion
+        defer_top_final = nu2 > 0 && top > base;
+        try {
+            lmgc(top);
+        } catch (...) {
+            defer_top_final = false;
+            throw;
+        }
+        if (defer_top_final) {
+            defer_top_final = false;
+            UGX_LAUNCH(jacobi_convert_dot<T>(t.N, t.dinv, t.sd, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
+        } else if (cd_out) {
             UGX_LAUNCH(convert_dot<T>(t.N, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
-        else
+        } else {
             UGX_LAUNCH((convert<T, double>(t.N, t.sc, c_lat, c.stream)));
+        }
     }
 
     // One cycle is about 60 launches, most of them on the small coarse levels

@@ -1054,6 +1054,11 @@ struct MfZMap {
 // enough for more resident waves, which matters more than the y halo of the
 // narrow bands (2 lines: FP32 Jacobi step 0.159 -> 0.140 ms, FP64 operator
 // 0.235 -> 0.191 ms against 4-line bands with 7 staged entries and 5 rows).
+// The plane chunks are sized so that the whole grid is resident at once: an
+// FP32 block holds 25.8 KB of LDS, so 6 fit on a CU (192 per XCD), and 129
+// bands x 11 chunks of about 24 planes need at most 17 x 11 = 187 of them per
+// XCD, where 16-plane chunks (17 per band) needed about 1.5 rounds (Jacobi
+// step 0.134 -> 0.118 ms). FP64 blocks (3 per CU) take two rounds either way.
 #ifndef UGX_MF_TY32
 #define UGX_MF_TY32 2
 #endif
@@ -1061,7 +1066,7 @@ struct MfZMap {
 #define UGX_MF_TY64 2
 #endif
 #ifndef UGX_MF_ZC32
-#define UGX_MF_ZC32 16
+#define UGX_MF_ZC32 24
 #endif
 #ifndef UGX_MF_ZC64
 #define UGX_MF_ZC64 24
@@ -1305,6 +1310,39 @@ __global__ __launch_bounds__(256) void k_jacobi_accumulate(int64_t n, const TV* 
             if (UGX_STREAM_IDX(e) < n)
                 stnt((TV)((double)cv[e] + (double)dv[e] * (double)sv[e]), sc + UGX_STREAM_IDX(e));
     }
+}
+
+// the last top-level post-smoothing step fused with the conversion of the
+// correction: z = (double)(sc + dinv .* sd), with the arithmetic of
+// k_jacobi_accumulate, and (partial != nullptr) partial sums of z.r as
+// k_convert_dot forms them
+template <typename TV>
+__global__ __launch_bounds__(256) void k_jacobi_convert_dot(int64_t n, const TV* __restrict__ dinv,
+                                                            const TV* __restrict__ sd, const TV* __restrict__ sc,
+                                                            double* __restrict__ z, const double* __restrict__ r,
+                                                            double* __restrict__ partial) {
+    double acc = 0.0;
+    UGX_STREAM_LOOP(n) {
+        TV dv[kStreamE], sv[kStreamE], cv[kStreamE];
+        double rv[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_SAFE(e, n);
+            dv[e] = ldnt(dinv + i);
+            sv[e] = ldnt(sd + i);
+            cv[e] = ldnt(sc + i);
+            rv[e] = partial ? ldnt(r + i) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            const int64_t i = UGX_STREAM_IDX(e);
+            if (i >= n) continue;
+            const double zi = (double)(TV)((double)cv[e] + (double)dv[e] * (double)sv[e]);
+            stnt(zi, z + i);
+            acc += zi * rv[e];
+        }
+    }
+    if (partial) block_partial(acc, partial);
 }
 
 // dinv from the DIA centre diagonal
@@ -2292,6 +2330,25 @@ int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream
 }
 template int jacobi_accumulate<double>(int64_t, const double*, const double*, double*, hipStream_t);
 template int jacobi_accumulate<float>(int64_t, const float*, const float*, float*, hipStream_t);
+
+template <typename TV>
+int jacobi_convert_dot(int64_t n, const TV* dinv, const TV* sd, const TV* sc, double* z, const double* r,
+                       double* partial, double* out, hipStream_t s) {
+    if (n <= 0) return 0;
+    if (!out) {
+        hipLaunchKernelGGL(k_jacobi_convert_dot<TV>, dim3(stream_grid(n)), dim3(256), 0, s, n, dinv, sd, sc, z, r,
+                           (double*)nullptr);
+        UGX_CHECK_LAUNCH();
+    }
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_jacobi_convert_dot<TV>, dim3(g), dim3(256), 0, s, n, dinv, sd, sc, z, r, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, out, 0);
+    UGX_CHECK_LAUNCH();
+}
+template int jacobi_convert_dot<double>(int64_t, const double*, const double*, const double*, double*, const double*,
+                                        double*, double*, hipStream_t);
+template int jacobi_convert_dot<float>(int64_t, const float*, const float*, const float*, double*, const double*,
+                                       double*, double*, hipStream_t);
 
 template <typename TM, typename TV>
 int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s) {

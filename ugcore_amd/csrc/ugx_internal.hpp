@@ -61,6 +61,11 @@ int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV
                     TV* sc, hipStream_t s);
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
+// z = (double)(sc + dinv .* sd) (arithmetic of jacobi_accumulate); with out,
+// also (z, r) into *out through partial (as convert_dot)
+template <typename TV>
+int jacobi_convert_dot(int64_t n, const TV* dinv, const TV* sd, const TV* sc, double* z, const double* r,
+                       double* partial, double* out, hipStream_t s);
 // symmetric (Dirichlet-eliminated) storage: nd/2+1 tiled upper diagonals
 template <typename TM>
 int dia_symmetrize(int64_t n, int nd, const int32_t* off, const uint8_t* dir, const TM* full, TM* up, hipStream_t s);
