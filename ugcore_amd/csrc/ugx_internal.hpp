@@ -100,6 +100,9 @@ int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, in
 // tiled diagonal storage of level operators (see kernels.hip)
 inline int64_t dia_tiled_index(int d, int64_t i, int nd) { return ((i >> 6) * nd + d) * 64 + (i & 63); }
 inline int64_t dia_tiled_size(int64_t n, int nd) { return ((n + 63) / 64) * 64 * nd; }
+// P1 transfers between lattice levels. In 3D with Wf = 2 Wc - 1 the Dirichlet
+// flags must mark exactly the lattice boundary (as the multigrid levels' flags
+// do): the tiled kernels take the Dirichlet rows from the lattice geometry.
 template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
                 hipStream_t s);
