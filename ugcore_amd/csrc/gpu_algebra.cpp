@@ -927,6 +927,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             UGX_LAUNCH(q1mf_coeff<T>(ne, 1.0 / G.n, disc->theta_device(), cell_c.get(), c.stream));
             cell_c64.resize(ne);
             UGX_LAUNCH(q1mf_coeff<double>(ne, 1.0 / G.n, disc->theta_device(), cell_c64.get(), c.stream));
+            UGX_LAUNCH(q1mf_prepare(G.W));
         }
         // Galerkin hierarchy (AssembledMultiGridCycle::init_rap_operator,
         // mg_solver_impl.hpp:829-1013): the assembled top-level operator, then

@@ -16,6 +16,11 @@
 // All kernels are stream ordered; none allocates. Wavefront size is 64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include <cstdint>
 
 #include "ugx_internal.hpp"
@@ -1056,7 +1061,7 @@ __device__ __forceinline__ void q1mf_block(const MfMap& m, const T* __restrict__
 // computed and stored after it, so every plane is read once per block and
 // its load latency overlaps the arithmetic.
 struct MfZMap {
-    int32_t W, TY, ZC, nchunks;
+    int32_t W, TY, nchunks;
     int32_t glo[8], gsz[8];
 };
 // tile shape of the z-marching kernels: lines per block and planes per block
@@ -1065,22 +1070,15 @@ struct MfZMap {
 // enough for more resident waves, which matters more than the y halo of the
 // narrow bands (2 lines: FP32 Jacobi step 0.159 -> 0.140 ms, FP64 operator
 // 0.235 -> 0.191 ms against 4-line bands with 7 staged entries and 5 rows).
-// The plane chunks are sized so that the whole grid is resident at once: an
-// FP32 block holds 25.8 KB of LDS, so 6 fit on a CU (192 per XCD), and 129
-// bands x 11 chunks of about 24 planes need at most 17 x 11 = 187 of them per
-// XCD, where 16-plane chunks (17 per band) needed about 1.5 rounds (Jacobi
-// step 0.134 -> 0.118 ms). FP64 blocks (3 per CU) take two rounds either way.
+// The number of plane chunks per band is chosen at launch so that the whole
+// grid is resident at once (make_mf_zmap asks the runtime how many blocks of
+// the kernel fit on a CU): at 257^3 with 16-plane chunks the FP32 grid needed
+// about 1.5 rounds, and one round measured 0.134 -> 0.118 ms.
 #ifndef UGX_MF_TY32
 #define UGX_MF_TY32 2
 #endif
 #ifndef UGX_MF_TY64
 #define UGX_MF_TY64 2
-#endif
-#ifndef UGX_MF_ZC32
-#define UGX_MF_ZC32 24
-#endif
-#ifndef UGX_MF_ZC64
-#define UGX_MF_ZC64 24
 #endif
 #ifndef UGX_MF_PF
 #define UGX_MF_PF 5
@@ -1088,8 +1086,22 @@ struct MfZMap {
 constexpr int kMfPrefetch = UGX_MF_PF;   // staged entries per thread and plane (P <= kMfPrefetch * 256)
 
 #ifndef UGX_MF_WAVES32
-#define UGX_MF_WAVES32 6
+#define UGX_MF_WAVES32 7
 #endif
+// planes in the staged LDS ring (FP32 / FP64): 4 = three in use and one being
+// filled; 3 = the filled plane overwrites the oldest one after a second
+// barrier per plane, which leaves room for more workgroups per CU. The cell
+// ring holds one plane fewer. Measured at 257^3 with one-round grids: 3 slots
+// let 7 FP32 workgroups (18.6 KB each, <= 72 VGPRs) and 4 FP64 workgroups
+// share a CU instead of 6 and 3: Jacobi step 0.114 -> 0.107 ms, FP64 operator
+// 0.185 -> 0.167 ms (8 FP32 workgroups need <= 64 VGPRs and spill: 0.124 ms).
+#ifndef UGX_MF_SLOTS32
+#define UGX_MF_SLOTS32 3
+#endif
+#ifndef UGX_MF_SLOTS64
+#define UGX_MF_SLOTS64 3
+#endif
+__host__ __device__ constexpr int mf_slots(size_t elem) { return elem == 4 ? UGX_MF_SLOTS32 : UGX_MF_SLOTS64; }
 #ifndef UGX_MF_ROWS
 #define UGX_MF_ROWS 3
 #endif
@@ -1133,7 +1145,8 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
     const int CX = n + 2, CL = (TY + 1) * CX;
     const int64_t W2 = (int64_t)W * W;
     T* ring = lds_base<T>();
-    T* cring = ring + 4 * P;
+    constexpr int SL = mf_slots(sizeof(T)), CS = SL - 1;
+    T* cring = ring + SL * P;
     const int tid = threadIdx.x, step = blockDim.x;
     auto plane_first = [&](int p) { return (int64_t)p * W2 + (int64_t)(j0 - 1) * W - 1; };
     auto cell_value = [&](int cz, int t) -> T {
@@ -1156,12 +1169,12 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
         }
     }
     for (int p = k0 - 1; p <= k0 + 1; ++p) {
-        T* dst = ring + (p & 3) * P;
+        T* dst = ring + ((p + SL) % SL) * P;
         const int64_t f0 = plane_first(p);
         for (int t = tid; t < P; t += step) dst[t] = staged(f0 + t);
     }
     for (int cz = k0 - 1; cz <= k0; ++cz) {
-        T* dst = cring + ((cz + 3) % 3) * CL;
+        T* dst = cring + ((cz + CS) % CS) * CL;
         for (int t = tid; t < CL; t += step) dst[t] = cell_value(cz, t);
     }
     __syncthreads();
@@ -1196,11 +1209,11 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             }
         }
         // 2. the operator on this thread's rows of plane k
-        const T* sm = ring + ((k - 1) & 3) * P;
-        const T* s0 = ring + (k & 3) * P;
-        const T* sp = ring + ((k + 1) & 3) * P;
-        const T* cm = cring + ((k + 2) % 3) * CL;
-        const T* c0 = cring + (k % 3) * CL;
+        const T* sm = ring + ((k - 1 + SL) % SL) * P;
+        const T* s0 = ring + (k % SL) * P;
+        const T* sp = ring + ((k + 1) % SL) * P;
+        const T* cm = cring + ((k - 1 + CS) % CS) * CL;
+        const T* c0 = cring + (k % CS) * CL;
         const bool plane_bnd = (k == 0 || k == W - 1);
 #pragma unroll
         for (int u = 0; u < kMfRows; ++u) {
@@ -1270,10 +1283,13 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                 dot_acc += (double)yv * (double)xi;
             }
         }
-        // 4. the prefetched plane k+2 and cell plane k+1 into the rings
+        // 4. the prefetched plane k+2 and cell plane k+1 into the rings (with
+        //    the short ring they replace planes k-1 and cell plane k-1, which
+        //    every thread must have finished reading)
+        if (SL == 3) __syncthreads();
         if (more) {
-            T* dst = ring + ((k + 2) & 3) * P;
-            T* cdst = cring + ((k + 1) % 3) * CL;
+            T* dst = ring + ((k + 2) % SL) * P;
+            T* cdst = cring + ((k + 1) % CS) * CL;
 #pragma unroll
             for (int u = 0; u < kMfPrefetch; ++u) {
                 const int t = tid + u * step;
@@ -2291,14 +2307,39 @@ static int64_t make_staged_map(int64_t n, int nd, const int32_t* off, size_t ele
     return 8 * (int64_t)maxg * W;
 }
 
-// z-marching plan: bands of TY lines grouped per XCD, chunks of ZC planes;
-// returns the grid (0 when the prefetch registers cannot hold a plane)
-static int64_t make_mf_zmap(int W, size_t elem, MfZMap& m, size_t& lds) {
+// workgroups of `kernel` (256 threads, `lds` bytes of LDS) resident on one CU,
+// and the device's CU count; both asked once and kept
+static int mf_resident_blocks(const void* kernel, size_t lds) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& e : cache)
+        if (e.first.first == kernel && e.first.second == lds) return e.second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, lds) != hipSuccess || nb < 1) nb = 1;
+    cache.push_back({{kernel, lds}, nb});
+    return nb;
+}
+static int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 8)
+            n = 8;
+        cus = n;
+    }
+    return cus;
+}
+
+// z-marching plan: bands of TY lines grouped per XCD (blockIdx % 8), and as
+// many plane chunks per band as make the grid resident at once (each XCD's
+// group of bands times the chunks fills at most its CUs' workgroup slots;
+// chunks of at least 4 planes); returns the grid (0 when the prefetch
+// registers cannot hold a plane)
+static int64_t make_mf_zmap(int W, size_t elem, const void* kernel, MfZMap& m, size_t& lds) {
     m = MfZMap{};
     m.W = W;
     m.TY = elem == 8 ? UGX_MF_TY64 : UGX_MF_TY32;
-    m.ZC = elem == 8 ? UGX_MF_ZC64 : UGX_MF_ZC32;
-    m.nchunks = (W + m.ZC - 1) / m.ZC;
     const int64_t P = (int64_t)m.TY * W + 2 * W + 2, CL = (int64_t)(m.TY + 1) * (W + 1);
     if (P > (int64_t)kMfPrefetch * 256 || CL > (int64_t)kMfPrefetch * 256 || W < 8) return 0;
     if ((int64_t)m.TY * W + 63 > (int64_t)kMfRows * 256) return 0;
@@ -2310,8 +2351,12 @@ static int64_t make_mf_zmap(int W, size_t elem, MfZMap& m, size_t& lds) {
         m.gsz[c] = (int32_t)(b1 - b0);
         maxg = m.gsz[c] > maxg ? m.gsz[c] : maxg;
     }
-    lds = (size_t)(4 * P + 3 * CL) * elem;
+    lds = (size_t)(mf_slots(elem) * P + (mf_slots(elem) - 1) * CL) * elem;
     if (lds > 64 * 1024) return 0;
+    const int64_t slots = (int64_t)(device_cus() / 8) * mf_resident_blocks(kernel, lds);
+    int64_t nch = maxg > 0 ? slots / maxg : 1;
+    nch = std::min<int64_t>(std::max<int64_t>(nch, 1), std::max(1, W / 4));
+    m.nchunks = (int32_t)nch;
     return 8 * (int64_t)maxg * m.nchunks;
 }
 
@@ -2415,7 +2460,7 @@ template <typename T>
 int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s) {
     MfZMap zm;
     size_t zlds;
-    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 0>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 0>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
                            sd_out, sc, 0.0, 0.0, (double*)nullptr);
         UGX_CHECK_LAUNCH();
@@ -2434,7 +2479,7 @@ template <typename T>
 int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s) {
     MfZMap zm;
     size_t zlds;
-    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 1>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 1>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, d_in,
                            d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr);
         UGX_CHECK_LAUNCH();
@@ -2454,7 +2499,7 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
               double* dot_partial, int* n_partial) {
     MfZMap zm;
     size_t zlds;
-    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), zm, zlds)) {
+    if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 2>), zm, zlds)) {
         if (dot_partial && n_partial && zgrid > *n_partial) return -1;
         hipLaunchKernelGGL((k_q1mf_z<T, 2>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, v, y,
                            (T*)nullptr, alpha, beta, dot_partial);
@@ -2471,6 +2516,19 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
 }
 template int q1mf_axpy<double>(int, const double*, double, const double*, double, const double*, double*, hipStream_t,
                                double*, int*);
+
+// asks the runtime for the residency of every z-marching kernel at lattice
+// width W, so that no query happens inside a captured cycle
+int q1mf_prepare(int W) {
+    MfZMap zm;
+    size_t lds;
+    make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 0>), zm, lds);
+    make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 1>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 0>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 1>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 2>), zm, lds);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 template <typename T>
 int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s) {

@@ -59,6 +59,9 @@ int dia_axpy(int64_t n, int nd, const int32_t* off, const TM* A, double alpha, c
 template <typename TM, typename TV>
 int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV* dinv, const TV* sd_in, TV* sd_out,
                     TV* sc, hipStream_t s);
+// residency queries of the matrix-free kernels at lattice width W, made
+// before any cycle is captured into a graph
+int q1mf_prepare(int W);
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
 // z = (double)(sc + dinv .* sd) (arithmetic of jacobi_accumulate); with out,
