@@ -458,8 +458,9 @@ class AssembledMultiGridCycle : public ILinearIterator {
     const int32_t* lattice_of_dof() const;
     int64_t num_inner_dofs() const;
     // c = B d for a defect d in lattice order that vanishes on Dirichlet DoFs;
+    // d carries the halo (the first smoothing step stages its planes from it);
     // with cd_out (device), also (c, d) into cd_out[0] (partial: device
-    // scratch of at least 1024 doubles)
+    // scratch of max_partials() doubles)
     void apply_lattice(double* c_lat, const double* d_lat, double* cd_out = nullptr, double* partial = nullptr);
     // y = alpha*v + beta*A*x with the surface operator in lattice order
     // (x must vanish on Dirichlet DoFs and carry the halo); with yx_out

@@ -231,6 +231,13 @@ int ugx_gmg_smooth_level(void* g, int level, int nsteps);
  * pointers to (n+1)^dim entries; x needs ugx_gmg_lattice_halo zero entries on
  * each side and must vanish on Dirichlet DoFs) */
 int ugx_gmg_surface_apply_lattice(void* g, double* y_dev, const double* x_dev);
+/* c = B*d with the cycle applied in lattice order (device pointers to (n+1)^dim
+ * FP64 entries, each with ugx_gmg_lattice_halo zero entries on either side; d
+ * must vanish on Dirichlet DoFs): the preconditioner of the lattice CG, equal
+ * to ugx_gmg_apply on the permuted vectors. Symmetric discretisations only.
+ * Replaces AssembledMultiGridCycle::apply (mg_solver_impl.hpp:175-275) for
+ * callers that keep their vectors in lattice order. */
+int ugx_gmg_apply_lattice(void* g, double* c_dev, const double* d_dev);
 int64_t ugx_gmg_lattice_halo(void* g);
 int ugx_gmg_level_num_diags(void* g, int level);
 int ugx_gmg_level_dia(void* g, int level, double* out_host);  /* ndiag x (n+1)^dim values, lattice numbering */

@@ -66,3 +66,33 @@ def test_graph_cycle_matches_eager(kind, refs, precision):
             assert steps == ref_steps
             assert np.array_equal(x, ref_x)
             assert np.array_equal(defs, ref_def)
+
+
+@pytest.mark.parametrize("refs,precision,graphs", [(5, "f32", True), (5, "f64", True), (5, "f32", False),
+                                                   (4, "f64", False)])
+def test_lattice_cycle_matches_surface_cycle(refs, precision, graphs):
+    """the cycle's lattice entry (its first step reads the FP64 defect
+    directly, its last update is fused with the conversion) against the
+    surface entry (separate conversion, zeroing and final update) on the same
+    defect: identical corrections"""
+    say(f"case lattice vs surface cycle refs={refs} {precision} graphs={graphs}")
+    u = U()
+    L = u.lib()
+    d, A, b = problem(0, refs)
+    g = u.GeometricMultiGrid(d, base_level=0, precision=precision, graphs=graphs)
+    g.init(A)
+    N = d.num_dofs()
+    dol = d.dof_of_lattice(refs)
+    rng = np.random.default_rng(11)
+    r = rng.normal(size=N)
+    r[d.num_inner(refs):] = 0.0          # Dirichlet DoFs come last
+    c_surf = u.Vector(N)
+    g.apply(c_surf, u.Vector(values=r))
+    H = L.ugx_gmg_lattice_halo(g.h)
+    d_lat = u.DeviceArray(np.concatenate([np.zeros(H), r[dol], np.zeros(H)]))
+    c_lat = u.DeviceArray(np.zeros(N + 2 * H))
+    for rep in range(2):
+        assert L.ugx_gmg_apply_lattice(g.h, c_lat.ptr + 8 * H, d_lat.ptr + 8 * H) == 0
+        u.synchronize()
+        c = c_lat.numpy()[H:H + N]
+        assert np.array_equal(c, c_surf.numpy()[dol])

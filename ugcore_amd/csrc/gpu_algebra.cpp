@@ -1084,6 +1084,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     // below it; the lattice entry fuses that update with its conversion
     // (defer_top_final).
     bool defer_top_final = false;
+    // the lattice entry's FP64 defect when the first top-level pre-smoothing
+    // step reads it directly (q1mf_jacobi_first) instead of a converted copy
+    const double* first_defect = nullptr;
 
     void base_solve(int lev) {
         auto& c = context();
@@ -1096,7 +1099,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
         for (int nu = 0; nu < nu1; ++nu) {
-            UGX_LAUNCH(op_jacobi_step(f));
+            if (nu == 0 && lev == top && first_defect) {
+                UGX_LAUNCH(q1mf_jacobi_first<T>(f.W, cell_c.get(), f.dinv, first_defect, f.sd2, f.sc, c.stream));
+                first_defect = nullptr;
+            } else {
+                UGX_LAUNCH(op_jacobi_step(f));
+            }
             std::swap(f.sd, f.sd2);
         }
         UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
@@ -1168,8 +1176,14 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         auto& c = context();
         GmgLevel<T>& t = L[top];
         sym = true;
-        UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
-        UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
+        // with pre-smoothing on the matrix-free top level the first step reads
+        // the FP64 defect itself and starts from a zero correction
+        if (mf && nu1 > 0 && top > base && q1mf_first_fits(t.W, sizeof(T))) {
+            first_defect = d_lat;
+        } else {
+            UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
+            UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
+        }
         // with post-smoothing below a coarser level the cycle ends with the
         // top level's correction update, fused here with the conversion
         defer_top_final = nu2 > 0 && top > base;
@@ -1177,6 +1191,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             lmgc(top);
         } catch (...) {
             defer_top_final = false;
+            first_defect = nullptr;
             throw;
         }
         if (defer_top_final) {

@@ -1109,11 +1109,16 @@ constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= k
 
 // MODE 0: fused Jacobi step (staged st = dinv.*sd; sd_out = sd - A st; sc += st)
 // MODE 1: d_out = d_in - A x;  MODE 2: y = alpha v + beta A x
+// MODE 3: the first Jacobi step of a cycle from the FP64 defect d64 and a zero
+//         correction: sd = (T)d64, st = dinv.*sd; sd_out = sd - A st; sc = st
+//         (the conversion rounds as k_convert does, so this equals k_convert,
+//         zeroing sc and MODE 0)
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? UGX_MF_WAVES32 : 1)))
 void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                                                 const T* __restrict__ b_in, const T* c_in, T* out0, T* out1,
-                                                double alpha, double beta, double* dot_partial) {
+                                                double alpha, double beta, double* dot_partial,
+                                                const double* __restrict__ d64) {
     // c_in and out0 may alias (d_out = d_in); every row is read and then
     // written by the same thread. MODE 2 with dot_partial: per-block partial
     // sums of y_i * x_i (fixed order) in dot_partial[blockIdx.x]
@@ -1155,7 +1160,9 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
         if (cz < 0 || cz >= n || cy < 0 || cy >= n || cx < 1 || cx > n) return (T)0;
         return cell[((int64_t)cz * n + cy) * n + cx - 1];
     };
-    auto staged = [&](int64_t j) -> T { return MODE == 0 ? a_in[j] * b_in[j] : a_in[j]; };
+    auto staged = [&](int64_t j) -> T {
+        return MODE == 0 ? a_in[j] * b_in[j] : MODE == 3 ? a_in[j] * (T)d64[j] : a_in[j];
+    };
     // the cell each prefetch slot stages sits at the same place of every cell
     // plane: its offset inside the plane, or -1 outside the lattice / the tile
     int coff[kMfPrefetch];
@@ -1192,6 +1199,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             const bool ok = more && t < P;
             pa[u] = ok ? a_in[f2 + t] : (T)0;
             if (MODE == 0) pb[u] = ok ? b_in[f2 + t] : (T)0;
+            if (MODE == 3) pb[u] = ok ? (T)d64[f2 + t] : (T)0;
             cv[u] = (more && coff[u] >= 0 && k + 1 < n) ? cell[(int64_t)(k + 1) * n * n + coff[u]] : (T)0;
         }
         T ra[kMfRows], rb[kMfRows];
@@ -1202,6 +1210,8 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             if (MODE == 0) {
                 ra[u] = ok ? b_in[base + r] : (T)0;   // sd_in
                 rb[u] = ok ? out1[base + r] : (T)0;   // sc
+            } else if (MODE == 3) {
+                ra[u] = ok ? (T)d64[base + r] : (T)0;   // sd_in
             } else if (MODE == 1) {
                 ra[u] = ok ? c_in[base + r] : (T)0;   // d_in
             } else {
@@ -1275,6 +1285,9 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             if (MODE == 0) {
                 out0[i] = ra[u] - sv;
                 out1[i] = rb[u] + xi;
+            } else if (MODE == 3) {
+                out0[i] = ra[u] - sv;
+                out1[i] = xi;
             } else if (MODE == 1) {
                 out0[i] = ra[u] - sv;
             } else {
@@ -1293,7 +1306,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
 #pragma unroll
             for (int u = 0; u < kMfPrefetch; ++u) {
                 const int t = tid + u * step;
-                if (t < P) dst[t] = MODE == 0 ? pa[u] * pb[u] : pa[u];
+                if (t < P) dst[t] = (MODE == 0 || MODE == 3) ? pa[u] * pb[u] : pa[u];
                 if (t < CL) cdst[t] = cv[u];
             }
         }
@@ -2462,7 +2475,7 @@ int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 0>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 0>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
-                           sd_out, sc, 0.0, 0.0, (double*)nullptr);
+                           sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -2476,12 +2489,33 @@ template int q1mf_jacobi_step<float>(int, const float*, const float*, const floa
 template int q1mf_jacobi_step<double>(int, const double*, const double*, const double*, double*, double*, hipStream_t);
 
 template <typename T>
+int q1mf_jacobi_first(int W, const T* cell, const T* dinv, const double* d64, T* sd_out, T* sc, hipStream_t s) {
+    MfZMap zm;
+    size_t zlds;
+    const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 3>), zm, zlds);
+    if (!zgrid) return -1;
+    hipLaunchKernelGGL((k_q1mf_z<T, 3>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, (const T*)nullptr,
+                       (const T*)nullptr, sd_out, sc, 0.0, 0.0, (double*)nullptr, d64);
+    UGX_CHECK_LAUNCH();
+}
+bool q1mf_first_fits(int W, size_t elem) {
+    MfZMap zm;
+    size_t lds;
+    const void* k = elem == 4 ? reinterpret_cast<const void*>(&k_q1mf_z<float, 3>)
+                              : reinterpret_cast<const void*>(&k_q1mf_z<double, 3>);
+    return make_mf_zmap(W, elem, k, zm, lds) != 0;
+}
+template int q1mf_jacobi_first<float>(int, const float*, const float*, const double*, float*, float*, hipStream_t);
+template int q1mf_jacobi_first<double>(int, const double*, const double*, const double*, double*, double*,
+                                       hipStream_t);
+
+template <typename T>
 int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s) {
     MfZMap zm;
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 1>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 1>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, d_in,
-                           d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr);
+                           d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -2502,7 +2536,7 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 2>), zm, zlds)) {
         if (dot_partial && n_partial && zgrid > *n_partial) return -1;
         hipLaunchKernelGGL((k_q1mf_z<T, 2>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, v, y,
-                           (T*)nullptr, alpha, beta, dot_partial);
+                           (T*)nullptr, alpha, beta, dot_partial, (const double*)nullptr);
         if (n_partial) *n_partial = (int)zgrid;
         UGX_CHECK_LAUNCH();
     }
@@ -2524,6 +2558,8 @@ int q1mf_prepare(int W) {
     size_t lds;
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 0>), zm, lds);
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 1>), zm, lds);
+    make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 3>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 3>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 0>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 1>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 2>), zm, lds);

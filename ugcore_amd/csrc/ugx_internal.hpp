@@ -62,6 +62,12 @@ int dia_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* A, const TV
 // residency queries of the matrix-free kernels at lattice width W, made
 // before any cycle is captured into a graph
 int q1mf_prepare(int W);
+// the first Jacobi step of a cycle from the FP64 defect d64 and a zero
+// correction (sd_out = (T)d64 - A st, sc = st, st = dinv .* (T)d64), when
+// q1mf_first_fits(W, sizeof(T)) (the z-marching plan applies)
+bool q1mf_first_fits(int W, size_t elem);
+template <typename T>
+int q1mf_jacobi_first(int W, const T* cell, const T* dinv, const double* d64, T* sd_out, T* sc, hipStream_t s);
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
 // z = (double)(sc + dinv .* sd) (arithmetic of jacobi_accumulate); with out,
