@@ -490,12 +490,14 @@ def main():
 
     t32 = float(np.mean(times["f32"]))
     t64 = float(np.mean(times["f64"]))
-    # launches per FP32-preconditioned solve: the fused top-level smoother runs
-    # 4 times per V(2,2) cycle (2 pre, 2 post incl. the final top-level defect
-    # update), one cycle per CG step plus the initial one; the CG operator
-    # (lattice path) once per step
+    # launches per FP32-preconditioned solve, one cycle per CG step plus the
+    # initial one: of the 4 top-level smoothing steps of a V(2,2) cycle the
+    # lattice path runs 2 as the fused Jacobi step timed here (the first one
+    # reads the FP64 defect in its own kernel mode, the last one is the
+    # correction update fused with the conversion), the surface path all 4;
+    # the CG operator once per step
     lattice = all(bool(solvers[p][1].used_lattice_path()) for p in solvers)
-    smooth_launches = 4 * (iters["f32"] + 1)
+    smooth_launches = (2 if lattice else 4) * (iters["f32"] + 1)
     op_launches = iters["f32"]
     t_op, op_bytes = (t_lat, lat_bytes) if lattice else (t_spmv, spmv_bytes)
     op_kernel = "k_q1mf_z<double,2>" if lattice else (
