@@ -1102,6 +1102,12 @@ constexpr int kMfPrefetch = UGX_MF_PF;   // staged entries per thread and plane 
 #define UGX_MF_SLOTS64 3
 #endif
 __host__ __device__ constexpr int mf_slots(size_t elem) { return elem == 4 ? UGX_MF_SLOTS32 : UGX_MF_SLOTS64; }
+// at most this many z-marching workgroups per CU are planned for: the FP32
+// d -= A x kernel fits 8 (61 VGPRs) but measured 114.7 us with 8 and 91.9 us
+// with 7 at 257^3; the other kernels fit 7 (FP32) or 4 (FP64) anyway
+#ifndef UGX_MF_MAXRES
+#define UGX_MF_MAXRES 7
+#endif
 #ifndef UGX_MF_ROWS
 #define UGX_MF_ROWS 3
 #endif
@@ -2366,7 +2372,7 @@ static int64_t make_mf_zmap(int W, size_t elem, const void* kernel, MfZMap& m, s
     }
     lds = (size_t)(mf_slots(elem) * P + (mf_slots(elem) - 1) * CL) * elem;
     if (lds > 64 * 1024) return 0;
-    const int64_t slots = (int64_t)(device_cus() / 8) * mf_resident_blocks(kernel, lds);
+    const int64_t slots = (int64_t)(device_cus() / 8) * std::min(UGX_MF_MAXRES, mf_resident_blocks(kernel, lds));
     int64_t nch = maxg > 0 ? slots / maxg : 1;
     nch = std::min<int64_t>(std::max<int64_t>(nch, 1), std::max(1, W / 4));
     m.nchunks = (int32_t)nch;
