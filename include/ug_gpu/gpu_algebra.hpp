@@ -290,6 +290,7 @@ class CG : public ILinearOperatorInverse {
     // lattice-path work space, kept across solves (halos stay zero)
     int64_t ws_n_ = -1, ws_h_ = -1;
     DeviceBuffer<double> ws_x_, ws_r_, ws_p_, ws_q_, ws_z_, ws_partial_, ws_sc_;
+    DeviceBuffer<float> ws_rf_;   // FP32 copy of r for an FP32 cycle (with halos)
 };
 
 class BiCGStab : public ILinearOperatorInverse {
@@ -460,8 +461,12 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // c = B d for a defect d in lattice order that vanishes on Dirichlet DoFs;
     // d carries the halo (the first smoothing step stages its planes from it);
     // with cd_out (device), also (c, d) into cd_out[0] (partial: device
-    // scratch of max_partials() doubles)
-    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out = nullptr, double* partial = nullptr);
+    // scratch of max_partials() doubles). d_lat_f32 (optional, with the halo):
+    // (float) d, read instead of d by the first step when
+    // lattice_uses_f32_defect(); same results
+    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out = nullptr, double* partial = nullptr,
+                       const float* d_lat_f32 = nullptr);
+    bool lattice_uses_f32_defect() const;
     // y = alpha*v + beta*A*x with the surface operator in lattice order
     // (x must vanish on Dirichlet DoFs and carry the halo); with yx_out
     // (device) also (y, x) into yx_out[0] (partial: device scratch of

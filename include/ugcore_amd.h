@@ -235,9 +235,13 @@ int ugx_gmg_surface_apply_lattice(void* g, double* y_dev, const double* x_dev);
  * FP64 entries, each with ugx_gmg_lattice_halo zero entries on either side; d
  * must vanish on Dirichlet DoFs): the preconditioner of the lattice CG, equal
  * to ugx_gmg_apply on the permuted vectors. Symmetric discretisations only.
+ * d_f32_dev (may be NULL; same layout and halo): (float) d, which an FP32 cycle
+ * reads instead of d when ugx_gmg_lattice_uses_f32_defect says so (the lattice
+ * CG writes it along with its residual); the result is the same.
  * Replaces AssembledMultiGridCycle::apply (mg_solver_impl.hpp:175-275) for
  * callers that keep their vectors in lattice order. */
-int ugx_gmg_apply_lattice(void* g, double* c_dev, const double* d_dev);
+int ugx_gmg_apply_lattice(void* g, double* c_dev, const double* d_dev, const float* d_f32_dev);
+int ugx_gmg_lattice_uses_f32_defect(void* g);
 int64_t ugx_gmg_lattice_halo(void* g);
 int ugx_gmg_level_num_diags(void* g, int level);
 int ugx_gmg_level_dia(void* g, int level, double* out_host);  /* ndiag x (n+1)^dim values, lattice numbering */

@@ -72,9 +72,9 @@ def test_graph_cycle_matches_eager(kind, refs, precision):
                                                    (4, "f64", False)])
 def test_lattice_cycle_matches_surface_cycle(refs, precision, graphs):
     """the cycle's lattice entry (its first step reads the FP64 defect
-    directly, its last update is fused with the conversion) against the
-    surface entry (separate conversion, zeroing and final update) on the same
-    defect: identical corrections"""
+    directly, or an FP32 copy of it, and its last update is fused with the
+    conversion) against the surface entry (separate conversion, zeroing and
+    final update) on the same defect: identical corrections"""
     say(f"case lattice vs surface cycle refs={refs} {precision} graphs={graphs}")
     u = U()
     L = u.lib()
@@ -90,9 +90,13 @@ def test_lattice_cycle_matches_surface_cycle(refs, precision, graphs):
     g.apply(c_surf, u.Vector(values=r))
     H = L.ugx_gmg_lattice_halo(g.h)
     d_lat = u.DeviceArray(np.concatenate([np.zeros(H), r[dol], np.zeros(H)]))
+    # the FP32 copy of the defect an FP32 cycle may start from
+    d_f32 = u.DeviceArray(np.concatenate([np.zeros(H), r[dol], np.zeros(H)]).astype(np.float32))
+    assert L.ugx_gmg_lattice_uses_f32_defect(g.h) == (1 if precision == "f32" else 0)
     c_lat = u.DeviceArray(np.zeros(N + 2 * H))
     for rep in range(2):
-        assert L.ugx_gmg_apply_lattice(g.h, c_lat.ptr + 8 * H, d_lat.ptr + 8 * H) == 0
-        u.synchronize()
-        c = c_lat.numpy()[H:H + N]
-        assert np.array_equal(c, c_surf.numpy()[dol])
+        for f32 in (None, d_f32.ptr + 4 * H):
+            assert L.ugx_gmg_apply_lattice(g.h, c_lat.ptr + 8 * H, d_lat.ptr + 8 * H, f32) == 0
+            u.synchronize()
+            c = c_lat.numpy()[H:H + N]
+            assert np.array_equal(c, c_surf.numpy()[dol])

@@ -553,14 +553,15 @@ int ugx_gmg_surface_apply_lattice(void* g, double* y, const double* x) {
         return 0;
     });
 }
-int ugx_gmg_apply_lattice(void* g, double* c, const double* d) {
+int ugx_gmg_apply_lattice(void* g, double* c, const double* d, const float* d_f32) {
     return guard([&] {
         auto& gm = *((GmgH*)g)->g;
         if (!c || !d) throw UGError("gmg_apply_lattice: null vector");
-        gm.apply_lattice(c, d);
+        gm.apply_lattice(c, d, nullptr, nullptr, d_f32);
         return 0;
     });
 }
+int ugx_gmg_lattice_uses_f32_defect(void* g) { return ((GmgH*)g)->g->lattice_uses_f32_defect() ? 1 : 0; }
 int64_t ugx_gmg_lattice_halo(void* g) { return ((GmgH*)g)->g->lattice_halo(); }
 int ugx_gmg_level_num_diags(void* g, int level) { return ((GmgH*)g)->g->level_num_diags(level); }
 int ugx_gmg_level_dia(void* g, int level, double* out) {

@@ -363,6 +363,15 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
     double* sc = ws_sc_.get();
     double* part = ws_partial_.get();
     ws_sc_.zero();
+    // an FP32 cycle starts from an FP32 copy of r that cg_update writes along
+    float* rfl = nullptr;
+    if (gmg.lattice_uses_f32_defect()) {
+        if (ws_rf_.size() != N + 2 * H) {
+            ws_rf_.resize(N + 2 * H);
+            ws_rf_.zero();   // the halos
+        }
+        rfl = ws_rf_.get() + H;
+    }
     UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
     UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), r.data(), rl, c.stream)));
     gmg.apply_lattice(zl, rl, sc + 0, part);   // z = B r, rho = (z, r)
@@ -377,7 +386,7 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
             UGX_LAUNCH(dot<double>(N, ql, pl, c.red_partial, sc + 1, false, c.stream));
         }
         UGX_LAUNCH(cg_alpha(sc, c.stream));
-        UGX_LAUNCH(cg_update(N, sc, pl, ql, xl, rl, part, c.stream));
+        UGX_LAUNCH(cg_update(N, sc, pl, ql, xl, rl, part, c.stream, rfl));
         UGX_HIP(hipMemcpyAsync(c.red_host, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         sync();
         if (c.red_host[1] != 0.0) {   // lambda == 0: breakdown (x, r unchanged)
@@ -386,7 +395,7 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
         }
         conv_->update_defect(std::sqrt(c.red_host[0]));
         if (conv_->iteration_ended()) break;
-        gmg.apply_lattice(zl, rl, sc + 4, part);   // z = B r, rho_new = (z, r)
+        gmg.apply_lattice(zl, rl, sc + 4, part, rfl);   // z = B r, rho_new = (z, r)
         UGX_LAUNCH(cg_beta(sc, c.stream));          // beta = rho_new / rho, rho = rho_new
         UGX_LAUNCH(cg_direction(N, sc, pl, zl, c.stream));
     }
@@ -808,7 +817,9 @@ struct AssembledMultiGridCycle::Impl {
     virtual void download_level_dia(int lev, double* out) const = 0;
     virtual int num_diags(int lev) const = 0;
     virtual void smooth(int lev, int nsteps) = 0;
-    virtual void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) = 0;
+    virtual void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial,
+                               const float* d_lat_f32) = 0;
+    virtual bool uses_f32_defect() const = 0;
     virtual bool surface_axpy(double* y, double alpha, const double* v, double beta, const double* x, double* yx_out,
                               double* partial) = 0;
     virtual int64_t halo() const = 0;
@@ -868,7 +879,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     bool rap = false;            // Galerkin coarse operators R A P below the top level
     bool graphs = true;          // replay the lattice cycle from captured HIP graphs
     struct CycleGraph {
-        const void* key[4];
+        const void* key[5];
         hipGraphExec_t exec;
     };
     std::vector<CycleGraph> cycle_graphs;
@@ -1085,8 +1096,11 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     // (defer_top_final).
     bool defer_top_final = false;
     // the lattice entry's FP64 defect when the first top-level pre-smoothing
-    // step reads it directly (q1mf_jacobi_first) instead of a converted copy
+    // step reads it directly (q1mf_jacobi_first) instead of a converted copy,
+    // or the caller's FP32 copy of it (q1mf_jacobi_first_t, FP32 levels)
     const double* first_defect = nullptr;
+    const float* first_defect_f32 = nullptr;
+    bool first_step_fused() const { return mf && nu1 > 0 && top > base && q1mf_first_fits(L[top].W, sizeof(T)); }
 
     void base_solve(int lev) {
         auto& c = context();
@@ -1099,9 +1113,19 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
         for (int nu = 0; nu < nu1; ++nu) {
-            if (nu == 0 && lev == top && first_defect) {
-                UGX_LAUNCH(q1mf_jacobi_first<T>(f.W, cell_c.get(), f.dinv, first_defect, f.sd2, f.sc, c.stream));
+            if (nu == 0 && lev == top && (first_defect || first_defect_f32)) {
+                if constexpr (std::is_same<T, float>::value) {
+                    if (first_defect_f32)
+                        UGX_LAUNCH(q1mf_jacobi_first_t<T>(f.W, cell_c.get(), f.dinv, first_defect_f32, f.sd2, f.sc,
+                                                          c.stream));
+                    else
+                        UGX_LAUNCH(
+                            q1mf_jacobi_first<T>(f.W, cell_c.get(), f.dinv, first_defect, f.sd2, f.sc, c.stream));
+                } else {
+                    UGX_LAUNCH(q1mf_jacobi_first<T>(f.W, cell_c.get(), f.dinv, first_defect, f.sd2, f.sc, c.stream));
+                }
                 first_defect = nullptr;
+                first_defect_f32 = nullptr;
             } else {
                 UGX_LAUNCH(op_jacobi_step(f));
             }
@@ -1172,14 +1196,15 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         sym = sym_ok;
     }
 
-    void run_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) {
+    void run_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial, const float* d_lat_f32) {
         auto& c = context();
         GmgLevel<T>& t = L[top];
         sym = true;
         // with pre-smoothing on the matrix-free top level the first step reads
         // the FP64 defect itself and starts from a zero correction
-        if (mf && nu1 > 0 && top > base && q1mf_first_fits(t.W, sizeof(T))) {
+        if (first_step_fused()) {
             first_defect = d_lat;
+            if (std::is_same<T, float>::value) first_defect_f32 = d_lat_f32;
         } else {
             UGX_LAUNCH((convert<double, T>(t.N, d_lat, t.sd, c.stream)));
             UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
@@ -1192,6 +1217,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         } catch (...) {
             defer_top_final = false;
             first_defect = nullptr;
+            first_defect_f32 = nullptr;
             throw;
         }
         if (defer_top_final) {
@@ -1209,23 +1235,29 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     // decisions and every level vector is rewritten before it is read, so it
     // is captured once per (c, d, cd_out, partial) and replayed as one graph
     // (kernel nodes only: the zeroing is a kernel, not a memset).
-    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) override {
+    // FP32 levels with the fused first step can start from the caller's FP32
+    // copy of the defect (rounded as the conversion would)
+    bool uses_f32_defect() const override { return std::is_same<T, float>::value && first_step_fused(); }
+
+    void apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial,
+                       const float* d_lat_f32) override {
         auto& c = context();
         if (!sym_ok) throw UGError("GMG::apply_lattice: needs a symmetric discretisation");
+        if (!uses_f32_defect()) d_lat_f32 = nullptr;
         if (!graphs) {
-            run_lattice(c_lat, d_lat, cd_out, partial);
+            run_lattice(c_lat, d_lat, cd_out, partial, d_lat_f32);
             return;
         }
-        const void* key[4] = {c_lat, d_lat, cd_out, partial};
+        const void* key[5] = {c_lat, d_lat, cd_out, partial, d_lat_f32};
         for (auto& g : cycle_graphs)
-            if (std::equal(key, key + 4, g.key)) {
+            if (std::equal(key, key + 5, g.key)) {
                 UGX_HIP(hipGraphLaunch(g.exec, c.stream));
                 return;
             }
         if (cycle_graphs.size() >= 4) clear_graphs();
         UGX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         try {
-            run_lattice(c_lat, d_lat, cd_out, partial);
+            run_lattice(c_lat, d_lat, cd_out, partial, d_lat_f32);
         } catch (...) {
             hipGraph_t broken = nullptr;
             (void)hipStreamEndCapture(c.stream, &broken);
@@ -1239,7 +1271,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) throw UGError(std::string("GMG: graph instantiation failed: ") + hipGetErrorString(e));
         CycleGraph g;
-        std::copy(key, key + 4, g.key);
+        std::copy(key, key + 5, g.key);
         g.exec = exec;
         cycle_graphs.push_back(g);
         UGX_HIP(hipGraphLaunch(exec, c.stream));
@@ -1320,10 +1352,12 @@ int64_t AssembledMultiGridCycle::lattice_halo() const { return impl_ ? impl_->ha
 const int32_t* AssembledMultiGridCycle::dof_of_lattice() const { return disc_->level(disc_->top_level()).dof_of_lat.get(); }
 const int32_t* AssembledMultiGridCycle::lattice_of_dof() const { return disc_->level(disc_->top_level()).lat_of_dof.get(); }
 int64_t AssembledMultiGridCycle::num_inner_dofs() const { return disc_->level(disc_->top_level()).n_inner; }
-void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial) {
+void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat, double* cd_out, double* partial,
+                                            const float* d_lat_f32) {
     if (!impl_) throw UGError("GMG: not initialised");
-    impl_->apply_lattice(c_lat, d_lat, cd_out, partial);
+    impl_->apply_lattice(c_lat, d_lat, cd_out, partial, d_lat_f32);
 }
+bool AssembledMultiGridCycle::lattice_uses_f32_defect() const { return impl_ && impl_->uses_f32_defect(); }
 bool AssembledMultiGridCycle::surface_axpy_lattice(double* y, double alpha, const double* v, double beta,
                                                    const double* x, double* yx_out, double* partial) {
     if (!impl_) throw UGError("GMG: not initialised");

@@ -489,7 +489,7 @@ __global__ void k_cg_beta(double* sc) {
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __restrict__ sc,
                                                    const double* __restrict__ p, const double* __restrict__ q,
                                                    double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial, float* __restrict__ rf) {
     const double alpha = sc[2];
     double acc = 0.0;
     UGX_STREAM_LOOP(n) {
@@ -509,6 +509,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __re
             stnt(1.0 * xv[e] + alpha * pv[e], x + i);
             const double ri = 1.0 * rv[e] + (-alpha) * qv[e];
             stnt(ri, r + i);
+            if (rf) stnt((float)ri, rf + i);   // the FP32 cycle's copy of the defect
             acc += ri * ri;
         }
     }
@@ -1119,6 +1120,8 @@ constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= k
 //         correction: sd = (T)d64, st = dinv.*sd; sd_out = sd - A st; sc = st
 //         (the conversion rounds as k_convert does, so this equals k_convert,
 //         zeroing sc and MODE 0)
+// MODE 4: the same from a defect already in T (b_in = sd): MODE 0 without
+//         reading sc
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? UGX_MF_WAVES32 : 1)))
 void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
@@ -1167,7 +1170,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
         return cell[((int64_t)cz * n + cy) * n + cx - 1];
     };
     auto staged = [&](int64_t j) -> T {
-        return MODE == 0 ? a_in[j] * b_in[j] : MODE == 3 ? a_in[j] * (T)d64[j] : a_in[j];
+        return (MODE == 0 || MODE == 4) ? a_in[j] * b_in[j] : MODE == 3 ? a_in[j] * (T)d64[j] : a_in[j];
     };
     // the cell each prefetch slot stages sits at the same place of every cell
     // plane: its offset inside the plane, or -1 outside the lattice / the tile
@@ -1204,7 +1207,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             const int t = tid + u * step;
             const bool ok = more && t < P;
             pa[u] = ok ? a_in[f2 + t] : (T)0;
-            if (MODE == 0) pb[u] = ok ? b_in[f2 + t] : (T)0;
+            if (MODE == 0 || MODE == 4) pb[u] = ok ? b_in[f2 + t] : (T)0;
             if (MODE == 3) pb[u] = ok ? (T)d64[f2 + t] : (T)0;
             cv[u] = (more && coff[u] >= 0 && k + 1 < n) ? cell[(int64_t)(k + 1) * n * n + coff[u]] : (T)0;
         }
@@ -1218,6 +1221,8 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                 rb[u] = ok ? out1[base + r] : (T)0;   // sc
             } else if (MODE == 3) {
                 ra[u] = ok ? (T)d64[base + r] : (T)0;   // sd_in
+            } else if (MODE == 4) {
+                ra[u] = ok ? b_in[base + r] : (T)0;     // sd_in
             } else if (MODE == 1) {
                 ra[u] = ok ? c_in[base + r] : (T)0;   // d_in
             } else {
@@ -1291,7 +1296,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             if (MODE == 0) {
                 out0[i] = ra[u] - sv;
                 out1[i] = rb[u] + xi;
-            } else if (MODE == 3) {
+            } else if (MODE == 3 || MODE == 4) {
                 out0[i] = ra[u] - sv;
                 out1[i] = xi;
             } else if (MODE == 1) {
@@ -1312,7 +1317,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
 #pragma unroll
             for (int u = 0; u < kMfPrefetch; ++u) {
                 const int t = tid + u * step;
-                if (t < P) dst[t] = (MODE == 0 || MODE == 3) ? pa[u] * pb[u] : pa[u];
+                if (t < P) dst[t] = (MODE == 0 || MODE == 3 || MODE == 4) ? pa[u] * pb[u] : pa[u];
                 if (t < CL) cdst[t] = cv[u];
             }
         }
@@ -2230,9 +2235,9 @@ int cg_beta(double* sc, hipStream_t s) {
     UGX_CHECK_LAUNCH();
 }
 int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
-              hipStream_t s) {
+              hipStream_t s, float* rf) {
     const int g = red_grid(n);
-    hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, s, n, sc, p, q, x, r, partial);
+    hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, s, n, sc, p, q, x, r, partial, rf);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, (double*)(sc + 5), 0);
     UGX_CHECK_LAUNCH();
 }
@@ -2511,6 +2516,17 @@ bool q1mf_first_fits(int W, size_t elem) {
                               : reinterpret_cast<const void*>(&k_q1mf_z<double, 3>);
     return make_mf_zmap(W, elem, k, zm, lds) != 0;
 }
+template <typename T>
+int q1mf_jacobi_first_t(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s) {
+    MfZMap zm;
+    size_t zlds;
+    const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 4>), zm, zlds);
+    if (!zgrid) return -1;
+    hipLaunchKernelGGL((k_q1mf_z<T, 4>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
+                       sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_jacobi_first_t<float>(int, const float*, const float*, const float*, float*, float*, hipStream_t);
 template int q1mf_jacobi_first<float>(int, const float*, const float*, const double*, float*, float*, hipStream_t);
 template int q1mf_jacobi_first<double>(int, const double*, const double*, const double*, double*, double*,
                                        hipStream_t);
@@ -2565,6 +2581,7 @@ int q1mf_prepare(int W) {
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 0>), zm, lds);
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 1>), zm, lds);
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 3>), zm, lds);
+    make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 4>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 3>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 0>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 1>), zm, lds);

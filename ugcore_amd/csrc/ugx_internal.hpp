@@ -40,8 +40,10 @@ int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool ta
 int sum_partials(int np, const double* partial, double* out, hipStream_t s);
 int cg_alpha(double* sc, hipStream_t s);
 int cg_beta(double* sc, hipStream_t s);
+// rf (nullable): also writes (float) r, the FP32 cycle's copy of the defect
 int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
-              hipStream_t s);
+              hipStream_t s,
+              float* rf = nullptr);
 int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s);
 template <typename T>
 int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s);
@@ -68,6 +70,9 @@ int q1mf_prepare(int W);
 bool q1mf_first_fits(int W, size_t elem);
 template <typename T>
 int q1mf_jacobi_first(int W, const T* cell, const T* dinv, const double* d64, T* sd_out, T* sc, hipStream_t s);
+// the same from a defect already in T (with the halo)
+template <typename T>
+int q1mf_jacobi_first_t(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s);
 template <typename TV>
 int jacobi_accumulate(int64_t n, const TV* dinv, const TV* sd, TV* sc, hipStream_t s);
 // z = (double)(sc + dinv .* sd) (arithmetic of jacobi_accumulate); with out,
