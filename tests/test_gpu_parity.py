@@ -237,6 +237,31 @@ def test_gmg_application(kind, refs):
     assert np.linalg.norm(cg - c_orc) <= 1e-12 * np.linalg.norm(c_orc)
 
 
+@pytest.mark.parametrize("kind,refs,nu1,nu2", [(0, 4, 2, 2), (1, 6, 2, 2), (0, 4, 1, 0), (0, 3, 0, 1)])
+def test_gmg_w_cycle_and_smoothing_counts(kind, refs, nu1, nu2):
+    """W-cycle (cycle 2) application and V-cycles with other smoothing counts
+    against the oracle's AssembledMultiGridCycle::apply (mg_solver_impl.hpp:
+    2090-2136: the coarse level is visited twice, so its correction must be
+    complete after the first visit)"""
+    say(f"case gmg W-cycle kind={kind} refs={refs} nu1={nu1} nu2={nu2}")
+    u = U()
+    p, theta, f = oracle_problem(kind, refs)
+    d, A, b = gpu_problem(kind, refs, theta, f)
+    rng = np.random.default_rng(9)
+    dv = rng.normal(size=p.num_dofs(refs))
+    dv[p.dirichlet(refs) == 1] = 0.0
+    for cycle in (2, 1):
+        for precision, tol in (("f64", 1e-12), ("f32", 1e-5)):
+            g = u.GeometricMultiGrid(d, base_level=0, nu1=nu1, nu2=nu2, cycle=cycle, omega=0.66,
+                                     precision=precision)
+            g.init(A)
+            c_orc = p.gmg_apply(dv, base_lev=0, nu1=nu1, nu2=nu2, cycle=cycle)
+            c = u.Vector(len(dv))
+            g.apply(c, u.Vector(values=dv))
+            err = np.linalg.norm(c.numpy() - c_orc) / np.linalg.norm(c_orc)
+            assert err <= tol, (cycle, precision, err)
+
+
 @pytest.mark.parametrize("kind,refs,solver", [(0, 3, "cg"), (0, 4, "cg"), (0, 5, "cg"), (1, 7, "cg"),
                                               (0, 4, "bicgstab"), (1, 6, "bicgstab")])
 def test_gmg_krylov_solve(kind, refs, solver):

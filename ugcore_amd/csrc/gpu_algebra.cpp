@@ -857,6 +857,10 @@ struct GmgLevel {
     DeviceBuffer<T> b_dinv, b_sc, b_sd, b_sd2, b_st;
     T *dinv = nullptr, *sc = nullptr, *sd = nullptr, *sd2 = nullptr, *st = nullptr;
     const uint8_t* dir = nullptr;
+    // the level's last post-smoothing update sc += dinv .* sd is left to the
+    // prolongation that reads sc next (set in a cycle, cleared by that
+    // prolongation)
+    bool acc_deferred = false;
 };
 
 template <typename T>
@@ -1100,6 +1104,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     // or the caller's FP32 copy of it (q1mf_jacobi_first_t, FP32 levels)
     const double* first_defect = nullptr;
     const float* first_defect_f32 = nullptr;
+    void clear_cycle_state() {
+        defer_top_final = false;
+        first_defect = nullptr;
+        first_defect_f32 = nullptr;
+        for (auto& l : L) l.acc_deferred = false;
+    }
     bool first_step_fused() const { return mf && nu1 > 0 && top > base && q1mf_first_fits(L[top].W, sizeof(T)); }
 
     void base_solve(int lev) {
@@ -1131,29 +1141,40 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             }
             std::swap(f.sd, f.sd2);
         }
-        UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
-        if (tets)
+        if (tets) {
+            UGX_LAUNCH(vec_set<T>(cl.N, cl.sc, 0.0, c.stream));
             UGX_LAUNCH(restrict_tet<T>(cl.W, f.W, cl.N, f.sd, disc->level(lev).pcode.get(), cl.dir, f.dir, cl.sd,
                                        c.stream));
-        else
-            UGX_LAUNCH(restrict_defect<T>(f.dim, cl.W, f.W, cl.N, f.sd, cl.dir, f.dir, cl.sd, c.stream));
+        } else {
+            // the restriction also clears the coarse correction
+            UGX_LAUNCH(restrict_defect<T>(f.dim, cl.W, f.W, cl.N, f.sd, cl.dir, f.dir, cl.sd, c.stream, cl.sc));
+        }
     }
 
     void prolongation_and_postsmooth(int lev) {
         auto& c = context();
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
-        if (tets)
+        if (tets) {
             UGX_LAUNCH(prolong_add_tet<T>(cl.W, f.W, f.N, cl.sc, disc->level(lev).pcode.get(), f.dir, f.st, f.sc,
                                           c.stream));
-        else
+        } else if (cl.acc_deferred) {
+            // the coarse level's last correction update, folded in
+            UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream, cl.dinv, cl.sd));
+        } else {
             UGX_LAUNCH(prolong_add<T>(f.dim, cl.W, f.W, f.N, cl.sc, f.dir, f.st, f.sc, c.stream));
+        }
+        cl.acc_deferred = false;
         if (nu2 == 0) return;
         UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
         for (int nu = 0; nu < nu2; ++nu) {
             if (nu < nu2 - 1) {
                 UGX_LAUNCH(op_jacobi_step(f));
                 std::swap(f.sd, f.sd2);
+            } else if (lev < top && !tets && cycle == 1) {
+                // folded into the prolongation to the next level (V-cycle only:
+                // a W-cycle visits the level again and needs sc updated)
+                f.acc_deferred = true;
             } else if (!(defer_top_final && lev == top)) {
                 UGX_LAUNCH(jacobi_accumulate<T>(f.N, f.dinv, f.sd, f.sc, c.stream));
             }
@@ -1191,7 +1212,12 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         UGX_LAUNCH((permute_to<double, T>(t.N, G.dof_of_lat.get(), d.data(), t.sd, c.stream)));
         cvec.set(0.0);
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
-        lmgc(top);
+        try {
+            lmgc(top);
+        } catch (...) {
+            clear_cycle_state();
+            throw;
+        }
         UGX_LAUNCH((permute_add_from<T, double>(t.N, G.lat_of_dof.get(), t.sc, cvec.data(), c.stream)));
         sym = sym_ok;
     }
@@ -1215,9 +1241,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         try {
             lmgc(top);
         } catch (...) {
-            defer_top_final = false;
-            first_defect = nullptr;
-            first_defect_f32 = nullptr;
+            clear_cycle_state();
             throw;
         }
         if (defer_top_final) {

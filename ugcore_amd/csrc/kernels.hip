@@ -1431,8 +1431,23 @@ __global__ void k_dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega
 // ---------------------------------------------------------------------------
 
 // prolongation with accumulation: st_f = sum_c P(f,c) sc_c ; sc_f += st_f
+// The coarse correction a prolongation reads: sc, or (with dinv) the value the
+// coarse level's last post-smoothing update sc + dinv .* sd would store, with
+// the arithmetic of k_jacobi_accumulate. Folding that update into the
+// prolongation saves its launch; the level's own sc is not read again.
 template <typename TV>
-__global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* __restrict__ scc,
+struct CoarseCorr {
+    const TV* sc;
+    const TV* dinv;
+    const TV* sd;
+    __device__ __forceinline__ TV operator()(int64_t j) const {
+        if (!dinv) return sc[j];
+        return (TV)((double)sc[j] + (double)dinv[j] * (double)sd[j]);
+    }
+};
+
+template <typename TV>
+__global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, int64_t n_f, CoarseCorr<TV> scc,
                                                      const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
                                                      TV* __restrict__ sc, FastDiv fw) {
     const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
@@ -1456,7 +1471,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
             for (int dz = 0; dz <= oz; ++dz)
                 for (int dy = 0; dy <= oy; ++dy)
                     for (int dx = 0; dx <= ox; ++dx)
-                        v += w * (double)scc[((int64_t)(z0 + dz) * Wc + (y0 + dy)) * Wc + (x0 + dx)];
+                        v += w * (double)scc(((int64_t)(z0 + dz) * Wc + (y0 + dy)) * Wc + (x0 + dx));
         }
     } else {
         // triangle mesh: odd coordinates lie on axis edges or on the (1,1) diagonal
@@ -1464,11 +1479,11 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
         if (bnd && dir_f[i] && nodd != 0) {
             v = 0.0;
         } else if (nodd == 0) {
-            v = (double)scc[(int64_t)y0 * Wc + x0];
+            v = (double)scc((int64_t)y0 * Wc + x0);
         } else if (nodd == 1) {
-            v = 0.5 * (double)scc[(int64_t)y0 * Wc + x0] + 0.5 * (double)scc[(int64_t)(y0 + oy) * Wc + (x0 + ox)];
+            v = 0.5 * (double)scc((int64_t)y0 * Wc + x0) + 0.5 * (double)scc((int64_t)(y0 + oy) * Wc + (x0 + ox));
         } else {
-            v = 0.5 * (double)scc[(int64_t)y0 * Wc + x0] + 0.5 * (double)scc[(int64_t)(y0 + 1) * Wc + (x0 + 1)];
+            v = 0.5 * (double)scc((int64_t)y0 * Wc + x0) + 0.5 * (double)scc((int64_t)(y0 + 1) * Wc + (x0 + 1));
         }
     }
     st[i] = (TV)v;
@@ -1482,7 +1497,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
 // neighbouring threads are contiguous). Weights and summation order as in
 // k_prolong_add, so the values are identical.
 template <typename TV>
-__device__ __forceinline__ void prolong3_point(int x, int y, int z, int Wc, int Wf, const TV* __restrict__ scc,
+__device__ __forceinline__ void prolong3_point(int x, int y, int z, int Wc, int Wf, const CoarseCorr<TV>& scc,
                                                const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
                                                TV* __restrict__ sc) {
     double v[2][2][2];
@@ -1493,7 +1508,7 @@ __device__ __forceinline__ void prolong3_point(int x, int y, int z, int Wc, int 
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
                 const bool in = x + dx < Wc && y + dy < Wc && z + dz < Wc;
-                v[dz][dy][dx] = in ? (double)scc[((int64_t)(z + dz) * Wc + (y + dy)) * Wc + (x + dx)] : 0.0;
+                v[dz][dy][dx] = in ? (double)scc(((int64_t)(z + dz) * Wc + (y + dy)) * Wc + (x + dx)) : 0.0;
             }
 #pragma unroll
     for (int oz = 0; oz < 2; ++oz)
@@ -1518,7 +1533,7 @@ __device__ __forceinline__ void prolong3_point(int x, int y, int z, int Wc, int 
             }
 }
 template <typename TV>
-__global__ __launch_bounds__(256) void k_prolong_add3(int Wc, int Wf, int64_t n_c, const TV* __restrict__ scc,
+__global__ __launch_bounds__(256) void k_prolong_add3(int Wc, int Wf, int64_t n_c, CoarseCorr<TV> scc,
                                                       const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
                                                       TV* __restrict__ sc, FastDiv fc) {
     const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
@@ -1533,9 +1548,10 @@ __global__ __launch_bounds__(256) void k_prolong_add3(int Wc, int Wf, int64_t n_
 template <typename TV>
 __global__ __launch_bounds__(256) void k_restrict(int dim, int Wc, int Wf, int64_t n_c, const TV* __restrict__ sdf,
                                                   const uint8_t* __restrict__ dir_c, const uint8_t* __restrict__ dir_f,
-                                                  TV* __restrict__ sdc, FastDiv fw) {
+                                                  TV* __restrict__ sdc, FastDiv fw, TV* __restrict__ zc) {
     const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (c >= n_c) return;
+    if (zc) zc[c] = (TV)0;   // the coarse correction starts from zero
     const uint32_t q1 = fdiv((uint32_t)c, fw);
     const int x = (int)((uint32_t)c - q1 * (uint32_t)Wc);
     const uint32_t q2 = dim == 3 ? fdiv(q1, fw) : 0u;
@@ -1620,7 +1636,7 @@ struct TransferTile {
 // line padded with one zero on either side
 template <typename TV>
 __global__ __launch_bounds__(256) void k_restrict3_tiled(int Wc, int Wf, int nbands, const TV* __restrict__ sdf,
-                                                         TV* __restrict__ sdc) {
+                                                         TV* __restrict__ sdc, TV* __restrict__ zc) {
     constexpr int TY = TransferTile<TV>::TY;
     constexpr int NL = 2 * TY + 1;
     const int z = blockIdx.x / nbands, band = blockIdx.x - z * nbands;
@@ -1661,6 +1677,7 @@ __global__ __launch_bounds__(256) void k_restrict3_tiled(int Wc, int Wf, int nba
         }
         const int y = y0 + ly;
         const int64_t c = ((int64_t)z * Wc + y) * Wc + x;
+        if (zc) zc[c] = (TV)0;   // the coarse correction starts from zero
         // fine point (2x + dx, 2y + dy, 2z + dz): plane 1 + dz, line 2 ly + 1 + dy, column 2x + 1 + dx
         auto at = [&](int dz, int dy, int dx) { return lds[((1 + dz) * NL + 2 * ly + 1 + dy) * LW + 2 * x + 1 + dx]; };
         if (x == 0 || y == 0 || z == 0 || x == Wc - 1 || y == Wc - 1 || z == Wc - 1) {
@@ -1686,7 +1703,7 @@ __global__ __launch_bounds__(256) void k_restrict3_tiled(int Wc, int Wf, int nba
 // fine lines 2y0 .. 2y0+2TY-1 of a coarse band; LDS holds coarse planes
 // fz/2 and fz/2 + 1, lines y0 .. y0+TY, each line padded with one zero
 template <typename TV>
-__global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nbands, const TV* __restrict__ scc,
+__global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nbands, CoarseCorr<TV> scc,
                                                         TV* __restrict__ st, TV* __restrict__ sc) {
     constexpr int TY = TransferTile<TV>::TY;
     constexpr int NL = TY + 1;
@@ -1704,7 +1721,7 @@ __global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nban
         const int pl = row >= NL ? 1 : 0, ln = row - pl * NL;
         const int cz = cz0 + pl, cy = y0 + ln;
         TV v = (TV)0;
-        if (cz < Wc && cy < Wc && xs < Wc) v = scc[((int64_t)cz * Wc + cy) * Wc + xs];
+        if (cz < Wc && cy < Wc && xs < Wc) v = scc(((int64_t)cz * Wc + cy) * Wc + xs);
         lds[t] = v;
         xs += 256;
         while (xs >= LW) {
@@ -2665,9 +2682,11 @@ template int dia_jacobi_diag<double, double>(int64_t, int, const double*, double
 
 template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
-                hipStream_t s) {
+                hipStream_t s, const TV* dinv_c, const TV* sd_c) {
     if (n_f <= 0) return 0;
     if (n_f >= ((int64_t)1 << 32)) return -1;
+    if ((dinv_c == nullptr) != (sd_c == nullptr)) return -1;
+    const CoarseCorr<TV> corr{scc, dinv_c, sd_c};
     if (dim == 3 && Wf == 2 * Wc - 1) {
         // measured on one MI355X, 129^3 -> 257^3: FP64 tiled 84 us against 118 us
         // for the per-coarse-point kernel; FP32 the other way round (62 / 46 us)
@@ -2676,24 +2695,26 @@ int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8
         if (sizeof(TV) == 8 && lds <= 64 * 1024) {
             const int nbands = (Wc + TY - 1) / TY;
             hipLaunchKernelGGL(k_prolong3_tiled<TV>, dim3((unsigned)((int64_t)Wf * nbands)), dim3(256), lds, s, Wc, Wf,
-                               nbands, scc, st, sc);
+                               nbands, corr, st, sc);
             UGX_CHECK_LAUNCH();
         }
         const int64_t n_c = (int64_t)Wc * Wc * Wc;
-        hipLaunchKernelGGL(k_prolong_add3<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, Wc, Wf, n_c, scc, dir_f, st,
+        hipLaunchKernelGGL(k_prolong_add3<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, Wc, Wf, n_c, corr, dir_f, st,
                            sc, make_fastdiv((uint32_t)Wc));
         UGX_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, scc, dir_f, st, sc,
-                       make_fastdiv((uint32_t)Wf));
+    hipLaunchKernelGGL(k_prolong_add<TV>, dim3((n_f + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_f, corr, dir_f, st,
+                       sc, make_fastdiv((uint32_t)Wf));
     UGX_CHECK_LAUNCH();
 }
-template int prolong_add<double>(int, int, int, int64_t, const double*, const uint8_t*, double*, double*, hipStream_t);
-template int prolong_add<float>(int, int, int, int64_t, const float*, const uint8_t*, float*, float*, hipStream_t);
+template int prolong_add<double>(int, int, int, int64_t, const double*, const uint8_t*, double*, double*, hipStream_t,
+                                 const double*, const double*);
+template int prolong_add<float>(int, int, int, int64_t, const float*, const uint8_t*, float*, float*, hipStream_t,
+                                const float*, const float*);
 
 template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
-                    TV* sdc, hipStream_t s) {
+                    TV* sdc, hipStream_t s, TV* zero_c) {
     if (n_c <= 0) return 0;
     if (n_c >= ((int64_t)1 << 32)) return -1;
     if (dim == 3 && Wf == 2 * Wc - 1) {
@@ -2704,16 +2725,18 @@ int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const u
         if (lds <= 64 * 1024) {
             const int nbands = (Wc + TY - 1) / TY;
             hipLaunchKernelGGL(k_restrict3_tiled<TV>, dim3((unsigned)((int64_t)Wc * nbands)), dim3(256), lds, s, Wc,
-                               Wf, nbands, sdf, sdc);
+                               Wf, nbands, sdf, sdc, zero_c);
             UGX_CHECK_LAUNCH();
         }
     }
     hipLaunchKernelGGL(k_restrict<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, dim, Wc, Wf, n_c, sdf, dir_c, dir_f, sdc,
-                       make_fastdiv((uint32_t)Wc));
+                       make_fastdiv((uint32_t)Wc), zero_c);
     UGX_CHECK_LAUNCH();
 }
-template int restrict_defect<double>(int, int, int, int64_t, const double*, const uint8_t*, const uint8_t*, double*, hipStream_t);
-template int restrict_defect<float>(int, int, int, int64_t, const float*, const uint8_t*, const uint8_t*, float*, hipStream_t);
+template int restrict_defect<double>(int, int, int, int64_t, const double*, const uint8_t*, const uint8_t*, double*,
+                                     hipStream_t, double*);
+template int restrict_defect<float>(int, int, int, int64_t, const float*, const uint8_t*, const uint8_t*, float*,
+                                    hipStream_t, float*);
 
 template <typename TV>
 int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s) {

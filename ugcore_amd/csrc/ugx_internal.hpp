@@ -117,12 +117,16 @@ inline int64_t dia_tiled_size(int64_t n, int nd) { return ((n + 63) / 64) * 64 *
 // P1 transfers between lattice levels. In 3D with Wf = 2 Wc - 1 the Dirichlet
 // flags must mark exactly the lattice boundary (as the multigrid levels' flags
 // do): the tiled kernels take the Dirichlet rows from the lattice geometry.
+// prolong_add with dinv_c and sd_c reads the coarse correction as
+// sc + dinv_c .* sd_c (the coarse level's last post-smoothing update, with the
+// arithmetic of jacobi_accumulate, which then need not run); restrict_defect
+// with zero_c also sets zero_c (the coarse correction) to 0
 template <typename TV>
 int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8_t* dir_f, TV* st, TV* sc,
-                hipStream_t s);
+                hipStream_t s, const TV* dinv_c = nullptr, const TV* sd_c = nullptr);
 template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
-                    TV* sdc, hipStream_t s);
+                    TV* sdc, hipStream_t s, TV* zero_c = nullptr);
 // x = U^{-1} L^{-1} P b with row-major triangular inverses; perm[i] = row of b
 template <typename TV>
 int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* perm, const TV* b, TV* x, hipStream_t s);
