@@ -291,6 +291,7 @@ class CG : public ILinearOperatorInverse {
     int64_t ws_n_ = -1, ws_h_ = -1;
     DeviceBuffer<double> ws_x_, ws_r_, ws_p_, ws_q_, ws_z_, ws_partial_, ws_sc_;
     DeviceBuffer<float> ws_rf_;   // FP32 copy of r for an FP32 cycle (with halos)
+    DeviceBuffer<double> ws_p2_;  // second direction buffer (fused direction update)
 };
 
 class BiCGStab : public ILinearOperatorInverse {
@@ -471,6 +472,12 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // (x must vanish on Dirichlet DoFs and carry the halo); with yx_out
     // (device) also (y, x) into yx_out[0] (partial: device scratch of
     // max_partials() doubles), false if that fusion is not available
+    // p_new = z + beta p_old (beta: device scalar, the arithmetic of the CG's
+    // direction update) and q = A p_new with (q, p_new) into qp_out[0] in one
+    // pass; p_new and p_old are distinct lattice vectors with the halo; false
+    // when the top level is not matrix-free
+    bool surface_direction_axpy_lattice(double* q, const double* p_old, const double* z, const double* beta,
+                                        double* p_new, double* qp_out, double* partial);
     bool surface_axpy_lattice(double* y, double alpha, const double* v, double beta, const double* x,
                               double* yx_out = nullptr, double* partial = nullptr);
     static constexpr int max_partials() { return 65536; }

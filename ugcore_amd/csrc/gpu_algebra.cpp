@@ -378,13 +378,21 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
     conv_->set_name(name());
     conv_->start_defect(dev_dot(N, rl, rl, true));
     UGX_HIP(hipMemcpyAsync(pl, zl, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
+    // second direction buffer: the fused update writes p_new beside p_old
+    if (ws_p2_.size() != N + 2 * H) {
+        ws_p2_.resize(N + 2 * H);
+        ws_p2_.zero();   // the halos
+    }
+    double* pl2 = ws_p2_.get() + H;
+    bool q_ready = false;   // q = A p and lambda already formed with the direction update
     bool ok = true;
     while (!conv_->iteration_ended()) {
         // q = A p, lambda = (q, p)
-        if (!gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl, sc + 1, part)) {
+        if (!q_ready && !gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl, sc + 1, part)) {
             gmg.surface_axpy_lattice(ql, 0.0, ql, 1.0, pl);
             UGX_LAUNCH(dot<double>(N, ql, pl, c.red_partial, sc + 1, false, c.stream));
         }
+        q_ready = false;
         UGX_LAUNCH(cg_alpha(sc, c.stream));
         UGX_LAUNCH(cg_update(N, sc, pl, ql, xl, rl, part, c.stream, rfl));
         UGX_HIP(hipMemcpyAsync(c.red_host, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -397,7 +405,13 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
         if (conv_->iteration_ended()) break;
         gmg.apply_lattice(zl, rl, sc + 4, part, rfl);   // z = B r, rho_new = (z, r)
         UGX_LAUNCH(cg_beta(sc, c.stream));          // beta = rho_new / rho, rho = rho_new
-        UGX_LAUNCH(cg_direction(N, sc, pl, zl, c.stream));
+        // p = z + beta p, fused with the next q = A p when the operator allows
+        if (gmg.surface_direction_axpy_lattice(ql, pl, zl, sc + 3, pl2, sc + 1, part)) {
+            std::swap(pl, pl2);
+            q_ready = true;
+        } else {
+            UGX_LAUNCH(cg_direction(N, sc, pl, zl, c.stream));
+        }
     }
     UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), xl, x.data(), c.stream)));
     UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, r.data(), c.stream)));
@@ -822,6 +836,8 @@ struct AssembledMultiGridCycle::Impl {
     virtual bool uses_f32_defect() const = 0;
     virtual bool surface_axpy(double* y, double alpha, const double* v, double beta, const double* x, double* yx_out,
                               double* partial) = 0;
+    virtual bool surface_direction_axpy(double* q, const double* p_old, const double* z, const double* beta,
+                                        double* p_new, double* qp_out, double* partial) = 0;
     virtual int64_t halo() const = 0;
 };
 
@@ -1321,6 +1337,20 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         return true;
     }
 
+    // p_new = z + beta p_old and q = A p_new with (q, p_new) in one pass
+    // (matrix-free top level only)
+    bool surface_direction_axpy(double* q, const double* p_old, const double* z, const double* beta, double* p_new,
+                                double* qp_out, double* partial) override {
+        if (!mf) return false;
+        auto& c = context();
+        int np = AssembledMultiGridCycle::max_partials();
+        if (q1mf_direction_axpy<double>(L[top].W, cell_c64.get(), p_old, z, beta, p_new, q, c.stream, partial, &np) !=
+            0)
+            return false;
+        UGX_LAUNCH(sum_partials(np, partial, qp_out, c.stream));
+        return true;
+    }
+
     int64_t halo() const override { return L[top].H; }
 
     void download_level_dia(int lev, double* out) const override {
@@ -1382,6 +1412,12 @@ void AssembledMultiGridCycle::apply_lattice(double* c_lat, const double* d_lat, 
     impl_->apply_lattice(c_lat, d_lat, cd_out, partial, d_lat_f32);
 }
 bool AssembledMultiGridCycle::lattice_uses_f32_defect() const { return impl_ && impl_->uses_f32_defect(); }
+bool AssembledMultiGridCycle::surface_direction_axpy_lattice(double* q, const double* p_old, const double* z,
+                                                             const double* beta, double* p_new, double* qp_out,
+                                                             double* partial) {
+    if (!impl_) throw UGError("GMG: not initialised");
+    return impl_->surface_direction_axpy(q, p_old, z, beta, p_new, qp_out, partial);
+}
 bool AssembledMultiGridCycle::surface_axpy_lattice(double* y, double alpha, const double* v, double beta,
                                                    const double* x, double* yx_out, double* partial) {
     if (!impl_) throw UGError("GMG: not initialised");

@@ -531,7 +531,7 @@ __global__ __launch_bounds__(256) void k_cg_direction(int64_t n, const double* _
         }
 #pragma unroll
         for (int e = 0; e < kStreamE; ++e)
-            if (UGX_STREAM_IDX(e) < n) stnt(beta * pv[e] + 1.0 * zv[e], p + UGX_STREAM_IDX(e));
+            if (UGX_STREAM_IDX(e) < n) stnt(fma(beta, pv[e], zv[e]), p + UGX_STREAM_IDX(e));
     }
 }
 
@@ -1122,6 +1122,10 @@ constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= k
 //         zeroing sc and MODE 0)
 // MODE 4: the same from a defect already in T (b_in = sd): MODE 0 without
 //         reading sc
+// MODE 5: the CG direction update fused with the operator: staged
+//         p_new = fma(beta, p_old, z) (a_in = p_old, b_in = z, beta = *d64,
+//         the arithmetic of k_cg_direction), out1 = p_new for the block's rows,
+//         y = A p_new and the (y, p_new) partials as MODE 2 with alpha 0
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? UGX_MF_WAVES32 : 1)))
 void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
@@ -1134,7 +1138,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
     double dot_acc = 0.0;
     __shared__ double dot_sh[4];
     auto finish_dot = [&]() {
-        if (MODE != 2 || !dot_partial) return;
+        if ((MODE != 2 && MODE != 5) || !dot_partial) return;
         double v = dot_acc;
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if ((threadIdx.x & 63) == 0) dot_sh[threadIdx.x >> 6] = v;
@@ -1169,8 +1173,12 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
         if (cz < 0 || cz >= n || cy < 0 || cy >= n || cx < 1 || cx > n) return (T)0;
         return cell[((int64_t)cz * n + cy) * n + cx - 1];
     };
+    const T bcg = MODE == 5 ? (T)*d64 : (T)0;   // the CG's beta (MODE 5)
     auto staged = [&](int64_t j) -> T {
-        return (MODE == 0 || MODE == 4) ? a_in[j] * b_in[j] : MODE == 3 ? a_in[j] * (T)d64[j] : a_in[j];
+        return (MODE == 0 || MODE == 4) ? a_in[j] * b_in[j]
+               : MODE == 3             ? a_in[j] * (T)d64[j]
+               : MODE == 5             ? (T)fma((double)bcg, (double)a_in[j], (double)b_in[j])
+                                       : a_in[j];
     };
     // the cell each prefetch slot stages sits at the same place of every cell
     // plane: its offset inside the plane, or -1 outside the lattice / the tile
@@ -1207,7 +1215,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             const int t = tid + u * step;
             const bool ok = more && t < P;
             pa[u] = ok ? a_in[f2 + t] : (T)0;
-            if (MODE == 0 || MODE == 4) pb[u] = ok ? b_in[f2 + t] : (T)0;
+            if (MODE == 0 || MODE == 4 || MODE == 5) pb[u] = ok ? b_in[f2 + t] : (T)0;
             if (MODE == 3) pb[u] = ok ? (T)d64[f2 + t] : (T)0;
             cv[u] = (more && coff[u] >= 0 && k + 1 < n) ? cell[(int64_t)(k + 1) * n * n + coff[u]] : (T)0;
         }
@@ -1304,6 +1312,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             } else {
                 const T yv = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
                 out0[i] = yv;
+                if (MODE == 5) out1[i] = xi;   // p_new
                 dot_acc += (double)yv * (double)xi;
             }
         }
@@ -1317,7 +1326,10 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
 #pragma unroll
             for (int u = 0; u < kMfPrefetch; ++u) {
                 const int t = tid + u * step;
-                if (t < P) dst[t] = (MODE == 0 || MODE == 3 || MODE == 4) ? pa[u] * pb[u] : pa[u];
+                if (t < P)
+                    dst[t] = (MODE == 0 || MODE == 3 || MODE == 4) ? pa[u] * pb[u]
+                             : MODE == 5 ? (T)fma((double)bcg, (double)pa[u], (double)pb[u])
+                                         : pa[u];
                 if (t < CL) cdst[t] = cv[u];
             }
         }
@@ -2590,6 +2602,25 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
 template int q1mf_axpy<double>(int, const double*, double, const double*, double, const double*, double*, hipStream_t,
                                double*, int*);
 
+// p_new = fma(beta, p_old, z) (beta = *beta_dev), y = A p_new and the (y, p_new)
+// partials in one pass; p_new must not alias p_old (other workgroups still
+// stage p_old around their bands)
+template <typename T>
+int q1mf_direction_axpy(int W, const T* cell, const T* p_old, const T* z, const double* beta_dev, T* p_new, T* y,
+                        hipStream_t s, double* dot_partial, int* n_partial) {
+    if (p_new == p_old) return -1;
+    MfZMap zm;
+    size_t zlds;
+    const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 5>), zm, zlds);
+    if (!zgrid || !dot_partial || !n_partial || zgrid > *n_partial) return -1;
+    hipLaunchKernelGGL((k_q1mf_z<T, 5>), dim3(zgrid), dim3(256), zlds, s, zm, cell, p_old, z, (const T*)nullptr, y,
+                       p_new, 0.0, 1.0, dot_partial, beta_dev);
+    *n_partial = (int)zgrid;
+    UGX_CHECK_LAUNCH();
+}
+template int q1mf_direction_axpy<double>(int, const double*, const double*, const double*, const double*, double*,
+                                         double*, hipStream_t, double*, int*);
+
 // asks the runtime for the residency of every z-marching kernel at lattice
 // width W, so that no query happens inside a captured cycle
 int q1mf_prepare(int W) {
@@ -2603,6 +2634,7 @@ int q1mf_prepare(int W) {
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 0>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 1>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 2>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 5>), zm, lds);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -70,6 +70,12 @@ int q1mf_prepare(int W);
 bool q1mf_first_fits(int W, size_t elem);
 template <typename T>
 int q1mf_jacobi_first(int W, const T* cell, const T* dinv, const double* d64, T* sd_out, T* sc, hipStream_t s);
+// the CG direction update fused with the operator: p_new = fma(*beta_dev,
+// p_old, z) (as cg_direction), y = A p_new, (y, p_new) partials; p_new and
+// p_old distinct, both with the halo
+template <typename T>
+int q1mf_direction_axpy(int W, const T* cell, const T* p_old, const T* z, const double* beta_dev, T* p_new, T* y,
+                        hipStream_t s, double* dot_partial, int* n_partial);
 // the same from a defect already in T (with the halo)
 template <typename T>
 int q1mf_jacobi_first_t(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s);
