@@ -278,14 +278,21 @@ class ILinearOperatorInverse {
 class CG : public ILinearOperatorInverse {
    public:
     const char* name() const override { return "CG"; }
+    bool apply(GPUVector& x, const GPUVector& b) override;
     bool apply_return_defect(GPUVector& x, GPUVector& b) override;
     // when on (default) and the preconditioner is a GMG initialised with this
-    // operator, the iteration runs on lattice-ordered copies of the vectors
+    // operator, the iteration runs on lattice-ordered copies of the vectors;
+    // with x and b zero on the Dirichlet DoFs the initial defect is formed
+    // there too (r = b - A x on the lattice operator)
     void set_lattice_fast_path(bool on) { fast_ = on; }
     bool last_used_lattice_path() const { return used_fast_; }
 
    private:
-    bool apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg);
+    int lattice_start_kind(const GPUVector& x, const GPUVector& b, int64_t ni);
+    // kind 0: src is the defect; 1: src = b and x = 0; 2: src = b, r = b - A x;
+    // the final defect is written to defect_out when given
+    bool apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCycle& gmg, int kind,
+                       GPUVector* defect_out);
     bool fast_ = true, used_fast_ = false;
     // lattice-path work space, kept across solves (halos stay zero)
     int64_t ws_n_ = -1, ws_h_ = -1;

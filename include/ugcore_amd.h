@@ -259,6 +259,10 @@ int ugx_solver_set_conv_check(void* s, int max_steps, double min_defect, double 
 int ugx_solver_init(void* s, void* matrix);
 /* returns 1 if converged, 0 if not, <0 on error */
 int ugx_solver_apply(void* s, void* x, void* b);
+/* ILinearOperatorInverse::apply_return_defect (operator/interface/linear_operator_inverse.h:188):
+ * solves A x = b from the initial x and leaves the final defect b - A x in b;
+ * 1 converged, 0 not converged, -1 error */
+int ugx_solver_apply_return_defect(void* s, void* x, void* b);
 int ugx_solver_steps(void* s);
 /* CG only: run on lattice-ordered copies of the vectors when the
  * preconditioner is a GMG built on the same assembly (1 = on, default) */

@@ -124,6 +124,7 @@ _SIGNATURES = {
     "ugx_solver_set_conv_check": (_I, [_P, _I, _D, _D, _I]),
     "ugx_solver_init": (_I, [_P, _P]),
     "ugx_solver_apply": (_I, [_P, _P, _P]),
+    "ugx_solver_apply_return_defect": (_I, [_P, _P, _P]),
     "ugx_solver_steps": (_I, [_P]),
     "ugx_solver_defects": (_I, [_P, _P, _I]),
     "ugx_solver_set_lattice_path": (_I, [_P, _I]),
@@ -542,6 +543,12 @@ class _Solver:
 
     def apply(self, x, b):
         rc = _check(lib().ugx_solver_apply(self.h, x.h, b.h), "solver.apply")
+        return rc == 1
+
+    def apply_return_defect(self, x, b):
+        """solves from the initial x and leaves the final defect b - A x in b
+        (linear_operator_inverse.h:188)"""
+        rc = _check(lib().ugx_solver_apply_return_defect(self.h, x.h, b.h), "solver.apply_return_defect")
         return rc == 1
 
     def steps(self):

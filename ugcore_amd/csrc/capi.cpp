@@ -613,6 +613,9 @@ int ugx_solver_init(void* s, void* matrix) {
 int ugx_solver_apply(void* s, void* x, void* b) {
     return guard([&] { return ((SolverH*)s)->s->apply(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });
 }
+int ugx_solver_apply_return_defect(void* s, void* x, void* b) {
+    return guard([&] { return ((SolverH*)s)->s->apply_return_defect(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });
+}
 int ugx_solver_steps(void* s) { return ((SolverH*)s)->s->step(); }
 int ugx_solver_set_restart(void* s, int n) {
     return guard([&] {

@@ -337,7 +337,47 @@ static double dev_dot(int64_t n, const double* a, const double* b, bool take_sqr
     return reduce_to_host(0);
 }
 
-bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg) {
+// How the lattice path can start for x and b (one host synchronisation for
+// three sums): 1 = x is zero and b vanishes on the Dirichlet DoFs (r = b);
+// 2 = x and b both vanish there (r = b - A x with the Dirichlet-eliminated
+// lattice operator, which equals A x for such x); 0 = neither.
+int CG::lattice_start_kind(const GPUVector& x, const GPUVector& b, int64_t ni) {
+    auto& c = context();
+    const int64_t N = x.size(), nt = N - ni;
+    if (b.size() != N || nt < 0) return 0;
+    UGX_LAUNCH(dot<double>(N, x.data(), x.data(), c.red_partial, c.red_out + 0, false, c.stream));
+    if (nt > 0) {
+        UGX_LAUNCH(dot<double>(nt, x.data() + ni, x.data() + ni, c.red_partial, c.red_out + 1, false, c.stream));
+        UGX_LAUNCH(dot<double>(nt, b.data() + ni, b.data() + ni, c.red_partial, c.red_out + 2, false, c.stream));
+    } else {
+        UGX_LAUNCH(vec_set<double>(2, c.red_out + 1, 0.0, c.stream));
+    }
+    UGX_HIP(hipMemcpyAsync(c.red_host, c.red_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    sync();
+    if (c.red_host[2] != 0.0) return 0;
+    if (c.red_host[0] == 0.0) return 1;
+    return c.red_host[1] == 0.0 ? 2 : 0;
+}
+
+bool CG::apply(GPUVector& x, const GPUVector& b) {
+    // the lattice path straight from b: no defect copy, and the final defect
+    // (which apply does not return) stays in lattice order
+    used_fast_ = false;
+    if (fast_ && op_) {
+        auto* gmg = dynamic_cast<AssembledMultiGridCycle*>(precond_.get());
+        if (gmg && gmg->lattice_ready(op_->get_matrix())) {
+            const int kind = lattice_start_kind(x, b, gmg->num_inner_dofs());
+            if (kind) {
+                used_fast_ = true;
+                return apply_lattice(x, b, *gmg, kind, nullptr);
+            }
+        }
+    }
+    return ILinearOperatorInverse::apply(x, b);
+}
+
+bool CG::apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCycle& gmg, int kind,
+                       GPUVector* defect_out) {
     // The iteration of cg.h:103-242 on lattice-ordered copies of x and r with
     // the surface operator and the preconditioner applied in lattice order.
     // Valid because p and z vanish on Dirichlet DoFs (r does on entry).
@@ -372,8 +412,14 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
         }
         rfl = ws_rf_.get() + H;
     }
-    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
-    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), r.data(), rl, c.stream)));
+    // the defect in lattice order: src is the defect (kind 0) or b (kinds 1, 2)
+    if (kind == 1) {
+        UGX_LAUNCH(vec_set<double>(N, xl, 0.0, c.stream));
+    } else {
+        UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
+    }
+    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), src.data(), rl, c.stream)));
+    if (kind == 2) gmg.surface_axpy_lattice(rl, 1.0, rl, -1.0, xl);   // r = b - A x
     gmg.apply_lattice(zl, rl, sc + 0, part);   // z = B r, rho = (z, r)
     conv_->set_name(name());
     conv_->start_defect(dev_dot(N, rl, rl, true));
@@ -414,25 +460,32 @@ bool CG::apply_lattice(GPUVector& x, GPUVector& r, AssembledMultiGridCycle& gmg)
         }
     }
     UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), xl, x.data(), c.stream)));
-    UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, r.data(), c.stream)));
+    if (defect_out) UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, defect_out->data(), c.stream)));
     sync();
     return ok && conv_->post();
 }
 
 bool CG::apply_return_defect(GPUVector& x, GPUVector& b) {
     // cg.h:103-242
+    used_fast_ = false;
+    auto* gmg = fast_ ? dynamic_cast<AssembledMultiGridCycle*>(precond_.get()) : nullptr;
+    if (gmg && !gmg->lattice_ready(op_->get_matrix())) gmg = nullptr;
+    if (gmg) {
+        // the initial defect formed in lattice order where x and b allow it
+        const int kind = lattice_start_kind(x, b, gmg->num_inner_dofs());
+        if (kind) {
+            used_fast_ = true;
+            return apply_lattice(x, b, *gmg, kind, &b);
+        }
+    }
     GPUVector& r = b;
     op_->apply_sub(r, x);
-    used_fast_ = false;
-    if (fast_) {
-        auto* gmg = dynamic_cast<AssembledMultiGridCycle*>(precond_.get());
-        if (gmg && gmg->lattice_ready(op_->get_matrix())) {
-            const int64_t ni = gmg->num_inner_dofs();
-            const double tail = r.size() > ni ? dev_dot(r.size() - ni, r.data() + ni, r.data() + ni, false) : 0.0;
-            if (tail == 0.0) {
-                used_fast_ = true;
-                return apply_lattice(x, r, *gmg);
-            }
+    if (gmg) {
+        const int64_t ni = gmg->num_inner_dofs();
+        const double tail = r.size() > ni ? dev_dot(r.size() - ni, r.data() + ni, r.data() + ni, false) : 0.0;
+        if (tail == 0.0) {
+            used_fast_ = true;
+            return apply_lattice(x, r, *gmg, 0, &r);
         }
     }
     GPUVector q(r.size()), z(x.size()), p(x.size());
