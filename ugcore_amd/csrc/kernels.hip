@@ -1770,6 +1770,123 @@ __global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nban
     }
 }
 
+// z-marching restriction: a workgroup owns TY coarse lines of a chunk of
+// coarse planes and keeps the fine planes 2z-1, 2z, 2z+1 of its band (lines
+// 2y0-1 .. 2y0+2TY-1, padded by one zero per side) in an LDS ring of three;
+// coarse plane z+1 needs two new fine planes, which are loaded into registers
+// while plane z is computed and then replace 2z-1 and 2z. Each fine value is
+// loaded once per chunk (plus one plane per chunk), by long-lived workgroups.
+// Arithmetic and order as k_restrict3_tiled.
+struct RzMap {
+    int32_t Wc, Wf, TY, nchunks;
+    int32_t glo[8], gsz[8];
+};
+constexpr int kRzSlots = 6;   // staged entries per thread and fine plane (5 lines x (Wf + 2) <= 6 * 256)
+
+template <typename TV>
+__global__ __launch_bounds__(256) void k_restrict3_z(RzMap m, const TV* __restrict__ sdf, TV* __restrict__ sdc,
+                                                     TV* __restrict__ zc) {
+    const int xc = blockIdx.x & 7, q = blockIdx.x >> 3, g = m.gsz[xc];
+    const int chunk = g == 0 ? 0 : q / g;
+    if (g == 0 || chunk >= m.nchunks) return;
+    const int band = m.glo[xc] + (q - chunk * g);
+    const int Wc = m.Wc, Wf = m.Wf, TY = m.TY;
+    const int y0 = band * TY, ny = min(TY, Wc - y0);
+    const int z0 = (int)((int64_t)chunk * Wc / m.nchunks), z1 = (int)((int64_t)(chunk + 1) * Wc / m.nchunks);
+    const int NL = 2 * TY + 1, LW = Wf + 2, PL = NL * LW;
+    const int tid = threadIdx.x;
+    const int64_t Wf2 = (int64_t)Wf * Wf;
+    TV* lds = lds_base<TV>();
+    // offset inside a fine plane of each staging slot (-1: outside the lattice
+    // or the tile); the same in every plane
+    int off[kRzSlots];
+#pragma unroll
+    for (int u = 0; u < kRzSlots; ++u) {
+        const int t = tid + u * 256;
+        off[u] = -1;
+        if (t < PL) {
+            const int ln = t / LW, xi = t - ln * LW;
+            const int fy = 2 * y0 - 1 + ln, fx = xi - 1;
+            if (fy >= 0 && fy < Wf && fx >= 0 && fx < Wf) off[u] = fy * Wf + fx;
+        }
+    }
+    auto slot = [&](int f) { return lds + ((f + 3) % 3) * PL; };
+    auto fetch = [&](int f, int u) -> TV {
+        return (f >= 0 && f < Wf && off[u] >= 0) ? sdf[(int64_t)f * Wf2 + off[u]] : (TV)0;
+    };
+    for (int f = 2 * z0 - 1; f <= 2 * z0 + 1; ++f) {
+        TV* dst = slot(f);
+#pragma unroll
+        for (int u = 0; u < kRzSlots; ++u) {
+            const int t = tid + u * 256;
+            if (t < PL) dst[t] = fetch(f, u);
+        }
+    }
+    __syncthreads();
+    // (line, column) of this thread's first output, advanced without divisions
+    int ly0 = 0, x0 = tid;
+    while (x0 >= Wc) {
+        x0 -= Wc;
+        ++ly0;
+    }
+    for (int z = z0; z < z1; ++z) {
+        const bool more = z + 1 < z1;
+        TV pa[kRzSlots], pb[kRzSlots];
+#pragma unroll
+        for (int u = 0; u < kRzSlots; ++u) {
+            pa[u] = more ? fetch(2 * z + 2, u) : (TV)0;
+            pb[u] = more ? fetch(2 * z + 3, u) : (TV)0;
+        }
+        const TV* sm = slot(2 * z - 1);
+        const TV* s0 = slot(2 * z);
+        const TV* sp = slot(2 * z + 1);
+        int ly = ly0, x = x0;
+        for (int t = tid; t < ny * Wc; t += 256, x += 256) {
+            while (x >= Wc) {
+                x -= Wc;
+                ++ly;
+            }
+            const int y = y0 + ly;
+            const int64_t c = ((int64_t)z * Wc + y) * Wc + x;
+            if (zc) zc[c] = (TV)0;   // the coarse correction starts from zero
+            auto at = [&](int dz, int dy, int dx) {
+                const TV* pl = dz < 0 ? sm : (dz > 0 ? sp : s0);
+                return pl[(2 * ly + 1 + dy) * LW + 2 * x + 1 + dx];
+            };
+            if (x == 0 || y == 0 || z == 0 || x == Wc - 1 || y == Wc - 1 || z == Wc - 1) {
+                sdc[c] = at(0, 0, 0);
+                continue;
+            }
+            double sum = 0.0;
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int nodd = (dx != 0) + (dy != 0) + (dz != 0);
+                        const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+                        sum += w * (double)at(dz, dy, dx);
+                    }
+            sdc[c] = (TV)sum;
+        }
+        __syncthreads();   // every thread is done with planes 2z-1 and 2z
+        if (more) {
+            TV* da = slot(2 * z + 2);
+            TV* db = slot(2 * z + 3);
+#pragma unroll
+            for (int u = 0; u < kRzSlots; ++u) {
+                const int t = tid + u * 256;
+                if (t < PL) {
+                    da[t] = pa[u];
+                    db[t] = pb[u];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 static size_t restrict3_tiled_lds(int Wf, size_t elem, int TY) { return (size_t)3 * (2 * TY + 1) * (Wf + 2) * elem; }
 static size_t prolong3_tiled_lds(int Wc, size_t elem, int TY) { return (size_t)2 * (TY + 1) * (Wc + 1) * elem; }
 
@@ -2744,11 +2861,51 @@ template int prolong_add<double>(int, int, int, int64_t, const double*, const ui
 template int prolong_add<float>(int, int, int, int64_t, const float*, const uint8_t*, float*, float*, hipStream_t,
                                 const float*, const float*);
 
+// plan of k_restrict3_z: 2-line bands (1-line when a staged plane would not
+// fit the slots) grouped per XCD, chunks so that the grid is one round;
+// returns the grid, 0 when the plan does not apply
+template <typename TV>
+static int64_t make_rz_map(int Wc, int Wf, RzMap& m, size_t& lds) {
+    m = RzMap{};
+    m.Wc = Wc;
+    m.Wf = Wf;
+    m.TY = 5 * (Wf + 2) <= kRzSlots * 256 ? 2 : 1;
+    const int64_t PL = (int64_t)(2 * m.TY + 1) * (Wf + 2);
+    if (PL > kRzSlots * 256 || Wc < 3) return 0;
+    lds = (size_t)(3 * PL) * sizeof(TV);
+    if (lds > 64 * 1024) return 0;
+    const int64_t nb = (Wc + m.TY - 1) / m.TY;
+    int32_t maxg = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int64_t b0 = c * nb / 8, b1 = (c + 1) * nb / 8;
+        m.glo[c] = (int32_t)b0;
+        m.gsz[c] = (int32_t)(b1 - b0);
+        maxg = m.gsz[c] > maxg ? m.gsz[c] : maxg;
+    }
+    if (maxg == 0) return 0;
+    const int64_t slots = (int64_t)(device_cus() / 8) *
+                          std::min(UGX_MF_MAXRES, mf_resident_blocks(reinterpret_cast<const void*>(&k_restrict3_z<TV>), lds));
+    int64_t nch = slots / maxg;
+    nch = std::min<int64_t>(std::max<int64_t>(nch, 1), std::max(1, Wc / 2));
+    m.nchunks = (int32_t)nch;
+    return 8 * (int64_t)maxg * m.nchunks;
+}
+
 template <typename TV>
 int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const uint8_t* dir_c, const uint8_t* dir_f,
                     TV* sdc, hipStream_t s, TV* zero_c) {
     if (n_c <= 0) return 0;
     if (n_c >= ((int64_t)1 << 32)) return -1;
+    if (dim == 3 && Wf == 2 * Wc - 1 && Wc >= 65) {
+        // large levels: the z-marching restriction (short-lived staging
+        // workgroups left the tiled kernel at about 1.1 TB/s at 257^3)
+        RzMap rm;
+        size_t rlds = 0;
+        if (const int64_t grid = make_rz_map<TV>(Wc, Wf, rm, rlds)) {
+            hipLaunchKernelGGL(k_restrict3_z<TV>, dim3((unsigned)grid), dim3(256), rlds, s, rm, sdf, sdc, zero_c);
+            UGX_CHECK_LAUNCH();
+        }
+    }
     if (dim == 3 && Wf == 2 * Wc - 1) {
         // measured on one MI355X: 257^3 -> 129^3 FP32 59 / 60 us, FP64 65 / 74 us,
         // 33^3 -> 17^3 about 4 / 8 us against the per-coarse-point kernel
