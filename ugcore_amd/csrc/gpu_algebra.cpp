@@ -418,9 +418,17 @@ bool CG::apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCyc
     } else {
         UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), x.data(), xl, c.stream)));
     }
-    UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), src.data(), rl, c.stream)));
+    // the FP32 copy of the defect comes with the permutation unless the
+    // defect is formed after it (kind 2)
+    const float* rf_first = nullptr;
+    if (rfl && kind != 2) {
+        UGX_LAUNCH(permute_to_dual(N, gmg.dof_of_lattice(), src.data(), rl, rfl, c.stream));
+        rf_first = rfl;
+    } else {
+        UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), src.data(), rl, c.stream)));
+    }
     if (kind == 2) gmg.surface_axpy_lattice(rl, 1.0, rl, -1.0, xl);   // r = b - A x
-    gmg.apply_lattice(zl, rl, sc + 0, part);   // z = B r, rho = (z, r)
+    gmg.apply_lattice(zl, rl, sc + 0, part, rf_first);   // z = B r, rho = (z, r)
     conv_->set_name(name());
     conv_->start_defect(dev_dot(N, rl, rl, true));
     UGX_HIP(hipMemcpyAsync(pl, zl, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));

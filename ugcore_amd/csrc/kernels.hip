@@ -385,6 +385,15 @@ __global__ void k_permute_to(int64_t n, const int32_t* __restrict__ dof_of_lat, 
     for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x)
         dst[l] = (TL)src[dof_of_lat[l]];
 }
+// the same into an FP64 and an FP32 lattice vector
+__global__ void k_permute_to_dual(int64_t n, const int32_t* __restrict__ dof_of_lat, const double* __restrict__ src,
+                                  double* __restrict__ dst, float* __restrict__ dst32) {
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+        const double v = src[dof_of_lat[l]];
+        dst[l] = v;
+        dst32[l] = (float)v;
+    }
+}
 template <typename TL, typename TS>
 __global__ void k_permute_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
                                TS* __restrict__ dst) {
@@ -2410,6 +2419,11 @@ int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hip
 }
 template int permute_to<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
 template int permute_to<double, float>(int64_t, const int32_t*, const double*, float*, hipStream_t);
+int permute_to_dual(int64_t n, const int32_t* dof_of_lat, const double* src, double* dst, float* dst32, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_permute_to_dual, dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst, dst32);
+    UGX_CHECK_LAUNCH();
+}
 
 template <typename TL, typename TS>
 int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s) {

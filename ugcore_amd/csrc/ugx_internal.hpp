@@ -49,6 +49,8 @@ template <typename T>
 int convert_dot(int64_t n, const T* c, double* z, const double* r, double* partial, double* out, hipStream_t s);
 template <typename TS, typename TL>
 int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hipStream_t s);
+// dst = src permuted into lattice order, dst32 = (float) dst
+int permute_to_dual(int64_t n, const int32_t* dof_of_lat, const double* src, double* dst, float* dst32, hipStream_t s);
 template <typename TL, typename TS>
 int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s);
 
