@@ -444,4 +444,27 @@ void orc_csr_op(int op, int64_t rows, int64_t cols, const int64_t* rowptr, const
     std::memcpy(dest, d.data(), sizeof(double) * nd);
 }
 
+// element matrices of n Q2 hexahedra: X [n][8][3] corners in the reference
+// order, k [n] (or NULL: 1), Ke [n][27][27], fe [n][27] (or NULL); mi [27][3]
+// receives the local multi-indices when not NULL
+void orc_q2_hex_elements(int64_t n, const double* X, const double* k, double f_const, double* Ke, double* fe,
+                         int32_t* mi) {
+    if (mi) {
+        int t[27][3];
+        orc::q2_hex_multi_index(t);
+        for (int a = 0; a < 27; ++a)
+            for (int d = 0; d < 3; ++d) mi[3 * a + d] = t[a][d];
+    }
+    for (int64_t e = 0; e < n; ++e) {
+        double Xe[8][3], K[27][27], F[27];
+        for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) Xe[a][d] = X[(e * 8 + a) * 3 + d];
+        orc::q2_hex_element(Xe, k ? k[e] : 1.0, f_const, K, F);
+        for (int a = 0; a < 27; ++a) {
+            for (int b = 0; b < 27; ++b) Ke[(e * 27 + a) * 27 + b] = K[a][b];
+            if (fe) fe[e * 27 + a] = F[a];
+        }
+    }
+}
+
 }  // extern "C"

@@ -172,6 +172,109 @@ void adjoint_gradient_q1_hex(const MultiGrid& mg, const Coefficient& k, const st
     }
 }
 
+// ---------------------------------------------------------------------------
+// Q2 (triquadratic Lagrange) hexahedron element matrices.
+// Local shape order: lagrange.cpp:236-253 (SetLagrangeMultiIndex) applied to
+// the reference hexahedron of reference_element.cpp (corners, EDGE->POINT and
+// FACE->POINT tables): vertices p*corner (lagrange.cpp:40-62), edge DoFs
+// i*co1 + (p-i)*co0 (:68-105), face DoFs p*co0 + i*(co1-co0) + j*(co3-co0)
+// (:110-160), the interior (:167-230). Shape i is the product of the 1D
+// equidistant Lagrange polynomials of its multi-index on the nodes {0, 1/2, 1}
+// (lagrange.h:2106-2190, EquidistantLagrange1D). Geometry: the trilinear map
+// of the 8 corners; 3x3x3 Gauss-Legendre points (exact for the Q2 stiffness
+// of a parallelepiped); k constant per element.
+// ---------------------------------------------------------------------------
+void q2_hex_multi_index(int mi[27][3]) {
+    static const int co[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    static const int edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                                    {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    static const int face[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                   {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
+    const int p = 2;
+    int n = 0;
+    for (int v = 0; v < 8; ++v, ++n)
+        for (int d = 0; d < 3; ++d) mi[n][d] = p * co[v][d];
+    for (int e = 0; e < 12; ++e)
+        for (int i = 1; i < p; ++i, ++n)
+            for (int d = 0; d < 3; ++d) mi[n][d] = i * co[edge[e][1]][d] + (p - i) * co[edge[e][0]][d];
+    for (int f = 0; f < 6; ++f) {
+        const int c0 = face[f][0], c1 = face[f][1], c2 = face[f][3];   // quadrilateral: co2 := corner 3
+        for (int j = 1; j < p; ++j)
+            for (int i = 1; i < p; ++i, ++n)
+                for (int d = 0; d < 3; ++d)
+                    mi[n][d] = p * co[c0][d] + i * (co[c1][d] - co[c0][d]) + j * (co[c2][d] - co[c0][d]);
+    }
+    for (int m2 = 1; m2 < p; ++m2)
+        for (int m1 = 1; m1 < p; ++m1)
+            for (int m0 = 1; m0 < p; ++m0, ++n) {
+                mi[n][0] = m0;
+                mi[n][1] = m1;
+                mi[n][2] = m2;
+            }
+}
+
+static double lagrange1d(int i, double x) {
+    // Lagrange polynomial of node i/2 on the nodes {0, 1/2, 1}
+    double v = 1.0;
+    for (int m = 0; m < 3; ++m)
+        if (m != i) v *= (x - 0.5 * m) / (0.5 * i - 0.5 * m);
+    return v;
+}
+static double dlagrange1d(int i, double x) {
+    double s = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        if (a == i) continue;
+        double t = 1.0 / (0.5 * i - 0.5 * a);
+        for (int m = 0; m < 3; ++m)
+            if (m != i && m != a) t *= (x - 0.5 * m) / (0.5 * i - 0.5 * m);
+        s += t;
+    }
+    return s;
+}
+
+void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27][27], double fe[27]) {
+    int mi[27][3];
+    q2_hex_multi_index(mi);
+    const double r = 0.5 * std::sqrt(0.6);
+    const double gx[3] = {0.5 - r, 0.5, 0.5 + r}, gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    for (int a = 0; a < 27; ++a) {
+        fe[a] = 0.0;
+        for (int b = 0; b < 27; ++b) Ke[a][b] = 0.0;
+    }
+    for (int qz = 0; qz < 3; ++qz)
+        for (int qy = 0; qy < 3; ++qy)
+            for (int qx = 0; qx < 3; ++qx) {
+                const double xi[3] = {gx[qx], gx[qy], gx[qz]};
+                const double w = gw[qx] * gw[qy] * gw[qz];
+                double phi1[8], dphi1[8][3];
+                q1_shape(xi, phi1, dphi1);
+                double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int a = 0; a < 8; ++a)
+                    for (int rr = 0; rr < 3; ++rr)
+                        for (int c = 0; c < 3; ++c) J[rr][c] += X[a][rr] * dphi1[a][c];
+                double Ji[3][3];
+                const double det = inv3(J, Ji);
+                const double wd = w * std::fabs(det);
+                double g[27][3], ph[27];
+                for (int a = 0; a < 27; ++a) {
+                    double dref[3];
+                    for (int d = 0; d < 3; ++d) {
+                        dref[d] = dlagrange1d(mi[a][d], xi[d]);
+                        for (int d2 = 0; d2 < 3; ++d2)
+                            if (d2 != d) dref[d] *= lagrange1d(mi[a][d2], xi[d2]);
+                    }
+                    ph[a] = lagrange1d(mi[a][0], xi[0]) * lagrange1d(mi[a][1], xi[1]) * lagrange1d(mi[a][2], xi[2]);
+                    for (int rr = 0; rr < 3; ++rr) g[a][rr] = Ji[0][rr] * dref[0] + Ji[1][rr] * dref[1] + Ji[2][rr] * dref[2];
+                }
+                for (int a = 0; a < 27; ++a) {
+                    for (int b = 0; b < 27; ++b)
+                        Ke[a][b] += wd * k * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
+                    fe[a] += wd * f_const * ph[a];
+                }
+            }
+}
+
 void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,
                       const std::vector<double>& f_at_dof, CSR& A, std::vector<double>& b) {
     const Level& L = mg.levels[lev];

@@ -146,6 +146,10 @@ struct Coefficient {
 // assembled in element creation order and scattered with AddLocalMatrixToGlobal
 // semantics (lib_disc/common/local_algebra.h:766-789); Dirichlet rows via
 // SetDirichletRow (lagrange_dirichlet_boundary_impl.h:753-915).
+// Q2 hexahedron: the reference's local shape order (27 multi-indices) and
+// the element stiffness k * int grad phi_a . grad phi_b and load f * int phi_a
+void q2_hex_multi_index(int mi[27][3]);
+void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27][27], double fe[27]);
 void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double f_const,
                      CSR& A, std::vector<double>& b);
 // FV1 (vertex-centred box method) on P1 triangles, diffusion k=1 (or k(x)),

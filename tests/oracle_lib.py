@@ -69,6 +69,7 @@ def lib():
             "orc_builder_csr": (I64, [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
             "orc_csr_op": (None, [C.c_int, I64, I64, I64P, I32P, F64P, C.c_double, C.c_void_p,
                                   C.c_double, F64P, F64P]),
+            "orc_q2_hex_elements": (None, [I64, P, P, C.c_double, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -305,3 +306,23 @@ def builder_csr(entries, rows, cols, mode=0, add=True):
     L.orc_builder_csr(h, mode, rp.ctypes.data, col.ctypes.data, val.ctypes.data)
     L.orc_builder_destroy(h)
     return CSR(rp, col, val, rows if mode else cols)
+
+
+def q2_hex_elements(X, k=None, f_const=0.0):
+    """Oracle Q2 hexahedron element matrices: X [n, 8, 3] corners in the
+    reference order, k [n] (None: 1); returns (Ke [n, 27, 27], fe [n, 27])"""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n = X.shape[0]
+    kk = None if k is None else np.ascontiguousarray(k, dtype=np.float64)
+    Ke = np.zeros((n, 27, 27))
+    fe = np.zeros((n, 27))
+    lib().orc_q2_hex_elements(n, X.ctypes.data, None if kk is None else kk.ctypes.data, f_const,
+                              Ke.ctypes.data, fe.ctypes.data, None)
+    return Ke, fe
+
+
+def q2_hex_multi_index():
+    """the 27 local multi-indices of the Q2 hexahedron, in the reference's shape order"""
+    mi = np.zeros((27, 3), np.int32)
+    lib().orc_q2_hex_elements(0, None, None, 0.0, None, None, mi.ctypes.data)
+    return mi
