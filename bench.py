@@ -131,6 +131,35 @@ def bench_poisson_2d(U, seed, reps):
             "assembly_s": d.assemble_seconds()}
 
 
+def bench_q2_elements(U, seed, reps, n_cells=64):
+    """Q2 hexahedron element matrices (one element per wavefront, FP64 matrix
+    cores) for the n_cells^3 elements of a jittered lattice, k = exp(theta)
+    per element: elements/s, the HBM rate of the element-matrix output, and
+    the FP64 rate of the contraction (27 x 27 x 81 useful multiply-adds per
+    element; 84 MFMA 16x16x4 steps issued)"""
+    rng = np.random.default_rng(seed)
+    n = n_cells ** 3
+    h = 1.0 / n_cells
+    e = np.arange(n)
+    base = np.stack([e % n_cells, (e // n_cells) % n_cells, e // (n_cells * n_cells)], axis=1) * h
+    corners = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    X = base[:, None, :] + corners[None] * h + rng.uniform(-0.1 * h, 0.1 * h, (n, 8, 3))
+    k = np.exp(rng.normal(scale=0.5, size=n))
+    dx, dk = U.DeviceArray(X), U.DeviceArray(k)
+    ke = U.DeviceArray(np.zeros(n * 729))
+    fe = U.DeviceArray(np.zeros(n * 27))
+    L = U.lib()
+    t = U.time_launches(lambda: L.ugx_q2_hex_element_matrices(n, dx.ptr, dk.ptr, 1.0, ke.ptr, fe.ptr), reps)
+    bytes_ = n * 8 * (24 + 1 + 729 + 27)
+    res = {"elements": n, "ms": t * 1e3, "elements_per_s": n / t, "GBps": bytes_ / t / 1e9,
+           "frac_of_8TBps": bytes_ / t / 1e9 / HBM_PEAK_GBS,
+           "useful_fp64_TFLOPs": n * 2 * 27 * 27 * 81 / t / 1e12,
+           "mfma_issued_fp64_TFLOPs": n * 84 * 16 * 16 * 4 * 2 / t / 1e12}
+    log(f"[bench] Q2 element matrices {n_cells}^3: {t*1e3:.3f} ms, {res['GBps']:.0f} GB/s written+read, "
+        f"{res['useful_fp64_TFLOPs']:.2f} TFLOP/s useful FP64")
+    return res
+
+
 def bench_galerkin(U, disc, A, b, base_level, reps):
     """the headline 256^3 Q1 problem with Galerkin coarse operators
     (set_rap: A_{l-1} = R_l A_l P_l from the assembled top level) instead of
@@ -549,6 +578,8 @@ def main():
         secondary["tet_fv1_upwind_96"] = bench_tet(U, 5, args.seed, max(1, args.steps // 2))
         log(f"[bench] secondary: Galerkin coarse operators at {n}^3")
         secondary[f"galerkin_q1_{n}"] = bench_galerkin(U, disc, A, b, args.base_level, args.steps)
+        log("[bench] secondary: Q2 element matrices")
+        secondary["q2_hex_element_matrices_64"] = bench_q2_elements(U, args.seed, args.kernel_reps)
 
     cpu = None
     if not args.skip_cpu:

@@ -160,6 +160,16 @@ int ugx_disc_set_jitter(void* disc, uint64_t seed, double amp);
  * rows zeroed)^T with identity Dirichlet rows, on A's structurally symmetric
  * pattern; a from ugx_disc_assemble_linear */
 int ugx_disc_adjoint_operator(void* disc, void* a, void* at);
+/* Element matrices of n Q2 (triquadratic Lagrange) hexahedra, the local
+ * stiffness k * int grad phi_a . grad phi_b and load f * int phi_a an IElemDisc
+ * adds per element (add_jac_A_elem / add_rhs_elem, elem_disc_interface.h:
+ * 470-752) in the local shape order of LagrangeLSFS<ReferenceHexahedron, 2>
+ * (lagrange.cpp:236-253, lagrange.h:2106-2190), trilinear geometry, 3x3x3
+ * Gauss points. Device pointers: corners [n][8][3] in the reference
+ * hexahedron order, k [n] (NULL: 1), ke [n][27][27] row-major, fe [n][27]
+ * (NULL: not formed). Enqueued on the library stream. */
+int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* k, double f_const, double* ke,
+                                double* fe);
 /* DomainDiscretization::assemble_linear(A, b) (domain_disc_impl.h:835-948) incl. Dirichlet rows */
 int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
 double ugx_disc_assemble_seconds(void* disc);

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "ugx_disc_set_theta_device": (_I, [_P, _P]),
     "ugx_disc_adjust_defect": (_I, [_P, _P]),
     "ugx_disc_adjoint_gradient": (_I, [_P, _P, _P, _P]),
+    "ugx_q2_hex_element_matrices": (_I, [_I64, _P, _P, _D, _P, _P]),
     "ugx_disc_set_source_const": (_I, [_P, _D]),
     "ugx_disc_set_source_dofs": (_I, [_P, _P]),
     "ugx_disc_assemble_linear": (_I, [_P, _P, _P]),
@@ -650,3 +651,21 @@ def SaveVectorForConnectionViewer(disc, v, filename):
     """a surface vector with the top-level DoF positions
     (SaveVectorForConnectionViewer, grid_function_util.h)"""
     write_connection_viewer_vector(filename, v.numpy(), disc.dof_positions())
+
+
+def q2_hex_element_matrices(corners, k=None, f_const=0.0):
+    """Q2 hexahedron element matrices on the device (ugx_q2_hex_element_matrices):
+    corners [n, 8, 3] in the reference hexahedron order, k [n] or None (1);
+    returns (Ke [n, 27, 27], fe [n, 27]) as numpy arrays"""
+    X = np.ascontiguousarray(corners, dtype=np.float64)
+    n = X.shape[0]
+    if X.shape[1:] != (8, 3):
+        raise ValueError("corners must have shape [n, 8, 3]")
+    dx = DeviceArray(X)
+    dk = DeviceArray(np.ascontiguousarray(k, dtype=np.float64)) if k is not None else None
+    ke = DeviceArray(np.zeros((n, 27, 27)))
+    fe = DeviceArray(np.zeros((n, 27)))
+    _check(lib().ugx_q2_hex_element_matrices(n, dx.ptr, dk.ptr if dk is not None else None, float(f_const),
+                                             ke.ptr, fe.ptr), "q2_hex_element_matrices")
+    synchronize()
+    return ke.numpy(), fe.numpy()

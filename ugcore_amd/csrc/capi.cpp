@@ -373,6 +373,16 @@ int ugx_disc_set_jitter(void* disc, uint64_t seed, double amp) {
         return 0;
     });
 }
+int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* k, double f_const, double* ke,
+                                double* fe) {
+    return guard([&] {
+        if (n < 0) throw UGError("ugx_q2_hex_element_matrices: negative element count");
+        if (n > 0 && (!corners || !ke)) throw UGError("ugx_q2_hex_element_matrices: null corners or output");
+        if (ugamd::q2_hex_elements(n, corners, k, f_const, ke, fe, context().stream) != 0)
+            throw UGError("ugx_q2_hex_element_matrices: launch failed");
+        return 0;
+    });
+}
 int ugx_disc_adjoint_operator(void* disc, void* a, void* at) {
     return guard([&] {
         ((DiscH*)disc)->d->assemble_adjoint_operator(*((MatrixH*)a)->A, *((MatrixH*)at)->A);

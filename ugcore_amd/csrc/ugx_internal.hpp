@@ -163,6 +163,12 @@ int adjoint_grad_q1_hex(int n, double h, const double* theta, const int32_t* dof
                         const double* u, const double* lambda, double* grad, hipStream_t s);
 int zero_range(int64_t lo, int64_t hi, double* x, hipStream_t s);
 
+// ---- Q2 hexahedron element matrices (q2_kernels.hip) ----
+// X [n][8][3] corners in the reference order, k [n] (or null: 1), Ke
+// [n][27][27] (row-major), fe [n][27] (or null); device pointers
+int q2_hex_elements(int64_t n, const double* X, const double* k, double f_const, double* Ke, double* fe,
+                    hipStream_t s);
+
 // ---- tetrahedral hierarchy (tet_kernels.hip) ----
 struct TetDisc {
     int n = 0;                    // cells per side of the level
