@@ -6,9 +6,10 @@
 // and fe[a] = sum_q w_q |det J_q| f phi_a over the 3x3x3 Gauss-Legendre
 // points, with the trilinear geometry of the 8 corners. Written as
 // Ke = B^T C B with B (81 x 27) the physical gradients, one row per point and
-// component, and C = diag(w_q |det J_q| k): the wave stages B (padded to
-// 84 x 32) in LDS and forms the four 16 x 16 tiles of Ke with
-// v_mfma_f64_16x16x4f64, 21 steps of 4 rows each.
+// component, and C = diag(w_q |det J_q| k): each lane forms the two entries
+// of B (padded to 84 x 32) it feeds to each v_mfma_f64_16x16x4f64 step in
+// registers, and the wave accumulates three 16 x 16 tiles of the symmetric Ke
+// over 21 steps of 4 rows (only the Jacobians sit in LDS).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,7 +30,6 @@ __constant__ int8_t kQ2Mi[27][3] = {
     {1, 2, 2}, {0, 1, 2}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
 
 constexpr int kQ2Rows = 84;   // 27 points x 3 components, padded to a multiple of 4
-constexpr int kQ2Ld = 33;     // LDS row stride of B (32 columns + 1)
 
 // 1D Lagrange polynomials of the nodes 0, 1/2, 1 and their derivatives
 __device__ __forceinline__ double q2_l(int i, double x) {
@@ -46,21 +46,46 @@ __device__ __forceinline__ double q2_gauss_w(int i) { return i == 1 ? 8.0 / 18.0
 
 typedef double q2d4 __attribute__((ext_vector_type(4)));
 
+// grad_xi phi_a at Gauss point q (q = qx + 3 qy + 9 qz), evaluated at compile
+// time from the same 1D polynomials and local order
+struct Q2RefGrad {
+    double g[27][27][3];
+};
+constexpr int8_t kQ2MiHost[27][3] = {
+    {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2}, {1, 0, 0},
+    {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 2},
+    {1, 2, 2}, {0, 1, 2}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
+constexpr double q2c_x(int i) { return i == 0 ? 0.11270166537925831 : i == 1 ? 0.5 : 0.8872983346207417; }
+constexpr double q2c_l(int i, double x) {
+    return i == 0 ? (1.0 - x) * (1.0 - 2.0 * x) : i == 1 ? 4.0 * x * (1.0 - x) : x * (2.0 * x - 1.0);
+}
+constexpr double q2c_dl(int i, double x) { return i == 0 ? 4.0 * x - 3.0 : i == 1 ? 4.0 - 8.0 * x : 4.0 * x - 1.0; }
+constexpr Q2RefGrad q2_ref_grad() {
+    Q2RefGrad t{};
+    for (int q = 0; q < 27; ++q)
+        for (int a = 0; a < 27; ++a) {
+            const double x0 = q2c_x(q % 3), x1 = q2c_x((q / 3) % 3), x2 = q2c_x(q / 9);
+            const int m0 = kQ2MiHost[a][0], m1 = kQ2MiHost[a][1], m2 = kQ2MiHost[a][2];
+            const double l0 = q2c_l(m0, x0), l1 = q2c_l(m1, x1), l2 = q2c_l(m2, x2);
+            t.g[q][a][0] = q2c_dl(m0, x0) * l1 * l2;
+            t.g[q][a][1] = l0 * q2c_dl(m1, x1) * l2;
+            t.g[q][a][2] = l0 * l1 * q2c_dl(m2, x2);
+        }
+    return t;
+}
+__constant__ Q2RefGrad kQ2Grad = q2_ref_grad();
+
 __global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double* __restrict__ X,
                                                         const double* __restrict__ kcoef, double f_const,
                                                         double* __restrict__ Ke, double* __restrict__ fe) {
     const int64_t e = blockIdx.x;
     if (e >= n) return;
-    __shared__ double Bs[kQ2Rows * kQ2Ld];
     __shared__ double Xs[24];
     __shared__ double Jis[27][9];
-    __shared__ double cs[27], fw[27];
+    __shared__ double cs[28], fw[27];
     const int l = threadIdx.x;
     if (l < 24) Xs[l] = X[e * 24 + l];
-    for (int t = l; t < kQ2Rows * kQ2Ld; t += 64) {
-        const int r = t / kQ2Ld, c = t - r * kQ2Ld;
-        if (r >= 81 || c >= 27) Bs[t] = 0.0;
-    }
+    if (l == 27) cs[27] = 0.0;   // the padding rows 81..83 of B
     __syncthreads();
     // 1. Jacobian of the trilinear map at point q = lane (x fastest)
     if (l < 27) {
@@ -94,17 +119,6 @@ __global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double*
         fw[l] = wd * f_const;
     }
     __syncthreads();
-    // 2. B[3q + r][a] = (J_q^-T grad_xi phi_a(xi_q))_r for the 729 (q, a) pairs
-    for (int t = l; t < 729; t += 64) {
-        const int q = t / 27, a = t - q * 27;
-        const double xi[3] = {q2_gauss(q % 3), q2_gauss((q / 3) % 3), q2_gauss(q / 9)};
-        const int m0 = kQ2Mi[a][0], m1 = kQ2Mi[a][1], m2 = kQ2Mi[a][2];
-        const double l0 = q2_l(m0, xi[0]), l1 = q2_l(m1, xi[1]), l2 = q2_l(m2, xi[2]);
-        const double d0 = q2_dl(m0, xi[0]) * l1 * l2, d1 = l0 * q2_dl(m1, xi[1]) * l2, d2 = l0 * l1 * q2_dl(m2, xi[2]);
-        const double* ji = Jis[q];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) Bs[(3 * q + r) * kQ2Ld + a] = ji[r] * d0 + ji[3 + r] * d1 + ji[6 + r] * d2;
-    }
     // load vector: fe[a] = sum_q w_q |det J_q| f phi_a(xi_q)
     if (fe && l < 27) {
         const int m0 = kQ2Mi[l][0], m1 = kQ2Mi[l][1], m2 = kQ2Mi[l][2];
@@ -113,31 +127,45 @@ __global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double*
             s += fw[q] * q2_l(m0, q2_gauss(q % 3)) * q2_l(m1, q2_gauss((q / 3) % 3)) * q2_l(m2, q2_gauss(q / 9));
         fe[e * 27 + l] = s;
     }
-    __syncthreads();
-    // 3. Ke = (C B)^T B in 16 x 16 tiles: lane l supplies A[l % 16][l / 16] and
-    //    B[l / 16][l % 16] of each 16x4 / 4x16 step; accumulator element v of
-    //    lane l is row l / 16 + 4 v, column l % 16 of the tile (the f64 layout,
-    //    measured on gfx950)
+    // 2. Ke = (C B)^T B in the 16 x 16 tiles (0,0), (0,1), (1,1); (1,0) is the
+    //    transpose of (0,1). Step kk covers the rows k = 4 kk .. 4 kk + 3 of B
+    //    (point k / 3, component k % 3; rows 81..83 are zero): lane
+    //    l = 16 lk + li forms B[4 kk + lk][a] = (J_q^-T grad_xi phi_a(xi_q))_(k % 3)
+    //    for the shapes a = li and 16 + li (zero beyond 26) and supplies them
+    //    as A[li][lk] (scaled by C) and B[lk][li] of the 16x4 / 4x16 step.
+    //    Accumulator element v of lane l is row l / 16 + 4 v, column l % 16 of
+    //    the tile (the f64 layout, measured on gfx950).
     const int li = l & 15, lk = l >> 4;
+    const int a1 = 16 + li < 27 ? 16 + li : 26;
+    const bool has1 = 16 + li < 27;
+    q2d4 acc00 = {0.0, 0.0, 0.0, 0.0}, acc01 = acc00, acc11 = acc00;
+    for (int kk = 0; kk < kQ2Rows / 4; ++kk) {
+        const int k = 4 * kk + lk;
+        const int q = k / 3, r = k - 3 * q;
+        double v0 = 0.0, v1 = 0.0, c = 0.0;
+        if (k < 81) {
+            const double j0 = Jis[q][r], j1 = Jis[q][3 + r], j2 = Jis[q][6 + r];
+            const double* g0 = kQ2Grad.g[q][li];
+            const double* g1 = kQ2Grad.g[q][a1];
+            c = cs[q];
+            v0 = j0 * g0[0] + j1 * g0[1] + j2 * g0[2];
+            if (has1) v1 = j0 * g1[0] + j1 * g1[1] + j2 * g1[2];
+        }
+        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v0, v0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v0, v1, acc01, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v1, v1, acc11, 0, 0, 0);
+    }
     double* out = Ke + e * 729;
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj) {
-            q2d4 acc = {0.0, 0.0, 0.0, 0.0};
-            for (int kk = 0; kk < kQ2Rows / 4; ++kk) {
-                const int k = 4 * kk + lk;
-                const double c = k < 81 ? cs[k / 3] : 0.0;
-                const double av = c * Bs[k * kQ2Ld + 16 * ti + li];
-                const double bv = Bs[k * kQ2Ld + 16 * tj + li];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = 16 * ti + lk + 4 * v, j = 16 * tj + li;
-                if (i < 27 && j < 27) out[i * 27 + j] = acc[v];
-            }
+    for (int v = 0; v < 4; ++v) {
+        const int i = lk + 4 * v;
+        out[i * 27 + li] = acc00[v];
+        if (has1) {
+            out[i * 27 + 16 + li] = acc01[v];
+            out[(16 + li) * 27 + i] = acc01[v];
         }
+        if (16 + i < 27 && has1) out[(16 + i) * 27 + 16 + li] = acc11[v];
+    }
 }
 
 }  // namespace
