@@ -19,25 +19,8 @@
 namespace ugamd {
 namespace {
 
-// Local shape order of LagrangeLSFS<ReferenceHexahedron, 2> (lagrange.cpp:
-// 236-253, lagrange.h:2106-2190): multi-indices of the vertices (2 x corner),
-// the edge midpoints of edges (0,1) (1,2) (2,3) (3,0) (0,4) (1,5) (2,6) (3,7)
-// (4,5) (5,6) (6,7) (7,4), the centres of faces {0,3,2,1} {0,1,5,4} {1,2,6,5}
-// {2,3,7,6} {3,0,4,7} {4,5,6,7} (reference_element.cpp), then the interior.
-__constant__ int8_t kQ2Mi[27][3] = {
-    {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2}, {1, 0, 0},
-    {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 2},
-    {1, 2, 2}, {0, 1, 2}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
-
 constexpr int kQ2Rows = 84;   // 27 points x 3 components, padded to a multiple of 4
 
-// 1D Lagrange polynomials of the nodes 0, 1/2, 1 and their derivatives
-__device__ __forceinline__ double q2_l(int i, double x) {
-    return i == 0 ? (1.0 - x) * (1.0 - 2.0 * x) : i == 1 ? 4.0 * x * (1.0 - x) : x * (2.0 * x - 1.0);
-}
-__device__ __forceinline__ double q2_dl(int i, double x) {
-    return i == 0 ? 4.0 * x - 3.0 : i == 1 ? 4.0 - 8.0 * x : 4.0 * x - 1.0;
-}
 __device__ __forceinline__ double q2_gauss(int i) {
     // 3-point Gauss-Legendre on [0, 1]: 1/2 -+ sqrt(3/5)/2
     return i == 0 ? 0.11270166537925831 : i == 1 ? 0.5 : 0.8872983346207417;
@@ -46,12 +29,19 @@ __device__ __forceinline__ double q2_gauss_w(int i) { return i == 1 ? 8.0 / 18.0
 
 typedef double q2d4 __attribute__((ext_vector_type(4)));
 
-// grad_xi phi_a at Gauss point q (q = qx + 3 qy + 9 qz), evaluated at compile
-// time from the same 1D polynomials and local order
+// grad_xi phi_a and phi_a at Gauss point q (q = qx + 3 qy + 9 qz), evaluated
+// at compile time from the 1D Lagrange polynomials of the nodes 0, 1/2, 1 and
+// the local order
 struct Q2RefGrad {
     double g[27][27][3];
+    double phi[27][27];   // phi_a at point q
 };
-constexpr int8_t kQ2MiHost[27][3] = {
+// Local shape order of LagrangeLSFS<ReferenceHexahedron, 2> (lagrange.cpp:
+// 236-253, lagrange.h:2106-2190): multi-indices of the vertices (2 x corner),
+// the edge midpoints of edges (0,1) (1,2) (2,3) (3,0) (0,4) (1,5) (2,6) (3,7)
+// (4,5) (5,6) (6,7) (7,4), the centres of faces {0,3,2,1} {0,1,5,4} {1,2,6,5}
+// {2,3,7,6} {3,0,4,7} {4,5,6,7} (reference_element.cpp), then the interior.
+constexpr int8_t kQ2Mi[27][3] = {
     {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2}, {1, 0, 0},
     {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 2},
     {1, 2, 2}, {0, 1, 2}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
@@ -65,26 +55,40 @@ constexpr Q2RefGrad q2_ref_grad() {
     for (int q = 0; q < 27; ++q)
         for (int a = 0; a < 27; ++a) {
             const double x0 = q2c_x(q % 3), x1 = q2c_x((q / 3) % 3), x2 = q2c_x(q / 9);
-            const int m0 = kQ2MiHost[a][0], m1 = kQ2MiHost[a][1], m2 = kQ2MiHost[a][2];
+            const int m0 = kQ2Mi[a][0], m1 = kQ2Mi[a][1], m2 = kQ2Mi[a][2];
             const double l0 = q2c_l(m0, x0), l1 = q2c_l(m1, x1), l2 = q2c_l(m2, x2);
             t.g[q][a][0] = q2c_dl(m0, x0) * l1 * l2;
             t.g[q][a][1] = l0 * q2c_dl(m1, x1) * l2;
             t.g[q][a][2] = l0 * l1 * q2c_dl(m2, x2);
+            t.phi[q][a] = l0 * l1 * l2;
         }
     return t;
 }
 __constant__ Q2RefGrad kQ2Grad = q2_ref_grad();
 
-__global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double* __restrict__ X,
-                                                        const double* __restrict__ kcoef, double f_const,
-                                                        double* __restrict__ Ke, double* __restrict__ fe) {
-    const int64_t e = blockIdx.x;
-    if (e >= n) return;
-    __shared__ double Xs[24];
-    __shared__ double Jis[27][9];
-    __shared__ double cs[28], fw[27];
-    const int l = threadIdx.x;
-    if (l < 24) Xs[l] = X[e * 24 + l];
+constexpr int kQ2Waves = 4;   // elements (wavefronts) per workgroup
+
+__global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, const double* __restrict__ X,
+                                                                   const double* __restrict__ kcoef, double f_const,
+                                                                   double* __restrict__ Ke, double* __restrict__ fe) {
+    const int w = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * kQ2Waves + w;
+    const bool valid = e < n;   // the last workgroup may hold fewer elements; every wave reaches the barriers
+    __shared__ double Xs_all[kQ2Waves][24];
+    __shared__ double Jis_all[kQ2Waves][27][9];
+    __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    double* Xs = Xs_all[w];
+    double(*Jis)[9] = Jis_all[w];
+    double* cs = cs_all[w];
+    double* fw = fw_all[w];
+    const int l = threadIdx.x & 63;
+    if (l < 24) {
+        // a wave without an element works on the unit cube and writes nothing
+        const int c = l / 3, d = l - 3 * c;
+        const int bit = d == 0 ? (c == 1 || c == 2 || c == 5 || c == 6) : d == 1 ? (c == 2 || c == 3 || c == 6 || c == 7)
+                                                                         : (c >= 4);
+        Xs[l] = valid ? X[e * 24 + l] : (double)bit;
+    }
     if (l == 27) cs[27] = 0.0;   // the padding rows 81..83 of B
     __syncthreads();
     // 1. Jacobian of the trilinear map at point q = lane (x fastest)
@@ -120,11 +124,9 @@ __global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double*
     }
     __syncthreads();
     // load vector: fe[a] = sum_q w_q |det J_q| f phi_a(xi_q)
-    if (fe && l < 27) {
-        const int m0 = kQ2Mi[l][0], m1 = kQ2Mi[l][1], m2 = kQ2Mi[l][2];
+    if (fe && valid && l < 27) {
         double s = 0.0;
-        for (int q = 0; q < 27; ++q)
-            s += fw[q] * q2_l(m0, q2_gauss(q % 3)) * q2_l(m1, q2_gauss((q / 3) % 3)) * q2_l(m2, q2_gauss(q / 9));
+        for (int q = 0; q < 27; ++q) s += fw[q] * kQ2Grad.phi[q][l];
         fe[e * 27 + l] = s;
     }
     // 2. Ke = (C B)^T B in the 16 x 16 tiles (0,0), (0,1), (1,1); (1,0) is the
@@ -155,6 +157,7 @@ __global__ __launch_bounds__(64) void k_q2_hex_elements(int64_t n, const double*
         acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v0, v1, acc01, 0, 0, 0);
         acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v1, v1, acc11, 0, 0, 0);
     }
+    if (!valid) return;
     double* out = Ke + e * 729;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -174,7 +177,8 @@ int q2_hex_elements(int64_t n, const double* X, const double* k, double f_const,
                     hipStream_t s) {
     if (n <= 0) return 0;
     if (!X || !Ke || n > (int64_t)INT32_MAX) return -1;
-    hipLaunchKernelGGL(k_q2_hex_elements, dim3((unsigned)n), dim3(64), 0, s, n, X, k, f_const, Ke, fe);
+    hipLaunchKernelGGL(k_q2_hex_elements, dim3((unsigned)((n + kQ2Waves - 1) / kQ2Waves)), dim3(64 * kQ2Waves), 0, s,
+                       n, X, k, f_const, Ke, fe);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
