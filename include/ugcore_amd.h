@@ -170,6 +170,16 @@ int ugx_disc_adjoint_operator(void* disc, void* a, void* at);
  * (NULL: not formed). Enqueued on the library stream. */
 int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* k, double f_const, double* ke,
                                 double* fe);
+/* assemble_linear (domain_disc_impl.h:835-948) of -div(k grad u) = f with
+ * Q2 Lagrange elements on the unit cube split into n^3 hexahedra, u = 0 on the
+ * boundary (identity rows on the kept pattern, SetDirichletRow,
+ * sparsematrix_util.h:850-861): k = exp(theta[e]) per element (theta_dev
+ * device pointer, x fastest, or NULL: 1), f constant. Nodes are the
+ * (2n+1)^3 lattice points numbered x fastest (not the reference's DoF order,
+ * which lib_grid fixes); element matrices as ugx_q2_hex_element_matrices,
+ * scattered colour by colour without atomics into a pattern built on the
+ * device. Fills the matrix and vector handles. */
+int ugx_q2_assemble_lattice(int n, const double* theta_dev, double f_const, void* matrix, void* rhs);
 /* DomainDiscretization::assemble_linear(A, b) (domain_disc_impl.h:835-948) incl. Dirichlet rows */
 int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
 double ugx_disc_assemble_seconds(void* disc);

@@ -81,6 +81,7 @@ _SIGNATURES = {
     "ugx_disc_adjust_defect": (_I, [_P, _P]),
     "ugx_disc_adjoint_gradient": (_I, [_P, _P, _P, _P]),
     "ugx_q2_hex_element_matrices": (_I, [_I64, _P, _P, _D, _P, _P]),
+    "ugx_q2_assemble_lattice": (_I, [_I, _P, _D, _P, _P]),
     "ugx_disc_set_source_const": (_I, [_P, _D]),
     "ugx_disc_set_source_dofs": (_I, [_P, _P]),
     "ugx_disc_assemble_linear": (_I, [_P, _P, _P]),
@@ -669,3 +670,16 @@ def q2_hex_element_matrices(corners, k=None, f_const=0.0):
                                              ke.ptr, fe.ptr), "q2_hex_element_matrices")
     synchronize()
     return ke.numpy(), fe.numpy()
+
+
+def q2_assemble_lattice(n, theta=None, f_const=1.0):
+    """Q2 assembly of -div(exp(theta) grad u) = f on the unit cube of n^3
+    hexahedra with u = 0 on the boundary (ugx_q2_assemble_lattice); returns
+    (SparseMatrix, Vector) on the device, nodes of the (2n+1)^3 lattice
+    numbered x fastest"""
+    A = SparseMatrix()
+    b = Vector(1)
+    th = DeviceArray(np.ascontiguousarray(theta, dtype=np.float64)) if theta is not None else None
+    _check(lib().ugx_q2_assemble_lattice(int(n), th.ptr if th is not None else None, float(f_const), A.h, b.h),
+           "q2_assemble_lattice")
+    return A, b

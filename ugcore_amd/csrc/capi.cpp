@@ -383,6 +383,40 @@ int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* 
         return 0;
     });
 }
+int ugx_q2_assemble_lattice(int n, const double* theta, double f_const, void* matrix, void* rhs) {
+    return guard([&] {
+        if (n < 1 || n > 600) throw UGError("ugx_q2_assemble_lattice: n must be in 1..600");
+        if (!matrix || !rhs) throw UGError("ugx_q2_assemble_lattice: null matrix or vector");
+        auto& c = context();
+        const int W = 2 * n + 1;
+        const int64_t N = (int64_t)W * W * W;
+        DeviceBuffer<int32_t> cnt(N);
+        if (ugamd::q2_lattice_count(n, cnt.get(), c.stream) != 0) throw UGError("ugx_q2_assemble_lattice: launch failed");
+        std::vector<int32_t> h(N + 1, 0);
+        cnt.download(h.data() + 1, N);
+        int64_t acc = 0;
+        for (int64_t i = 1; i <= N; ++i) {
+            acc += h[i];
+            if (acc >= ((int64_t)1 << 31)) throw UGError("ugx_q2_assemble_lattice: nnz exceeds int32 range");
+            h[i] = (int32_t)acc;
+        }
+        GPUSparseMatrix& A = *((MatrixH*)matrix)->A;
+        GPUVector& b = ((VectorH*)rhs)->v;
+        A.allocate(N, N, acc);
+        if (hipMemcpyAsync(A.rowptr(), h.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, c.stream) !=
+                hipSuccess ||
+            hipMemsetAsync(A.val(), 0, sizeof(double) * acc, c.stream) != hipSuccess)
+            throw UGError("ugx_q2_assemble_lattice: copy failed");
+        b.resize(N);
+        b.set(0.0);
+        if (ugamd::q2_lattice_fill(n, A.rowptr(), A.col(), c.stream) != 0 ||
+            ugamd::q2_lattice_assemble(n, theta, f_const, A.rowptr(), A.col(), A.val(), b.data(), c.stream) != 0)
+            throw UGError("ugx_q2_assemble_lattice: launch failed");
+        if (hipStreamSynchronize(c.stream) != hipSuccess) throw UGError("ugx_q2_assemble_lattice: stream failed");
+        A.finalize();
+        return 0;
+    });
+}
 int ugx_disc_adjoint_operator(void* disc, void* a, void* at) {
     return guard([&] {
         ((DiscH*)disc)->d->assemble_adjoint_operator(*((MatrixH*)a)->A, *((MatrixH*)at)->A);

@@ -12,6 +12,7 @@
 // over 21 steps of 4 rows (only the Jacobians sit in LDS).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "ugx_internal.hpp"
@@ -68,26 +69,19 @@ __constant__ Q2RefGrad kQ2Grad = q2_ref_grad();
 
 constexpr int kQ2Waves = 4;   // elements (wavefronts) per workgroup
 
-__global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, const double* __restrict__ X,
-                                                                   const double* __restrict__ kcoef, double f_const,
-                                                                   double* __restrict__ Ke, double* __restrict__ fe) {
-    const int w = threadIdx.x >> 6;
-    const int64_t e = (int64_t)blockIdx.x * kQ2Waves + w;
-    const bool valid = e < n;   // the last workgroup may hold fewer elements; every wave reaches the barriers
-    __shared__ double Xs_all[kQ2Waves][24];
-    __shared__ double Jis_all[kQ2Waves][27][9];
-    __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
-    double* Xs = Xs_all[w];
-    double(*Jis)[9] = Jis_all[w];
-    double* cs = cs_all[w];
-    double* fw = fw_all[w];
-    const int l = threadIdx.x & 63;
+// The element computation of one wavefront (lane l): corners from src.corner(l)
+// for l < 24 (valid elements) and src.k(), then sink.ke(i, j, v) for the 729
+// entries and sink.fe(a, v) for the 27 load entries. Every wave of the
+// workgroup calls it once (it holds two barriers); invalid waves work on the
+// unit cube and emit nothing.
+template <class Src, class Sink>
+__device__ __forceinline__ void q2_element_wave(int l, bool valid, const Src& src, Sink& sink, double f_const,
+                                                double* Xs, double (*Jis)[9], double* cs, double* fw, bool want_fe) {
     if (l < 24) {
-        // a wave without an element works on the unit cube and writes nothing
         const int c = l / 3, d = l - 3 * c;
         const int bit = d == 0 ? (c == 1 || c == 2 || c == 5 || c == 6) : d == 1 ? (c == 2 || c == 3 || c == 6 || c == 7)
                                                                          : (c >= 4);
-        Xs[l] = valid ? X[e * 24 + l] : (double)bit;
+        Xs[l] = valid ? src.corner(l) : (double)bit;
     }
     if (l == 27) cs[27] = 0.0;   // the padding rows 81..83 of B
     __syncthreads();
@@ -119,15 +113,15 @@ __global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, co
         ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
         ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
         const double wd = q2_gauss_w(qx) * q2_gauss_w(qy) * q2_gauss_w(qz) * fabs(det);
-        cs[l] = wd * (kcoef ? kcoef[e] : 1.0);
+        cs[l] = wd * (valid ? src.k() : 1.0);
         fw[l] = wd * f_const;
     }
     __syncthreads();
     // load vector: fe[a] = sum_q w_q |det J_q| f phi_a(xi_q)
-    if (fe && valid && l < 27) {
+    if (want_fe && valid && l < 27) {
         double s = 0.0;
         for (int q = 0; q < 27; ++q) s += fw[q] * kQ2Grad.phi[q][l];
-        fe[e * 27 + l] = s;
+        sink.fe(l, s);
     }
     // 2. Ke = (C B)^T B in the 16 x 16 tiles (0,0), (0,1), (1,1); (1,0) is the
     //    transpose of (0,1). Step kk covers the rows k = 4 kk .. 4 kk + 3 of B
@@ -158,16 +152,151 @@ __global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, co
         acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(c * v1, v1, acc11, 0, 0, 0);
     }
     if (!valid) return;
-    double* out = Ke + e * 729;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = lk + 4 * v;
-        out[i * 27 + li] = acc00[v];
+        sink.ke(i, li, acc00[v]);
         if (has1) {
-            out[i * 27 + 16 + li] = acc01[v];
-            out[(16 + li) * 27 + i] = acc01[v];
+            sink.ke(i, 16 + li, acc01[v]);
+            sink.ke(16 + li, i, acc01[v]);
         }
-        if (16 + i < 27 && has1) out[(16 + i) * 27 + 16 + li] = acc11[v];
+        if (16 + i < 27 && has1) sink.ke(16 + i, 16 + li, acc11[v]);
+    }
+}
+
+struct Q2ArraySrc {
+    const double* X;
+    const double* kc;
+    int64_t e;
+    __device__ double corner(int l) const { return X[e * 24 + l]; }
+    __device__ double k() const { return kc ? kc[e] : 1.0; }
+};
+struct Q2ArraySink {
+    double* Kout;
+    double* fout;
+    __device__ void ke(int i, int j, double v) { Kout[i * 27 + j] = v; }
+    __device__ void fe(int a, double v) { fout[a] = v; }
+};
+
+__global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, const double* __restrict__ X,
+                                                                   const double* __restrict__ kcoef, double f_const,
+                                                                   double* __restrict__ Ke, double* __restrict__ fe) {
+    const int w = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * kQ2Waves + w;
+    const bool valid = e < n;   // the last workgroup may hold fewer elements; every wave reaches the barriers
+    __shared__ double Xs_all[kQ2Waves][24];
+    __shared__ double Jis_all[kQ2Waves][27][9];
+    __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    Q2ArraySrc src{X, kcoef, e};
+    Q2ArraySink sink{Ke + (valid ? e : 0) * 729, fe ? fe + (valid ? e : 0) * 27 : nullptr};
+    q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
+                    fe != nullptr);
+}
+
+// ---- assembly on the unit-cube lattice of n^3 elements ----
+// Q2 nodes on the (2n+1)^3 lattice, numbered x fastest; element (ex, ey, ez)
+// holds the nodes 2 e + m for its local multi-indices m. Two nodes are coupled
+// when they share an element, which factorises per direction: an even
+// coordinate c (a lattice plane of vertices) reaches c-2 .. c+2, an odd one
+// (inside one element) c-1 .. c+1.
+__device__ __forceinline__ void q2_range(int c, int W, int& lo, int& hi) {
+    const int r = (c & 1) ? 1 : 2;
+    lo = c - r < 0 ? 0 : c - r;
+    hi = c + r > W - 1 ? W - 1 : c + r;
+}
+
+__global__ void k_q2_lattice_count(int W, int64_t N, int32_t* cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)((i / W) % W), z = (int)(i / ((int64_t)W * W));
+        int a, b, c, d, e, f;
+        q2_range(x, W, a, b);
+        q2_range(y, W, c, d);
+        q2_range(z, W, e, f);
+        cnt[i] = (b - a + 1) * (d - c + 1) * (f - e + 1);
+    }
+}
+
+__global__ void k_q2_lattice_fill(int W, int64_t N, const int32_t* __restrict__ rowptr, int32_t* __restrict__ col) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)((i / W) % W), z = (int)(i / ((int64_t)W * W));
+        int x0, x1, y0, y1, z0, z1;
+        q2_range(x, W, x0, x1);
+        q2_range(y, W, y0, y1);
+        q2_range(z, W, z0, z1);
+        int32_t p = rowptr[i];
+        for (int zz = z0; zz <= z1; ++zz)
+            for (int yy = y0; yy <= y1; ++yy)
+                for (int xx = x0; xx <= x1; ++xx) col[p++] = (int32_t)(((int64_t)zz * W + yy) * W + xx);
+    }
+}
+
+__device__ __forceinline__ int32_t q2_slot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                           int32_t r, int32_t c) {
+    int32_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;   // the pattern contains (r, c) by construction
+}
+
+struct Q2LatticeSrc {
+    int n;
+    double h;
+    int ex, ey, ez;
+    const double* theta;
+    __device__ double corner(int l) const {
+        const int c = l / 3, d = l - 3 * c;
+        const int bit = d == 0 ? (c == 1 || c == 2 || c == 5 || c == 6) : d == 1 ? (c == 2 || c == 3 || c == 6 || c == 7)
+                                                                         : (c >= 4);
+        const int e = d == 0 ? ex : d == 1 ? ey : ez;
+        return (double)(e + bit) * h;
+    }
+    __device__ double k() const { return theta ? exp(theta[((int64_t)ez * n + ey) * n + ex]) : 1.0; }
+};
+struct Q2CsrSink {
+    int W, ex, ey, ez;
+    const int32_t* rowptr;
+    const int32_t* col;
+    double* val;
+    double* rhs;
+    __device__ int32_t node(int a) const {
+        return (int32_t)(((int64_t)(2 * ez + kQ2Mi[a][2]) * W + (2 * ey + kQ2Mi[a][1])) * W + (2 * ex + kQ2Mi[a][0]));
+    }
+    __device__ void ke(int i, int j, double v) { val[q2_slot(rowptr, col, node(i), node(j))] += v; }
+    __device__ void fe(int a, double v) { rhs[node(a)] += v; }
+};
+
+// one colour (element parities px, py, pz): no two of its elements share a node
+__global__ __launch_bounds__(64 * kQ2Waves) void k_q2_lattice_assemble(int n, int color, const double* theta,
+                                                                       double f_const, const int32_t* rowptr,
+                                                                       const int32_t* col, double* val, double* rhs) {
+    const int hn = (n + 1) / 2;
+    const int w = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * kQ2Waves + w;
+    const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
+    const int tx = (int)(t % hn), ty = (int)((t / hn) % hn), tz = (int)(t / ((int64_t)hn * hn));
+    const int ex = 2 * tx + px, ey = 2 * ty + py, ez = 2 * tz + pz;
+    const bool valid = t < (int64_t)hn * hn * hn && ex < n && ey < n && ez < n;
+    __shared__ double Xs_all[kQ2Waves][24];
+    __shared__ double Jis_all[kQ2Waves][27][9];
+    __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    Q2LatticeSrc src{n, 1.0 / n, ex, ey, ez, theta};
+    Q2CsrSink sink{2 * n + 1, ex, ey, ez, rowptr, col, val, rhs};
+    q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
+                    rhs != nullptr);
+}
+
+// Dirichlet rows (u = 0 on the boundary of the cube): row := unit row on its
+// pattern, rhs := 0
+__global__ void k_q2_lattice_dirichlet(int W, int64_t N, const int32_t* __restrict__ rowptr,
+                                       const int32_t* __restrict__ col, double* val, double* rhs) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)((i / W) % W), z = (int)(i / ((int64_t)W * W));
+        if (x > 0 && y > 0 && z > 0 && x < W - 1 && y < W - 1 && z < W - 1) continue;
+        for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) val[p] = col[p] == (int32_t)i ? 1.0 : 0.0;
+        rhs[i] = 0.0;
     }
 }
 
@@ -179,6 +308,34 @@ int q2_hex_elements(int64_t n, const double* X, const double* k, double f_const,
     if (!X || !Ke || n > (int64_t)INT32_MAX) return -1;
     hipLaunchKernelGGL(k_q2_hex_elements, dim3((unsigned)((n + kQ2Waves - 1) / kQ2Waves)), dim3(64 * kQ2Waves), 0, s,
                        n, X, k, f_const, Ke, fe);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int q2_lattice_count(int n, int32_t* cnt, hipStream_t s) {
+    const int W = 2 * n + 1;
+    const int64_t N = (int64_t)W * W * W;
+    hipLaunchKernelGGL(k_q2_lattice_count, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 65536)), dim3(256), 0, s,
+                       W, N, cnt);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int q2_lattice_fill(int n, const int32_t* rowptr, int32_t* col, hipStream_t s) {
+    const int W = 2 * n + 1;
+    const int64_t N = (int64_t)W * W * W;
+    hipLaunchKernelGGL(k_q2_lattice_fill, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 65536)), dim3(256), 0, s,
+                       W, N, rowptr, col);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int q2_lattice_assemble(int n, const double* theta, double f_const, const int32_t* rowptr, const int32_t* col,
+                        double* val, double* rhs, hipStream_t s) {
+    const int hn = (n + 1) / 2;
+    const int64_t per = (int64_t)hn * hn * hn;
+    for (int color = 0; color < 8; ++color)
+        hipLaunchKernelGGL(k_q2_lattice_assemble, dim3((unsigned)((per + kQ2Waves - 1) / kQ2Waves)),
+                           dim3(64 * kQ2Waves), 0, s, n, color, theta, f_const, rowptr, col, val, rhs);
+    const int W = 2 * n + 1;
+    const int64_t N = (int64_t)W * W * W;
+    hipLaunchKernelGGL(k_q2_lattice_dirichlet, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 65536)), dim3(256),
+                       0, s, W, N, rowptr, col, val, rhs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

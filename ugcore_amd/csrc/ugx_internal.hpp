@@ -168,6 +168,13 @@ int zero_range(int64_t lo, int64_t hi, double* x, hipStream_t s);
 // [n][27][27] (row-major), fe [n][27] (or null); device pointers
 int q2_hex_elements(int64_t n, const double* X, const double* k, double f_const, double* Ke, double* fe,
                     hipStream_t s);
+// Q2 on the unit-cube lattice of n^3 elements, nodes on the (2n+1)^3 lattice
+// (x fastest): row lengths, sorted columns, colour-ordered element scatter
+// (k = exp(theta) per element, theta null: 1; f constant) and Dirichlet rows
+int q2_lattice_count(int n, int32_t* cnt, hipStream_t s);
+int q2_lattice_fill(int n, const int32_t* rowptr, int32_t* col, hipStream_t s);
+int q2_lattice_assemble(int n, const double* theta, double f_const, const int32_t* rowptr, const int32_t* col,
+                        double* val, double* rhs, hipStream_t s);
 
 // ---- tetrahedral hierarchy (tet_kernels.hip) ----
 struct TetDisc {
