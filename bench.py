@@ -158,6 +158,18 @@ def bench_q2_elements(U, seed, reps, n_cells=64):
            "mfma_issued_fp64_TFLOPs": n * 63 * 16 * 16 * 4 * 2 / t / 1e12}
     log(f"[bench] Q2 element matrices {n_cells}^3: {t*1e3:.3f} ms, {res['GBps']:.0f} GB/s written+read, "
         f"{res['useful_fp64_TFLOPs']:.2f} TFLOP/s useful FP64")
+    del dx, dk, ke, fe
+    # the whole Q2 system of the unit cube (pattern on the device, host scan of
+    # the row lengths, colour-ordered scatter, Dirichlet rows), wall clock
+    theta = rng.normal(scale=0.5, size=n)
+    U.q2_assemble_lattice(n_cells, theta, 1.0)
+    t0 = time.perf_counter()
+    A, b = U.q2_assemble_lattice(n_cells, theta, 1.0)
+    t_asm = time.perf_counter() - t0
+    res["lattice_assembly_s"] = t_asm
+    res["lattice_assembly_dofs"] = (2 * n_cells + 1) ** 3
+    res["lattice_assembly_nnz"] = A.nnz()
+    log(f"[bench] Q2 lattice assembly {n_cells}^3 elements: {t_asm*1e3:.1f} ms, {A.nnz()} nonzeros")
     return res
 
 
