@@ -37,6 +37,7 @@ struct UGError : public std::runtime_error {
 // device context and buffers
 // ---------------------------------------------------------------------------
 struct DeviceContext {
+    int device = -1;                 // the device the context was created on
     hipStream_t stream = nullptr;
     double* red_partial = nullptr;   // 65536 partial sums (one per block of a reduction)
     double* red_out = nullptr;       // 8 results
@@ -44,6 +45,7 @@ struct DeviceContext {
     int* flag = nullptr;             // device error flag
 };
 DeviceContext& context();            // initialises the device on first use
+DeviceContext* context_if_ready();   // nullptr before the first context() (allocates nothing)
 void sync();
 
 template <typename T>
@@ -127,16 +129,34 @@ class GPUSparseMatrix {
     int32_t* rowptr() const { return rowptr_.get(); }
     int32_t* col() const { return col_.get(); }
     double* val() const { return val_.get(); }
-    // build the sliced-ELL SpMV layout from the CSR (after the values are set;
+    // the values were changed through the raw CSR pointers: drops the
+    // provenance (operators derived from an assembly, such as the matrix-free
+    // and lattice-order paths of the multigrid, no longer apply to this
+    // matrix) and rebuilds the SpMV layout from the CSR
+    void finalize() {
+        set_origin(nullptr);
+        build_spmv_layout();
+    }
+    // write access to the CSR values is handed out: the provenance is dropped
+    // and SpMV reads the CSR itself until finalize() rebuilds the layout
+    void values_edited_externally() {
+        set_origin(nullptr);
+        sell_valid_ = false;
+    }
+    // build the sliced-ELL SpMV layout from the current CSR (provenance kept;
     // assembly and uploads call it); until then SpMV runs on the CSR directly
-    void finalize();
+    void build_spmv_layout();
     bool uses_sliced_ell() const { return sell_valid_; }
     // provenance: the discretisation that assembled the values and a counter
     // bumped on every assembly/upload (used to validate derived operators)
     const void* origin() const { return origin_; }
     uint64_t version() const { return version_; }
     void set_origin(const void* o) { origin_ = o; ++version_; }
-    void set_use_sliced_ell(bool on) { sell_enabled_ = on; if (!on) sell_valid_ = false; }
+    void set_use_sliced_ell(bool on) {
+        sell_enabled_ = on;
+        if (!on) sell_valid_ = false;
+        else if (!sell_valid_) build_spmv_layout();
+    }
     // the row permutation that fills the sliced-ELL slices (rows order[64t..
     // 64t+63] form slice t); it groups rows whose columns lie close together so
     // that the x gathers of a wavefront stay local. Results do not depend on
@@ -426,10 +446,31 @@ class AssembledMultiGridCycle : public ILinearIterator {
     explicit AssembledMultiGridCycle(std::shared_ptr<DomainDiscretization> disc);
     ~AssembledMultiGridCycle() override;
     const char* name() const override { return "GMG"; }
-    void set_base_level(int l) { base_lev_ = l; }
-    void set_num_presmooth(int n) { nu1_ = n; }
-    void set_num_postsmooth(int n) { nu2_ = n; }
-    void set_cycle_type(int c) { cycle_ = c; }
+    // cycle types of AssembledMultiGridCycle (mg_solver.h:339-341)
+    static const int _V_ = 1;
+    static const int _W_ = 2;
+    static const int _F_ = -1;
+    void set_base_level(int l) {
+        if (l < 0) throw UGError("GMG::set_base_level: level must be >= 0");
+        base_lev_ = l;
+    }
+    void set_num_presmooth(int n) {
+        if (n < 0) throw UGError("GMG::set_num_presmooth: number of steps must be >= 0");
+        nu1_ = n;
+    }
+    void set_num_postsmooth(int n) {
+        if (n < 0) throw UGError("GMG::set_num_postsmooth: number of steps must be >= 0");
+        nu2_ = n;
+    }
+    // _V_ (1), _W_ (2), any gamma >= 1 (gamma coarse cycles per level), or
+    // _F_ (-1: one F-cycle then one V-cycle on the next coarser level,
+    // mg_solver_impl.hpp:2115-2118); 0 and other negative values are refused
+    // (they would run no coarse correction at all)
+    void set_cycle_type(int c) {
+        if (c != _F_ && c < 1) throw UGError("GMG::set_cycle_type: cycle must be 1 (V), 2 (W), gamma > 2 or -1 (F)");
+        cycle_ = c;
+    }
+    int cycle_type() const { return cycle_; }
     void set_smoother_damp(double w) { omega_ = w; }
     void set_precision(Precision p) { prec_ = p; }
     // precondition the adjoint operator: level operators A_l,elim^T (the

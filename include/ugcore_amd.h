@@ -166,15 +166,17 @@ int ugx_disc_adjoint_operator(void* disc, void* a, void* at);
  * 470-752) in the local shape order of LagrangeLSFS<ReferenceHexahedron, 2>
  * (lagrange.cpp:236-253, lagrange.h:2106-2190), trilinear geometry, 3x3x3
  * Gauss points. Device pointers: corners [n][8][3] in the reference
- * hexahedron order, k [n] (NULL: 1), ke [n][27][27] row-major, fe [n][27]
- * (NULL: not formed). Enqueued on the library stream. */
+ * hexahedron order, k [n] (NULL: 1; must hold n entries), ke [n][27][27]
+ * row-major, fe [n][27] (NULL: not formed). Enqueued on the library stream. */
 int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* k, double f_const, double* ke,
                                 double* fe);
 /* assemble_linear (domain_disc_impl.h:835-948) of -div(k grad u) = f with
  * Q2 Lagrange elements on the unit cube split into n^3 hexahedra, u = 0 on the
  * boundary (identity rows on the kept pattern, SetDirichletRow,
  * sparsematrix_util.h:850-861): k = exp(theta[e]) per element (theta_dev
- * device pointer, x fastest, or NULL: 1), f constant. Nodes are the
+ * device pointer to n^3 entries, x fastest, or NULL: 1), f constant;
+ * 1 <= n <= 161 so that nnz = (8n+1)^3 fits int32 (checked before any
+ * allocation). Nodes are the
  * (2n+1)^3 lattice points numbered x fastest (not the reference's DoF order,
  * which lib_grid fixes); element matrices as ugx_q2_hex_element_matrices,
  * scattered colour by colour without atomics into a pattern built on the
@@ -192,6 +194,11 @@ int64_t ugx_matrix_nnz(void* m);
 int ugx_matrix_upload_csr(void* m, int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col,
                           const double* val);
 int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val);
+/* Raw device CSR pointers (write access): from this call on the matrix no
+ * longer counts as the discretization's assembly, so no operator derived from
+ * that assembly (matrix-free top level, lattice-order CG) is used with it, and
+ * SpMV reads the CSR until ugx_matrix_finalize rebuilds the sliced-ELL layout
+ * from the edited values. */
 int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val);
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w);   /* vectors are handles */
 /* tuning: lanes of a wavefront that share one CSR row in this matrix's SpMV (4..64, power of 2) */
@@ -208,6 +215,7 @@ int ugx_matrix_set_spmv_layout(void* m, int sliced_ell);
  * reference (SparseMatrix has one storage, sparsematrix.h). */
 int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n);
 int ugx_matrix_has_row_order(void* m);
+/* after editing values through ugx_matrix_device_csr: rebuilds the SpMV layout */
 int ugx_matrix_finalize(void* m);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
 
@@ -220,9 +228,13 @@ int ugx_vector_upload(void* v, const double* host);
 int ugx_vector_download(void* v, double* host);
 int ugx_vector_set(void* v, double value);
 
-/* AssembledMultiGridCycle (mg_solver_impl.hpp:175-2136): V(nu1,nu2) or W cycle,
- * damped Jacobi smoother, P1 transfer, dense LU on the base level.
- * precision: 0 = FP64 level operators, 1 = FP32 level operators (base LU FP64). */
+/* AssembledMultiGridCycle (mg_solver_impl.hpp:175-2136): damped Jacobi
+ * smoother, P1 transfer, dense LU on the base level.
+ * cycle: 1 = V, 2 = W, gamma > 2, or -1 = F (one F-cycle then one V-cycle on
+ * each coarser level, mg_solver_impl.hpp:2115-2118; mg_solver.h:339-341);
+ * nu1, nu2 >= 0 smoothing steps; 0 <= base_level <= top level (checked at
+ * ugx_gmg_init). precision: 0 = FP64 level operators, 1 = FP32 level
+ * operators (base LU FP64). Anything else returns NULL with a message. */
 void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision);
 void ugx_gmg_destroy(void* g);
 /* before ugx_gmg_init: 1 = precondition the adjoint operator (level operators

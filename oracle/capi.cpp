@@ -23,6 +23,7 @@ struct Problem {
     std::vector<double> x;
     TetParams tp;                       // kind 2: velocity, jitter
     BiCGStabOptions bicg;               // BiCGStab restarts
+    bool record_x = false;              // keep every iterate of the next solves
     SolveResult last;
     double t_assemble = 0, t_setup = 0, t_solve = 0;
     std::string err;
@@ -249,8 +250,9 @@ int orc_solve(void* h, int solver, int precond, int base_lev, int nu1, int nu2, 
         cc.max_steps = max_steps;
         cc.min_defect = min_def;
         cc.rel_reduction = rel_red;
-        p->last = solver == 0 ? cg_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc)
-                              : bicgstab_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc, p->bicg);
+        p->last = solver == 0 ? cg_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc, p->record_x)
+                              : bicgstab_solve(p->A[top], precond ? &g : nullptr, p->x, p->b[top], cc, p->bicg,
+                                               p->record_x);
         double t2 = now();
         p->t_setup = t1 - t0;
         p->t_solve = t2 - t1;
@@ -377,6 +379,44 @@ int orc_solve_adjoint(void* h, int solver, const double* rhs, double* lambda_out
         auto r = solver == 0 ? cg_solve(g.A[top], &g, x, b, cc) : bicgstab_solve(g.A[top], &g, x, b, cc, p->bicg);
         std::memcpy(lambda_out, x.data(), sizeof(double) * n);
         p->last = r;
+        return r.ok ? r.steps : -1 - r.steps;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -100000;
+    }
+}
+
+// iterate recording for the solves of orc_solve (tests pin the Krylov
+// methods by properties of their iterates)
+void orc_set_record_x(void* h, int on) { ((Problem*)h)->record_x = on != 0; }
+int orc_iterates_count(void* h) { return (int)((Problem*)h)->last.xs.size(); }
+int orc_iterate_get(void* h, int k, double* out) {
+    auto* p = (Problem*)h;
+    if (k < 0 || k >= (int)p->last.xs.size()) return -1;
+    std::memcpy(out, p->last.xs[k].data(), sizeof(double) * p->last.xs[k].size());
+    return 0;
+}
+
+// CG (solver 0) or BiCGStab (1) without a preconditioner on a caller's CSR
+// (row pointers int64) from x (in/out); returns steps if converged, -1-steps
+// if not (breakdown, NaN defect or step limit), -100000 on an exception.
+// Failure-handling tests drive it with singular and NaN inputs.
+int orc_csr_solve(int solver, int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+                  const double* b, double* x, double rel_red, double min_def, int max_steps) {
+    try {
+        CSR A;
+        A.n_rows = n;
+        A.n_cols = n;
+        A.rowptr.assign(rowptr, rowptr + n + 1);
+        A.col.assign(col, col + rowptr[n]);
+        A.val.assign(val, val + rowptr[n]);
+        std::vector<double> xv(x, x + n), bv(b, b + n);
+        ConvCheck cc;
+        cc.max_steps = max_steps;
+        cc.min_defect = min_def;
+        cc.rel_reduction = rel_red;
+        SolveResult r = solver == 0 ? cg_solve(A, nullptr, xv, bv, cc) : bicgstab_solve(A, nullptr, xv, bv, cc);
+        std::memcpy(x, xv.data(), sizeof(double) * n);
         return r.ok ? r.steps : -1 - r.steps;
     } catch (std::exception& e) {
         g_err = e.what();

@@ -230,7 +230,7 @@ struct GMG {
     // AssembledMultiGridCycle (mg_solver_impl.hpp), full refinement, LEVEL view
     int base_lev = 0, top_lev = 0;
     int nu1 = 2, nu2 = 2;
-    int cycle = 1;                    // 1 = V, 2 = W
+    int cycle = 1;                    // 1 = V, 2 = W, gamma > 2, -1 = F (mg_solver.h:339-341)
     double omega = 0.66;
     std::vector<CSR> A;               // level matrices
     std::vector<CSR> P, R;            // P[l]: l-1 -> l (rows fine), R[l]: l -> l-1
@@ -240,7 +240,7 @@ struct GMG {
     std::vector<std::vector<double>> sc, sd, st;
     void init();
     void apply(std::vector<double>& c, const std::vector<double>& d);
-    void lmgc(int lev);
+    void lmgc(int lev, int type);
     void presmooth_and_restriction(int lev);
     void prolongation_and_postsmooth(int lev);
     void base_solve(int lev);
@@ -251,11 +251,15 @@ struct SolveResult {
     bool ok = false;
     int steps = 0;
     std::vector<double> defects;
+    // with record_x: the iterate at every recorded defect (CG: x_k after each
+    // step; BiCGStab: after each half step and each full step)
+    bool record_x = false;
+    std::vector<std::vector<double>> xs;
 };
 
 // CG::apply_return_defect (linear_solver/cg.h:103-242)
 SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                     ConvCheck cc);
+                     ConvCheck cc, bool record_x = false);
 // BiCGStab::apply_return_defect (linear_solver/bicgstab.h:112-385), no restarts
 // BiCGStab::set_restart / set_min_orthogonality (bicgstab.h:386-390)
 struct BiCGStabOptions {
@@ -263,6 +267,6 @@ struct BiCGStabOptions {
     double min_ortho = 0.0;
 };
 SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                           ConvCheck cc, BiCGStabOptions opt = BiCGStabOptions());
+                           ConvCheck cc, BiCGStabOptions opt = BiCGStabOptions(), bool record_x = false);
 
 }  // namespace orc

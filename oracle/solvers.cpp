@@ -91,6 +91,11 @@ void DenseLU::apply(std::vector<double>& x, const std::vector<double>& b) const 
 // DoFs (init_index_mappings, mg_solver_impl.hpp:1280-1370).
 
 void GMG::init() {
+    // set_cycle_type accepts V (1), W (2), F (-1) and gamma > 2; 0 and other
+    // negative values would run no coarse correction and are refused
+    if (cycle != -1 && cycle < 1) throw std::runtime_error("oracle GMG: cycle must be 1, 2, gamma > 2 or -1 (F)");
+    if (nu1 < 0 || nu2 < 0) throw std::runtime_error("oracle GMG: smoothing step counts must be >= 0");
+    if (base_lev < 0 || base_lev > top_lev) throw std::runtime_error("oracle GMG: base level out of range");
     const int nl = top_lev + 1;
     sc.assign(nl, {});
     sd.assign(nl, {});
@@ -113,16 +118,22 @@ void GMG::apply(std::vector<double>& c, const std::vector<double>& d) {
     sd[top_lev] = d;
     std::fill(c.begin(), c.end(), 0.0);
     std::fill(sc[top_lev].begin(), sc[top_lev].end(), 0.0);
-    lmgc(top_lev);
+    lmgc(top_lev, cycle);
     for (size_t i = 0; i < c.size(); ++i) c[i] += sc[top_lev][i];
 }
 
-void GMG::lmgc(int lev) {
+void GMG::lmgc(int lev, int type) {
     // mg_solver_impl.hpp:2090-2136
     if (lev == base_lev) { base_solve(top_lev); return; }
     presmooth_and_restriction(lev);
-    if (lev - 1 == base_lev) base_solve(lev - 1);
-    else for (int i = 0; i < cycle; ++i) lmgc(lev - 1);
+    if (lev - 1 == base_lev) {
+        base_solve(lev - 1);                       // on the base level invert only once
+    } else if (type == -1) {
+        lmgc(lev - 1, -1);                         // F-cycle: one F-cycle on lev-1,
+        lmgc(lev - 1, 1);                          // followed by a V-cycle
+    } else {
+        for (int i = 0; i < type; ++i) lmgc(lev - 1, type);
+    }
     prolongation_and_postsmooth(lev);
 }
 
@@ -169,13 +180,16 @@ void GMG::base_solve(int lev) {
 // ---- CG (cg.h:103-242) ----------------------------------------------------------
 
 SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                     ConvCheck cc) {
+                     ConvCheck cc, bool record_x) {
     SolveResult res;
+    res.record_x = record_x;
+    auto rec = [&]() { if (record_x) res.xs.push_back(x); };
     std::vector<double> r = b;
     apply_sub(A, r, x);
     std::vector<double> q(r.size(), 0.0), z(r.size(), 0.0), p(r.size(), 0.0);
     if (precond) precond->apply(z, r); else z = r;
     cc.start(norm(r));
+    rec();
     p = z;
     double rho_old = dot(z, r), rho;
     while (!cc.iteration_ended()) {
@@ -186,6 +200,7 @@ SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const s
         for (size_t i = 0; i < x.size(); ++i) x[i] = 1.0 * x[i] + alpha * p[i];
         for (size_t i = 0; i < r.size(); ++i) r[i] = 1.0 * r[i] + (-alpha) * q[i];
         cc.update(norm(r));
+        rec();
         if (cc.iteration_ended()) break;
         if (precond) precond->apply(z, r); else z = r;
         rho = dot(z, r);
@@ -203,13 +218,16 @@ SolveResult cg_solve(const CSR& A, GMG* precond, std::vector<double>& x, const s
 // when |(r0, r)| <= opt.min_ortho ||r|| ||r0|| (bicgstab.h:157-215) -------------------
 
 SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, const std::vector<double>& b,
-                           ConvCheck cc, BiCGStabOptions opt) {
+                           ConvCheck cc, BiCGStabOptions opt, bool record_x) {
     SolveResult res;
+    res.record_x = record_x;
+    auto rec = [&]() { if (record_x) res.xs.push_back(x); };
     const size_t n = b.size();
     std::vector<double> r = b;
     apply_sub(A, r, x);
     std::vector<double> r0(n), p(n), v(n), t(n), s(n), q(n);
     cc.start(norm(r));
+    rec();
     double rho = 1, alpha = 1, omega = 1, norm_r0 = 0.0;
     bool restart = true;
     auto fail = [&]() { res.ok = false; res.steps = cc.step; res.defects = cc.defects; return res; };
@@ -240,6 +258,7 @@ SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, c
         for (size_t i = 0; i < n; ++i) x[i] = 1.0 * x[i] + alpha * q[i];
         for (size_t i = 0; i < n; ++i) s[i] = 1.0 * r[i] + (-alpha) * v[i];
         cc.update(norm(s));
+        rec();
         if (cc.iteration_ended()) { r = s; break; }
         if (precond) precond->apply(q, s); else q = s;
         axpy(A, t, 0.0, t, 1.0, q);
@@ -250,6 +269,7 @@ SolveResult bicgstab_solve(const CSR& A, GMG* precond, std::vector<double>& x, c
         for (size_t i = 0; i < n; ++i) x[i] = 1.0 * x[i] + omega * q[i];
         for (size_t i = 0; i < n; ++i) r[i] = 1.0 * s[i] + (-omega) * t[i];
         cc.update(norm(r));
+        rec();
         if (omega == 0.0) return fail();
     }
     res.ok = cc.post();

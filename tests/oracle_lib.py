@@ -70,6 +70,11 @@ def lib():
             "orc_csr_op": (None, [C.c_int, I64, I64, I64P, I32P, F64P, C.c_double, C.c_void_p,
                                   C.c_double, F64P, F64P]),
             "orc_q2_hex_elements": (None, [I64, P, P, C.c_double, P, P, P]),
+            "orc_set_record_x": (None, [P, C.c_int]),
+            "orc_iterates_count": (C.c_int, [P]),
+            "orc_iterate_get": (C.c_int, [P, C.c_int, F64P]),
+            "orc_csr_solve": (C.c_int, [C.c_int, I64, I64P, I32P, F64P, F64P, F64P, C.c_double, C.c_double,
+                                        C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -228,6 +233,23 @@ class Problem:
         lib().orc_defects_get(self.h, d.ctypes.data, n)
         return {"ok": r >= 0, "steps": r if r >= 0 else -1 - r, "x": x, "defects": d}
 
+    def iterates(self, solver="cg", gmg=True, base_lev=0, nu1=2, nu2=2, cycle=1, omega=0.66, rel_red=1e-10,
+                 max_steps=200):
+        """solve as solve() and return every recorded iterate (CG: x_0, x_1,
+        ...; BiCGStab: x_0, then x after each half and each full step) with
+        the solve's result"""
+        lib().orc_set_record_x(self.h, 1)
+        try:
+            res = self.solve(solver, gmg, base_lev, nu1, nu2, cycle, omega, rel_red, 1e-50, max_steps)
+            n = lib().orc_iterates_count(self.h)
+            top = self.num_levels - 1
+            xs = np.zeros((n, self.num_dofs(top)), np.float64)
+            for k in range(n):
+                lib().orc_iterate_get(self.h, k, xs[k])
+        finally:
+            lib().orc_set_record_x(self.h, 0)
+        return xs, res
+
     def gmg_apply(self, d, base_lev=0, nu1=2, nu2=2, cycle=1, omega=0.66):
         d = np.ascontiguousarray(d, np.float64)
         c = np.zeros_like(d)
@@ -283,6 +305,21 @@ class Problem:
         out = np.zeros(3, np.float64)
         lib().orc_timings(self.h, out)
         return {"assemble": out[0], "setup": out[1], "solve": out[2]}
+
+
+def csr_solve(A, b, x0=None, solver="cg", rel_red=1e-10, min_def=1e-50, max_steps=200):
+    """CG or BiCGStab without a preconditioner on a scipy CSR matrix (oracle
+    restatement); returns (converged, steps, x)"""
+    n = A.shape[0]
+    rp = np.ascontiguousarray(A.indptr, np.int64)
+    col = np.ascontiguousarray(A.indices, np.int32)
+    val = np.ascontiguousarray(A.data, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    r = lib().orc_csr_solve(0 if solver == "cg" else 1, n, rp, col, val, np.ascontiguousarray(b, np.float64), x,
+                            rel_red, min_def, max_steps)
+    if r == -100000:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return (r >= 0, r if r >= 0 else -1 - r, x)
 
 
 def make_theta(n, seed, passes=3):

@@ -462,16 +462,27 @@ class DomainDiscretization:
         return buf.numpy()
 
 
+_CYCLES = {"V": 1, "W": 2, "F": -1}
+_PRECISIONS = {"f64": 0, "float64": 0, 0: 0, "f32": 1, "float32": 1, 1: 1}
+
+
 class GeometricMultiGrid:
-    """AssembledMultiGridCycle: V(nu1,nu2) (cycle=1) or W (cycle=2), damped
-    Jacobi smoother, P1 transfer, dense FP64 LU on the base level.
-    precision 'f64' or 'f32' for the level operators; rap=True builds the
-    coarse levels as Galerkin products R A P (set_rap)."""
+    """AssembledMultiGridCycle: damped Jacobi smoother, P1 transfer, dense
+    FP64 LU on the base level. cycle: 1 or "V", 2 or "W", gamma > 2, -1 or
+    "F" (set_cycle_type, mg_solver.h:155-163); anything else raises
+    ValueError. precision 'f64' or 'f32' for the level operators; rap=True
+    builds the coarse levels as Galerkin products R A P (set_rap)."""
 
     def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64", transposed=False,
                  matrix_free=True, rap=False, graphs=True):
         self.disc = disc
-        p = 1 if precision in ("f32", "float32", 1) else 0
+        if isinstance(cycle, str):
+            if cycle.strip() not in _CYCLES:
+                raise ValueError(f"GeometricMultiGrid: cycle '{cycle}' not supported (V, W or F)")
+            cycle = _CYCLES[cycle.strip()]
+        if precision not in _PRECISIONS:
+            raise ValueError(f"GeometricMultiGrid: precision {precision!r} not supported (f64 or f32)")
+        p = _PRECISIONS[precision]
         self.h = _handle(lib().ugx_gmg_create(disc.h, base_level, nu1, nu2, cycle, float(omega), p),
                          "GeometricMultiGrid")
         if transposed:
@@ -662,8 +673,12 @@ def q2_hex_element_matrices(corners, k=None, f_const=0.0):
     n = X.shape[0]
     if X.shape[1:] != (8, 3):
         raise ValueError("corners must have shape [n, 8, 3]")
+    if k is not None:
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        if k.size != n:
+            raise ValueError(f"k must hold one coefficient per element ({n}), got {k.size}")
     dx = DeviceArray(X)
-    dk = DeviceArray(np.ascontiguousarray(k, dtype=np.float64)) if k is not None else None
+    dk = DeviceArray(k) if k is not None else None
     ke = DeviceArray(np.zeros((n, 27, 27)))
     fe = DeviceArray(np.zeros((n, 27)))
     _check(lib().ugx_q2_hex_element_matrices(n, dx.ptr, dk.ptr if dk is not None else None, float(f_const),
@@ -672,14 +687,25 @@ def q2_hex_element_matrices(corners, k=None, f_const=0.0):
     return ke.numpy(), fe.numpy()
 
 
+# largest n with nnz = (8n+1)^3 < 2^31 (int32 column indices)
+Q2_LATTICE_MAX_N = 161
+
+
 def q2_assemble_lattice(n, theta=None, f_const=1.0):
     """Q2 assembly of -div(exp(theta) grad u) = f on the unit cube of n^3
     hexahedra with u = 0 on the boundary (ugx_q2_assemble_lattice); returns
     (SparseMatrix, Vector) on the device, nodes of the (2n+1)^3 lattice
     numbered x fastest"""
+    n = int(n)
+    if not 1 <= n <= Q2_LATTICE_MAX_N:
+        raise ValueError(f"n must be in 1..{Q2_LATTICE_MAX_N} (the pattern's nnz must fit int32)")
+    if theta is not None:
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        if theta.size != n ** 3:
+            raise ValueError(f"theta must hold one value per element ({n ** 3}), got {theta.size}")
     A = SparseMatrix()
     b = Vector(1)
-    th = DeviceArray(np.ascontiguousarray(theta, dtype=np.float64)) if theta is not None else None
+    th = DeviceArray(theta) if theta is not None else None
     _check(lib().ugx_q2_assemble_lattice(int(n), th.ptr if th is not None else None, float(f_const), A.h, b.h),
            "q2_assemble_lattice")
     return A, b

@@ -385,7 +385,10 @@ int ugx_q2_hex_element_matrices(int64_t n, const double* corners, const double* 
 }
 int ugx_q2_assemble_lattice(int n, const double* theta, double f_const, void* matrix, void* rhs) {
     return guard([&] {
-        if (n < 1 || n > 600) throw UGError("ugx_q2_assemble_lattice: n must be in 1..600");
+        // nnz = (8n+1)^3 (an even node couples to 5 nodes per direction, an
+        // odd one to 3: 2n+1 + 4n + 2n = 8n+1 per direction) must fit int32:
+        // checked before anything is allocated
+        if (n < 1 || n > 161) throw UGError("ugx_q2_assemble_lattice: n must be in 1..161 (nnz must fit int32)");
         if (!matrix || !rhs) throw UGError("ugx_q2_assemble_lattice: null matrix or vector");
         auto& c = context();
         const int W = 2 * n + 1;
@@ -413,7 +416,8 @@ int ugx_q2_assemble_lattice(int n, const double* theta, double f_const, void* ma
             ugamd::q2_lattice_assemble(n, theta, f_const, A.rowptr(), A.col(), A.val(), b.data(), c.stream) != 0)
             throw UGError("ugx_q2_assemble_lattice: launch failed");
         if (hipStreamSynchronize(c.stream) != hipSuccess) throw UGError("ugx_q2_assemble_lattice: stream failed");
-        A.finalize();
+        A.set_origin(nullptr);
+        A.build_spmv_layout();
         return 0;
     });
 }
@@ -464,6 +468,10 @@ int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val)
 }
 int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val) {
     auto& A = *((MatrixH*)m)->A;
+    // the caller may write through these pointers: from here on the matrix is
+    // no longer the assembly's (no matrix-free or lattice-order operator is
+    // derived from it) and SpMV reads the CSR until ugx_matrix_finalize
+    A.values_edited_externally();
     *rowptr = A.rowptr();
     *col = A.col();
     *val = A.val();
@@ -485,7 +493,6 @@ int ugx_matrix_set_spmv_layout(void* m, int sliced_ell) {
     return guard([&] {
         auto& A = *((MatrixH*)m)->A;
         A.set_use_sliced_ell(sliced_ell != 0);
-        if (sliced_ell) A.finalize();
         return 0;
     });
 }
@@ -567,7 +574,9 @@ int ugx_gmg_set_transposed(void* g, int transposed) {
 }
 void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision) {
     return guard_ptr([&] {
-        auto* h = new GmgH();
+        if (!disc) throw UGError("ugx_gmg_create: null discretization");
+        if (precision != 0 && precision != 1) throw UGError("ugx_gmg_create: precision must be 0 (FP64) or 1 (FP32)");
+        std::unique_ptr<GmgH> h(new GmgH());
         h->g = std::make_shared<AssembledMultiGridCycle>(((DiscH*)disc)->d);
         h->g->set_base_level(base_level);
         h->g->set_num_presmooth(nu1);
@@ -575,7 +584,7 @@ void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, do
         h->g->set_cycle_type(cycle);
         h->g->set_smoother_damp(omega);
         h->g->set_precision(precision == 1 ? Precision::F32 : Precision::F64);
-        return (void*)h;
+        return (void*)h.release();
     });
 }
 void ugx_gmg_destroy(void* g) { delete (GmgH*)g; }
@@ -652,7 +661,11 @@ int ugx_solver_set_conv_check(void* s, int max_steps, double min_defect, double 
     });
 }
 int ugx_solver_init(void* s, void* matrix) {
-    return guard([&] { return ((SolverH*)s)->s->init(((MatrixH*)matrix)->op) ? 0 : -1; });
+    return guard([&] {
+        if (!((SolverH*)s)->s->init(((MatrixH*)matrix)->op))
+            throw UGError("solver init: the preconditioner's set-up failed (Jacobi: zero diagonal entry)");
+        return 0;
+    });
 }
 int ugx_solver_apply(void* s, void* x, void* b) {
     return guard([&] { return ((SolverH*)s)->s->apply(((VectorH*)x)->v, ((VectorH*)b)->v) ? 1 : 0; });

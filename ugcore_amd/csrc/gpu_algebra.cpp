@@ -36,13 +36,19 @@ static double wall() {
 // ---------------------------------------------------------------------------
 // context / buffers
 // ---------------------------------------------------------------------------
+static DeviceContext g_ctx;
+static bool g_ctx_ready = false;
+
+DeviceContext* context_if_ready() { return g_ctx_ready ? &g_ctx : nullptr; }
+
 DeviceContext& context() {
-    static DeviceContext ctx;
-    static bool init = false;
+    DeviceContext& ctx = g_ctx;
+    bool& init = g_ctx_ready;
     if (!init) {
         int n = 0;
         UGX_HIP(hipGetDeviceCount(&n));
         if (n < 1) throw UGError("no HIP device available");
+        UGX_HIP(hipGetDevice(&ctx.device));
         UGX_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
         UGX_HIP(hipMalloc(&ctx.red_partial, 65536 * sizeof(double)));
         UGX_HIP(hipMalloc(&ctx.red_out, 8 * sizeof(double)));
@@ -169,7 +175,7 @@ static int lanes_for(int64_t rows, int64_t nnz) {
     return 32;
 }
 
-void GPUSparseMatrix::finalize() {
+void GPUSparseMatrix::build_spmv_layout() {
     sell_valid_ = false;
     if (!sell_enabled_ || rows_ == 0) return;
     auto& c = context();
@@ -207,7 +213,7 @@ void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
 void GPUSparseMatrix::set_row_order(const int32_t* order, int64_t n) {
     if (!order) {
         row_order_.resize(0);
-        finalize();
+        build_spmv_layout();
         return;
     }
     if (n != rows_) throw UGError("set_row_order: the order must list every row once");
@@ -219,7 +225,7 @@ void GPUSparseMatrix::set_row_order(const int32_t* order, int64_t n) {
     }
     row_order_.resize(n);
     row_order_.upload(order, n);
-    finalize();
+    build_spmv_layout();
 }
 
 void GPUSparseMatrix::set_row_order_device(const int32_t* order, int64_t n) {
@@ -237,7 +243,7 @@ void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_
     rowptr_.upload(rowptr, rows + 1);
     col_.upload(col, nnz_);
     val_.upload(val, nnz_);
-    finalize();
+    build_spmv_layout();
 }
 
 void GPUSparseMatrix::download_csr(int32_t* rowptr, int32_t* col, double* val) const {
@@ -823,7 +829,7 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     int64_t lat_points = 1;
     for (int d = 0; d < L.dim; ++d) lat_points *= L.W;
     if (lat_points == L.N) A.set_row_order_device(L.dof_of_lat.get(), L.N);
-    A.finalize();
+    A.build_spmv_layout();
     t_assemble_ = wall() - t0;
 }
 
@@ -845,7 +851,7 @@ void DomainDiscretization::assemble_adjoint_operator(const GPUSparseMatrix& A, G
     if (bad) throw UGError("assemble_adjoint_operator: pattern is not structurally symmetric");
     AT.set_origin(nullptr);
     if (A.has_row_order()) AT.set_row_order_device(L.dof_of_lat.get(), L.N);
-    AT.finalize();
+    AT.build_spmv_layout();
 }
 
 void DomainDiscretization::level_dia_layout(int lev, int& nd, std::vector<int32_t>& offsets) const {
@@ -1228,7 +1234,9 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         }
     }
 
-    void prolongation_and_postsmooth(int lev) {
+    // last: the caller prolongates this level's correction next (the level is
+    // not visited again before that)
+    void prolongation_and_postsmooth(int lev, bool last) {
         auto& c = context();
         GmgLevel<T>& f = L[lev];
         GmgLevel<T>& cl = L[lev - 1];
@@ -1248,9 +1256,10 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
             if (nu < nu2 - 1) {
                 UGX_LAUNCH(op_jacobi_step(f));
                 std::swap(f.sd, f.sd2);
-            } else if (lev < top && !tets && cycle == 1) {
-                // folded into the prolongation to the next level (V-cycle only:
-                // a W-cycle visits the level again and needs sc updated)
+            } else if (lev < top && !tets && last) {
+                // folded into the prolongation to the next level (only on the
+                // level's last visit: an earlier visit of a W- or F-cycle is
+                // followed by another one that smooths with sc updated)
                 f.acc_deferred = true;
             } else if (!(defer_top_final && lev == top)) {
                 UGX_LAUNCH(jacobi_accumulate<T>(f.N, f.dinv, f.sd, f.sc, c.stream));
@@ -1258,15 +1267,24 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         }
     }
 
-    void lmgc(int lev) {
+    // mg_solver_impl.hpp:2090-2136: the base level is solved once; an F-cycle
+    // (type -1) runs one F-cycle and then one V-cycle on the coarser level,
+    // a V/W/gamma-cycle runs gamma cycles of its own type there
+    void lmgc(int lev, int type, bool last) {
         if (lev == base) {
             base_solve(top);
             return;
         }
         presmooth_and_restriction(lev);
-        if (lev - 1 == base) base_solve(lev - 1);
-        else for (int i = 0; i < cycle; ++i) lmgc(lev - 1);
-        prolongation_and_postsmooth(lev);
+        if (lev - 1 == base) {
+            base_solve(lev - 1);
+        } else if (type == AssembledMultiGridCycle::_F_) {
+            lmgc(lev - 1, AssembledMultiGridCycle::_F_, false);
+            lmgc(lev - 1, AssembledMultiGridCycle::_V_, true);
+        } else {
+            for (int i = 0; i < type; ++i) lmgc(lev - 1, type, i == type - 1);
+        }
+        prolongation_and_postsmooth(lev, last);
     }
 
     void apply(GPUVector& cvec, const GPUVector& d) override {
@@ -1290,7 +1308,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         cvec.set(0.0);
         UGX_LAUNCH(vec_set<T>(t.N, t.sc, 0.0, c.stream));
         try {
-            lmgc(top);
+            lmgc(top, cycle, true);
         } catch (...) {
             clear_cycle_state();
             throw;
@@ -1316,7 +1334,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         // top level's correction update, fused here with the conversion
         defer_top_final = nu2 > 0 && top > base;
         try {
-            lmgc(top);
+            lmgc(top, cycle, true);
         } catch (...) {
             clear_cycle_state();
             throw;
