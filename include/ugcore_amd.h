@@ -69,6 +69,18 @@ int ugx_vec_set_f64(int64_t n, double* x, double value, void* stream);
  * two-pass reduction; synchronises and writes the result to host memory. */
 int ugx_dot_f64(int64_t n, const double* a, const double* b, double* result_host, void* stream);
 int ugx_norm_f64(int64_t n, const double* a, double* result_host, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ugx_dot_f64 without any allocation. The partial sums go
+ * to a fixed device array of the library (65536 doubles = 512 KiB per device,
+ * held until the process ends; the reduction grid never needs more, so n is
+ * not limited further), the result to result_dev (device memory, written on
+ * the stream, no synchronisation). Same kernels and grid as ugx_dot_f64, so
+ * the same value bit for bit. stream NULL means the library's stream, which
+ * exists only after ugx_init() (the call fails otherwise). Every call on one
+ * device shares that device's array: calls that overlap on one device race. */
+int ugx_dot_f64_noalloc(int64_t n, const double* a, const double* b, double* result_dev, void* stream);
+
 
 /* ---- kernel entry points: Jacobi (lib_algebra/operator/preconditioner/jacobi.h:155-249,
  * replacing GPUJacobi::preprocess + CUDA_JacobiApply, gpujacobi.h:113-144, common_cuda.cu:46-52) */

@@ -46,6 +46,7 @@ _SIGNATURES = {
     "ugx_vec_set_f64": (_I, [_I64, _P, _D, _P]),
     "ugx_dot_f64": (_I, [_I64, _P, _P, _P, _P]),
     "ugx_norm_f64": (_I, [_I64, _P, _P, _P]),
+    "ugx_dot_f64_noalloc": (_I, [_I64, _P, _P, _P, _P]),
     "ugx_jacobi_diag_inv_f64": (_I, [_I64, _P, _P, _P, _D, _P, _P]),
     "ugx_jacobi_apply_f64": (_I, [_I64, _P, _P, _P, _P]),
     "ugx_hierarchy_numbering": (_I, [_I, _I, _I, _P, _P]),

@@ -68,13 +68,39 @@ struct SolverH {
     std::shared_ptr<ILinearOperatorInverse> s;
 };
 
+// ---- the allocation-free dot product (ugx_dot_f64_noalloc) -------------------------
+// Its partial sums live in this file-scope device array instead of the context's
+// buffer: one double per workgroup of the reduction grid, which is capped at
+// 65536 workgroups, so 512 KiB per device, held until the process ends. In host
+// code `scratch` names a host-side stand-in, never the device memory: the device
+// address of each device's copy is taken with hipGetSymbolAddress (at ugx_init,
+// or at the entry point's first call on a device) and only that address is used.
+constexpr int kNoAllocPartials = 65536;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};   // the address of `scratch` per device
+
+// the device address of `scratch` on device `dev`, which must be current
+double* scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw UGError("device index out of range for the no-alloc scratch table");
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            throw UGError("hipGetSymbolAddress failed for the no-alloc scratch array");
+        g_scratch_dev[dev] = (double*)p;
+    }
+    return g_scratch_dev[dev];
+}
+
 }  // namespace
 
 extern "C" {
 
 int ugx_init(void) {
     return guard([] {
-        context();
+        auto& c = context();
+        scratch_address(c.device);
         return 0;
     });
 }
@@ -212,6 +238,29 @@ int ugx_dot_f64(int64_t n, const double* a, const double* b, double* result_host
             hipStreamSynchronize(s) != hipSuccess)
             throw UGError("dot readback failed");
         *result_host = c.red_host[0];
+        return 0;
+    });
+}
+int ugx_dot_f64_noalloc(int64_t n, const double* a, const double* b, double* result_dev, void* stream) {
+    return guard([&] {
+        if (n < 0) throw UGError("ugx_dot_f64_noalloc: negative length");
+        if (!result_dev || (n > 0 && (!a || !b))) throw UGError("ugx_dot_f64_noalloc: null pointer");
+        const int np = ugamd::dot_partials(n);
+        if (np > kNoAllocPartials)
+            throw UGError("ugx_dot_f64_noalloc: the reduction needs more partial sums than the 65536 of its scratch");
+        hipStream_t s = (hipStream_t)stream;
+        if (!s) {
+            // the library's stream exists only after the set-up, which allocates
+            DeviceContext* c = context_if_ready();
+            if (!c) throw UGError("ugx_dot_f64_noalloc: pass a stream or call ugx_init() first");
+            s = c->stream;
+        }
+        int dev = -1;
+        if (hipGetDevice(&dev) != hipSuccess) throw UGError("ugx_dot_f64_noalloc: hipGetDevice failed");
+        double* part = scratch_address(dev);
+        // same kernels and grid as ugx_dot_f64: the same result bit for bit
+        if (ugamd::dot<double>(n, a, b, part, result_dev, false, s) != 0)
+            throw UGError("ugx_dot_f64_noalloc: launch failed");
         return 0;
     });
 }

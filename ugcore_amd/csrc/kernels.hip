@@ -2377,6 +2377,8 @@ int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool ta
 }
 template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
 
+int dot_partials(int64_t n) { return red_grid(n); }
+
 int sum_partials(int np, const double* partial, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, np, partial, out, 0);
     UGX_CHECK_LAUNCH();

@@ -35,6 +35,9 @@ int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const 
                     int* bad, hipStream_t s);
 template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s);
+// number of partial sums dot() writes for n entries (one per workgroup of its
+// grid, at most 65536)
+int dot_partials(int64_t n);
 // CG with device scalars sc[0..6] = rho, lambda, alpha, beta, rho_new, |r|^2,
 // breakdown flag
 int sum_partials(int np, const double* partial, double* out, hipStream_t s);
