@@ -58,6 +58,23 @@ def test_boxes_unit_coefficient_and_no_source():
     assert np.all(fg == 0.0)
 
 
+def test_null_load_vector_output():
+    """fe = NULL (documented in the C ABI): the element matrices alone, equal
+    to those of a call that also writes the load vectors"""
+    say("case q2 fe=NULL")
+    u = U()
+    rng = np.random.default_rng(5)
+    n = 70
+    X = np.repeat(CORNERS[None], n, axis=0) + rng.uniform(-0.1, 0.1, (n, 8, 3))
+    k = np.exp(rng.normal(size=n))
+    K_ref, _ = u.q2_hex_element_matrices(X, k, 2.0)
+    dx, dk = u.DeviceArray(np.ascontiguousarray(X)), u.DeviceArray(k)
+    ke = u.DeviceArray(np.zeros((n, 27, 27)))
+    assert u.lib().ugx_q2_hex_element_matrices(n, dx.ptr, dk.ptr, 2.0, ke.ptr, None) == 0
+    u.synchronize()
+    assert np.array_equal(ke.numpy(), K_ref)
+
+
 def test_empty_batch():
     say("case q2 empty batch")
     Kg, fg = U().q2_hex_element_matrices(np.zeros((0, 8, 3)))
@@ -103,7 +120,9 @@ def _oracle_lattice_system(n, theta, f):
     return P.indptr.astype(np.int64), P.indices.astype(np.int64), dense[r_idx, P.indices], b, A
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4])
+# n = 7: several workgroups per colour launch, and odd n leaves colour slots
+# without elements
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7])
 def test_lattice_assembly_matches_oracle_elements(n):
     say(f"case q2 lattice assembly n={n}")
     theta = np.random.default_rng(n).normal(scale=0.5, size=n ** 3)

@@ -52,7 +52,8 @@ def edit_diagonal(A, row, factor):
     """scale A(row, row) through the raw device pointers"""
     u = U()
     rp, col, val = A.download_csr()
-    k = rp[row] + int(np.nonzero(col[rp[row]:rp[row + 1]] == row)[0][0])
+    # Python ints: rp is int32 and a device address does not fit in it
+    k = int(rp[row]) + int(np.nonzero(col[rp[row]:rp[row + 1]] == row)[0][0])
     new = np.array([val[k] * factor])
     _, _, vptr = A.device_csr()
     assert u.lib().ugx_memcpy_htod(C.c_void_p(vptr + 8 * k), new.ctypes.data_as(C.c_void_p), 8) == 0
