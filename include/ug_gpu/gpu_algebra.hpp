@@ -24,6 +24,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace ug {
@@ -116,6 +117,43 @@ class GPUSparseMatrix {
     int64_t num_rows() const { return rows_; }
     int64_t num_cols() const { return cols_; }
     int64_t total_num_connections() const { return nnz_; }
+
+    // ---- pattern creation on the host (sparsematrix_impl.h:90-145, 553-700) ----
+    // resize_and_clear starts a host-side build: operator()(r, c) inserts the
+    // entry (value 0) on first access and keeps every row sorted by column, as
+    // SparseMatrix does; finalize() moves the rows into the device CSR, after
+    // which the device arrays are the truth and operator() throws (values are
+    // then written by the device assembly or the device operations below).
+    void resize_and_clear(int64_t rows, int64_t cols);
+    // keeps the entries (downloading a device CSR into a host build); entries
+    // in columns >= cols are dropped, rows beyond `rows` are removed or added
+    void resize_and_keep_values(int64_t rows, int64_t cols);
+    // all values 0, pattern kept (host build or device)
+    void clear_retain_structure();
+    bool building() const { return building_; }
+    double& operator()(int64_t r, int64_t c);
+    // the stored value, 0 for an entry that is not stored (host build or device)
+    double get(int64_t r, int64_t c) const;
+    bool has_connection(int64_t r, int64_t c) const;
+    // row access (begin_row/end_row): number of stored entries of row r and a
+    // host copy of its columns and values (device CSR: downloaded)
+    int64_t num_connections(int64_t r) const;
+    void row(int64_t r, int32_t* cols, double* vals) const;
+
+    // ---- device operations (sparsematrix_impl.h:148-240, 341-370) ----
+    // this = scale * B^T keeping every stored entry of B, explicit zeros
+    // included (set_as_transpose_of) or dropping zero entries
+    // (set_as_transpose_of2); rows sorted by column (= source row)
+    void set_as_transpose_of(const GPUSparseMatrix& B, double scale = 1.0) { transpose_from(B, scale, false); }
+    void set_as_transpose_of2(const GPUSparseMatrix& B, double scale = 1.0) { transpose_from(B, scale, true); }
+    void set_as_copy_of(const GPUSparseMatrix& B, double scale = 1.0);
+    void scale(double a);
+    // dest = a1*v1 + b1*A^T*w1 (v1 may alias dest; w1 must not)
+    void axpy_transposed(GPUVector& dest, double a1, const GPUVector& v1, double b1, const GPUVector& w1) const;
+    void apply_transposed(GPUVector& dest, const GPUVector& w) const { axpy_transposed(dest, 0.0, dest, 1.0, w); }
+    // SetDirichletRow (sparsematrix_util.h:850-861) for every listed row: all
+    // entries 0, the diagonal 1; every listed row must store its diagonal
+    void set_dirichlet_rows(const int32_t* rows_host, int64_t n);
     // create from host CSR (row pointers int32, columns sorted per row)
     void set_from_host_csr(int64_t rows, int64_t cols, const int32_t* rowptr, const int32_t* col, const double* val);
     // allocate the device pattern (values zero)
@@ -129,17 +167,17 @@ class GPUSparseMatrix {
     int32_t* rowptr() const { return rowptr_.get(); }
     int32_t* col() const { return col_.get(); }
     double* val() const { return val_.get(); }
-    // the values were changed through the raw CSR pointers: drops the
-    // provenance (operators derived from an assembly, such as the matrix-free
-    // and lattice-order paths of the multigrid, no longer apply to this
-    // matrix) and rebuilds the SpMV layout from the CSR
-    void finalize() {
-        set_origin(nullptr);
-        build_spmv_layout();
-    }
+    // ends a host build (resize_and_clear + operator()): the sorted rows become
+    // the device CSR (SparseMatrix::defragment, sparsematrix_impl.h:703-757).
+    // Otherwise: the values were changed through the raw CSR pointers, so the
+    // provenance is dropped (operators derived from an assembly, such as the
+    // matrix-free and lattice-order paths of the multigrid, no longer apply to
+    // this matrix) and the SpMV layout is rebuilt from the CSR
+    void finalize();
     // write access to the CSR values is handed out: the provenance is dropped
     // and SpMV reads the CSR itself until finalize() rebuilds the layout
     void values_edited_externally() {
+        require_device("device_csr");
         set_origin(nullptr);
         sell_valid_ = false;
     }
@@ -173,8 +211,22 @@ class GPUSparseMatrix {
     }
 
    private:
+    void require_device(const char* what) const;
+    void transpose_from(const GPUSparseMatrix& B, double scale, bool drop_zeros);
+    // the values changed on the device: derived operators and the SpMV layout
+    // no longer apply
+    void values_changed_on_device();
     int64_t rows_ = 0, cols_ = 0, nnz_ = 0;
     int lanes_ = 8;
+    // host build (pattern creation): one sorted (column, value) list per row
+    bool building_ = false;
+    std::vector<std::vector<std::pair<int32_t, double>>> hrows_;
+    // transpose of the current values without zero entries (axpy_transposed),
+    // valid while t_version_ == version_
+    mutable uint64_t t_version_ = ~(uint64_t)0;
+    mutable int64_t t_nnz_ = 0;
+    mutable DeviceBuffer<int32_t> t_rowptr_, t_col_;
+    mutable DeviceBuffer<double> t_val_;
     DeviceBuffer<int32_t> rowptr_, col_;
     DeviceBuffer<double> val_;
     bool sell_enabled_ = true, sell_valid_ = false;
@@ -338,6 +390,68 @@ class BiCGStab : public ILinearOperatorInverse {
     double min_ortho_ = 0.0;
 };
 
+// Element-loop assembly from caller-supplied connectivity: the element loop of
+// DomainDiscretization::assemble_linear / assemble_jacobian / assemble_defect
+// (domain_disc_impl.h:469-948; elem_disc_assemble_util.h:326-413, 577-687,
+// 902-1003) over one element type, with the element -> DoF table that
+// DoFDistribution::indices gives per element (dof_distribution.cpp:1117-1206)
+// and the DoF coordinates. Element types: P1 triangles with FV1 (box method),
+// P1 tetrahedra with FV1 and optional full-upwind convection, Q1 hexahedra with
+// FE (2x2x2 Gauss); -div(k grad u) (+ div(v u)) = f, k per element, f constant
+// or per DoF (hexahedra: constant), homogeneous Dirichlet values on the listed
+// DoFs. The CSR pattern (every element coupling, rows sorted), the CSR slot of
+// every local entry and an element colouring are built once on the device;
+// every assembly is one launch per colour without atomics (elem_assembly.hip).
+class ElementDiscretization {
+   public:
+    enum ElemType { TRIANGLE_FV1 = 0, TETRAHEDRON_FV1 = 1, HEXAHEDRON_Q1 = 2 };
+    static int vertices_of(int elem_type) { return elem_type == 0 ? 3 : (elem_type == 1 ? 4 : 8); }
+    static int dim_of(int elem_type) { return elem_type == 0 ? 2 : 3; }
+    // elem_dofs: num_elems x vertices_of(type) DoFs in the reference element's
+    // local vertex order; dof_coords: num_dofs x dim_of(type). on_device: the
+    // two arrays are device pointers (copied), otherwise host arrays.
+    ElementDiscretization(int elem_type, int64_t num_elems, const int32_t* elem_dofs, int64_t num_dofs,
+                          const double* dof_coords, bool on_device = false);
+    int elem_type() const { return type_; }
+    int64_t num_elems() const { return ne_; }
+    int64_t num_dofs() const { return ndofs_; }
+    int64_t nnz() const { return nnz_; }
+    int num_colours() const { return (int)colour_start_.size() - 1; }
+    // diffusion k per element (host or device array of num_elems; nullptr: 1)
+    void set_diffusion(const double* k, bool on_device = false);
+    void set_source(double f) { f_const_ = f; has_f_dof_ = false; }
+    void set_source_dofs(const double* f_host);          // FV1 only
+    void set_velocity(double vx, double vy, double vz);  // tetrahedra only
+    // homogeneous Dirichlet DoFs (value 0): rows become identity rows
+    // (SetDirichletRow), rhs and defect entries 0
+    void set_dirichlet(const int32_t* dofs_host, int64_t n);
+    void set_dirichlet_device(const int32_t* dofs_dev, int64_t n);
+    // A and b (A is given this discretisation's pattern on its first use)
+    void assemble_linear(GPUSparseMatrix& A, GPUVector& b);
+    // J = dd/du; the element discretisations here are linear in u, so J is the
+    // stiffness matrix of assemble_linear
+    void assemble_jacobian(GPUSparseMatrix& J, const GPUVector& u);
+    // d = sum_e (A_e u_e - f_e), Dirichlet entries 0
+    void assemble_defect(GPUVector& d, const GPUVector& u);
+    // a copy of the pattern (rowptr ndofs + 1, col nnz), host arrays
+    void download_pattern(int32_t* rowptr, int32_t* col) const;
+
+   private:
+    void build();
+    void run(double* val, double* rhs, const double* u, double* d);
+    void give_pattern(GPUSparseMatrix& A);
+    int type_, nv_, dim_;
+    int64_t ne_, ndofs_, nnz_ = 0;
+    DeviceBuffer<int32_t> dofs_, rowptr_, col_, order_, slot_, dir_;
+    DeviceBuffer<double> coords_, kel_, f_dof_;
+    std::vector<int64_t> colour_start_;
+    double f_const_ = 0.0;
+    bool has_k_ = false, has_f_dof_ = false;
+    double vel_[3] = {0.0, 0.0, 0.0};
+    bool conv_ = false;
+    int64_t ndir_ = 0;
+};
+
 // ---------------------------------------------------------------------------
 // structured discretisation + multigrid
 // ---------------------------------------------------------------------------
@@ -392,6 +506,16 @@ class DomainDiscretization {
     void set_source_dofs(const double* f_host);   // FV1: per top-level DoF
     // assemble the top-level (surface) CSR in DoF numbering and the rhs
     void assemble_linear(GPUSparseMatrix& A, GPUVector& b);
+    // J = dd/du (domain_disc_impl.h:469-600): the discretisations here are
+    // linear in u, so J is the matrix of assemble_linear (Dirichlet rows set)
+    void assemble_jacobian(GPUSparseMatrix& J, const GPUVector& u);
+    // d = sum_e (A_e u_e - f_e) with the Dirichlet entries 0
+    // (domain_disc_impl.h:646-788), by the element loop over the top level's
+    // connectivity (ElementDiscretization built on the device)
+    void assemble_defect(GPUVector& d, const GPUVector& u);
+    // the element-loop discretisation of the top level (built on first use;
+    // rebuilt when theta, the source, velocity or jitter change)
+    ElementDiscretization& element_discretization();
     // assemble a level operator in lattice numbering as DIA values (27 or 9
     // diagonals of length N, FP64), Dirichlet rows set
     void assemble_level_dia(int lev, DeviceBuffer<double>& dia, int& nd, std::vector<int32_t>& offsets);
@@ -427,6 +551,12 @@ class DomainDiscretization {
 
    private:
     void build_host_levels(int kind, int num_refs, int ordering);
+    int dim_of_kind() const { return kind_ == 1 ? 2 : 3; }
+    // the element-loop discretisation of the top level and the jitter its
+    // coordinates were made with
+    std::unique_ptr<ElementDiscretization> elem_;
+    uint64_t elem_jseed_ = 0;
+    double elem_jamp_ = -1.0;
     int kind_;
     int refs_;
     std::vector<std::unique_ptr<LevelGrid>> levels_;

@@ -81,7 +81,6 @@ int ugx_norm_f64(int64_t n, const double* a, double* result_host, void* stream);
  * device shares that device's array: calls that overlap on one device race. */
 int ugx_dot_f64_noalloc(int64_t n, const double* a, const double* b, double* result_dev, void* stream);
 
-
 /* ---- kernel entry points: Jacobi (lib_algebra/operator/preconditioner/jacobi.h:155-249,
  * replacing GPUJacobi::preprocess + CUDA_JacobiApply, gpujacobi.h:113-144, common_cuda.cu:46-52) */
 /* dinv_i = 1/(a_ii*(1/omega)); returns -2 if a diagonal entry is zero */
@@ -199,6 +198,48 @@ int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs);
 double ugx_disc_assemble_seconds(void* disc);
 
 /* GPUSparseMatrix (matrix_type) wrapped as MatrixOperator (matrix_operator.h:47) */
+/* DomainDiscretization::assemble_jacobian (domain_disc_impl.h:469-600): the
+ * discretisations here are linear in u, so J is the matrix of
+ * ugx_disc_assemble_linear with its Dirichlet rows */
+int ugx_disc_assemble_jacobian(void* disc, void* matrix, void* u);
+/* DomainDiscretization::assemble_defect (domain_disc_impl.h:646-788;
+ * AssembleDefect, elem_disc_assemble_util.h:577-687): d = sum_e (A_e u_e - f_e)
+ * by the element loop over the top level, Dirichlet entries 0
+ * (DirichletBoundary::adjust_defect, lagrange_dirichlet_boundary_impl.h:920-1035) */
+int ugx_disc_assemble_defect(void* disc, void* d, void* u);
+
+/* ---- element-loop assembly from caller-supplied connectivity -------------
+ * The element loop of DomainDiscretization::assemble_linear / _jacobian /
+ * _defect (domain_disc_impl.h:469-948; StdGlobAssembler,
+ * elem_disc_assemble_util.h:326-413, 577-687, 902-1003) over one element type.
+ * elem_dofs: num_elems x nv DoF indices per element in the reference element's
+ * local vertex order (what DoFDistribution::indices gives per element,
+ * dof_distribution.cpp:1117-1206); dof_coords: num_dofs x dim coordinates.
+ * elem_type 0: P1 triangles, FV1 (nv 3, dim 2); 1: P1 tetrahedra, FV1 with
+ * optional full-upwind convection (nv 4, dim 3); 2: Q1 hexahedra, FE with
+ * 2x2x2 Gauss (nv 8, dim 3). Host arrays. num_elems * nv^2 < 2^31.
+ * The CSR pattern holds every element coupling (rows sorted by column, zero
+ * couplings kept, as AddLocalMatrixToGlobal's insertion, local_algebra.h:766-789);
+ * it, the CSR slot of every local entry and an element colouring are built
+ * once on the device here; each assembly is one atomic-free launch per colour. */
+void* ugx_elemdisc_create(int elem_type, int64_t num_elems, const int32_t* elem_dofs, int64_t num_dofs,
+                          const double* dof_coords);
+void ugx_elemdisc_destroy(void* e);
+int64_t ugx_elemdisc_nnz(void* e);
+int ugx_elemdisc_num_colours(void* e);
+/* diffusion k per element (host array of num_elems; NULL: 1) */
+int ugx_elemdisc_set_diffusion(void* e, const double* k_host);
+/* source f: a constant, or per DoF (FV1 types); FV1 takes f |scv|, Q1 f int phi */
+int ugx_elemdisc_set_source_const(void* e, double f);
+int ugx_elemdisc_set_source_dofs(void* e, const double* f_host);
+/* convection velocity (tetrahedra only; full upwind) */
+int ugx_elemdisc_set_velocity(void* e, double vx, double vy, double vz);
+/* homogeneous Dirichlet DoFs: identity rows (SetDirichletRow), rhs and defect 0 */
+int ugx_elemdisc_set_dirichlet(void* e, const int32_t* dofs_host, int64_t n);
+int ugx_elemdisc_assemble_linear(void* e, void* matrix, void* rhs);
+int ugx_elemdisc_assemble_jacobian(void* e, void* matrix, void* u);
+int ugx_elemdisc_assemble_defect(void* e, void* d, void* u);
+
 void* ugx_matrix_create(void);
 void ugx_matrix_destroy(void* m);
 int64_t ugx_matrix_num_rows(void* m);
@@ -230,6 +271,54 @@ int ugx_matrix_has_row_order(void* m);
 /* after editing values through ugx_matrix_device_csr: rebuilds the SpMV layout */
 int ugx_matrix_finalize(void* m);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */
+
+/* ---- the SparseMatrix surface of the reference's callers
+ * (lib_algebra/cpu_algebra/sparsematrix.h:98-752, sparsematrix_impl.h) ---- */
+int64_t ugx_matrix_num_cols(void* m);
+/* the device CSR for reading only (the provenance is kept, unlike
+ * ugx_matrix_device_csr): for kernels of callers that read the matrix */
+int ugx_matrix_device_csr_readonly(void* m, const int32_t** rowptr, const int32_t** col, const double** val);
+/* dest[i] = beta*(A w)[i] for rows with entries, other rows untouched
+ * (apply_ignore_zero_rows, sparsematrix_impl.h:271-288) */
+int ugx_matrix_apply_ignore_zero_rows(void* m, void* dest, double beta, void* w);
+/* resize_and_clear (sparsematrix_impl.h:90-109): drops the device CSR and
+ * starts a host-side pattern build; entries are created by ugx_matrix_entry
+ * and moved to the device by ugx_matrix_finalize (defragment,
+ * sparsematrix_impl.h:703-757). Sizes below 2^31. */
+int ugx_matrix_resize_and_clear(void* m, int64_t rows, int64_t cols);
+/* resize_and_keep_values (sparsematrix_impl.h:111-133): a host build that
+ * keeps the entries (a device CSR is downloaded into it), dropping columns >= cols */
+int ugx_matrix_resize_and_keep_values(void* m, int64_t rows, int64_t cols);
+/* clear_retain_structure (sparsematrix_impl.h:136-140): values 0, pattern kept */
+int ugx_matrix_clear_retain_structure(void* m);
+/* 1 between resize_and_clear and finalize */
+int ugx_matrix_is_building(void* m);
+/* operator()(r, c) (sparsematrix.h:293-300; get_index, sparsematrix_impl.h:553-700):
+ * the address of the entry's host-side value, the entry created with value 0
+ * on first access (rows stay sorted by column); valid until the next
+ * insertion. Host build only: NULL once the matrix is finalised on the device. */
+double* ugx_matrix_entry(void* m, int64_t r, int64_t c);
+/* const operator()(r, c): *value = the stored value or 0; returns 1 if the
+ * entry is stored, 0 if not (host build or device CSR) */
+int ugx_matrix_get(void* m, int64_t r, int64_t c, double* value);
+/* begin_row/end_row (sparsematrix.h:484-487): the number of stored entries of
+ * row r, and a host copy of its columns and values (either pointer may be NULL) */
+int64_t ugx_matrix_row_length(void* m, int64_t r);
+int ugx_matrix_row(void* m, int64_t r, int32_t* cols, double* vals);
+/* m = scale * src^T on the device: keep_zeros 1 keeps every stored entry, as
+ * set_as_transpose_of (sparsematrix_impl.h:148-183); 0 drops zero entries, as
+ * set_as_transpose_of2 (:185-240). Rows sorted by column. */
+int ugx_matrix_set_as_transpose_of(void* m, void* src, double scale, int keep_zeros);
+/* set_as_copy_of (sparsematrix_impl.h:471-481) and scale (sparsematrix_impl.h:486-496) */
+int ugx_matrix_set_as_copy_of(void* m, void* src, double scale);
+int ugx_matrix_scale(void* m, double a);
+/* dest = alpha*v1 + beta*A^T*w (axpy_transposed, sparsematrix_impl.h:341-370):
+ * per column in ascending row, zero entries skipped; v1 may be dest */
+int ugx_matrix_axpy_transposed(void* m, void* dest, double alpha, void* v1, double beta, void* w);
+/* SetDirichletRow (algebra_common/sparsematrix_util.h:850-861) for every
+ * listed row: entries 0, diagonal 1. On the device every listed row must
+ * store its diagonal (the pattern is fixed there). */
+int ugx_matrix_set_dirichlet_rows(void* m, const int32_t* rows_host, int64_t n);
 
 /* GPUVector (vector_type) */
 void* ugx_vector_create(int64_t n);

@@ -39,3 +39,26 @@ def test_host_numbering_error_path():
     rc = U.lib().ugx_hierarchy_numbering(0, 3, 7, out.ctypes.data, None)
     assert rc < 0
     assert b"out of range" in U.lib().ugx_last_error()
+
+
+def test_q2_argument_validation_before_any_device_work():
+    """short coefficient arrays and lattice sizes whose pattern would not fit
+    int32 are refused before a device buffer is made (no GPU needed)"""
+    import pytest
+    X = np.zeros((3, 8, 3))
+    with pytest.raises(ValueError, match="one coefficient per element"):
+        U.q2_hex_element_matrices(X, np.ones(2))
+    with pytest.raises(ValueError, match="one coefficient per element"):
+        U.q2_hex_element_matrices(X, 1.0)
+    with pytest.raises(ValueError, match="shape"):
+        U.q2_hex_element_matrices(np.zeros((3, 4, 3)))
+    with pytest.raises(ValueError, match="one value per element"):
+        U.q2_assemble_lattice(3, np.zeros(26))
+    for n in (0, U.Q2_LATTICE_MAX_N + 1, 600):
+        with pytest.raises(ValueError, match="int32"):
+            U.q2_assemble_lattice(n)
+    # the C entry point itself refuses n = 162 ((8n+1)^3 > 2^31 - 1) up front
+    assert (8 * 161 + 1) ** 3 < 2 ** 31 <= (8 * 162 + 1) ** 3
+    assert U.lib().ugx_q2_assemble_lattice(162, None, 1.0, None, None) < 0
+    assert b"1..161" in U.lib().ugx_last_error()
+    assert U.lib().ugx_q2_hex_element_matrices(-1, None, None, 0.0, None, None) < 0

@@ -87,6 +87,20 @@ _SIGNATURES = {
     "ugx_disc_set_source_dofs": (_I, [_P, _P]),
     "ugx_disc_assemble_linear": (_I, [_P, _P, _P]),
     "ugx_disc_assemble_seconds": (_D, [_P]),
+    "ugx_disc_assemble_jacobian": (_I, [_P, _P, _P]),
+    "ugx_disc_assemble_defect": (_I, [_P, _P, _P]),
+    "ugx_elemdisc_create": (_P, [_I, _I64, _P, _I64, _P]),
+    "ugx_elemdisc_destroy": (None, [_P]),
+    "ugx_elemdisc_nnz": (_I64, [_P]),
+    "ugx_elemdisc_num_colours": (_I, [_P]),
+    "ugx_elemdisc_set_diffusion": (_I, [_P, _P]),
+    "ugx_elemdisc_set_source_const": (_I, [_P, _D]),
+    "ugx_elemdisc_set_source_dofs": (_I, [_P, _P]),
+    "ugx_elemdisc_set_velocity": (_I, [_P, _D, _D, _D]),
+    "ugx_elemdisc_set_dirichlet": (_I, [_P, _P, _I64]),
+    "ugx_elemdisc_assemble_linear": (_I, [_P, _P, _P]),
+    "ugx_elemdisc_assemble_jacobian": (_I, [_P, _P, _P]),
+    "ugx_elemdisc_assemble_defect": (_I, [_P, _P, _P]),
     "ugx_matrix_create": (_P, []),
     "ugx_matrix_destroy": (None, [_P]),
     "ugx_matrix_num_rows": (_I64, [_P]),
@@ -101,6 +115,22 @@ _SIGNATURES = {
     "ugx_matrix_has_row_order": (_I, [_P]),
     "ugx_matrix_finalize": (_I, [_P]),
     "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
+    "ugx_matrix_num_cols": (_I64, [_P]),
+    "ugx_matrix_device_csr_readonly": (_I, [_P, _P, _P, _P]),
+    "ugx_matrix_apply_ignore_zero_rows": (_I, [_P, _P, _D, _P]),
+    "ugx_matrix_resize_and_clear": (_I, [_P, _I64, _I64]),
+    "ugx_matrix_resize_and_keep_values": (_I, [_P, _I64, _I64]),
+    "ugx_matrix_clear_retain_structure": (_I, [_P]),
+    "ugx_matrix_is_building": (_I, [_P]),
+    "ugx_matrix_entry": (C.POINTER(C.c_double), [_P, _I64, _I64]),
+    "ugx_matrix_get": (_I, [_P, _I64, _I64, _P]),
+    "ugx_matrix_row_length": (_I64, [_P, _I64]),
+    "ugx_matrix_row": (_I, [_P, _I64, _P, _P]),
+    "ugx_matrix_set_as_transpose_of": (_I, [_P, _P, _D, _I]),
+    "ugx_matrix_set_as_copy_of": (_I, [_P, _P, _D]),
+    "ugx_matrix_scale": (_I, [_P, _D]),
+    "ugx_matrix_axpy_transposed": (_I, [_P, _P, _D, _P, _D, _P]),
+    "ugx_matrix_set_dirichlet_rows": (_I, [_P, _P, _I64]),
     "ugx_vector_create": (_P, [_I64]),
     "ugx_vector_destroy": (None, [_P]),
     "ugx_vector_size": (_I64, [_P]),
@@ -316,8 +346,94 @@ class SparseMatrix:
 
     def device_csr(self):
         rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
-        lib().ugx_matrix_device_csr(self.h, C.byref(rp), C.byref(col), C.byref(val))
+        _check(lib().ugx_matrix_device_csr(self.h, C.byref(rp), C.byref(col), C.byref(val)),
+               "SparseMatrix.device_csr")
         return rp.value, col.value, val.value
+
+    # ---- the SparseMatrix surface (sparsematrix.h / sparsematrix_impl.h) ----
+    def num_cols(self):
+        return lib().ugx_matrix_num_cols(self.h)
+
+    def resize_and_clear(self, rows, cols):
+        """start a host-side pattern build (SparseMatrix::resize_and_clear)"""
+        _check(lib().ugx_matrix_resize_and_clear(self.h, int(rows), int(cols)), "SparseMatrix.resize_and_clear")
+
+    def resize_and_keep_values(self, rows, cols):
+        _check(lib().ugx_matrix_resize_and_keep_values(self.h, int(rows), int(cols)),
+               "SparseMatrix.resize_and_keep_values")
+
+    def clear_retain_structure(self):
+        _check(lib().ugx_matrix_clear_retain_structure(self.h), "SparseMatrix.clear_retain_structure")
+
+    def building(self):
+        return bool(lib().ugx_matrix_is_building(self.h))
+
+    def _entry(self, r, c):
+        p = lib().ugx_matrix_entry(self.h, int(r), int(c))
+        if not p:
+            raise UGError(f"SparseMatrix({r}, {c}): {lib().ugx_last_error().decode()}")
+        return p
+
+    def __setitem__(self, rc, value):
+        """A[r, c] = v: operator()(r, c) = v (creates the entry; host build only)"""
+        self._entry(*rc)[0] = float(value)
+
+    def add(self, r, c, value):
+        """A(r, c) += v (creates the entry with 0 first; host build only)"""
+        p = self._entry(r, c)
+        p[0] = p[0] + float(value)
+
+    def __getitem__(self, rc):
+        """const operator()(r, c): the stored value or 0"""
+        v = C.c_double()
+        _check(lib().ugx_matrix_get(self.h, int(rc[0]), int(rc[1]), C.byref(v)), "SparseMatrix.get")
+        return v.value
+
+    def has_connection(self, r, c):
+        v = C.c_double()
+        return _check(lib().ugx_matrix_get(self.h, int(r), int(c), C.byref(v)), "SparseMatrix.get") == 1
+
+    def row(self, r):
+        """(columns, values) of row r (begin_row/end_row)"""
+        n = lib().ugx_matrix_row_length(self.h, int(r))
+        if n < 0:
+            raise UGError(f"SparseMatrix.row: {lib().ugx_last_error().decode()}")
+        cols = np.empty(n, np.int32)
+        vals = np.empty(n, np.float64)
+        _check(lib().ugx_matrix_row(self.h, int(r), _ptr(cols), _ptr(vals)), "SparseMatrix.row")
+        return cols, vals
+
+    def finalize(self):
+        """end a host build (the rows become the device CSR), or rebuild the
+        SpMV layout after raw edits through device_csr()"""
+        _check(lib().ugx_matrix_finalize(self.h), "SparseMatrix.finalize")
+
+    def set_as_transpose_of(self, B, scale=1.0):
+        """scale * B^T keeping explicit zeros (set_as_transpose_of)"""
+        _check(lib().ugx_matrix_set_as_transpose_of(self.h, B.h, float(scale), 1), "SparseMatrix.set_as_transpose_of")
+
+    def set_as_transpose_of2(self, B, scale=1.0):
+        """scale * B^T without zero entries (set_as_transpose_of2)"""
+        _check(lib().ugx_matrix_set_as_transpose_of(self.h, B.h, float(scale), 0), "SparseMatrix.set_as_transpose_of2")
+
+    def set_as_copy_of(self, B, scale=1.0):
+        _check(lib().ugx_matrix_set_as_copy_of(self.h, B.h, float(scale)), "SparseMatrix.set_as_copy_of")
+
+    def scale(self, a):
+        _check(lib().ugx_matrix_scale(self.h, float(a)), "SparseMatrix.scale")
+
+    def axpy_transposed(self, dest, alpha, v1, beta, w):
+        """dest = alpha*v1 + beta*A^T*w"""
+        _check(lib().ugx_matrix_axpy_transposed(self.h, dest.h, float(alpha), v1.h, float(beta), w.h),
+               "SparseMatrix.axpy_transposed")
+
+    def apply_transposed(self, dest, w):
+        self.axpy_transposed(dest, 0.0, dest, 1.0, w)
+
+    def set_dirichlet_rows(self, rows):
+        """SetDirichletRow for every listed row"""
+        r = np.ascontiguousarray(rows, np.int32)
+        _check(lib().ugx_matrix_set_dirichlet_rows(self.h, _ptr(r), r.size), "SparseMatrix.set_dirichlet_rows")
 
     def axpy(self, dest, alpha, v1, beta, w):
         _check(lib().ugx_matrix_axpy(self.h, dest.h, float(alpha), v1.h, float(beta), w.h), "SparseMatrix.axpy")
@@ -441,6 +557,14 @@ class DomainDiscretization:
 
     def assemble_linear(self, A, b):
         _check(lib().ugx_disc_assemble_linear(self.h, A.h, b.h), "assemble_linear")
+
+    def assemble_jacobian(self, J, u):
+        """J = dd/du at u (the discretisations are linear: the stiffness matrix)"""
+        _check(lib().ugx_disc_assemble_jacobian(self.h, J.h, u.h), "assemble_jacobian")
+
+    def assemble_defect(self, d, u):
+        """d = sum_e (A_e u_e - f_e), Dirichlet entries 0 (element loop)"""
+        _check(lib().ugx_disc_assemble_defect(self.h, d.h, u.h), "assemble_defect")
 
     def assemble_seconds(self):
         return lib().ugx_disc_assemble_seconds(self.h)
@@ -664,6 +788,75 @@ def SaveVectorForConnectionViewer(disc, v, filename):
     """a surface vector with the top-level DoF positions
     (SaveVectorForConnectionViewer, grid_function_util.h)"""
     write_connection_viewer_vector(filename, v.numpy(), disc.dof_positions())
+
+
+class ElementDiscretization:
+    """Element-loop assembly from caller-supplied connectivity
+    (ugx_elemdisc_*): elem_type "triangle_fv1", "tetrahedron_fv1" or
+    "hexahedron_q1"; elem_dofs [num_elems, nv] in the reference element's local
+    vertex order; dof_coords [num_dofs, dim]. The pattern, the slot of every
+    local entry and the element colouring are built on the device here."""
+
+    TYPES = {"triangle_fv1": (0, 3, 2), "tetrahedron_fv1": (1, 4, 3), "hexahedron_q1": (2, 8, 3)}
+
+    def __init__(self, elem_type, elem_dofs, dof_coords):
+        if elem_type not in self.TYPES:
+            raise ValueError(f"elem_type must be one of {sorted(self.TYPES)}")
+        t, nv, dim = self.TYPES[elem_type]
+        E = np.ascontiguousarray(elem_dofs, np.int32)
+        X = np.ascontiguousarray(dof_coords, np.float64)
+        if E.ndim != 2 or E.shape[1] != nv:
+            raise ValueError(f"elem_dofs must have shape [num_elems, {nv}]")
+        if X.ndim != 2 or X.shape[1] != dim:
+            raise ValueError(f"dof_coords must have shape [num_dofs, {dim}]")
+        self.num_elems, self.num_dofs = E.shape[0], X.shape[0]
+        self.h = _handle(lib().ugx_elemdisc_create(t, E.shape[0], _ptr(E), X.shape[0], _ptr(X)),
+                         "ElementDiscretization")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_elemdisc_destroy(self.h)
+            self.h = None
+
+    def nnz(self):
+        return lib().ugx_elemdisc_nnz(self.h)
+
+    def num_colours(self):
+        return lib().ugx_elemdisc_num_colours(self.h)
+
+    def set_diffusion(self, k):
+        if k is None:
+            _check(lib().ugx_elemdisc_set_diffusion(self.h, None), "set_diffusion")
+            return
+        k = np.ascontiguousarray(k, np.float64)
+        if k.size != self.num_elems:
+            raise ValueError(f"one diffusion value per element ({self.num_elems}) expected, got {k.size}")
+        _check(lib().ugx_elemdisc_set_diffusion(self.h, _ptr(k)), "set_diffusion")
+
+    def set_source(self, f):
+        if np.isscalar(f):
+            _check(lib().ugx_elemdisc_set_source_const(self.h, float(f)), "set_source")
+        else:
+            f = np.ascontiguousarray(f, np.float64)
+            if f.size != self.num_dofs:
+                raise ValueError(f"one source value per DoF ({self.num_dofs}) expected, got {f.size}")
+            _check(lib().ugx_elemdisc_set_source_dofs(self.h, _ptr(f)), "set_source")
+
+    def set_velocity(self, v):
+        _check(lib().ugx_elemdisc_set_velocity(self.h, float(v[0]), float(v[1]), float(v[2])), "set_velocity")
+
+    def set_dirichlet(self, dofs):
+        d = np.ascontiguousarray(dofs, np.int32)
+        _check(lib().ugx_elemdisc_set_dirichlet(self.h, _ptr(d), d.size), "set_dirichlet")
+
+    def assemble_linear(self, A, b):
+        _check(lib().ugx_elemdisc_assemble_linear(self.h, A.h, b.h), "assemble_linear")
+
+    def assemble_jacobian(self, J, u):
+        _check(lib().ugx_elemdisc_assemble_jacobian(self.h, J.h, u.h), "assemble_jacobian")
+
+    def assemble_defect(self, d, u):
+        _check(lib().ugx_elemdisc_assemble_defect(self.h, d.h, u.h), "assemble_defect")
 
 
 def q2_hex_element_matrices(corners, k=None, f_const=0.0):

@@ -13,7 +13,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libugcore_amd.so")
 
-SOURCES = ["kernels.hip", "tet_kernels.hip", "rap_kernels.hip", "mesh_kernels.hip", "q2_kernels.hip", "hierarchy.cpp", "gpu_algebra.cpp", "io.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "tet_kernels.hip", "rap_kernels.hip", "mesh_kernels.hip", "q2_kernels.hip", "matrix_kernels.hip", "elem_assembly.hip", "hierarchy.cpp", "gpu_algebra.cpp", "io.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("UGX_OFFLOAD_ARCH", "gfx950")
 
@@ -48,7 +48,28 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_ug4py(verbose)
     return LIB
+
+
+def build_ug4py(verbose=True):
+    """the pybind11 module ug4py (host code over the C ABI, linked against
+    libugcore_amd.so in the same directory)"""
+    import sysconfig
+    import pybind11
+    out = os.path.join(PKG, "ug4py" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-fvisibility=hidden",
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "ug4py.cpp"),
+           "-L", PKG, "-l:libugcore_amd.so", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ug4py build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -58,6 +58,9 @@ struct VectorH {
 struct DiscH {
     std::shared_ptr<DomainDiscretization> d;
 };
+struct ElemH {
+    std::unique_ptr<ElementDiscretization> e;
+};
 struct GmgH {
     std::shared_ptr<AssembledMultiGridCycle> g;
 };
@@ -493,6 +496,85 @@ int ugx_disc_assemble_linear(void* disc, void* matrix, void* rhs) {
     });
 }
 double ugx_disc_assemble_seconds(void* disc) { return ((DiscH*)disc)->d->assemble_seconds(); }
+int ugx_disc_assemble_jacobian(void* disc, void* matrix, void* u) {
+    return guard([&] {
+        ((DiscH*)disc)->d->assemble_jacobian(*((MatrixH*)matrix)->A, ((VectorH*)u)->v);
+        return 0;
+    });
+}
+int ugx_disc_assemble_defect(void* disc, void* d, void* u) {
+    return guard([&] {
+        ((DiscH*)disc)->d->assemble_defect(((VectorH*)d)->v, ((VectorH*)u)->v);
+        return 0;
+    });
+}
+
+// ---- element-loop assembly from connectivity -------------------------------------------
+
+void* ugx_elemdisc_create(int elem_type, int64_t num_elems, const int32_t* elem_dofs, int64_t num_dofs,
+                          const double* dof_coords) {
+    return guard_ptr([&] {
+        auto* h = new ElemH();
+        try {
+            h->e.reset(new ElementDiscretization(elem_type, num_elems, elem_dofs, num_dofs, dof_coords, false));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return (void*)h;
+    });
+}
+void ugx_elemdisc_destroy(void* e) { delete (ElemH*)e; }
+int64_t ugx_elemdisc_nnz(void* e) { return ((ElemH*)e)->e->nnz(); }
+int ugx_elemdisc_num_colours(void* e) { return ((ElemH*)e)->e->num_colours(); }
+int ugx_elemdisc_set_diffusion(void* e, const double* k) {
+    return guard([&] {
+        ((ElemH*)e)->e->set_diffusion(k, false);
+        return 0;
+    });
+}
+int ugx_elemdisc_set_source_const(void* e, double f) {
+    ((ElemH*)e)->e->set_source(f);
+    return 0;
+}
+int ugx_elemdisc_set_source_dofs(void* e, const double* f) {
+    return guard([&] {
+        if (!f) throw UGError("ugx_elemdisc_set_source_dofs: null source");
+        ((ElemH*)e)->e->set_source_dofs(f);
+        return 0;
+    });
+}
+int ugx_elemdisc_set_velocity(void* e, double vx, double vy, double vz) {
+    return guard([&] {
+        ((ElemH*)e)->e->set_velocity(vx, vy, vz);
+        return 0;
+    });
+}
+int ugx_elemdisc_set_dirichlet(void* e, const int32_t* dofs, int64_t n) {
+    return guard([&] {
+        if (n < 0 || (n > 0 && !dofs)) throw UGError("ugx_elemdisc_set_dirichlet: bad list");
+        ((ElemH*)e)->e->set_dirichlet(dofs, n);
+        return 0;
+    });
+}
+int ugx_elemdisc_assemble_linear(void* e, void* matrix, void* rhs) {
+    return guard([&] {
+        ((ElemH*)e)->e->assemble_linear(*((MatrixH*)matrix)->A, ((VectorH*)rhs)->v);
+        return 0;
+    });
+}
+int ugx_elemdisc_assemble_jacobian(void* e, void* matrix, void* u) {
+    return guard([&] {
+        ((ElemH*)e)->e->assemble_jacobian(*((MatrixH*)matrix)->A, ((VectorH*)u)->v);
+        return 0;
+    });
+}
+int ugx_elemdisc_assemble_defect(void* e, void* d, void* u) {
+    return guard([&] {
+        ((ElemH*)e)->e->assemble_defect(((VectorH*)d)->v, ((VectorH*)u)->v);
+        return 0;
+    });
+}
 
 // ---- matrices / vectors --------------------------------------------------------------
 
@@ -516,15 +598,17 @@ int ugx_matrix_download_csr(void* m, int32_t* rowptr, int32_t* col, double* val)
     });
 }
 int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val) {
-    auto& A = *((MatrixH*)m)->A;
-    // the caller may write through these pointers: from here on the matrix is
-    // no longer the assembly's (no matrix-free or lattice-order operator is
-    // derived from it) and SpMV reads the CSR until ugx_matrix_finalize
-    A.values_edited_externally();
-    *rowptr = A.rowptr();
-    *col = A.col();
-    *val = A.val();
-    return 0;
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        // the caller may write through these pointers: from here on the matrix is
+        // no longer the assembly's (no matrix-free or lattice-order operator is
+        // derived from it) and SpMV reads the CSR until ugx_matrix_finalize
+        A.values_edited_externally();
+        *rowptr = A.rowptr();
+        *col = A.col();
+        *val = A.val();
+        return 0;
+    });
 }
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w) {
     return guard([&] {
@@ -561,6 +645,105 @@ int ugx_matrix_finalize(void* m) {
 int ugx_matrix_apply_sub(void* m, void* d, void* x) {
     return guard([&] {
         ((MatrixH*)m)->A->matmul_minus(((VectorH*)d)->v, ((VectorH*)x)->v);
+        return 0;
+    });
+}
+int64_t ugx_matrix_num_cols(void* m) { return ((MatrixH*)m)->A->num_cols(); }
+int ugx_matrix_device_csr_readonly(void* m, const int32_t** rowptr, const int32_t** col, const double** val) {
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        if (A.building()) throw UGError("ugx_matrix_device_csr_readonly: the matrix is being built on the host");
+        *rowptr = A.rowptr();
+        *col = A.col();
+        *val = A.val();
+        return 0;
+    });
+}
+int ugx_matrix_apply_ignore_zero_rows(void* m, void* dest, double beta, void* w) {
+    return guard([&] {
+        ((MatrixH*)m)->A->apply_ignore_zero_rows(((VectorH*)dest)->v, beta, ((VectorH*)w)->v);
+        return 0;
+    });
+}
+int ugx_matrix_resize_and_clear(void* m, int64_t rows, int64_t cols) {
+    return guard([&] {
+        ((MatrixH*)m)->A->resize_and_clear(rows, cols);
+        return 0;
+    });
+}
+int ugx_matrix_resize_and_keep_values(void* m, int64_t rows, int64_t cols) {
+    return guard([&] {
+        ((MatrixH*)m)->A->resize_and_keep_values(rows, cols);
+        return 0;
+    });
+}
+int ugx_matrix_clear_retain_structure(void* m) {
+    return guard([&] {
+        ((MatrixH*)m)->A->clear_retain_structure();
+        return 0;
+    });
+}
+int ugx_matrix_is_building(void* m) { return ((MatrixH*)m)->A->building() ? 1 : 0; }
+double* ugx_matrix_entry(void* m, int64_t r, int64_t c) {
+    try {
+        return &(*((MatrixH*)m)->A)(r, c);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+int ugx_matrix_get(void* m, int64_t r, int64_t c, double* value) {
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        const bool stored = A.has_connection(r, c);
+        *value = A.get(r, c);
+        return stored ? 1 : 0;
+    });
+}
+int64_t ugx_matrix_row_length(void* m, int64_t r) {
+    try {
+        return ((MatrixH*)m)->A->num_connections(r);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+int ugx_matrix_row(void* m, int64_t r, int32_t* cols, double* vals) {
+    return guard([&] {
+        ((MatrixH*)m)->A->row(r, cols, vals);
+        return 0;
+    });
+}
+int ugx_matrix_set_as_transpose_of(void* m, void* src, double scale, int keep_zeros) {
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        const auto& B = *((MatrixH*)src)->A;
+        if (keep_zeros) A.set_as_transpose_of(B, scale);
+        else A.set_as_transpose_of2(B, scale);
+        return 0;
+    });
+}
+int ugx_matrix_set_as_copy_of(void* m, void* src, double scale) {
+    return guard([&] {
+        ((MatrixH*)m)->A->set_as_copy_of(*((MatrixH*)src)->A, scale);
+        return 0;
+    });
+}
+int ugx_matrix_scale(void* m, double a) {
+    return guard([&] {
+        ((MatrixH*)m)->A->scale(a);
+        return 0;
+    });
+}
+int ugx_matrix_axpy_transposed(void* m, void* dest, double alpha, void* v1, double beta, void* w) {
+    return guard([&] {
+        ((MatrixH*)m)->A->axpy_transposed(((VectorH*)dest)->v, alpha, ((VectorH*)v1)->v, beta, ((VectorH*)w)->v);
+        return 0;
+    });
+}
+int ugx_matrix_set_dirichlet_rows(void* m, const int32_t* rows_host, int64_t n) {
+    return guard([&] {
+        ((MatrixH*)m)->A->set_dirichlet_rows(rows_host, n);
         return 0;
     });
 }

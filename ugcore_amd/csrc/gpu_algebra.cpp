@@ -177,7 +177,7 @@ static int lanes_for(int64_t rows, int64_t nnz) {
 
 void GPUSparseMatrix::build_spmv_layout() {
     sell_valid_ = false;
-    if (!sell_enabled_ || rows_ == 0) return;
+    if (!sell_enabled_ || rows_ == 0 || building_) return;
     auto& c = context();
     const int64_t nsl = (rows_ + 63) / 64;
     sell_width_.resize(nsl);
@@ -198,6 +198,8 @@ void GPUSparseMatrix::build_spmv_layout() {
 }
 
 void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
+    building_ = false;
+    std::vector<std::vector<std::pair<int32_t, double>>>().swap(hrows_);
     sell_valid_ = false;
     row_order_.resize(0);
     if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix: nnz exceeds int32 range");
@@ -246,13 +248,296 @@ void GPUSparseMatrix::set_from_host_csr(int64_t rows, int64_t cols, const int32_
     build_spmv_layout();
 }
 
+void GPUSparseMatrix::require_device(const char* what) const {
+    if (building_)
+        throw UGError(std::string("GPUSparseMatrix::") + what +
+                      ": the matrix is being built on the host (call finalize() first)");
+}
+
+void GPUSparseMatrix::values_changed_on_device() {
+    set_origin(nullptr);
+    build_spmv_layout();
+}
+
+void GPUSparseMatrix::resize_and_clear(int64_t rows, int64_t cols) {
+    if (rows < 0 || cols < 0 || rows >= ((int64_t)1 << 31) || cols >= ((int64_t)1 << 31))
+        throw UGError("GPUSparseMatrix::resize_and_clear: sizes must be in [0, 2^31)");
+    // release the device CSR and its derived layouts
+    rowptr_.release();
+    col_.release();
+    val_.release();
+    sell_width_.release();
+    sell_col_.release();
+    sell_val_.release();
+    sell_ptr_.release();
+    row_order_.release();
+    sell_valid_ = false;
+    t_rowptr_.release();
+    t_col_.release();
+    t_val_.release();
+    set_origin(nullptr);
+    rows_ = rows;
+    cols_ = cols;
+    nnz_ = 0;
+    hrows_.assign((size_t)rows, {});
+    building_ = true;
+}
+
+void GPUSparseMatrix::resize_and_keep_values(int64_t rows, int64_t cols) {
+    if (rows < 0 || cols < 0 || rows >= ((int64_t)1 << 31) || cols >= ((int64_t)1 << 31))
+        throw UGError("GPUSparseMatrix::resize_and_keep_values: sizes must be in [0, 2^31)");
+    if (!building_) {
+        // move the device CSR into a host build
+        std::vector<int32_t> rp(rows_ + 1), cl(nnz_);
+        std::vector<double> vl(nnz_);
+        if (rows_ > 0) download_csr(rp.data(), cl.data(), vl.data());
+        const int64_t old_rows = rows_;
+        resize_and_clear(old_rows, cols_);
+        for (int64_t r = 0; r < old_rows; ++r) {
+            auto& row = hrows_[(size_t)r];
+            row.reserve((size_t)(rp[r + 1] - rp[r]));
+            for (int32_t k = rp[r]; k < rp[r + 1]; ++k) row.emplace_back(cl[k], vl[k]);
+        }
+        nnz_ = (int64_t)cl.size();
+    }
+    hrows_.resize((size_t)rows);
+    nnz_ = 0;
+    for (auto& row : hrows_) {
+        if (cols < cols_) {
+            auto it = std::lower_bound(row.begin(), row.end(), std::make_pair((int32_t)cols, -HUGE_VAL));
+            row.erase(it, row.end());
+        }
+        nnz_ += (int64_t)row.size();
+    }
+    rows_ = rows;
+    cols_ = cols;
+}
+
+void GPUSparseMatrix::clear_retain_structure() {
+    if (building_) {
+        for (auto& row : hrows_)
+            for (auto& e : row) e.second = 0.0;
+        return;
+    }
+    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, val_.get(), 0.0, val_.get(), context().stream));
+    values_changed_on_device();
+}
+
+double& GPUSparseMatrix::operator()(int64_t r, int64_t c) {
+    if (!building_)
+        throw UGError("GPUSparseMatrix::operator(): the pattern is finalised on the device; entries are created "
+                      "only between resize_and_clear and finalize");
+    if (r < 0 || r >= rows_ || c < 0 || c >= cols_) throw UGError("GPUSparseMatrix::operator(): index out of range");
+    auto& row = hrows_[(size_t)r];
+    auto it = std::lower_bound(row.begin(), row.end(), (int32_t)c,
+                               [](const std::pair<int32_t, double>& e, int32_t col) { return e.first < col; });
+    if (it == row.end() || it->first != (int32_t)c) {
+        it = row.insert(it, std::make_pair((int32_t)c, 0.0));
+        ++nnz_;
+    }
+    return it->second;
+}
+
+int64_t GPUSparseMatrix::num_connections(int64_t r) const {
+    if (r < 0 || r >= rows_) throw UGError("GPUSparseMatrix::num_connections: row out of range");
+    if (building_) return (int64_t)hrows_[(size_t)r].size();
+    int32_t b[2];
+    rowptr_.download(b, 2, r);
+    return b[1] - b[0];
+}
+
+void GPUSparseMatrix::row(int64_t r, int32_t* cols, double* vals) const {
+    if (r < 0 || r >= rows_) throw UGError("GPUSparseMatrix::row: row out of range");
+    if (building_) {
+        const auto& row = hrows_[(size_t)r];
+        for (size_t k = 0; k < row.size(); ++k) {
+            if (cols) cols[k] = row[k].first;
+            if (vals) vals[k] = row[k].second;
+        }
+        return;
+    }
+    int32_t b[2];
+    rowptr_.download(b, 2, r);
+    if (b[1] > b[0]) {
+        if (cols) col_.download(cols, b[1] - b[0], b[0]);
+        if (vals) val_.download(vals, b[1] - b[0], b[0]);
+    }
+}
+
+double GPUSparseMatrix::get(int64_t r, int64_t c) const {
+    if (r < 0 || r >= rows_ || c < 0 || c >= cols_) throw UGError("GPUSparseMatrix::get: index out of range");
+    const int64_t n = num_connections(r);
+    std::vector<int32_t> cl((size_t)n);
+    std::vector<double> vl((size_t)n);
+    row(r, cl.data(), vl.data());
+    auto it = std::lower_bound(cl.begin(), cl.end(), (int32_t)c);
+    return (it != cl.end() && *it == (int32_t)c) ? vl[(size_t)(it - cl.begin())] : 0.0;
+}
+
+bool GPUSparseMatrix::has_connection(int64_t r, int64_t c) const {
+    if (r < 0 || r >= rows_ || c < 0 || c >= cols_) return false;
+    const int64_t n = num_connections(r);
+    std::vector<int32_t> cl((size_t)n);
+    row(r, cl.data(), nullptr);
+    return std::binary_search(cl.begin(), cl.end(), (int32_t)c);
+}
+
+void GPUSparseMatrix::finalize() {
+    if (building_) {
+        std::vector<int32_t> rp((size_t)rows_ + 1, 0), cl;
+        std::vector<double> vl;
+        int64_t nnz = 0;
+        for (const auto& row : hrows_) nnz += (int64_t)row.size();
+        if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix::finalize: nnz exceeds int32 range");
+        cl.reserve((size_t)nnz);
+        vl.reserve((size_t)nnz);
+        for (int64_t r = 0; r < rows_; ++r) {
+            for (const auto& e : hrows_[(size_t)r]) {
+                cl.push_back(e.first);
+                vl.push_back(e.second);
+            }
+            rp[(size_t)r + 1] = (int32_t)cl.size();
+        }
+        building_ = false;
+        std::vector<std::vector<std::pair<int32_t, double>>>().swap(hrows_);
+        set_from_host_csr(rows_, cols_, rp.data(), cl.data(), vl.data());
+        return;
+    }
+    set_origin(nullptr);
+    build_spmv_layout();
+}
+
+void GPUSparseMatrix::transpose_from(const GPUSparseMatrix& B, double scale, bool drop_zeros) {
+    B.require_device("set_as_transpose_of");
+    if (&B == this) throw UGError("GPUSparseMatrix::set_as_transpose_of: source and destination are the same matrix");
+    auto& c = context();
+    const int64_t nnz = B.nnz_;
+    DeviceBuffer<uint32_t> key(nnz), key2(nnz);
+    DeviceBuffer<int32_t> idx(nnz), idx2(nnz), row_of(nnz);
+    size_t tb = 0;
+    if (transpose_sort_bytes(nnz, (int32_t)B.cols_, &tb) != 0) throw UGError("set_as_transpose_of: sort set-up failed");
+    DeviceBuffer<uint8_t> temp((int64_t)tb);
+    DeviceBuffer<int32_t> rpt(B.cols_ + 1);
+    UGX_LAUNCH(transpose_csr(B.rows_, (int32_t)B.cols_, nnz, B.rowptr_.get(), B.col_.get(), B.val_.get(),
+                             drop_zeros ? 1 : 0, key.get(), key2.get(), idx.get(), idx2.get(), row_of.get(),
+                             temp.get(), tb, rpt.get(), c.stream));
+    int32_t kept = 0;
+    rpt.download(&kept, 1, B.cols_);
+    allocate(B.cols_, B.rows_, kept);
+    set_origin(nullptr);
+    UGX_HIP(hipMemcpyAsync(rowptr_.get(), rpt.get(), sizeof(int32_t) * (size_t)(rows_ + 1), hipMemcpyDeviceToDevice,
+                           c.stream));
+    UGX_LAUNCH(transpose_gather(kept, idx2.get(), row_of.get(), B.val_.get(), scale, col_.get(), val_.get(),
+                                c.stream));
+    sync();
+    build_spmv_layout();
+}
+
+void GPUSparseMatrix::set_as_copy_of(const GPUSparseMatrix& B, double scale) {
+    B.require_device("set_as_copy_of");
+    if (&B == this) {
+        if (scale != 1.0) this->scale(scale);
+        return;
+    }
+    auto& c = context();
+    allocate(B.rows_, B.cols_, B.nnz_);
+    set_origin(nullptr);
+    UGX_HIP(hipMemcpyAsync(rowptr_.get(), B.rowptr_.get(), sizeof(int32_t) * (size_t)(rows_ + 1),
+                           hipMemcpyDeviceToDevice, c.stream));
+    if (nnz_ > 0) {
+        UGX_HIP(hipMemcpyAsync(col_.get(), B.col_.get(), sizeof(int32_t) * (size_t)nnz_, hipMemcpyDeviceToDevice,
+                               c.stream));
+        UGX_LAUNCH(scale_values(nnz_, B.val_.get(), scale, val_.get(), c.stream));
+    }
+    if (B.has_row_order()) {
+        row_order_.resize(rows_);
+        UGX_HIP(hipMemcpyAsync(row_order_.get(), B.row_order_.get(), sizeof(int32_t) * (size_t)rows_,
+                               hipMemcpyDeviceToDevice, c.stream));
+    }
+    sync();
+    build_spmv_layout();
+}
+
+void GPUSparseMatrix::scale(double a) {
+    if (building_) {
+        for (auto& row : hrows_)
+            for (auto& e : row) e.second *= a;
+        return;
+    }
+    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, val_.get(), a, val_.get(), context().stream));
+    values_changed_on_device();
+}
+
+void GPUSparseMatrix::axpy_transposed(GPUVector& dest, double a1, const GPUVector& v1, double b1,
+                                      const GPUVector& w1) const {
+    require_device("axpy_transposed");
+    if (dest.size() != cols_ || v1.size() != cols_ || w1.size() != rows_)
+        throw UGError("GPUSparseMatrix::axpy_transposed size mismatch");
+    if (&dest == &w1) throw UGError("GPUSparseMatrix::axpy_transposed: dest must not alias the multiplied vector");
+    auto& c = context();
+    if (t_version_ != version_ || t_rowptr_.size() != cols_ + 1) {
+        // the transpose without zero entries (the product skips them)
+        const int64_t nnz = nnz_;
+        DeviceBuffer<uint32_t> key(nnz), key2(nnz);
+        DeviceBuffer<int32_t> idx(nnz), idx2(nnz), row_of(nnz);
+        size_t tb = 0;
+        if (transpose_sort_bytes(nnz, (int32_t)cols_, &tb) != 0) throw UGError("axpy_transposed: sort set-up failed");
+        DeviceBuffer<uint8_t> temp((int64_t)tb);
+        t_rowptr_.resize(cols_ + 1);
+        UGX_LAUNCH(transpose_csr(rows_, (int32_t)cols_, nnz, rowptr_.get(), col_.get(), val_.get(), 1, key.get(),
+                                 key2.get(), idx.get(), idx2.get(), row_of.get(), temp.get(), tb, t_rowptr_.get(),
+                                 c.stream));
+        int32_t kept = 0;
+        t_rowptr_.download(&kept, 1, cols_);
+        t_col_.resize(kept);
+        t_val_.resize(kept);
+        UGX_LAUNCH(transpose_gather(kept, idx2.get(), row_of.get(), val_.get(), 1.0, t_col_.get(), t_val_.get(),
+                                    c.stream));
+        sync();
+        t_nnz_ = kept;
+        t_version_ = version_;
+    }
+    UGX_LAUNCH(ugamd::axpy_transposed(cols_, t_rowptr_.get(), t_col_.get(), t_val_.get(), a1, v1.data(),
+                                      &dest == &v1, b1, w1.data(), dest.data(), c.stream));
+}
+
+void GPUSparseMatrix::set_dirichlet_rows(const int32_t* rows_host, int64_t n) {
+    if (n <= 0) return;
+    for (int64_t i = 0; i < n; ++i)
+        if (rows_host[i] < 0 || rows_host[i] >= rows_ || rows_host[i] >= cols_)
+            throw UGError("GPUSparseMatrix::set_dirichlet_rows: row out of range");
+    if (building_) {
+        for (int64_t i = 0; i < n; ++i) {
+            for (auto& e : hrows_[(size_t)rows_host[i]]) e.second = 0.0;
+            (*this)(rows_host[i], rows_host[i]) = 1.0;
+        }
+        return;
+    }
+    auto& c = context();
+    DeviceBuffer<int32_t> rows(n);
+    rows.upload(rows_host, n);
+    UGX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    UGX_LAUNCH(missing_diagonals(n, rows.get(), rowptr_.get(), col_.get(), c.flag, c.stream));
+    int missing = 0;
+    UGX_HIP(hipMemcpyAsync(&missing, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    sync();
+    if (missing)
+        throw UGError("GPUSparseMatrix::set_dirichlet_rows: a listed row stores no diagonal entry (the device pattern "
+                      "is fixed; add the diagonal while building the pattern)");
+    UGX_LAUNCH(dirichlet_rows(n, rows.get(), rowptr_.get(), col_.get(), val_.get(), c.stream));
+    sync();
+    values_changed_on_device();
+}
+
 void GPUSparseMatrix::download_csr(int32_t* rowptr, int32_t* col, double* val) const {
+    require_device("download_csr");
     if (rowptr) rowptr_.download(rowptr, rows_ + 1);
     if (col) col_.download(col, nnz_);
     if (val) val_.download(val, nnz_);
 }
 
 void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, double b1, const GPUVector& w1) const {
+    require_device("axpy");
     if (dest.size() != rows_ || w1.size() != cols_) throw UGError("GPUSparseMatrix::axpy size mismatch");
     if (&dest == &w1) throw UGError("GPUSparseMatrix::axpy: dest must not alias the multiplied vector");
     if (sell_valid_)
@@ -266,6 +551,7 @@ void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, doub
 }
 
 void GPUSparseMatrix::apply_ignore_zero_rows(GPUVector& dest, double beta, const GPUVector& w) const {
+    require_device("apply_ignore_zero_rows");
     if (sell_valid_)
         UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
                                               sell_val_.get(), 0.0, dest.data(), beta, w.data(), dest.data(), 1,
@@ -767,6 +1053,276 @@ void DomainDiscretization::adjoint_gradient(const GPUVector& u, const GPUVector&
     }
     UGX_LAUNCH(adjoint_grad_q1_hex(L.n, 1.0 / L.n, theta_.get(), L.dof_of_lat.get(), L.n_inner, u.data(),
                                    lambda.data(), grad_dev, context().stream));
+}
+
+// ---------------------------------------------------------------------------
+// ElementDiscretization: element loop over caller-supplied connectivity
+// ---------------------------------------------------------------------------
+ElementDiscretization::ElementDiscretization(int elem_type, int64_t num_elems, const int32_t* elem_dofs,
+                                             int64_t num_dofs, const double* dof_coords, bool on_device)
+    : type_(elem_type), ne_(num_elems), ndofs_(num_dofs) {
+    if (elem_type < 0 || elem_type > 2)
+        throw UGError("ElementDiscretization: element type 0 (triangle, FV1), 1 (tetrahedron, FV1) or 2 "
+                      "(hexahedron, Q1 FE)");
+    nv_ = vertices_of(elem_type);
+    dim_ = dim_of(elem_type);
+    if (num_elems < 0 || num_dofs < 0 || num_dofs >= ((int64_t)1 << 31))
+        throw UGError("ElementDiscretization: element and DoF counts must be >= 0 and below 2^31");
+    if (num_elems * nv_ * nv_ >= ((int64_t)1 << 31))
+        throw UGError("ElementDiscretization: num_elems * vertices^2 must stay below 2^31");
+    if (num_elems > 0 && (!elem_dofs || !dof_coords)) throw UGError("ElementDiscretization: null connectivity");
+    if (!on_device)
+        for (int64_t i = 0; i < ne_ * nv_; ++i)
+            if (elem_dofs[i] < 0 || elem_dofs[i] >= ndofs_)
+                throw UGError("ElementDiscretization: DoF index out of range in the element table");
+    auto& c = context();
+    dofs_.resize(ne_ * nv_);
+    coords_.resize(ndofs_ * dim_);
+    if (on_device) {
+        if (ne_ > 0)
+            UGX_HIP(hipMemcpyAsync(dofs_.get(), elem_dofs, sizeof(int32_t) * ne_ * nv_, hipMemcpyDeviceToDevice,
+                                   c.stream));
+        if (ndofs_ > 0)
+            UGX_HIP(hipMemcpyAsync(coords_.get(), dof_coords, sizeof(double) * ndofs_ * dim_,
+                                   hipMemcpyDeviceToDevice, c.stream));
+    } else {
+        dofs_.upload(elem_dofs, ne_ * nv_);
+        coords_.upload(dof_coords, ndofs_ * dim_);
+    }
+    build();
+}
+
+void ElementDiscretization::build() {
+    auto& c = context();
+    const int64_t n = ne_ * nv_ * nv_;
+    // the pattern and the slot of every local entry
+    size_t sort_b = 0, scan_b = 0;
+    if (elem_pattern_temp_bytes(ne_, nv_, ndofs_, &sort_b, &scan_b) != 0)
+        throw UGError("ElementDiscretization: sort set-up failed");
+    DeviceBuffer<uint64_t> key(n), key2(n);
+    DeviceBuffer<int32_t> idx(n), idx2(n), head(n), run(n), nnz_dev(1), slot_local(n);
+    DeviceBuffer<uint8_t> temp((int64_t)std::max(sort_b, scan_b));
+    rowptr_.resize(ndofs_ + 1);
+    UGX_LAUNCH(elem_pattern(ne_, nv_, ndofs_, dofs_.get(), key.get(), key2.get(), idx.get(), idx2.get(), head.get(),
+                            run.get(), temp.get(), sort_b, scan_b, rowptr_.get(), nnz_dev.get(), c.stream));
+    int32_t nnz = 0;
+    nnz_dev.download(&nnz, 1);
+    nnz_ = nnz;
+    col_.resize(nnz_);
+    UGX_LAUNCH(elem_pattern_fill(ne_, nv_, key2.get(), idx2.get(), head.get(), run.get(), col_.get(),
+                                 slot_local.get(), c.stream));
+    key.release();
+    key2.release();
+    head.release();
+    run.release();
+    idx.release();
+    idx2.release();
+    // DoF -> element incidence, then the colouring rounds
+    const int64_t ni = ne_ * nv_;
+    size_t inc_b = 0;
+    if (elem_incidence_temp_bytes(ni, ndofs_, &inc_b) != 0 || elem_incidence_temp_bytes(ne_, 64, &sort_b) != 0)
+        throw UGError("ElementDiscretization: sort set-up failed");
+    DeviceBuffer<uint32_t> ikey(std::max(ni, ne_)), ikey2(std::max(ni, ne_));
+    DeviceBuffer<int32_t> iidx(std::max(ni, ne_)), inc_t(ni), inc_ptr(ndofs_ + 1), colour(ne_), flags(2);
+    DeviceBuffer<uint8_t> itemp((int64_t)std::max(inc_b, sort_b));
+    UGX_LAUNCH(elem_incidence(ne_, nv_, ndofs_, dofs_.get(), ikey.get(), ikey2.get(), iidx.get(), inc_t.get(),
+                              itemp.get(), inc_b, inc_ptr.get(), c.stream));
+    UGX_HIP(hipMemsetAsync(colour.get(), 0xff, sizeof(int32_t) * (size_t)std::max<int64_t>(ne_, 1), c.stream));
+    UGX_HIP(hipMemsetAsync(flags.get(), 0, sizeof(int32_t) * 2, c.stream));
+    int32_t done[2] = {0, 0};
+    int rounds = 0;
+    while (done[0] < ne_) {
+        UGX_LAUNCH(elem_colour_round(ne_, nv_, dofs_.get(), inc_ptr.get(), inc_t.get(), colour.get(), flags.get(),
+                                     flags.get() + 1, c.stream));
+        flags.download(done, 2);
+        if (done[1]) throw UGError("ElementDiscretization: an element has more than 63 neighbouring colours");
+        if (++rounds > 100000) throw UGError("ElementDiscretization: the colouring does not terminate");
+    }
+    // the number of colours: 1 + the largest colour
+    int ncol = 0;
+    if (ne_ > 0) {
+        std::vector<int32_t> col_h(ne_);
+        colour.download(col_h.data(), ne_);
+        for (int32_t x : col_h) ncol = std::max(ncol, x + 1);
+    }
+    order_.resize(ne_);
+    DeviceBuffer<int32_t> colour_ptr(ncol + 1);
+    UGX_LAUNCH(elem_colour_order(ne_, ncol, colour.get(), ikey.get(), ikey2.get(), iidx.get(), order_.get(),
+                                 itemp.get(), sort_b, colour_ptr.get(), c.stream));
+    std::vector<int32_t> cp(ncol + 1);
+    colour_ptr.download(cp.data(), ncol + 1);
+    colour_start_.assign(cp.begin(), cp.end());
+    slot_.resize(n);
+    UGX_LAUNCH(elem_slots_by_position(ne_, nv_, order_.get(), slot_local.get(), slot_.get(), c.stream));
+    sync();
+}
+
+void ElementDiscretization::set_diffusion(const double* k, bool on_device) {
+    if (!k) {
+        has_k_ = false;
+        return;
+    }
+    kel_.resize(ne_);
+    if (on_device)
+        UGX_HIP(hipMemcpyAsync(kel_.get(), k, sizeof(double) * ne_, hipMemcpyDeviceToDevice, context().stream));
+    else
+        kel_.upload(k, ne_);
+    has_k_ = true;
+}
+
+void ElementDiscretization::set_source_dofs(const double* f_host) {
+    if (type_ == HEXAHEDRON_Q1) throw UGError("ElementDiscretization: the Q1 source is a constant");
+    f_dof_.resize(ndofs_);
+    f_dof_.upload(f_host, ndofs_);
+    has_f_dof_ = true;
+}
+
+void ElementDiscretization::set_velocity(double vx, double vy, double vz) {
+    if (type_ != TETRAHEDRON_FV1 && (vx != 0.0 || vy != 0.0 || vz != 0.0))
+        throw UGError("ElementDiscretization: convection is available for tetrahedra (FV1, full upwind)");
+    vel_[0] = vx;
+    vel_[1] = vy;
+    vel_[2] = vz;
+    conv_ = vx != 0.0 || vy != 0.0 || vz != 0.0;
+}
+
+void ElementDiscretization::set_dirichlet(const int32_t* dofs_host, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (dofs_host[i] < 0 || dofs_host[i] >= ndofs_) throw UGError("ElementDiscretization: Dirichlet DoF out of range");
+    dir_.resize(n);
+    if (n > 0) dir_.upload(dofs_host, n);
+    ndir_ = n;
+}
+
+void ElementDiscretization::set_dirichlet_device(const int32_t* dofs_dev, int64_t n) {
+    dir_.resize(n);
+    if (n > 0)
+        UGX_HIP(hipMemcpyAsync(dir_.get(), dofs_dev, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, context().stream));
+    ndir_ = n;
+}
+
+void ElementDiscretization::run(double* val, double* rhs, const double* u, double* d) {
+    auto& c = context();
+    ugamd::ElemLaunch L;
+    L.ne = ne_;
+    L.order = order_.get();
+    L.dofs = dofs_.get();
+    L.coords = coords_.get();
+    L.kel = has_k_ ? kel_.get() : nullptr;
+    L.f_const = f_const_;
+    L.f_dof = has_f_dof_ ? f_dof_.get() : nullptr;
+    for (int k = 0; k < 3; ++k) L.vel[k] = vel_[k];
+    L.conv = conv_ ? 1 : 0;
+    L.slot = slot_.get();
+    L.val = val;
+    L.rhs = rhs;
+    L.u = u;
+    L.d = d;
+    for (int col = 0; col + 1 < (int)colour_start_.size(); ++col)
+        UGX_LAUNCH(elem_assemble_colour(type_, L, colour_start_[col], colour_start_[col + 1] - colour_start_[col],
+                                        c.stream));
+}
+
+void ElementDiscretization::give_pattern(GPUSparseMatrix& A) {
+    auto& c = context();
+    A.allocate(ndofs_, ndofs_, nnz_);
+    A.set_origin(nullptr);
+    UGX_HIP(hipMemcpyAsync(A.rowptr(), rowptr_.get(), sizeof(int32_t) * (ndofs_ + 1), hipMemcpyDeviceToDevice,
+                           c.stream));
+    if (nnz_ > 0)
+        UGX_HIP(hipMemcpyAsync(A.col(), col_.get(), sizeof(int32_t) * nnz_, hipMemcpyDeviceToDevice, c.stream));
+}
+
+void ElementDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
+    auto& c = context();
+    give_pattern(A);
+    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, A.val(), 0.0, A.val(), c.stream));
+    b.resize(ndofs_);
+    b.set(0.0);
+    run(A.val(), b.data(), nullptr, nullptr);
+    if (ndir_ > 0) {
+        UGX_LAUNCH(dirichlet_rows(ndir_, dir_.get(), A.rowptr(), A.col(), A.val(), c.stream));
+        UGX_LAUNCH(zero_listed(ndir_, dir_.get(), b.data(), c.stream));
+    }
+    sync();
+    A.build_spmv_layout();
+}
+
+void ElementDiscretization::assemble_jacobian(GPUSparseMatrix& J, const GPUVector& u) {
+    if (u.size() != ndofs_) throw UGError("ElementDiscretization::assemble_jacobian: size mismatch");
+    auto& c = context();
+    give_pattern(J);
+    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, J.val(), 0.0, J.val(), c.stream));
+    run(J.val(), nullptr, nullptr, nullptr);
+    if (ndir_ > 0) UGX_LAUNCH(dirichlet_rows(ndir_, dir_.get(), J.rowptr(), J.col(), J.val(), c.stream));
+    sync();
+    J.build_spmv_layout();
+}
+
+void ElementDiscretization::assemble_defect(GPUVector& d, const GPUVector& u) {
+    if (u.size() != ndofs_) throw UGError("ElementDiscretization::assemble_defect: size mismatch");
+    if (&d == &u) throw UGError("ElementDiscretization::assemble_defect: d must not alias u");
+    d.resize(ndofs_);
+    d.set(0.0);
+    run(nullptr, nullptr, u.data(), d.data());
+    if (ndir_ > 0) UGX_LAUNCH(zero_listed(ndir_, dir_.get(), d.data(), context().stream));
+}
+
+void ElementDiscretization::download_pattern(int32_t* rowptr, int32_t* col) const {
+    if (rowptr) rowptr_.download(rowptr, ndofs_ + 1);
+    if (col && nnz_ > 0) col_.download(col, nnz_);
+}
+
+ElementDiscretization& DomainDiscretization::element_discretization() {
+    auto& c = context();
+    LevelGrid& L = *levels_.back();
+    const int type = kind_ == 0 ? ElementDiscretization::HEXAHEDRON_Q1
+                                : (kind_ == 1 ? ElementDiscretization::TRIANGLE_FV1 : ElementDiscretization::TETRAHEDRON_FV1);
+    const int nv = ElementDiscretization::vertices_of(type);
+    const int64_t n = L.n;
+    const int64_t ne = kind_ == 0 ? n * n * n : (kind_ == 1 ? 2 * n * n : 6 * n * n * n);
+    DeviceBuffer<int32_t> dofs(ne * nv);
+    DeviceBuffer<double> kel(ne);
+    UGX_LAUNCH(lattice_connectivity(kind_, L.n, L.dof_of_lat.get(), kind_ == 2 ? L.cell_tets.get() : nullptr,
+                                    has_theta_ ? theta_.get() : nullptr, dofs.get(), kel.get(), c.stream));
+    if (!elem_ || elem_jseed_ != jseed_ || elem_jamp_ != jamp_) {
+        const int dim = dim_of_kind();
+        DeviceBuffer<double> X(L.N * dim);
+        UGX_LAUNCH(ugamd::dof_positions(tet_disc(*this, L, top_level(), nullptr), kind_, dim, L.W, L.N,
+                                        L.lat_of_dof.get(), X.get(), c.stream));
+        elem_.reset(new ElementDiscretization(type, ne, dofs.get(), L.N, X.get(), true));
+        std::vector<int32_t> dir((size_t)(L.N - L.n_inner));
+        for (int64_t i = L.n_inner; i < L.N; ++i) dir[(size_t)(i - L.n_inner)] = (int32_t)i;
+        elem_->set_dirichlet(dir.data(), (int64_t)dir.size());
+        elem_jseed_ = jseed_;
+        elem_jamp_ = jamp_;
+    }
+    elem_->set_diffusion(has_theta_ ? kel.get() : nullptr, true);
+    if (kind_ == 1) {
+        if (has_f_) {
+            std::vector<double> f((size_t)L.N);
+            f_dof_.download(f.data(), L.N);
+            elem_->set_source_dofs(f.data());
+        } else {
+            elem_->set_source(0.0);   // the triangle discretisation takes its source per DoF only
+        }
+    } else {
+        elem_->set_source(f_const_);
+    }
+    if (kind_ == 2) elem_->set_velocity(vel_[0], vel_[1], vel_[2]);
+    sync();
+    return *elem_;
+}
+
+void DomainDiscretization::assemble_jacobian(GPUSparseMatrix& J, const GPUVector& u) {
+    if (u.size() != levels_.back()->N) throw UGError("assemble_jacobian: size mismatch");
+    GPUVector b(levels_.back()->N);
+    assemble_linear(J, b);
+}
+
+void DomainDiscretization::assemble_defect(GPUVector& d, const GPUVector& u) {
+    if (u.size() != levels_.back()->N) throw UGError("assemble_defect: size mismatch");
+    element_discretization().assemble_defect(d, u);
 }
 
 void DomainDiscretization::set_source_dofs(const double* f_host) {

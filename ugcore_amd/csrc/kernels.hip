@@ -23,6 +23,7 @@
 
 #include <cstdint>
 
+#include "elem_local.hpp"
 #include "ugx_internal.hpp"
 
 namespace ugamd {
@@ -2009,67 +2010,21 @@ __device__ __forceinline__ int32_t csr_slot(const int32_t* __restrict__ rowptr, 
 }
 
 // Q1 FE element matrix of a hexahedron with trilinear geometry, 2x2x2 Gauss,
-// k(x) = exp(theta[fine element containing x]); rhs f_const * int phi_i.
-// Local vertex order is the reference hexahedron's (x fastest, bottom then top).
+// k(x) = exp(theta[fine element containing x]) (1 without theta); rhs
+// f_const * int phi_i. Local vertex order is the reference hexahedron's.
 __device__ void q1_hex_element(const double X[8][3], const double* __restrict__ theta, int nfine, double f_const,
                                double Ke[8][8], double fe[8]) {
-    const double g = 0.28867513459481287;   // 0.5/sqrt(3)
-    for (int a = 0; a < 8; ++a) {
-        fe[a] = 0.0;
-        for (int b = 0; b < 8; ++b) Ke[a][b] = 0.0;
-    }
-    for (int q = 0; q < 8; ++q) {
-        const double xi[3] = {(q & 1) ? 0.5 + g : 0.5 - g, (q & 2) ? 0.5 + g : 0.5 - g, (q & 4) ? 0.5 + g : 0.5 - g};
-        double phi[8], dphi[8][3];
-        for (int a = 0; a < 8; ++a) {
-            const int bx = (a == 1 || a == 2 || a == 5 || a == 6), by = (a == 2 || a == 3 || a == 6 || a == 7), bz = (a >= 4);
-            const double fx = bx ? xi[0] : 1.0 - xi[0], fy = by ? xi[1] : 1.0 - xi[1], fz = bz ? xi[2] : 1.0 - xi[2];
-            const double gx = bx ? 1.0 : -1.0, gy = by ? 1.0 : -1.0, gz = bz ? 1.0 : -1.0;
-            phi[a] = fx * fy * fz;
-            dphi[a][0] = gx * fy * fz;
-            dphi[a][1] = fx * gy * fz;
-            dphi[a][2] = fx * fy * gz;
-        }
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, xq[3] = {0, 0, 0};
-        for (int a = 0; a < 8; ++a)
-            for (int r = 0; r < 3; ++r) {
-                xq[r] += phi[a] * X[a][r];
-                for (int c = 0; c < 3; ++c) J[r][c] += X[a][r] * dphi[a][c];
-            }
-        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        const double id = 1.0 / det;
-        double Ji[3][3];
-        Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
-        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-        const double w = 0.125 * fabs(det);
-        double kq = 1.0;
-        if (theta) {
+    q1_hex_element_k(
+        X,
+        [=](const double* xq) {
+            if (!theta) return 1.0;
             int cx = (int)floor(xq[0] * nfine), cy = (int)floor(xq[1] * nfine), cz = (int)floor(xq[2] * nfine);
             cx = cx < 0 ? 0 : (cx >= nfine ? nfine - 1 : cx);
             cy = cy < 0 ? 0 : (cy >= nfine ? nfine - 1 : cy);
             cz = cz < 0 ? 0 : (cz >= nfine ? nfine - 1 : cz);
-            kq = exp(theta[((int64_t)cz * nfine + cy) * nfine + cx]);
-        }
-        double grad[8][3];
-        for (int a = 0; a < 8; ++a)
-            for (int r = 0; r < 3; ++r)
-                grad[a][r] = Ji[0][r] * dphi[a][0] + Ji[1][r] * dphi[a][1] + Ji[2][r] * dphi[a][2];
-        const double wk = w * kq;
-        for (int a = 0; a < 8; ++a) {
-            for (int b = 0; b < 8; ++b)
-                Ke[a][b] += wk * (grad[a][0] * grad[b][0] + grad[a][1] * grad[b][1] + grad[a][2] * grad[b][2]);
-            fe[a] += w * f_const * phi[a];
-        }
-    }
+            return exp(theta[((int64_t)cz * nfine + cy) * nfine + cx]);
+        },
+        f_const, Ke, fe);
 }
 
 // corner c of lattice element (ex,ey,ez): reference hexahedron order
@@ -2179,43 +2134,21 @@ __global__ void k_zero_range(int64_t lo, int64_t hi, double* x) {
         x[i] = 0.0;
 }
 
-// FV1 on P1 triangles (box method), lattice squares split along (1,1).
+// FV1 on P1 triangles (box method), lattice squares split along (1,1), k =
+// exp(theta[fine square containing the scvf's integration point]).
 // Colour = (sx%2, sy%2, which triangle).
-template <typename TM>
 __device__ void fv1_tri_element(const double X[3][2], const double* __restrict__ theta, int nfine, double Ke[3][3],
                                 double& area) {
-    const double J00 = X[1][0] - X[0][0], J01 = X[2][0] - X[0][0];
-    const double J10 = X[1][1] - X[0][1], J11 = X[2][1] - X[0][1];
-    const double det = J00 * J11 - J01 * J10;
-    area = 0.5 * fabs(det);
-    const double gr[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
-    double grad[3][2];
-    for (int i = 0; i < 3; ++i) {
-        grad[i][0] = (J11 * gr[i][0] - J10 * gr[i][1]) / det;
-        grad[i][1] = (-J01 * gr[i][0] + J00 * gr[i][1]) / det;
-    }
-    const double bx = (X[0][0] + X[1][0] + X[2][0]) / 3.0, by = (X[0][1] + X[1][1] + X[2][1]) / 3.0;
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) Ke[a][b] = 0.0;
-    for (int e = 0; e < 3; ++e) {
-        const int fr = e, to = (e + 1) % 3;
-        const double mx = 0.5 * (X[fr][0] + X[to][0]), my = 0.5 * (X[fr][1] + X[to][1]);
-        double nx = by - my, ny = -(bx - mx);
-        if (nx * (X[to][0] - X[fr][0]) + ny * (X[to][1] - X[fr][1]) < 0) { nx = -nx; ny = -ny; }
-        double kq = 1.0;
-        if (theta) {
-            const double ipx = 0.5 * (mx + bx), ipy = 0.5 * (my + by);
-            int cx = (int)floor(ipx * nfine), cy = (int)floor(ipy * nfine);
+    fv1_tri_element_k(
+        X,
+        [=](const double* ip) {
+            if (!theta) return 1.0;
+            int cx = (int)floor(ip[0] * nfine), cy = (int)floor(ip[1] * nfine);
             cx = cx < 0 ? 0 : (cx >= nfine ? nfine - 1 : cx);
             cy = cy < 0 ? 0 : (cy >= nfine ? nfine - 1 : cy);
-            kq = exp(theta[(int64_t)cy * nfine + cx]);
-        }
-        for (int sh = 0; sh < 3; ++sh) {
-            const double flux = kq * (grad[sh][0] * nx + grad[sh][1] * ny);
-            Ke[fr][sh] -= flux;
-            Ke[to][sh] += flux;
-        }
-    }
+            return exp(theta[(int64_t)cy * nfine + cx]);
+        },
+        Ke, area);
 }
 
 template <typename TM>
@@ -2244,7 +2177,7 @@ __global__ __launch_bounds__(128) void k_assemble_fv1_tri(int n, int color, doub
         lat[a] = (int64_t)yy * W + xx;
     }
     double Ke[3][3], area;
-    fv1_tri_element<TM>(X, theta, nfine, Ke, area);
+    fv1_tri_element(X, theta, nfine, Ke, area);
     if (rowptr) {
         int32_t dof[3];
         for (int a = 0; a < 3; ++a) dof[a] = dof_of_lat[lat[a]];

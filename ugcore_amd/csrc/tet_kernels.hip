@@ -11,6 +11,7 @@
 
 #include <cstdint>
 
+#include "elem_local.hpp"
 #include "ugx_internal.hpp"
 
 namespace ugamd {
@@ -20,13 +21,6 @@ namespace {
 __device__ __forceinline__ int64_t tile_index(int d, int64_t i, int nd) {
     return ((i >> 6) * nd + d) * 64 + (i & 63);
 }
-
-// ReferenceTetrahedron tables (reference_element.cpp:547-760)
-__constant__ int8_t kEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-__constant__ int8_t kFace[4][3] = {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}};
-__constant__ int8_t kEdgeFace[6][2] = {{0, 3}, {0, 1}, {0, 2}, {3, 2}, {1, 3}, {2, 1}};
-__constant__ int8_t kPointEdge[4][3] = {{2, 0, 3}, {0, 1, 4}, {1, 2, 5}, {3, 4, 5}};
-__constant__ int8_t kPointFace[4][3] = {{0, 2, 3}, {0, 3, 1}, {0, 1, 2}, {3, 2, 1}};
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
@@ -38,12 +32,6 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
 __device__ __forceinline__ double uniform_open(uint64_t seed, uint64_t counter) {
     const uint64_t h = splitmix(splitmix(seed) ^ counter);
     return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-__device__ __forceinline__ void cross(const double* a, const double* b, double* c) {
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
 // corner c of cell (i,j,k): lattice offsets
@@ -69,107 +57,6 @@ __device__ __forceinline__ void vertex_position(const TetDisc& P, int i, int j, 
     }
 }
 
-__device__ __forceinline__ double hex_volume(const double (*x)[3]) {
-    double x50[3], x14[3], x10[3], x20[3], x60[3], x24[3], x30[3], x70[3], m1[3], n1[3], m2[3], n2[3];
-    for (int d = 0; d < 3; ++d) {
-        x50[d] = x[5][d] - x[0][d];
-        x14[d] = x[1][d] - x[4][d];
-        x10[d] = x[1][d] - x[0][d];
-        x20[d] = x[2][d] - x[0][d];
-        x60[d] = x[6][d] - x[0][d];
-        x24[d] = x[2][d] - x[4][d];
-        x30[d] = x[3][d] - x[0][d];
-        x70[d] = x[7][d] - x[0][d];
-    }
-    cross(x50, x14, m1);
-    cross(x10, x20, n1);
-    cross(x60, x24, m2);
-    cross(x20, x30, n2);
-    double s1 = 0, s2 = 0;
-    for (int d = 0; d < 3; ++d) {
-        s1 += (n1[d] + m1[d]) * x60[d];
-        s2 += (n2[d] + m2[d]) * x70[d];
-    }
-    return (1.0 / 6.0) * (s1 + s2);
-}
-
-// FV1 element of a tetrahedron (fv1_geom.cpp:258-463): scvf i per edge with
-// corners [edge mid, centre of face kEdgeFace[i][0], element centre, centre of
-// face kEdgeFace[i][1]], normal 0.5 (c2-c0) x (c3-c1) from edge vertex 0 to 1;
-// diffusion k grad(phi).n subtracted from `from`, added to `to`; full upwind
-// convection (conv_shape.h:278-330); scv volumes by the hexahedron formula.
-__device__ void fv1_tet_element(const double (*X)[3], const double* kip, const double* vel, bool conv,
-                                double Ke[4][4], double vol[4]) {
-    double M[6][3], F[4][3], C[3];
-    for (int e = 0; e < 6; ++e)
-        for (int d = 0; d < 3; ++d) M[e][d] = (X[kEdge[e][0]][d] + X[kEdge[e][1]][d]) * 0.5;
-    for (int f = 0; f < 4; ++f)
-        for (int d = 0; d < 3; ++d) F[f][d] = (X[kFace[f][0]][d] + X[kFace[f][1]][d] + X[kFace[f][2]][d]) / 3.0;
-    for (int d = 0; d < 3; ++d) C[d] = (X[0][d] + X[1][d] + X[2][d] + X[3][d]) * 0.25;
-    // P1 gradients: rows of J^{-1}, J = [X1-X0, X2-X0, X3-X0]
-    double J[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) J[r][c] = X[c + 1][r] - X[0][r];
-    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    const double id = 1.0 / det;
-    double g[4][3];
-    g[1][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
-    g[1][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-    g[1][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-    g[2][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-    g[2][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-    g[2][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-    g[3][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-    g[3][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-    g[3][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    for (int d = 0; d < 3; ++d) g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) Ke[i][j] = 0.0;
-    for (int i = 0; i < 6; ++i) {
-        const double* c0 = M[i];
-        const double* c1 = F[kEdgeFace[i][0]];
-        const double* c3 = F[kEdgeFace[i][1]];
-        double a[3], b[3], nrm[3];
-        for (int d = 0; d < 3; ++d) {
-            a[d] = C[d] - c0[d];
-            b[d] = c3[d] - c1[d];
-        }
-        cross(a, b, nrm);
-        for (int d = 0; d < 3; ++d) nrm[d] *= 0.5;
-        const int from = kEdge[i][0], to = kEdge[i][1];
-        for (int sh = 0; sh < 4; ++sh) {
-            const double flux = kip[i] * (g[sh][0] * nrm[0] + g[sh][1] * nrm[1] + g[sh][2] * nrm[2]);
-            Ke[from][sh] -= flux;
-            Ke[to][sh] += flux;
-        }
-        if (conv) {
-            const double q = vel[0] * nrm[0] + vel[1] * nrm[1] + vel[2] * nrm[2];
-            const int up = q >= 0.0 ? from : to;
-            Ke[from][up] += q;
-            Ke[to][up] -= q;
-        }
-    }
-    if (!vol) return;
-    for (int i = 0; i < 4; ++i) {
-        double H[8][3];
-        const int8_t* pe = kPointEdge[i];
-        const int8_t* pf = kPointFace[i];
-        for (int d = 0; d < 3; ++d) {
-            H[0][d] = X[i][d];
-            H[1][d] = M[pe[1]][d];
-            H[2][d] = F[pf[0]][d];
-            H[3][d] = M[pe[0]][d];
-            H[4][d] = M[pe[2]][d];
-            H[5][d] = F[pf[2]][d];
-            H[6][d] = C[d];
-            H[7][d] = F[pf[1]][d];
-        }
-        vol[i] = hex_volume(H);
-    }
-}
-
 // the tetrahedron `code` of cell (ci,cj,ck): lattice coordinates of its corners
 __device__ __forceinline__ void tet_corners(uint16_t code, int ci, int cj, int ck, int (*c)[3]) {
     for (int q = 0; q < 4; ++q) {
@@ -188,9 +75,9 @@ __device__ __forceinline__ double tet_kip(const TetDisc& P, const int (*c)[3], i
     int64_t cell[3];
     for (int d = 0; d < 3; ++d) {
         auto Q = [&](int q) { return (int64_t)c[q][d] << P.shift; };
-        const int8_t* f0 = kFace[kEdgeFace[i][0]];
-        const int8_t* f1 = kFace[kEdgeFace[i][1]];
-        const int64_t s48 = 6 * (Q(kEdge[i][0]) + Q(kEdge[i][1])) + 4 * (Q(f0[0]) + Q(f0[1]) + Q(f0[2])) +
+        const int8_t* f0 = kTetFace[kTetEdgeFace[i][0]];
+        const int8_t* f1 = kTetFace[kTetEdgeFace[i][1]];
+        const int64_t s48 = 6 * (Q(kTetEdge[i][0]) + Q(kTetEdge[i][1])) + 4 * (Q(f0[0]) + Q(f0[1]) + Q(f0[2])) +
                             3 * (Q(0) + Q(1) + Q(2) + Q(3)) + 4 * (Q(f1[0]) + Q(f1[1]) + Q(f1[2]));
         int64_t cc = s48 / 48;
         if (cc >= P.n_fine) cc = P.n_fine - 1;

@@ -221,4 +221,76 @@ int csr_transpose_map(int64_t n, const int32_t* rowptr, const int32_t* col, int3
 int csr_adjoint_values(int64_t n, int64_t n_inner, const int32_t* rowptr, const int32_t* col, const int32_t* tp,
                        const double* val, double* valT, hipStream_t s);
 
+// ---- SparseMatrix surface (matrix_kernels.hip) ----
+// temporary bytes of the transpose's radix sort for nnz entries
+int transpose_sort_bytes(int64_t nnz, int32_t ncols, size_t* bytes);
+// stable sort of (column, entry) pairs; drop_zeros sends zero entries past the
+// last column; rowptr_t (ncols + 1) is read off the sorted keys (key2), the
+// sorted entry indices are in idx2; key, key2, idx, idx2, row_of hold nnz
+int transpose_csr(int64_t nrows, int32_t ncols, int64_t nnz, const int32_t* rowptr, const int32_t* col,
+                  const double* val, int drop_zeros, uint32_t* key, uint32_t* key2, int32_t* idx, int32_t* idx2,
+                  int32_t* row_of, void* temp, size_t temp_bytes, int32_t* rowptr_t, hipStream_t s);
+// col_t[k] = source row, val_t[k] = scale * value of sorted entry k < n
+int transpose_gather(int64_t n, const int32_t* idx_sorted, const int32_t* row_of, const double* val, double scale,
+                     int32_t* col_t, double* val_t, hipStream_t s);
+// dest = a1*v1 + b1*A^T w over the transpose (rp, cl, vl) of A (n = A's columns)
+int axpy_transposed(int64_t n, const int32_t* rp, const int32_t* cl, const double* vl, double a1, const double* v1,
+                    bool v1_is_dest, double b1, const double* w, double* dest, hipStream_t s);
+// *missing_flag = 1 if a listed row stores no diagonal entry
+int missing_diagonals(int64_t nlist, const int32_t* rows, const int32_t* rowptr, const int32_t* col,
+                      int* missing_flag, hipStream_t s);
+// SetDirichletRow for the listed rows (every one stores its diagonal)
+int dirichlet_rows(int64_t nlist, const int32_t* rows, const int32_t* rowptr, const int32_t* col, double* val,
+                   hipStream_t s);
+int scale_values(int64_t n, const double* src, double a, double* dst, hipStream_t s);
+
+// ---- element-loop assembly from connectivity (elem_assembly.hip) ----
+// temporary bytes of the pattern's sort and scan for ne elements of nv DoFs
+int elem_pattern_temp_bytes(int64_t ne, int nv, int64_t ndofs, size_t* sort_bytes, size_t* scan_bytes);
+// sorts the element couplings (keys/idx into key2/idx2), marks run heads,
+// scans them (run_incl), fills rowptr (ndofs + 1) and *nnz_dev; buffers of
+// ne*nv*nv entries, temp of max(sort_bytes, scan_bytes)
+int elem_pattern(int64_t ne, int nv, int64_t ndofs, const int32_t* dofs, uint64_t* key, uint64_t* key2, int32_t* idx,
+                 int32_t* idx2, int32_t* head, int32_t* run_incl, void* temp, size_t sort_bytes, size_t scan_bytes,
+                 int32_t* rowptr, int32_t* nnz_dev, hipStream_t s);
+// the columns (nnz) and the CSR slot of every local entry (ne*nv*nv, index e*nv*nv + a*nv + b)
+int elem_pattern_fill(int64_t ne, int nv, const uint64_t* key2, const int32_t* idx2, const int32_t* head,
+                      const int32_t* run_incl, int32_t* col, int32_t* slot_local, hipStream_t s);
+int elem_incidence_temp_bytes(int64_t n, int64_t maxkey, size_t* bytes);
+// DoF -> incidences (t = e*nv + a) sorted by DoF: inc_ptr (ndofs + 1), inc_t (ne*nv)
+int elem_incidence(int64_t ne, int nv, int64_t ndofs, const int32_t* dofs, uint32_t* key, uint32_t* key2, int32_t* idx,
+                   int32_t* inc_t, void* temp, size_t temp_bytes, int32_t* inc_ptr, hipStream_t s);
+// one colouring round; *coloured += elements coloured, *overflow = 1 past 64 colours
+int elem_colour_round(int64_t ne, int nv, const int32_t* dofs, const int32_t* inc_ptr, const int32_t* inc_t,
+                      int32_t* colour, int* coloured, int* overflow, hipStream_t s);
+// elements sorted by colour (order) and colour_ptr (ncolours + 1)
+int elem_colour_order(int64_t ne, int ncolours, const int32_t* colour, uint32_t* key, uint32_t* key2, int32_t* idx,
+                      int32_t* order, void* temp, size_t temp_bytes, int32_t* colour_ptr, hipStream_t s);
+int elem_slots_by_position(int64_t ne, int nv, const int32_t* order, const int32_t* slot_local, int32_t* slot,
+                           hipStream_t s);
+struct ElemLaunch {
+    int64_t ne = 0;
+    const int32_t* order = nullptr;
+    const int32_t* dofs = nullptr;
+    const double* coords = nullptr;
+    const double* kel = nullptr;
+    double f_const = 0.0;
+    const double* f_dof = nullptr;
+    double vel[3] = {0.0, 0.0, 0.0};
+    int conv = 0;
+    const int32_t* slot = nullptr;
+    double* val = nullptr;
+    double* rhs = nullptr;
+    const double* u = nullptr;
+    double* d = nullptr;
+};
+// one colour's elements (positions pos0 .. pos0+cnt-1 of the colour order);
+// elem_type 0 P1 triangle FV1, 1 P1 tetrahedron FV1, 2 Q1 hexahedron FE
+int elem_assemble_colour(int elem_type, const ElemLaunch& L, int64_t pos0, int64_t cnt, hipStream_t s);
+int zero_listed(int64_t n, const int32_t* idx, double* x, hipStream_t s);
+// element -> DoF table and k = exp(theta) per element of a lattice domain's
+// top level (kind 0 hexahedra, 1 triangles, 2 tetrahedra)
+int lattice_connectivity(int kind, int n, const int32_t* dof_of_lat, const uint16_t* cell_tets, const double* theta,
+                         int32_t* dofs, double* kel, hipStream_t s);
+
 }  // namespace ugamd
