@@ -180,6 +180,7 @@ class GPUSparseMatrix {
         require_device("device_csr");
         set_origin(nullptr);
         sell_valid_ = false;
+        dict_valid_ = false;
     }
     // build the sliced-ELL SpMV layout from the current CSR (provenance kept;
     // assembly and uploads call it); until then SpMV runs on the CSR directly
@@ -203,6 +204,22 @@ class GPUSparseMatrix {
     void set_row_order(const int32_t* order, int64_t n);
     // the same from device memory that is a permutation by construction
     void set_row_order_device(const int32_t* order, int64_t n);
+    // the slices of the dictionary layout (spmv_dict.hip): slot s (a multiple
+    // of 64 of them) holds row order[s] or -1, every row exactly once; slices
+    // whose entries touch at most 255 distinct columns (64 lattice points of a
+    // 4x4x4 block) take the layout, which stages x at those columns once per
+    // slice and stores one byte per column index. Device array (checked on
+    // the host unless by_construction, for orders the library generates);
+    // nullptr clears.
+    void set_slice_order_device(const int32_t* order, int64_t nslots, bool by_construction = false);
+    bool uses_dictionary() const { return dict_valid_; }
+    // 0: CSR, 1: sliced ELL, 2: the dictionary layout where a slice order
+    // exists (else sliced ELL)
+    void set_spmv_layout(int layout) {
+        dict_enabled_ = layout == 2;
+        sell_enabled_ = layout >= 1;
+        build_spmv_layout();
+    }
     bool has_row_order() const { return rows_ > 0 && row_order_.size() == rows_; }
     int lanes() const { return lanes_; }
     void set_lanes(int g) {
@@ -235,6 +252,13 @@ class GPUSparseMatrix {
     DeviceBuffer<int32_t> sell_width_, sell_col_, row_order_;
     DeviceBuffer<int64_t> sell_ptr_;
     DeviceBuffer<double> sell_val_;
+    // dictionary layout
+    bool build_dictionary();
+    bool dict_enabled_ = true, dict_valid_ = false;
+    DeviceBuffer<int32_t> slice_order_, d_width_, d_dptr_, d_dict_;
+    DeviceBuffer<int64_t> d_vptr_, d_iptr_;
+    DeviceBuffer<double> d_val_;
+    DeviceBuffer<uint32_t> d_idx_;
 };
 
 struct AlgebraType {

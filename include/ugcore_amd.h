@@ -256,10 +256,16 @@ int ugx_matrix_device_csr(void* m, int32_t** rowptr, int32_t** col, double** val
 int ugx_matrix_axpy(void* m, void* dest, double alpha, void* v1, double beta, void* w);   /* vectors are handles */
 /* tuning: lanes of a wavefront that share one CSR row in this matrix's SpMV (4..64, power of 2) */
 int ugx_matrix_set_spmv_lanes(void* m, int lanes);
-/* SpMV layout: 1 = sliced ELL (64-row slices, built from the CSR; default),
- * 0 = CSR directly. Call ugx_matrix_finalize after changing values through
- * the raw CSR pointers of ugx_matrix_device_csr. */
-int ugx_matrix_set_spmv_layout(void* m, int sliced_ell);
+/* SpMV layout: 2 = sliced ELL with a column dictionary per slice where the
+ * matrix has a slice order (operators the lattice discretisations assemble:
+ * 64 rows of a 4x4x4 or 8x8 block per slice, one-byte column positions, x
+ * staged once per slice; default), else sliced ELL; 1 = sliced ELL (64-row
+ * slices); 0 = CSR directly. Every layout is built from the CSR and sums each
+ * row in the same order as the sliced ELL. Call ugx_matrix_finalize after
+ * changing values through the raw CSR pointers of ugx_matrix_device_csr. */
+int ugx_matrix_set_spmv_layout(void* m, int layout);
+/* the layout SpMV runs on now: 0 CSR, 1 sliced ELL, 2 dictionary */
+int ugx_matrix_spmv_layout(void* m);
 /* Row permutation that fills the sliced-ELL slices (host array of n = rows
  * entries, checked to be a permutation; order = NULL clears it). Rows whose
  * columns lie close together should be neighbours in it, so that the column

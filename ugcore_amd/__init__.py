@@ -8,11 +8,50 @@ BiCGStab, Jacobi, ConvCheck (StdConvCheck), SparseMatrix, Vector.
 
 The shared library is built in-tree (ugcore_amd/libugcore_amd.so). If it is
 missing or cannot be loaded the package raises; there is no CPU fallback.
+
+HIP runtime: when torch is installed, importing this package first loads
+torch's own HIP runtime (torch/lib/libamdhip64.so, the same soname as the
+system runtime this library links), so that the library binds to it and one
+runtime serves both in the process. Device tensors and library buffers then
+share one address space and can be exchanged by pointer (ugcore_amd.adjoint,
+ugcore_amd.ug4py). UGX_HIP_RUNTIME=system keeps the system runtime.
 """
 import ctypes as C
+import importlib.util
 import os
 
 import numpy as np
+
+
+def _bind_torch_runtime():
+    """load torch's HIP runtime before this library (see the module notes);
+    returns its path, or None when torch or its runtime is absent"""
+    if os.environ.get("UGX_HIP_RUNTIME", "").lower() == "system":
+        return None
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.origin:
+        return None
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if not os.path.exists(path):
+        return None
+    try:
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+    except OSError:
+        return None
+    return path
+
+
+TORCH_HIP_RUNTIME = _bind_torch_runtime()
+
+
+def hip_runtimes():
+    """the HIP runtime libraries mapped into this process (one when this
+    library and torch share a runtime)"""
+    with open("/proc/self/maps") as f:
+        return sorted({line.split()[-1] for line in f if "libamdhip64" in line})
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libugcore_amd.so")
@@ -111,6 +150,7 @@ _SIGNATURES = {
     "ugx_matrix_axpy": (_I, [_P, _P, _D, _P, _D, _P]),
     "ugx_matrix_set_spmv_lanes": (_I, [_P, _I]),
     "ugx_matrix_set_spmv_layout": (_I, [_P, _I]),
+    "ugx_matrix_spmv_layout": (_I, [_P]),
     "ugx_matrix_set_row_order": (_I, [_P, _P, _I64]),
     "ugx_matrix_has_row_order": (_I, [_P]),
     "ugx_matrix_finalize": (_I, [_P]),
@@ -444,9 +484,19 @@ class SparseMatrix:
     def apply_sub(self, d, x):
         _check(lib().ugx_matrix_apply_sub(self.h, d.h, x.h), "SparseMatrix.apply_sub")
 
+    _LAYOUTS = {"csr": 0, "sell": 1, "dict": 2}
+
     def set_spmv_layout(self, layout):
-        """'sell' (sliced ELL, default) or 'csr'"""
-        _check(lib().ugx_matrix_set_spmv_layout(self.h, 1 if layout == "sell" else 0), "set_spmv_layout")
+        """'dict' (sliced ELL with a column dictionary per slice where the
+        matrix has a slice order, else sliced ELL; default), 'sell' (sliced
+        ELL) or 'csr'"""
+        if layout not in self._LAYOUTS:
+            raise ValueError(f"layout must be one of {sorted(self._LAYOUTS)}")
+        _check(lib().ugx_matrix_set_spmv_layout(self.h, self._LAYOUTS[layout]), "set_spmv_layout")
+
+    def spmv_layout(self):
+        """the layout SpMV runs on: 'csr', 'sell' or 'dict'"""
+        return {0: "csr", 1: "sell", 2: "dict"}[lib().ugx_matrix_spmv_layout(self.h)]
 
     def set_row_order(self, order):
         """Row permutation that fills the sliced-ELL slices (None clears it);

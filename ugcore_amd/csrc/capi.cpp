@@ -622,12 +622,17 @@ int ugx_matrix_set_spmv_lanes(void* m, int lanes) {
         return 0;
     });
 }
-int ugx_matrix_set_spmv_layout(void* m, int sliced_ell) {
+int ugx_matrix_set_spmv_layout(void* m, int layout) {
     return guard([&] {
+        if (layout < 0 || layout > 2) throw UGError("ugx_matrix_set_spmv_layout: 0 (CSR), 1 (sliced ELL) or 2 (dictionary)");
         auto& A = *((MatrixH*)m)->A;
-        A.set_use_sliced_ell(sliced_ell != 0);
+        A.set_spmv_layout(layout);
         return 0;
     });
+}
+int ugx_matrix_spmv_layout(void* m) {
+    auto& A = *((MatrixH*)m)->A;
+    return A.uses_dictionary() ? 2 : (A.uses_sliced_ell() ? 1 : 0);
 }
 int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n) {
     return guard([&] {

@@ -175,9 +175,87 @@ static int lanes_for(int64_t rows, int64_t nnz) {
     return 32;
 }
 
+void GPUSparseMatrix::set_slice_order_device(const int32_t* order, int64_t nslots, bool by_construction) {
+    if (!order) {
+        slice_order_.resize(0);
+        build_spmv_layout();
+        return;
+    }
+    if (nslots <= 0 || nslots % 64 != 0) throw UGError("set_slice_order_device: the slots must be a multiple of 64");
+    if (by_construction) {
+        slice_order_.resize(nslots);
+        UGX_HIP(hipMemcpyAsync(slice_order_.get(), order, sizeof(int32_t) * (size_t)nslots, hipMemcpyDeviceToDevice,
+                               context().stream));
+        build_spmv_layout();
+        return;
+    }
+    // every row exactly once
+    std::vector<int32_t> h((size_t)nslots);
+    UGX_HIP(hipMemcpy(h.data(), order, sizeof(int32_t) * (size_t)nslots, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> seen((size_t)rows_, 0);
+    int64_t cnt = 0;
+    for (int32_t r : h) {
+        if (r < 0) continue;
+        if (r >= rows_ || seen[(size_t)r]) throw UGError("set_slice_order_device: not a placement of the rows");
+        seen[(size_t)r] = 1;
+        ++cnt;
+    }
+    if (cnt != rows_) throw UGError("set_slice_order_device: rows are missing from the slices");
+    slice_order_.resize(nslots);
+    slice_order_.upload(h.data(), nslots);
+    build_spmv_layout();
+}
+
+bool GPUSparseMatrix::build_dictionary() {
+    auto& c = context();
+    const int64_t ns = slice_order_.size() / 64;
+    d_width_.resize(ns);
+    UGX_LAUNCH(dict_widths(ns, rows_, rowptr_.get(), slice_order_.get(), d_width_.get(), c.stream));
+    DeviceBuffer<int32_t> cnt(ns);
+    UGX_LAUNCH(dict_count(ns, rows_, rowptr_.get(), col_.get(), slice_order_.get(), d_width_.get(), cnt.get(),
+                          c.stream));
+    std::vector<int32_t> w((size_t)ns), u((size_t)ns);
+    d_width_.download(w.data(), ns);
+    cnt.download(u.data(), ns);
+    for (int64_t t = 0; t < ns; ++t)
+        if (u[(size_t)t] > 255) return false;   // too many columns (or too wide) for one-byte positions
+    std::vector<int32_t> dptr((size_t)ns + 1, 0);
+    std::vector<int64_t> vptr((size_t)ns + 1, 0), iptr((size_t)ns + 1, 0);
+    for (int64_t t = 0; t < ns; ++t) {
+        dptr[(size_t)t + 1] = dptr[(size_t)t] + u[(size_t)t];
+        vptr[(size_t)t + 1] = vptr[(size_t)t] + (int64_t)128 * ((w[(size_t)t] + 1) / 2);
+        iptr[(size_t)t + 1] = iptr[(size_t)t] + (int64_t)64 * ((w[(size_t)t] + 3) / 4);
+    }
+    d_dptr_.resize(ns + 1);
+    d_dptr_.upload(dptr.data(), ns + 1);
+    d_vptr_.resize(ns + 1);
+    d_vptr_.upload(vptr.data(), ns + 1);
+    d_iptr_.resize(ns + 1);
+    d_iptr_.upload(iptr.data(), ns + 1);
+    d_dict_.resize(std::max<int64_t>(dptr[(size_t)ns], 1));
+    d_val_.resize(std::max<int64_t>(vptr[(size_t)ns], 2));
+    d_idx_.resize(std::max<int64_t>(iptr[(size_t)ns], 1));
+    UGX_LAUNCH(dict_fill(ns, rows_, rowptr_.get(), col_.get(), val_.get(), slice_order_.get(), d_width_.get(),
+                         d_dptr_.get(), d_vptr_.get(), d_iptr_.get(), d_dict_.get(), d_val_.get(), d_idx_.get(),
+                         c.stream));
+    sync();
+    return true;
+}
+
 void GPUSparseMatrix::build_spmv_layout() {
     sell_valid_ = false;
-    if (!sell_enabled_ || rows_ == 0 || building_) return;
+    dict_valid_ = false;
+    if (rows_ == 0 || building_) return;
+    if (dict_enabled_ && slice_order_.size() > 0 && build_dictionary()) {
+        dict_valid_ = true;
+        // the sliced-ELL copy is not needed next to the dictionary layout
+        sell_col_.release();
+        sell_val_.release();
+        return;
+    }
+    d_val_.release();
+    d_idx_.release();
+    if (!sell_enabled_) return;
     auto& c = context();
     const int64_t nsl = (rows_ + 63) / 64;
     sell_width_.resize(nsl);
@@ -201,6 +279,8 @@ void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
     building_ = false;
     std::vector<std::vector<std::pair<int32_t, double>>>().swap(hrows_);
     sell_valid_ = false;
+    dict_valid_ = false;
+    slice_order_.resize(0);
     row_order_.resize(0);
     if (nnz >= ((int64_t)1 << 31)) throw UGError("GPUSparseMatrix: nnz exceeds int32 range");
     rows_ = rows;
@@ -540,7 +620,11 @@ void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, doub
     require_device("axpy");
     if (dest.size() != rows_ || w1.size() != cols_) throw UGError("GPUSparseMatrix::axpy size mismatch");
     if (&dest == &w1) throw UGError("GPUSparseMatrix::axpy: dest must not alias the multiplied vector");
-    if (sell_valid_)
+    if (dict_valid_)
+        UGX_LAUNCH(dict_spmv(rows_, slice_order_.size() / 64, d_width_.get(), d_dptr_.get(), d_dict_.get(),
+                             d_vptr_.get(), d_iptr_.get(), d_val_.get(), d_idx_.get(), slice_order_.get(), a1,
+                             v1.data(), b1, w1.data(), dest.data(), 0, rowptr_.get(), context().stream));
+    else if (sell_valid_)
         UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
                                               sell_val_.get(), a1, v1.data(), b1, w1.data(), dest.data(), 0,
                                               rowptr_.get(), has_row_order() ? row_order_.get() : nullptr,
@@ -552,7 +636,11 @@ void GPUSparseMatrix::axpy(GPUVector& dest, double a1, const GPUVector& v1, doub
 
 void GPUSparseMatrix::apply_ignore_zero_rows(GPUVector& dest, double beta, const GPUVector& w) const {
     require_device("apply_ignore_zero_rows");
-    if (sell_valid_)
+    if (dict_valid_)
+        UGX_LAUNCH(dict_spmv(rows_, slice_order_.size() / 64, d_width_.get(), d_dptr_.get(), d_dict_.get(),
+                             d_vptr_.get(), d_iptr_.get(), d_val_.get(), d_idx_.get(), slice_order_.get(), 0.0,
+                             dest.data(), beta, w.data(), dest.data(), 1, rowptr_.get(), context().stream));
+    else if (sell_valid_)
         UGX_LAUNCH((sell_spmv<double, double>(rows_, sell_ptr_.get(), sell_width_.get(), sell_col_.get(),
                                               sell_val_.get(), 0.0, dest.data(), beta, w.data(), dest.data(), 1,
                                               rowptr_.get(), has_row_order() ? row_order_.get() : nullptr,
@@ -1384,8 +1472,15 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     A.set_origin(this);
     int64_t lat_points = 1;
     for (int d = 0; d < L.dim; ++d) lat_points *= L.W;
-    if (lat_points == L.N) A.set_row_order_device(L.dof_of_lat.get(), L.N);
-    A.build_spmv_layout();
+    if (lat_points == L.N) {
+        A.set_row_order_device(L.dof_of_lat.get(), L.N);
+        const int64_t ns = lattice_block_slots(L.dim, L.W);
+        DeviceBuffer<int32_t> so(ns);
+        UGX_LAUNCH(lattice_block_order(L.dim, L.W, L.dof_of_lat.get(), so.get(), c.stream));
+        A.set_slice_order_device(so.get(), ns, true);   // builds the SpMV layout
+    } else {
+        A.build_spmv_layout();
+    }
     t_assemble_ = wall() - t0;
 }
 
@@ -1406,8 +1501,15 @@ void DomainDiscretization::assemble_adjoint_operator(const GPUSparseMatrix& A, G
     UGX_HIP(hipMemcpy(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) throw UGError("assemble_adjoint_operator: pattern is not structurally symmetric");
     AT.set_origin(nullptr);
-    if (A.has_row_order()) AT.set_row_order_device(L.dof_of_lat.get(), L.N);
-    AT.build_spmv_layout();
+    if (A.has_row_order()) {
+        AT.set_row_order_device(L.dof_of_lat.get(), L.N);
+        const int64_t ns = lattice_block_slots(L.dim, L.W);
+        DeviceBuffer<int32_t> so(ns);
+        UGX_LAUNCH(lattice_block_order(L.dim, L.W, L.dof_of_lat.get(), so.get(), c.stream));
+        AT.set_slice_order_device(so.get(), ns, true);
+    } else {
+        AT.build_spmv_layout();
+    }
 }
 
 void DomainDiscretization::level_dia_layout(int lev, int& nd, std::vector<int32_t>& offsets) const {

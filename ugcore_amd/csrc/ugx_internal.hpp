@@ -293,4 +293,23 @@ int zero_listed(int64_t n, const int32_t* idx, double* x, hipStream_t s);
 int lattice_connectivity(int kind, int n, const int32_t* dof_of_lat, const uint16_t* cell_tets, const double* theta,
                          int32_t* dofs, double* kel, hipStream_t s);
 
+// ---- sliced ELL with a column dictionary per slice (spmv_dict.hip) ----
+// slot s of the slice order holds row order[s] or -1; a slice is 64 slots
+int dict_widths(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* order, int32_t* width,
+                hipStream_t s);
+// distinct columns per slice (INT_MAX: the slice is wider than the layout takes)
+int dict_count(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* col, const int32_t* order,
+               const int32_t* width, int32_t* ucount, hipStream_t s);
+int dict_fill(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val,
+              const int32_t* order, const int32_t* width, const int32_t* dptr, const int64_t* vptr,
+              const int64_t* iptr, int32_t* dict, double* sval, uint32_t* sidx, hipStream_t s);
+// y = alpha*v1 + beta*A*x (mode 0) or y = beta*A*x on rows with entries (mode 1)
+int dict_spmv(int64_t nrows, int64_t nslices, const int32_t* width, const int32_t* dptr, const int32_t* dict,
+              const int64_t* vptr, const int64_t* iptr, const double* sval, const uint32_t* sidx,
+              const int32_t* order, double alpha, const double* v1, double beta, const double* x, double* y, int mode,
+              const int32_t* rowptr, hipStream_t s);
+// the slots of a lattice's rows in blocks of 64 points (4x4x4 or 8x8)
+int64_t lattice_block_slots(int dim, int W);
+int lattice_block_order(int dim, int W, const int32_t* dof_of_lat, int32_t* order, hipStream_t s);
+
 }  // namespace ugamd
