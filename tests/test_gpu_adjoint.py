@@ -1,6 +1,7 @@
-"""GPU tests of the adjoint gradient (torch.autograd through the C ABI):
+"""GPU tests of the adjoint gradient (torch.autograd over the ug4py bindings):
 the element kernel against the oracle, the full gradient against central
-differences and against the oracle's two-solve gradient.
+differences and against the oracle's two-solve gradient, with host tensors and
+with device tensors (which give the same gradient bit for bit).
 
 Tolerances: gradient vs finite differences and vs the two-solve gradient
 1e-6 relative; element kernel 1e-12 relative."""
@@ -19,19 +20,17 @@ def say(*a):
     sys.stdout.flush()
 
 
-# theta, u and the gradient live in host tensors; the library moves them to
-# and from the device itself
-DEV = "cpu"
+DEVICES = ["cpu", "cuda"]
 
 
-def setup(refs, seed=7, seed_true=11, n_obs=64, noise=1e-4):
+def setup(refs, seed=7, seed_true=11, n_obs=64, noise=1e-4, DEV="cpu"):
     from ugcore_amd import adjoint as ADJ
     import torch
     n = 2 ** refs
     theta = torch.tensor(O.make_theta(n, seed, 3), dtype=torch.float64, device=DEV)
     prob = ADJ.DiffusionProblem(refs, reduction=1e-14, max_steps=200)
     obs_lat = ADJ.observation_points(n, n_obs, 5)
-    dol = prob.disc.dof_of_lattice(refs)
+    dol = prob.dof_of_lattice()
     obs = torch.tensor(dol[obs_lat], dtype=torch.int64, device=DEV)
     # data from a second field theta* plus N(0, noise^2)
     with torch.no_grad():
@@ -67,10 +66,11 @@ def test_gradient_kernel_matches_oracle(refs):
     assert np.abs(g - go).max() <= 1e-12 * np.abs(go).max()
 
 
-def test_gradient_vs_central_differences():
-    say("case adjoint fd")
+@pytest.mark.parametrize("DEV", DEVICES)
+def test_gradient_vs_central_differences(DEV):
+    say(f"case adjoint fd {DEV}")
     refs = 4
-    ADJ, torch, prob, theta, obs, d = setup(refs)
+    ADJ, torch, prob, theta, obs, d = setup(refs, DEV=DEV)
     th = theta.clone().requires_grad_(True)
     J = loss(ADJ, prob, th, obs, d)
     J.backward()
@@ -86,15 +86,17 @@ def test_gradient_vs_central_differences():
             jm = loss(ADJ, prob, theta - h * v, obs, d).item()
             fd = (jp - jm) / (2 * h)
             ad = float((g * v).sum())
+            assert g.device.type == DEV
             worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-300))
     say(f"adjoint vs central differences: worst relative difference {worst:.2e}")
     assert worst <= 1e-6
 
 
-def test_gradient_vs_oracle_two_solves():
-    say("case adjoint two-solve")
+@pytest.mark.parametrize("DEV", DEVICES)
+def test_gradient_vs_oracle_two_solves(DEV):
+    say(f"case adjoint two-solve {DEV}")
     refs = 4
-    ADJ, torch, prob, theta, obs, d = setup(refs)
+    ADJ, torch, prob, theta, obs, d = setup(refs, DEV=DEV)
     th = theta.clone().requires_grad_(True)
     J = loss(ADJ, prob, th, obs, d)
     J.backward()
@@ -112,10 +114,11 @@ def test_gradient_vs_oracle_two_solves():
     assert np.abs(g - go).max() <= 1e-6 * np.abs(go).max()
 
 
-def test_gradient_without_stored_operator():
+@pytest.mark.parametrize("DEV", DEVICES)
+def test_gradient_without_stored_operator(DEV):
     """keep_operator=False drops A after the forward solve and re-assembles it
     in backward; the gradient is unchanged."""
-    say("case adjoint recompute")
+    say(f"case adjoint recompute {DEV}")
     refs = 3
     from ugcore_amd import adjoint as ADJ
     import torch
@@ -130,3 +133,23 @@ def test_gradient_without_stored_operator():
         (0.5 * (u[obs] ** 2).sum()).backward()
         grads.append(th.grad.detach().cpu().numpy())
     assert np.abs(grads[0] - grads[1]).max() <= 1e-12 * np.abs(grads[0]).max()
+
+
+def test_device_and_host_tensors_give_the_same_gradient():
+    """the device-tensor path moves data by device copies only; the gradient
+    equals the host-tensor one bit for bit, and torch and the library share one
+    HIP runtime"""
+    say("case adjoint device == host")
+    import ugcore_amd as U
+    refs = 4
+    grads, us = {}, {}
+    for DEV in DEVICES:
+        ADJ, torch, prob, theta, obs, d = setup(refs, DEV=DEV)
+        th = theta.clone().requires_grad_(True)
+        u = ADJ.solve(th, prob)
+        (0.5 * ((u[obs] - d) ** 2).sum()).backward()
+        grads[DEV] = th.grad.detach().cpu().numpy()
+        us[DEV] = u.detach().cpu().numpy()
+    assert len(U.hip_runtimes()) == 1
+    assert np.array_equal(us["cpu"], us["cuda"])
+    assert np.array_equal(grads["cpu"], grads["cuda"])

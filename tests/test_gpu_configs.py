@@ -128,22 +128,23 @@ def test_q1_256_mixed_precision_properties():
     assert abs(steps["f32"] - steps["f64"]) <= 2
 
 
-def test_adjoint_64_4096_observations():
-    say("case 64^3 adjoint gradient, 4096 observations")
+@pytest.mark.parametrize("DEV", ["cuda", "cpu"])
+def test_adjoint_64_4096_observations(DEV):
+    say(f"case 64^3 adjoint gradient, 4096 observations, {DEV} tensors")
     from ugcore_amd import adjoint as ADJ
     import torch
     refs = 6
     n = 64
     n_obs = 4096
-    theta = torch.tensor(O.make_theta(n, 7, 3), dtype=torch.float64)
+    theta = torch.tensor(O.make_theta(n, 7, 3), dtype=torch.float64, device=DEV)
     prob = ADJ.DiffusionProblem(refs, reduction=1e-14, max_steps=200)
     obs_lat = ADJ.observation_points(n, n_obs, 5)
-    obs = torch.tensor(prob.disc.dof_of_lattice(refs)[obs_lat], dtype=torch.int64)
+    obs = torch.tensor(prob.dof_of_lattice()[obs_lat], dtype=torch.int64, device=DEV)
     assert len(set(obs.tolist())) == n_obs
     with torch.no_grad():
-        u_true = ADJ.solve(torch.tensor(O.make_theta(n, 11, 3), dtype=torch.float64), prob)
+        u_true = ADJ.solve(torch.tensor(O.make_theta(n, 11, 3), dtype=torch.float64, device=DEV), prob)
     rng = np.random.default_rng(3)
-    dat = u_true[obs] + torch.tensor(1e-2 * rng.normal(size=n_obs))   # N(0, 1e-4): std 1e-2
+    dat = u_true[obs] + torch.tensor(1e-2 * rng.normal(size=n_obs), device=DEV)   # N(0, 1e-4): std 1e-2
 
     def loss(th):
         uu = ADJ.solve(th, prob)
@@ -152,7 +153,8 @@ def test_adjoint_64_4096_observations():
     th = theta.clone().requires_grad_(True)
     J = loss(th)
     J.backward()
-    g = th.grad.detach().numpy().copy()
+    assert th.grad.device == theta.device
+    g = th.grad.detach().cpu().numpy().copy()
     say(f"  J = {J.item():.6e}, forward {prob.last_steps['forward']} / adjoint {prob.last_steps['adjoint']} "
         f"CG steps")
 
@@ -167,7 +169,7 @@ def test_adjoint_64_4096_observations():
     with torch.no_grad():
         for k in range(16):
             v = rng.normal(size=theta.numel())
-            vt = torch.tensor(v)
+            vt = torch.tensor(v, device=DEV)
             up = ADJ.solve(theta + h * vt, prob)[obs]
             um = ADJ.solve(theta - h * vt, prob)[obs]
             fd = float((0.5 * (up - um) * (up + um - 2.0 * dat)).sum()) / (2 * h)
@@ -178,12 +180,12 @@ def test_adjoint_64_4096_observations():
 
     # the oracle's two-solve gradient, elementwise
     p = O.Problem(0, refs)
-    p.set_theta(theta.numpy())
+    p.set_theta(theta.cpu().numpy())
     p.assemble_all()
     uo = p.solve(base_lev=0, rel_red=1e-14)["x"]
-    o = obs.numpy()
+    o = obs.cpu().numpy()
     rhs = np.zeros_like(uo)
-    np.add.at(rhs, o, uo[o] - dat.numpy())
+    np.add.at(rhs, o, uo[o] - dat.cpu().numpy())
     lam = p.solve_rhs(rhs, base_lev=0, rel_red=1e-14)
     go = p.adjoint_gradient(uo, lam)
     err = np.abs(g - go).max() / np.abs(go).max()
