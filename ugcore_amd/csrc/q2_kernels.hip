@@ -216,29 +216,37 @@ __global__ void k_q2_lattice_count(int W, int64_t N, int32_t* cnt) {
     }
 }
 
-__global__ void k_q2_lattice_fill(int W, int64_t N, const int32_t* __restrict__ rowptr, int32_t* __restrict__ col) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % W), y = (int)((i / W) % W), z = (int)(i / ((int64_t)W * W));
-        int x0, x1, y0, y1, z0, z1;
-        q2_range(x, W, x0, x1);
-        q2_range(y, W, y0, y1);
-        q2_range(z, W, z0, z1);
-        int32_t p = rowptr[i];
-        for (int zz = z0; zz <= z1; ++zz)
-            for (int yy = y0; yy <= y1; ++yy)
-                for (int xx = x0; xx <= x1; ++xx) col[p++] = (int32_t)(((int64_t)zz * W + yy) * W + xx);
+// one wavefront per row: lane l writes entries l, l + 64 of the row's
+// (z, y, x)-ordered box of columns, so the row's writes are contiguous
+__global__ __launch_bounds__(256) void k_q2_lattice_fill(int W, int64_t N, const int32_t* __restrict__ rowptr,
+                                                         int32_t* __restrict__ col) {
+    const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int x = (int)(i % W), y = (int)((i / W) % W), z = (int)(i / ((int64_t)W * W));
+    int x0, x1, y0, y1, z0, z1;
+    q2_range(x, W, x0, x1);
+    q2_range(y, W, y0, y1);
+    q2_range(z, W, z0, z1);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, len = nx * ny * (z1 - z0 + 1);
+    const int32_t p = rowptr[i];
+    for (int e = lane; e < len; e += 64) {
+        const int xx = x0 + e % nx, yy = y0 + (e / nx) % ny, zz = z0 + e / (nx * ny);
+        col[p + e] = (int32_t)(((int64_t)zz * W + yy) * W + xx);
     }
 }
 
-__device__ __forceinline__ int32_t q2_slot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                           int32_t r, int32_t c) {
-    int32_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (col[mid] < c) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;   // the pattern contains (r, c) by construction
+// the CSR slot of (r, c): the row's columns are its box [x0,x1] x [y0,y1] x
+// [z0,z1] in (z, y, x) order, so the position follows from the offsets
+__device__ __forceinline__ int32_t q2_slot(int W, const int32_t* __restrict__ rowptr, int32_t r, int32_t c) {
+    const int x = r % W, y = (r / W) % W, z = r / (W * W);
+    const int cx = c % W, cy = (c / W) % W, cz = c / (W * W);
+    int x0, x1, y0, y1, z0, z1;
+    q2_range(x, W, x0, x1);
+    q2_range(y, W, y0, y1);
+    q2_range(z, W, z0, z1);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+    return rowptr[r] + ((cz - z0) * ny + (cy - y0)) * nx + (cx - x0);
 }
 
 struct Q2LatticeSrc {
@@ -264,7 +272,7 @@ struct Q2CsrSink {
     __device__ int32_t node(int a) const {
         return (int32_t)(((int64_t)(2 * ez + kQ2Mi[a][2]) * W + (2 * ey + kQ2Mi[a][1])) * W + (2 * ex + kQ2Mi[a][0]));
     }
-    __device__ void ke(int i, int j, double v) { val[q2_slot(rowptr, col, node(i), node(j))] += v; }
+    __device__ void ke(int i, int j, double v) { val[q2_slot(W, rowptr, node(i), node(j))] += v; }
     __device__ void fe(int a, double v) { rhs[node(a)] += v; }
 };
 
@@ -321,8 +329,7 @@ int q2_lattice_count(int n, int32_t* cnt, hipStream_t s) {
 int q2_lattice_fill(int n, const int32_t* rowptr, int32_t* col, hipStream_t s) {
     const int W = 2 * n + 1;
     const int64_t N = (int64_t)W * W * W;
-    hipLaunchKernelGGL(k_q2_lattice_fill, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 65536)), dim3(256), 0, s,
-                       W, N, rowptr, col);
+    hipLaunchKernelGGL(k_q2_lattice_fill, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, W, N, rowptr, col);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int q2_lattice_assemble(int n, const double* theta, double f_const, const int32_t* rowptr, const int32_t* col,
