@@ -179,9 +179,17 @@ class GPUSparseMatrix {
     void values_edited_externally() {
         require_device("device_csr");
         set_origin(nullptr);
+        pattern_id_ = fresh_pattern_id();   // the raw pointers also reach the pattern
         sell_valid_ = false;
         dict_valid_ = false;
     }
+    // identifies the current pattern: a fresh process-wide id on every
+    // allocation, host build and hand-out of the raw CSR pointers
+    uint64_t pattern_id() const { return pattern_id_; }
+    // the values changed on the device and the pattern did not: the SpMV
+    // layout takes the new values (only the values are rewritten when the
+    // layout was built for the current pattern, else it is rebuilt)
+    void refresh_layout_values();
     // build the sliced-ELL SpMV layout from the current CSR (provenance kept;
     // assembly and uploads call it); until then SpMV runs on the CSR directly
     void build_spmv_layout();
@@ -249,6 +257,8 @@ class GPUSparseMatrix {
     bool sell_enabled_ = true, sell_valid_ = false;
     const void* origin_ = nullptr;
     uint64_t version_ = 0;
+    static uint64_t fresh_pattern_id();
+    uint64_t pattern_id_ = 0, layout_pattern_id_ = ~(uint64_t)0;
     DeviceBuffer<int32_t> sell_width_, sell_col_, row_order_;
     DeviceBuffer<int64_t> sell_ptr_;
     DeviceBuffer<double> sell_val_;
@@ -463,7 +473,10 @@ class ElementDiscretization {
    private:
     void build();
     void run(double* val, double* rhs, const double* u, double* d);
-    void give_pattern(GPUSparseMatrix& A);
+    // hands the pattern to A (true: A still held it from the last assembly)
+    bool give_pattern(GPUSparseMatrix& A);
+    void finish_matrix(GPUSparseMatrix& A, bool kept);
+    uint64_t asm_pattern_id_ = 0;
     int type_, nv_, dim_;
     int64_t ne_, ndofs_, nnz_ = 0;
     DeviceBuffer<int32_t> dofs_, rowptr_, col_, order_, slot_, dir_;
@@ -591,6 +604,18 @@ class DomainDiscretization {
     uint64_t jseed_ = 0;
     double jamp_ = 0.0;
     double t_assemble_ = 0.0;
+    // pattern id of the matrix this discretisation last assembled: a matrix
+    // that still holds it keeps its pattern (and SpMV layout) on re-assembly
+    uint64_t asm_pattern_id_ = 0;
+    // Q1 unit-cube tables of the row-wise lattice assembly and exp(theta) per
+    // fine element (recomputed after the next set_theta)
+    DeviceBuffer<double> q1_tab_, kel_;
+    bool kel_valid_ = false;
+    // row start of every top-level lattice point in the lattice pattern's CSR
+    // (kinds 0, 1: the pattern depends on the numbering alone)
+    DeviceBuffer<int32_t> rp_lat_;
+    const double* exp_theta_device();
+    const double* q1_tables_device();
 };
 
 enum class Precision { F64 = 0, F32 = 1 };

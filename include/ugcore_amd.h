@@ -274,6 +274,14 @@ int ugx_matrix_spmv_layout(void* m);
  * reference (SparseMatrix has one storage, sparsematrix.h). */
 int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n);
 int ugx_matrix_has_row_order(void* m);
+/* Slices of the dictionary layout (host array of nslots entries, a multiple of
+ * 64): slice t holds rows order[64t .. 64t+63], -1 marks an empty slot, every
+ * row appears exactly once (checked). The layout is used when every slice's
+ * entries touch at most 255 distinct columns and no row of a slice is longer
+ * than 64; otherwise SpMV keeps the sliced ELL. order = NULL clears it.
+ * Assembly by a discretization sets 4x4x4 (8x8) lattice blocks. No
+ * counterpart in the reference. */
+int ugx_matrix_set_slice_order(void* m, const int32_t* order, int64_t nslots);
 /* after editing values through ugx_matrix_device_csr: rebuilds the SpMV layout */
 int ugx_matrix_finalize(void* m);
 int ugx_matrix_apply_sub(void* m, void* d, void* x);                                      /* d -= A*x */

@@ -152,6 +152,7 @@ _SIGNATURES = {
     "ugx_matrix_set_spmv_layout": (_I, [_P, _I]),
     "ugx_matrix_spmv_layout": (_I, [_P]),
     "ugx_matrix_set_row_order": (_I, [_P, _P, _I64]),
+    "ugx_matrix_set_slice_order": (_I, [_P, _P, _I64]),
     "ugx_matrix_has_row_order": (_I, [_P]),
     "ugx_matrix_finalize": (_I, [_P]),
     "ugx_matrix_apply_sub": (_I, [_P, _P, _P]),
@@ -507,11 +508,19 @@ class SparseMatrix:
         o = np.ascontiguousarray(order, dtype=np.int32)
         _check(lib().ugx_matrix_set_row_order(self.h, o.ctypes.data, o.size), "set_row_order")
 
+    def set_slice_order(self, order):
+        """Slices of the dictionary layout: slice t holds rows
+        order[64t .. 64t+63] (-1: empty slot), every row once; None clears it.
+        SpMV uses the layout when every slice touches at most 255 distinct
+        columns (and no row of a slice is longer than 64)."""
+        if order is None:
+            _check(lib().ugx_matrix_set_slice_order(self.h, None, 0), "set_slice_order")
+            return
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        _check(lib().ugx_matrix_set_slice_order(self.h, o.ctypes.data, o.size), "set_slice_order")
+
     def has_row_order(self):
         return bool(lib().ugx_matrix_has_row_order(self.h))
-
-    def finalize(self):
-        _check(lib().ugx_matrix_finalize(self.h), "SparseMatrix.finalize")
 
     def set_spmv_lanes(self, lanes):
         _check(lib().ugx_matrix_set_spmv_lanes(self.h, int(lanes)), "SparseMatrix.set_spmv_lanes")

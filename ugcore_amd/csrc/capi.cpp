@@ -634,6 +634,21 @@ int ugx_matrix_spmv_layout(void* m) {
     auto& A = *((MatrixH*)m)->A;
     return A.uses_dictionary() ? 2 : (A.uses_sliced_ell() ? 1 : 0);
 }
+int ugx_matrix_set_slice_order(void* m, const int32_t* order, int64_t nslots) {
+    return guard([&] {
+        auto& A = *((MatrixH*)m)->A;
+        if (!order) {
+            A.set_slice_order_device(nullptr, 0);
+            return 0;
+        }
+        if (nslots <= 0 || nslots % 64 != 0)
+            throw UGError("ugx_matrix_set_slice_order: the slots must be a positive multiple of 64");
+        DeviceBuffer<int32_t> d(nslots);
+        d.upload(order, nslots);
+        A.set_slice_order_device(d.get(), nslots);   // checked on the host
+        return 0;
+    });
+}
 int ugx_matrix_set_row_order(void* m, const int32_t* order, int64_t n) {
     return guard([&] {
         ((MatrixH*)m)->A->set_row_order(order, n);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <cstdint>
 
@@ -309,7 +310,41 @@ __global__ void k_lattice_tet_dofs(int n, const uint16_t* __restrict__ cell_tets
     kel[e] = theta ? exp(theta[cell]) : 1.0;
 }
 
+struct WidenI32 {
+    __host__ __device__ int64_t operator()(int32_t v) const { return v; }
+};
+
+__global__ void k_rowptr_from_incl(int64_t n, const int64_t* __restrict__ incl, int32_t* __restrict__ rowptr) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > n) return;
+    rowptr[i] = i == 0 ? 0 : (int32_t)incl[i - 1];
+}
+
 }  // namespace
+
+// ---- row pointers from row counts (scanned in 64 bits) -------------------------
+
+int rowptr_scan_temp_bytes(int64_t n, size_t* bytes) {
+    *bytes = 0;
+    if (n <= 0) return 0;
+    auto in = rocprim::make_transform_iterator((const int32_t*)nullptr, WidenI32());
+    return rocprim::inclusive_scan(nullptr, *bytes, in, (int64_t*)nullptr, (size_t)n, rocprim::plus<int64_t>()) ==
+                   hipSuccess
+               ? 0
+               : -1;
+}
+
+int rowptr_from_counts(int64_t n, const int32_t* cnt, int64_t* incl, void* temp, size_t bytes, int32_t* rowptr,
+                       hipStream_t s) {
+    if (n > 0) {
+        auto in = rocprim::make_transform_iterator(cnt, WidenI32());
+        size_t b = bytes;
+        if (rocprim::inclusive_scan(temp, b, in, incl, (size_t)n, rocprim::plus<int64_t>(), s) != hipSuccess)
+            return -1;
+    }
+    hipLaunchKernelGGL(k_rowptr_from_incl, dim3(blocks(n + 1, 256)), dim3(256), 0, s, n, incl, rowptr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 // ---- pattern -------------------------------------------------------------------
 

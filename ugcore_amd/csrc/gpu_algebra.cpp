@@ -1,4 +1,5 @@
 // Host implementation of the GPUAlgebra classes (include/ug_gpu/gpu_algebra.hpp).
+#include <atomic>
 #include "ug_gpu/gpu_algebra.hpp"
 
 #include <algorithm>
@@ -242,9 +243,33 @@ bool GPUSparseMatrix::build_dictionary() {
     return true;
 }
 
+uint64_t GPUSparseMatrix::fresh_pattern_id() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
+void GPUSparseMatrix::refresh_layout_values() {
+    if (rows_ == 0 || building_) return;
+    if (layout_pattern_id_ != pattern_id_ || (!dict_valid_ && !sell_valid_)) {
+        build_spmv_layout();
+        return;
+    }
+    auto& c = context();
+    if (dict_valid_) {
+        UGX_LAUNCH(dict_values(slice_order_.size() / 64, rows_, rowptr_.get(), val_.get(), slice_order_.get(),
+                               d_width_.get(), d_vptr_.get(), d_val_.get(), c.stream));
+    } else {
+        const int32_t* order = has_row_order() ? row_order_.get() : nullptr;
+        UGX_LAUNCH(sell_fill<double>(rows_, cols_, rowptr_.get(), col_.get(), val_.get(), order, sell_ptr_.get(),
+                                     sell_width_.get(), sell_col_.get(), sell_val_.get(), c.stream, true));
+    }
+    sync();
+}
+
 void GPUSparseMatrix::build_spmv_layout() {
     sell_valid_ = false;
     dict_valid_ = false;
+    layout_pattern_id_ = pattern_id_;
     if (rows_ == 0 || building_) return;
     if (dict_enabled_ && slice_order_.size() > 0 && build_dictionary()) {
         dict_valid_ = true;
@@ -277,6 +302,7 @@ void GPUSparseMatrix::build_spmv_layout() {
 
 void GPUSparseMatrix::allocate(int64_t rows, int64_t cols, int64_t nnz) {
     building_ = false;
+    pattern_id_ = fresh_pattern_id();
     std::vector<std::vector<std::pair<int32_t, double>>>().swap(hrows_);
     sell_valid_ = false;
     dict_valid_ = false;
@@ -336,7 +362,7 @@ void GPUSparseMatrix::require_device(const char* what) const {
 
 void GPUSparseMatrix::values_changed_on_device() {
     set_origin(nullptr);
-    build_spmv_layout();
+    refresh_layout_values();
 }
 
 void GPUSparseMatrix::resize_and_clear(int64_t rows, int64_t cols) {
@@ -352,6 +378,12 @@ void GPUSparseMatrix::resize_and_clear(int64_t rows, int64_t cols) {
     sell_ptr_.release();
     row_order_.release();
     sell_valid_ = false;
+    slice_order_.release();
+    d_dict_.release();
+    d_val_.release();
+    d_idx_.release();
+    dict_valid_ = false;
+    pattern_id_ = fresh_pattern_id();
     t_rowptr_.release();
     t_col_.release();
     t_val_.release();
@@ -1112,6 +1144,7 @@ void DomainDiscretization::set_theta(const double* theta_host) {
     theta_.resize(n);
     theta_.upload(theta_host, n);
     has_theta_ = true;
+    kel_valid_ = false;
 }
 
 void DomainDiscretization::set_theta_device(const double* theta_dev) {
@@ -1120,6 +1153,27 @@ void DomainDiscretization::set_theta_device(const double* theta_dev) {
     theta_.resize(n);
     UGX_HIP(hipMemcpyAsync(theta_.get(), theta_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, context().stream));
     has_theta_ = true;
+    kel_valid_ = false;
+}
+
+const double* DomainDiscretization::exp_theta_device() {
+    if (!has_theta_) return nullptr;
+    if (!kel_valid_) {
+        kel_.resize(theta_.size());
+        UGX_LAUNCH(exp_values(theta_.size(), theta_.get(), kel_.get(), context().stream));
+        kel_valid_ = true;
+    }
+    return kel_.get();
+}
+
+const double* DomainDiscretization::q1_tables_device() {
+    if (q1_tab_.size() != kQ1Tab) {
+        double tab[kQ1Tab];
+        q1_lattice_tables(tab);
+        q1_tab_.resize(kQ1Tab);
+        q1_tab_.upload(tab, kQ1Tab);
+    }
+    return q1_tab_.get();
 }
 
 void DomainDiscretization::adjust_defect(GPUVector& d) {
@@ -1311,7 +1365,11 @@ void ElementDiscretization::run(double* val, double* rhs, const double* u, doubl
                                         c.stream));
 }
 
-void ElementDiscretization::give_pattern(GPUSparseMatrix& A) {
+bool ElementDiscretization::give_pattern(GPUSparseMatrix& A) {
+    // a matrix this object assembled whose pattern is unchanged keeps it
+    if (A.origin() == this && A.pattern_id() == asm_pattern_id_ && !A.building() && A.num_rows() == ndofs_ &&
+        A.total_num_connections() == nnz_)
+        return true;
     auto& c = context();
     A.allocate(ndofs_, ndofs_, nnz_);
     A.set_origin(nullptr);
@@ -1319,11 +1377,19 @@ void ElementDiscretization::give_pattern(GPUSparseMatrix& A) {
                            c.stream));
     if (nnz_ > 0)
         UGX_HIP(hipMemcpyAsync(A.col(), col_.get(), sizeof(int32_t) * nnz_, hipMemcpyDeviceToDevice, c.stream));
+    return false;
+}
+
+void ElementDiscretization::finish_matrix(GPUSparseMatrix& A, bool kept) {
+    A.set_origin(this);
+    if (kept) A.refresh_layout_values();
+    else A.build_spmv_layout();
+    asm_pattern_id_ = A.pattern_id();
 }
 
 void ElementDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     auto& c = context();
-    give_pattern(A);
+    const bool kept = give_pattern(A);
     if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, A.val(), 0.0, A.val(), c.stream));
     b.resize(ndofs_);
     b.set(0.0);
@@ -1333,18 +1399,18 @@ void ElementDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
         UGX_LAUNCH(zero_listed(ndir_, dir_.get(), b.data(), c.stream));
     }
     sync();
-    A.build_spmv_layout();
+    finish_matrix(A, kept);
 }
 
 void ElementDiscretization::assemble_jacobian(GPUSparseMatrix& J, const GPUVector& u) {
     if (u.size() != ndofs_) throw UGError("ElementDiscretization::assemble_jacobian: size mismatch");
     auto& c = context();
-    give_pattern(J);
+    const bool kept = give_pattern(J);
     if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, J.val(), 0.0, J.val(), c.stream));
     run(J.val(), nullptr, nullptr, nullptr);
     if (ndir_ > 0) UGX_LAUNCH(dirichlet_rows(ndir_, dir_.get(), J.rowptr(), J.col(), J.val(), c.stream));
     sync();
-    J.build_spmv_layout();
+    finish_matrix(J, kept);
 }
 
 void ElementDiscretization::assemble_defect(GPUVector& d, const GPUVector& u) {
@@ -1424,45 +1490,70 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     auto& c = context();
     const double t0 = wall();
     LevelGrid& L = *levels_.back();
-    DeviceBuffer<int32_t> cnt(L.N);
-    DeviceBuffer<uint32_t> mask;
-    if (kind_ == 2) {
-        mask.resize(L.N);
-        mask.zero();
-        UGX_LAUNCH(tet_mask(L.n, L.cell_tets.get(), mask.get(), c.stream));
-        UGX_LAUNCH(pattern_count_mask(L.N, L.lat_of_dof.get(), mask.get(), cnt.get(), c.stream));
-    } else {
-        UGX_LAUNCH(pattern_count(L.dim, L.W, L.N, L.lat_of_dof.get(), cnt.get(), c.stream));
+    // the pattern is built once: a matrix this discretisation assembled whose
+    // pattern has not changed since keeps it, and its SpMV layout takes the
+    // new values in place
+    const bool reuse = A.origin() == this && A.pattern_id() == asm_pattern_id_ && !A.building() &&
+                       A.num_rows() == L.N && A.num_cols() == L.N;
+    if (!reuse) {
+        DeviceBuffer<int32_t> cnt(L.N);
+        DeviceBuffer<uint32_t> mask;
+        if (kind_ == 2) {
+            mask.resize(L.N);
+            mask.zero();
+            UGX_LAUNCH(tet_mask(L.n, L.cell_tets.get(), mask.get(), c.stream));
+            UGX_LAUNCH(pattern_count_mask(L.N, L.lat_of_dof.get(), mask.get(), cnt.get(), c.stream));
+        } else {
+            UGX_LAUNCH(pattern_count(L.dim, L.W, L.N, L.lat_of_dof.get(), cnt.get(), c.stream));
+        }
+        // row pointers scanned on the device (64-bit sums, total checked)
+        DeviceBuffer<int64_t> incl(L.N);
+        size_t tb = 0;
+        if (rowptr_scan_temp_bytes(L.N, &tb) != 0) throw UGError("assemble_linear: scan set-up failed");
+        DeviceBuffer<uint8_t> temp((int64_t)std::max<size_t>(tb, 1));
+        DeviceBuffer<int32_t> rowptr(L.N + 1);
+        UGX_LAUNCH(rowptr_from_counts(L.N, cnt.get(), incl.get(), temp.get(), tb, rowptr.get(), c.stream));
+        int64_t nnz = 0;
+        if (L.N > 0) incl.download(&nnz, 1, L.N - 1);
+        if (nnz >= ((int64_t)1 << 31)) throw UGError("assemble_linear: nnz exceeds int32 range");
+        A.allocate(L.N, L.N, nnz);
+        UGX_HIP(hipMemcpyAsync(A.rowptr(), rowptr.get(), sizeof(int32_t) * (L.N + 1), hipMemcpyDeviceToDevice,
+                               c.stream));
+        if (kind_ == 2) {
+            UGX_LAUNCH(pattern_fill_mask(L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), mask.get(), A.rowptr(),
+                                         A.col(), c.stream));
+        } else {
+            // row start per lattice point (the same for every matrix of this
+            // pattern), then one lattice point per 32 lanes, columns by rank
+            if (rp_lat_.size() != L.N) {
+                rp_lat_.resize(L.N);
+                UGX_LAUNCH(rowptr_of_lattice(L.N, L.dof_of_lat.get(), A.rowptr(), rp_lat_.get(), c.stream));
+            }
+            UGX_LAUNCH(lattice_pattern(L.dim, L.W, L.dof_of_lat.get(), rp_lat_.get(), A.col(), c.stream));
+        }
     }
-    std::vector<int32_t> h(L.N + 1, 0);
-    cnt.download(h.data() + 1, L.N);
-    int64_t acc = 0;
-    for (int64_t i = 1; i <= L.N; ++i) {
-        acc += h[i];
-        if (acc >= ((int64_t)1 << 31)) throw UGError("assemble_linear: nnz exceeds int32 range");
-        h[i] = (int32_t)acc;
-    }
-    A.allocate(L.N, L.N, acc);
-    UGX_HIP(hipMemcpyAsync(A.rowptr(), h.data(), sizeof(int32_t) * (L.N + 1), hipMemcpyHostToDevice, c.stream));
-    if (kind_ == 2)
-        UGX_LAUNCH(pattern_fill_mask(L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), mask.get(), A.rowptr(), A.col(),
-                                     c.stream));
-    else
-        UGX_LAUNCH(pattern_fill(L.dim, L.W, L.N, L.lat_of_dof.get(), L.dof_of_lat.get(), A.rowptr(), A.col(),
-                                c.stream));
-    UGX_HIP(hipMemsetAsync(A.val(), 0, sizeof(double) * acc, c.stream));
+    const int64_t acc = A.total_num_connections();
     b.resize(L.N);
-    b.set(0.0);
     const double hsz = 1.0 / L.n;
     const int nfine = 1 << refs_;
+    if (kind_ == 0) {
+        // row by row: every entry and every rhs entry written once
+        if (nfine != L.n) throw UGError("assemble_linear: the top level must be the fine level");
+        if (rp_lat_.size() != L.N) {
+            rp_lat_.resize(L.N);
+            UGX_LAUNCH(rowptr_of_lattice(L.N, L.dof_of_lat.get(), A.rowptr(), rp_lat_.get(), c.stream));
+        }
+        UGX_LAUNCH(q1_lattice_csr(L.n, hsz, f_const_, q1_tables_device(), exp_theta_device(), L.dof_of_lat.get(),
+                                  rp_lat_.get(), A.val(), b.data(), c.stream));
+    } else {
+        UGX_HIP(hipMemsetAsync(A.val(), 0, sizeof(double) * acc, c.stream));
+        b.set(0.0);
+    }
     if (kind_ == 2) {
         UGX_LAUNCH(assemble_fv1_tet<double>(tet_disc(*this, L, top_level(), theta_device()), L.cell_tets.get(),
                                             L.dof_of_lat.get(), A.rowptr(), A.col(), A.val(), nullptr, b.data(),
                                             c.stream));
-    } else if (kind_ == 0) {
-        UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), A.rowptr(),
-                                           A.col(), A.val(), nullptr, 0, b.data(), c.stream));
-    } else {
+    } else if (kind_ == 1) {
         UGX_LAUNCH(assemble_fv1_tri<double>(L.n, hsz, theta_device(), nfine, has_f_ ? f_dof_.get() : nullptr,
                                             L.dof_of_lat.get(), A.rowptr(), A.col(), A.val(), nullptr, 0, b.data(),
                                             c.stream));
@@ -1470,17 +1561,22 @@ void DomainDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     UGX_LAUNCH(dirichlet_csr<double>(L.N, L.n_inner, A.rowptr(), A.col(), A.val(), b.data(), c.stream));
     sync();
     A.set_origin(this);
-    int64_t lat_points = 1;
-    for (int d = 0; d < L.dim; ++d) lat_points *= L.W;
-    if (lat_points == L.N) {
-        A.set_row_order_device(L.dof_of_lat.get(), L.N);
-        const int64_t ns = lattice_block_slots(L.dim, L.W);
-        DeviceBuffer<int32_t> so(ns);
-        UGX_LAUNCH(lattice_block_order(L.dim, L.W, L.dof_of_lat.get(), so.get(), c.stream));
-        A.set_slice_order_device(so.get(), ns, true);   // builds the SpMV layout
+    if (reuse) {
+        A.refresh_layout_values();
     } else {
-        A.build_spmv_layout();
+        int64_t lat_points = 1;
+        for (int d = 0; d < L.dim; ++d) lat_points *= L.W;
+        if (lat_points == L.N) {
+            A.set_row_order_device(L.dof_of_lat.get(), L.N);
+            const int64_t ns = lattice_block_slots(L.dim, L.W);
+            DeviceBuffer<int32_t> so(ns);
+            UGX_LAUNCH(lattice_block_order(L.dim, L.W, L.dof_of_lat.get(), so.get(), c.stream));
+            A.set_slice_order_device(so.get(), ns, true);   // builds the SpMV layout
+        } else {
+            A.build_spmv_layout();
+        }
     }
+    asm_pattern_id_ = A.pattern_id();
     t_assemble_ = wall() - t0;
 }
 
@@ -1537,8 +1633,7 @@ void DomainDiscretization::assemble_level_dia(int lev, DeviceBuffer<double>& dia
                                             L.dof_of_lat.get(), nullptr, nullptr, nullptr, dia.get(), nullptr,
                                             c.stream));
     } else if (kind_ == 0) {
-        UGX_LAUNCH(assemble_q1_hex<double>(L.n, hsz, theta_device(), nfine, f_const_, L.dof_of_lat.get(), nullptr,
-                                           nullptr, nullptr, dia.get(), L.N, nullptr, c.stream));
+        UGX_LAUNCH(q1_lattice_dia(L.n, nfine, hsz, q1_tables_device(), exp_theta_device(), dia.get(), c.stream));
     } else {
         UGX_LAUNCH(assemble_fv1_tri<double>(L.n, hsz, theta_device(), nfine, nullptr, L.dof_of_lat.get(), nullptr,
                                             nullptr, nullptr, dia.get(), L.N, nullptr, c.stream));

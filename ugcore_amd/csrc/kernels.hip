@@ -101,9 +101,6 @@ __device__ __forceinline__ bool map_row(const RowMap& m, int64_t n, int64_t& i) 
 // values of diagonal d for rows 64t..64t+63 are contiguous, and the ND
 // diagonals of one tile follow each other, so a wavefront streams one
 // contiguous ND*64*sizeof(T) block.
-__host__ __device__ __forceinline__ int64_t dia_index(int d, int64_t i, int nd) {
-    return ((i >> 6) * nd + d) * 64 + (i & 63);
-}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum_group(T v, int G) {
@@ -205,25 +202,59 @@ __global__ void k_sell_widths(int64_t nrows, const int32_t* __restrict__ rowptr,
     width[t] = w;
 }
 
+// one wavefront per slice: the slice's entries move through LDS in chunks of
+// 8 per row, read row by row (contiguous CSR runs) and written column by
+// column (contiguous 64-entry runs of the slice block); padding: the row's own
+// index as column (0 beyond the columns), value 0. vals_only: the columns are
+// already in place (same pattern), only the values are rewritten
 template <typename TM>
-__global__ void k_sell_fill(int64_t nrows, int64_t ncols, const int32_t* __restrict__ rowptr,
-                            const int32_t* __restrict__ col, const TM* __restrict__ val,
-                            const int32_t* __restrict__ order, const int64_t* __restrict__ slice_ptr,
-                            const int32_t* __restrict__ width, int32_t* __restrict__ scol, TM* __restrict__ sval) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t t = p >> 6;
-    if (t >= (nrows + 63) / 64) return;
-    const int lane = (int)(p & 63);
+__global__ __launch_bounds__(64) void k_sell_fill(int64_t nrows, int64_t ncols, const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col, const TM* __restrict__ val,
+                                                  const int32_t* __restrict__ order, const int64_t* __restrict__ slice_ptr,
+                                                  const int32_t* __restrict__ width, int32_t* __restrict__ scol,
+                                                  TM* __restrict__ sval, int vals_only) {
+    constexpr int C = 8;
+    __shared__ int32_t rbs[64], lens[64];
+    __shared__ int32_t ct[C * 64];
+    __shared__ TM vt[C * 64];
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t p = t * 64 + lane;
+    const int64_t r = p < nrows ? sell_row(order, p) : p;
+    rbs[lane] = r < nrows ? rowptr[r] : 0;
+    lens[lane] = r < nrows ? rowptr[r + 1] - rowptr[r] : 0;
+    const int32_t pad_col = (int32_t)(r < ncols ? r : 0);
     const int32_t w = width[t];
     const int64_t base = slice_ptr[t] + lane;
-    const int64_t r = p < nrows ? sell_row(order, p) : p;
-    const int32_t rb = r < nrows ? rowptr[r] : 0;
-    const int32_t len = r < nrows ? rowptr[r + 1] - rb : 0;
-    const int32_t pad_col = (int32_t)(r < ncols ? r : 0);
-    for (int32_t k = 0; k < w; ++k) {
-        const bool real = k < len;
-        scol[base + (int64_t)64 * k] = real ? col[rb + k] : pad_col;
-        sval[base + (int64_t)64 * k] = real ? val[rb + k] : (TM)0;
+    __syncthreads();
+    for (int k0 = 0; k0 < w; k0 += C) {
+#pragma unroll
+        for (int it = 0; it < C; ++it) {
+            const int e = it * 64 + lane;
+            const int j = e / C, kk = e % C, k = k0 + kk;
+            int32_t c = -1;
+            TM v = (TM)0;
+            if (k < lens[j]) {
+                const int64_t a = (int64_t)rbs[j] + k;
+                v = val[a];
+                if (!vals_only) c = col[a];
+            }
+            const int slot = kk * 64 + ((j + 8 * kk) & 63);   // rotated: fewer bank conflicts
+            ct[slot] = c;
+            vt[slot] = v;
+        }
+        __syncthreads();
+        const int kc = w - k0 < C ? w - k0 : C;
+        for (int kk = 0; kk < kc; ++kk) {
+            const int slot = kk * 64 + ((lane + 8 * kk) & 63);
+            const int64_t o = base + (int64_t)64 * (k0 + kk);
+            if (!vals_only) {
+                const int32_t c = ct[slot];
+                scol[o] = c >= 0 ? c : pad_col;
+            }
+            sval[o] = vt[slot];
+        }
+        __syncthreads();
     }
 }
 
@@ -597,7 +628,7 @@ __global__ __launch_bounds__(256) void k_dia_apply_sub(int64_t n, DiaOffsets o, 
     if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_tiled_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -616,7 +647,7 @@ __global__ __launch_bounds__(256) void k_dia_axpy(int64_t n, DiaOffsets o, const
     if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_tiled_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -639,7 +670,7 @@ __global__ __launch_bounds__(256) void k_dia_jacobi_step(int64_t n, DiaOffsets o
     if (!map_row(rm, n, i)) return;
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_index(d, i, ND)];
+    for (int d = 0; d < ND; ++d) a[d] = (T)A[dia_tiled_index(d, i, ND)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int64_t j = i + o.off[d];
@@ -674,9 +705,9 @@ __global__ void k_dia_symmetrize(int64_t n, int nd, DiaOffsets o, const uint8_t*
         for (int u = 0; u < nu; ++u) {
             const int d = nd / 2 + u;
             const int64_t j = i + o.off[d];
-            TM v = full[dia_index(d, i, nd)];
+            TM v = full[dia_tiled_index(d, i, nd)];
             if (u != 0 && j >= 0 && j < n && dir[j]) v = (TM)0;   // eliminate Dirichlet columns
-            up[dia_index(u, i, nu)] = v;
+            up[dia_tiled_index(u, i, nu)] = v;
         }
     }
 }
@@ -694,10 +725,10 @@ __device__ __forceinline__ void dias_apply_sub_row(int64_t i, int64_t n, const D
     constexpr int NU = ND / 2 + 1;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_tiled_index(u, i, NU)];
 #pragma unroll
     for (int d = 0; d < ND / 2; ++d)
-        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
+        a[d] = (T)U[dia_tiled_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -725,10 +756,10 @@ __global__ __launch_bounds__(256) void k_dias_axpy(int64_t n, DiaOffsets o, cons
     if (!map_row(rm, n, i)) return;
     T a[ND], xv[ND];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_tiled_index(u, i, NU)];
 #pragma unroll
     for (int d = 0; d < ND / 2; ++d)
-        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
+        a[d] = (T)U[dia_tiled_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) xv[d] = (T)x[i + o.off[d]];
     T s = (T)0;
@@ -747,10 +778,10 @@ __device__ __forceinline__ void dias_jacobi_row(int64_t i, int64_t n, const DiaO
     constexpr int NU = ND / 2 + 1;
     T a[ND], dv[ND], sv[ND];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_index(u, i, NU)];
+    for (int u = 0; u < NU; ++u) a[ND / 2 + u] = (T)U[dia_tiled_index(u, i, NU)];
 #pragma unroll
     for (int d = 0; d < ND / 2; ++d)
-        a[d] = (T)U[dia_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
+        a[d] = (T)U[dia_tiled_index(ND - 1 - d - ND / 2, clamp_row<ND>(i + o.off[d], n), NU)];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int64_t j = i + o.off[d];
@@ -849,9 +880,9 @@ template <typename TM, typename T>
 __device__ __forceinline__ void load_sym27(const TM* __restrict__ U, const DiaOffsets& o, int64_t i, int64_t n,
                                            T (&a)[27]) {
 #pragma unroll
-    for (int u = 0; u < 14; ++u) a[13 + u] = (T)U[dia_index(u, i, 14)];
+    for (int u = 0; u < 14; ++u) a[13 + u] = (T)U[dia_tiled_index(u, i, 14)];
 #pragma unroll
-    for (int d = 0; d < 13; ++d) a[d] = (T)U[dia_index(26 - d - 13, clamp_row<27>(i + o.off[d], n), 14)];
+    for (int d = 0; d < 13; ++d) a[d] = (T)U[dia_tiled_index(26 - d - 13, clamp_row<27>(i + o.off[d], n), 14)];
 }
 
 // d_out = d_in - A*x
@@ -1439,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_jacobi_convert_dot(int64_t n, const TV*
 template <typename TM, typename TV>
 __global__ void k_dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double m = (double)dia[dia_index(nd / 2, i, nd)];
+        double m = (double)dia[dia_tiled_index(nd / 2, i, nd)];
         m *= 1.0 / omega;
         if (m == 0.0) { bad[0] = 1; dinv[i] = (TV)0; }
         else dinv[i] = (TV)(1.0 / m);
@@ -1970,34 +2001,6 @@ __global__ void k_pattern_count(int dim, int W, int64_t n, const int32_t* lat_of
     }
 }
 
-__global__ void k_pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat,
-                               const int32_t* rowptr, int32_t* col) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t l = lat_of_dof[r];
-        const int x = (int)(l % W), y = (int)((l / W) % W);
-        const int z = dim == 3 ? (int)(l / ((int64_t)W * W)) : 0;
-        int32_t c[27];
-        int k = 0;
-        for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int xx = x + dx, yy = y + dy, zz = z + dz;
-                    if (xx < 0 || xx >= W || yy < 0 || yy >= W || zz < 0 || zz >= W) continue;
-                    if (dim == 2 && dx * dy == -1) continue;   // no anti-diagonal edges
-                    c[k++] = dof_of_lat[((int64_t)zz * W + yy) * W + xx];
-                }
-        // insertion sort (rows are short)
-        for (int a = 1; a < k; ++a) {
-            int32_t v = c[a];
-            int b = a - 1;
-            while (b >= 0 && c[b] > v) { c[b + 1] = c[b]; --b; }
-            c[b + 1] = v;
-        }
-        const int32_t base = rowptr[r];
-        for (int a = 0; a < k; ++a) col[base + a] = c[a];
-    }
-}
-
 __device__ __forceinline__ int32_t csr_slot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                             int32_t r, int32_t c) {
     int32_t lo = rowptr[r], hi = rowptr[r + 1] - 1;
@@ -2032,62 +2035,6 @@ __device__ __forceinline__ void hex_corner(int c, int& dx, int& dy, int& dz) {
     dx = (c == 1 || c == 2 || c == 5 || c == 6);
     dy = (c == 2 || c == 3 || c == 6 || c == 7);
     dz = (c >= 4);
-}
-
-// One colour (ex%2, ey%2, ez%2) == (cx,cy,cz): elements of one colour share no
-// vertex, so the scatter needs no atomics.
-// Output target: CSR in DoF numbering (rowptr != nullptr) or DIA in lattice
-// numbering (dia != nullptr). rhs (DoF or lattice numbering) may be null.
-template <typename TM>
-__global__ __launch_bounds__(128) void k_assemble_q1_hex(int n, int color, double h, const double* __restrict__ theta,
-                                                         int nfine, double f_const, const int32_t* __restrict__ dof_of_lat,
-                                                         const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                         TM* __restrict__ csr_val, TM* __restrict__ dia, int64_t dia_n,
-                                                         double* __restrict__ rhs) {
-    const int hn = (n + 1) / 2;   // elements per dim of this colour (upper bound)
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int cx = color & 1, cy = (color >> 1) & 1, cz = (color >> 2) & 1;
-    const int64_t nper = (int64_t)hn * hn * hn;
-    if (e >= nper) return;
-    const int ex = 2 * (int)(e % hn) + cx, ey = 2 * (int)((e / hn) % hn) + cy, ez = 2 * (int)(e / ((int64_t)hn * hn)) + cz;
-    if (ex >= n || ey >= n || ez >= n) return;
-    const int W = n + 1;
-    double X[8][3];
-    int64_t lat[8];
-    for (int a = 0; a < 8; ++a) {
-        int dx, dy, dz;
-        hex_corner(a, dx, dy, dz);
-        X[a][0] = (ex + dx) * h;
-        X[a][1] = (ey + dy) * h;
-        X[a][2] = (ez + dz) * h;
-        lat[a] = ((int64_t)(ez + dz) * W + (ey + dy)) * W + (ex + dx);
-    }
-    double Ke[8][8], fe[8];
-    q1_hex_element(X, theta, nfine, f_const, Ke, fe);
-    if (rowptr) {
-        int32_t dof[8];
-        for (int a = 0; a < 8; ++a) dof[a] = dof_of_lat[lat[a]];
-        for (int a = 0; a < 8; ++a) {
-            for (int b = 0; b < 8; ++b) {
-                const int32_t k = csr_slot(rowptr, col, dof[a], dof[b]);
-                csr_val[k] = (TM)((double)csr_val[k] + Ke[a][b]);
-            }
-            if (rhs) rhs[dof[a]] += fe[a];
-        }
-    } else {
-        for (int a = 0; a < 8; ++a) {
-            int ax, ay, az;
-            hex_corner(a, ax, ay, az);
-            for (int b = 0; b < 8; ++b) {
-                int bx, by, bz;
-                hex_corner(b, bx, by, bz);
-                const int d = ((bz - az + 1) * 3 + (by - ay + 1)) * 3 + (bx - ax + 1);
-                TM* p = dia + dia_index(d, lat[a], 27);
-                *p = (TM)((double)*p + Ke[a][b]);
-            }
-            if (rhs) rhs[lat[a]] += fe[a];
-        }
-    }
 }
 
 // Adjoint element gradient on the finest level (one element per thread, no
@@ -2193,7 +2140,7 @@ __global__ __launch_bounds__(128) void k_assemble_fv1_tri(int n, int color, doub
             for (int b = 0; b < 3; ++b) {
                 const int dx = (int)((lat[b] % W) - (lat[a] % W)), dy = (int)((lat[b] / W) - (lat[a] / W));
                 const int d = (dy + 1) * 3 + (dx + 1);
-                TM* p = dia + dia_index(d, lat[a], 9);
+                TM* p = dia + dia_tiled_index(d, lat[a], 9);
                 *p = (TM)((double)*p + Ke[a][b]);
             }
             if (rhs && f_at_dof) rhs[lat[a]] += f_at_dof[dof_of_lat[lat[a]]] * (area / 3.0);
@@ -2217,7 +2164,7 @@ template <typename TM>
 __global__ void k_dirichlet_dia(int64_t n, int nd, int centre, const uint8_t* dir, TM* dia) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (!dir[i]) continue;
-        for (int d = 0; d < nd; ++d) dia[dia_index(d, i, nd)] = (TM)(d == centre ? 1.0 : 0.0);
+        for (int d = 0; d < nd; ++d) dia[dia_tiled_index(d, i, nd)] = (TM)(d == centre ? 1.0 : 0.0);
     }
 }
 
@@ -2896,24 +2843,6 @@ int pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t*
     UGX_CHECK_LAUNCH();
 }
 
-int pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const int32_t* rowptr,
-                 int32_t* col, hipStream_t s) {
-    hipLaunchKernelGGL(k_pattern_fill, dim3(vec_grid(n)), dim3(256), 0, s, dim, W, n, lat_of_dof, dof_of_lat, rowptr, col);
-    UGX_CHECK_LAUNCH();
-}
-
-template <typename TM>
-int assemble_q1_hex(int n, double h, const double* theta, int nfine, double f_const, const int32_t* dof_of_lat,
-                    const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
-                    hipStream_t s) {
-    const int hn = (n + 1) / 2;
-    const int64_t per = (int64_t)hn * hn * hn;
-    for (int color = 0; color < 8; ++color)
-        hipLaunchKernelGGL(k_assemble_q1_hex<TM>, dim3((per + 127) / 128), dim3(128), 0, s, n, color, h, theta, nfine, f_const, dof_of_lat, rowptr, col, csr_val, dia, dia_n, rhs);
-    UGX_CHECK_LAUNCH();
-}
-template int assemble_q1_hex<double>(int, double, const double*, int, double, const int32_t*, const int32_t*, const int32_t*, double*, double*, int64_t, double*, hipStream_t);
-
 template <typename TM>
 int assemble_fv1_tri(int n, double h, const double* theta, int nfine, const double* f_at_dof, const int32_t* dof_of_lat,
                      const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
@@ -2967,15 +2896,15 @@ int sell_widths(int64_t nrows, const int32_t* rowptr, const int32_t* order, int3
 template <typename TM>
 int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
               const int32_t* order, const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval,
-              hipStream_t s) {
-    const int64_t nth = ((nrows + 63) / 64) * 64;
-    if (nth <= 0) return 0;
-    hipLaunchKernelGGL(k_sell_fill<TM>, dim3((nth + 255) / 256), dim3(256), 0, s, nrows, ncols, rowptr, col, val,
-                       order, slice_ptr, width, scol, sval);
+              hipStream_t s, bool vals_only) {
+    const int64_t nsl = (nrows + 63) / 64;
+    if (nsl <= 0) return 0;
+    hipLaunchKernelGGL(k_sell_fill<TM>, dim3((unsigned)nsl), dim3(64), 0, s, nrows, ncols, rowptr, col, val, order,
+                       slice_ptr, width, scol, sval, vals_only ? 1 : 0);
     UGX_CHECK_LAUNCH();
 }
 template int sell_fill<double>(int64_t, int64_t, const int32_t*, const int32_t*, const double*, const int32_t*,
-                               const int64_t*, const int32_t*, int32_t*, double*, hipStream_t);
+                               const int64_t*, const int32_t*, int32_t*, double*, hipStream_t, bool);
 
 template <typename TM, typename TV>
 int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,

@@ -33,10 +33,6 @@ namespace ugamd {
 
 namespace {
 
-__host__ __device__ __forceinline__ int64_t rap_dia_index(int d, int64_t i, int nd) {
-    return ((i >> 6) * nd + d) * 64 + (i & 63);
-}
-
 // P1 weight for a fine-minus-twice-coarse offset diff (each component in
 // {-1, 0, 1}); the triangles of the 2D lattice are split along (1,1)
 template <int DIM>
@@ -60,7 +56,7 @@ __global__ __launch_bounds__(256) void k_rap_dia(int Wf, int Wc, const uint8_t* 
     // the lattice boundary is the Dirichlet boundary of these hierarchies
     const bool boundary = cx == 0 || cx == Wc - 1 || cy == 0 || cy == Wc - 1 || (DIM == 3 && (cz == 0 || cz == Wc - 1));
     if (dir_c[ci] || boundary) {
-        for (int o = 0; o < ND; ++o) Ac[rap_dia_index(o, ci, ND)] = o == ND / 2 ? 1.0 : 0.0;
+        for (int o = 0; o < ND; ++o) Ac[dia_tiled_index(o, ci, ND)] = o == ND / 2 ? 1.0 : 0.0;
         return;
     }
     const int64_t Wf2 = (int64_t)Wf * Wf;
@@ -87,7 +83,7 @@ __global__ __launch_bounds__(256) void k_rap_dia(int Wf, int Wc, const uint8_t* 
                                 for (int ex = ex0; ex <= ex1; ++ex) {
                                     const int e = DIM == 3 ? (ez + 1) * 9 + (ey + 1) * 3 + (ex + 1)
                                                            : (ey + 1) * 3 + (ex + 1);
-                                    const double a = Af[rap_dia_index(e, fi, ND)];
+                                    const double a = Af[dia_tiled_index(e, fi, ND)];
                                     if (a == 0.0) continue;
                                     const int64_t gi = DIM == 3 ? fi + (int64_t)ez * Wf2 + (int64_t)ey * Wf + ex
                                                                 : fi + (int64_t)ey * Wf + ex;
@@ -101,7 +97,7 @@ __global__ __launch_bounds__(256) void k_rap_dia(int Wf, int Wc, const uint8_t* 
                                 }
                     }
         }
-        Ac[rap_dia_index(o, ci, ND)] = acc;
+        Ac[dia_tiled_index(o, ci, ND)] = acc;
     }
 }
 

@@ -32,127 +32,239 @@ namespace {
 
 constexpr int kDictSlots = 255;       // dictionary entries per slice (position 255 = zero)
 constexpr int kDictMaxWidth = 64;     // longest row of a slice that the layout takes
-constexpr int kSortCap = 64 * kDictMaxWidth;
+constexpr int kHashSlots = 512;       // LDS hash of a slice's columns (at most 255 + 64 occupied)
+constexpr int kChunk = 8;             // entries per row moved through LDS at a time
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t > 0 ? (n + t - 1) / t : 1); }
 
-// the columns of slice t's entries sorted in LDS (pads INT_MAX); returns the
-// number of sorted keys (a power of two)
-__device__ int slice_sorted_columns(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                    const int32_t* __restrict__ order, int64_t t, int w, int32_t* keys) {
-    int n = 64 * w, P = 1;
-    while (P < n) P <<= 1;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-        int32_t v = INT_MAX;
-        if (e < n) {
-            const int k = e >> 6, lane = e & 63;
-            const int32_t r = order[t * 64 + lane];
-            if (r >= 0 && r < nrows) {
-                const int32_t rb = rowptr[r], len = rowptr[r + 1] - rb;
-                if (k < len) v = col[rb + k];
-            }
-        }
-        keys[e] = v;
-    }
-    __syncthreads();
-    // bitonic sort, ascending
-    for (int size = 2; size <= P; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = threadIdx.x; i < P; i += blockDim.x) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    const int32_t a = keys[i], b = keys[j];
-                    if ((a > b) == up) {
-                        keys[i] = b;
-                        keys[j] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    return P;
+__device__ __forceinline__ int hash_slot(int32_t c) { return (int)(((uint32_t)c * 2654435761u) >> 23); }
+
+// slot of column c, which is in the table
+__device__ __forceinline__ int hash_find(const int32_t* hk, int32_t c) {
+    int h = hash_slot(c);
+    while (hk[h] != c) h = (h + 1) & (kHashSlots - 1);
+    return h;
 }
 
-// distinct columns per slice (INT_MAX if the slice's width exceeds the layout)
-__global__ __launch_bounds__(256) void k_dict_count(int64_t nrows, const int32_t* __restrict__ rowptr,
-                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ order,
-                                                    const int32_t* __restrict__ width, int32_t* __restrict__ ucount) {
-    __shared__ int32_t keys[kSortCap];
-    __shared__ int32_t red[4];
+// Kernels below run one 64-lane workgroup (one wavefront) per slice. The
+// slice's rows: start and length of lane l's row in rbs[l], lens[l] (length 0
+// for an empty lane)
+__device__ __forceinline__ void slice_rows(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                           const int32_t* __restrict__ order, int64_t t, int32_t* rbs, int32_t* lens) {
+    const int lane = threadIdx.x;
+    const int32_t r = order[t * 64 + lane];
+    int32_t rb = 0, len = 0;
+    if (r >= 0 && r < nrows) {
+        rb = rowptr[r];
+        len = rowptr[r + 1] - rb;
+    }
+    rbs[lane] = rb;
+    lens[lane] = len;
+}
+
+// inserts the slice's columns into the LDS hash table hk (all -1 on entry;
+// *cnt 0): the entries are read row by row (lane e of a round reads entry
+// e % w of row e / w), so a round reads contiguous CSR runs. Returns the
+// number of distinct columns, or a number above 255 as soon as a round ends
+// above it (then the table is incomplete). A round adds at most 64 keys, so
+// at most 319 of the 512 slots are ever occupied and every probe ends.
+__device__ int slice_hash(int w, const int32_t* rbs, const int32_t* lens, const int32_t* __restrict__ col,
+                          int32_t* hk, int* cnt) {
+    const int lane = threadIdx.x;
+    const int total = 64 * w;
+    for (int base = 0; base < total; base += 64) {
+        const int e = base + lane;
+        if (e < total) {
+            const int j = e / w, k = e - j * w;
+            if (k < lens[j]) {
+                const int32_t c = col[rbs[j] + k];
+                int h = hash_slot(c);
+                for (;;) {
+                    const int32_t old = atomicCAS(&hk[h], -1, c);
+                    if (old == -1) {
+                        atomicAdd(cnt, 1);
+                        break;
+                    }
+                    if (old == c) break;
+                    h = (h + 1) & (kHashSlots - 1);
+                }
+            }
+        }
+        __syncthreads();
+        const int u = *cnt;
+        __syncthreads();
+        if (u > kDictSlots) return u;
+    }
+    return *cnt;
+}
+
+__device__ __forceinline__ void cmp_swap(int32_t& a, int32_t& b, bool up) {
+    if ((a > b) == up) {
+        const int32_t t = a;
+        a = b;
+        b = t;
+    }
+}
+
+// bitonic sort of the 256 keys v[0..3] of the wave (key 4*lane + m in v[m]),
+// ascending
+__device__ __forceinline__ void wave_sort256(int32_t v[4]) {
+    const int lane = threadIdx.x;
+    for (int size = 2; size <= 256; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= 4) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int i = 4 * lane + m;
+                    const int32_t o = __shfl_xor(v[m], stride >> 2, 64);
+                    const bool up = (i & size) == 0, lower = (i & stride) == 0;
+                    v[m] = (lower == up) ? min(v[m], o) : max(v[m], o);
+                }
+            } else {
+                const bool up = ((4 * lane) & size) == 0;   // size >= 4 here, or size 2 (both halves alike below)
+                if (stride == 2) {
+                    cmp_swap(v[0], v[2], up);
+                    cmp_swap(v[1], v[3], up);
+                } else if (size == 2) {
+                    cmp_swap(v[0], v[1], true);
+                    cmp_swap(v[2], v[3], false);
+                } else {
+                    cmp_swap(v[0], v[1], up);
+                    cmp_swap(v[2], v[3], up);
+                }
+            }
+        }
+    }
+}
+
+// distinct columns per slice (above 255, or INT_MAX when the slice is wider
+// than the layout takes: the slice does not fit)
+__global__ __launch_bounds__(64) void k_dict_count(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ order,
+                                                   const int32_t* __restrict__ width, int32_t* __restrict__ ucount) {
+    __shared__ int32_t hk[kHashSlots];
+    __shared__ int32_t rbs[64], lens[64];
+    __shared__ int cnt;
     const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
     const int w = width[t];
     if (w > kDictMaxWidth) {
-        if (threadIdx.x == 0) ucount[t] = INT_MAX;
+        if (lane == 0) ucount[t] = INT_MAX;
         return;
     }
-    const int P = slice_sorted_columns(nrows, rowptr, col, order, t, w, keys);
-    int c = 0;
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-        c += (keys[i] != INT_MAX && (i == 0 || keys[i] != keys[i - 1])) ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    for (int h = lane; h < kHashSlots; h += 64) hk[h] = -1;
+    if (lane == 0) cnt = 0;
+    slice_rows(nrows, rowptr, order, t, rbs, lens);
     __syncthreads();
-    if (threadIdx.x == 0) ucount[t] = red[0] + red[1] + red[2] + red[3];
+    const int u = slice_hash(w, rbs, lens, col, hk, &cnt);
+    if (lane == 0) ucount[t] = u;
 }
 
-// the dictionary, positions and values of every slice
-__global__ __launch_bounds__(256) void k_dict_fill(int64_t nrows, const int32_t* __restrict__ rowptr,
-                                                   const int32_t* __restrict__ col, const double* __restrict__ val,
-                                                   const int32_t* __restrict__ order, const int32_t* __restrict__ width,
-                                                   const int32_t* __restrict__ dptr, const int64_t* __restrict__ vptr,
-                                                   const int64_t* __restrict__ iptr, int32_t* __restrict__ dict,
-                                                   double* __restrict__ sval, uint32_t* __restrict__ sidx) {
-    __shared__ int32_t keys[kSortCap];
-    __shared__ int32_t uniq[kDictSlots];
-    __shared__ int nuniq;
-    const int64_t t = blockIdx.x;
-    const int w = width[t];
-    const int P = slice_sorted_columns(nrows, rowptr, col, order, t, w, keys);
-    if (threadIdx.x == 0) {
-        // sequential compaction of at most 255 distinct keys (the count pass
-        // checked the bound)
-        int u = 0;
-        for (int i = 0; i < P && keys[i] != INT_MAX; ++i)
-            if (i == 0 || keys[i] != keys[i - 1]) uniq[u++] = keys[i];
-        nuniq = u;
-    }
-    __syncthreads();
-    const int u = nuniq;
-    for (int j = threadIdx.x; j < u; j += blockDim.x) dict[dptr[t] + j] = uniq[j];
-    // one thread per (word, lane): four entries k = 4q .. 4q+3 of lane l
-    const int nq = (w + 3) >> 2;
-    for (int e = threadIdx.x; e < nq * 64; e += blockDim.x) {
-        const int q = e >> 6, lane = e & 63;
-        const int32_t r = order[t * 64 + lane];
-        int32_t rb = 0, len = 0;
-        if (r >= 0 && r < nrows) {
-            rb = rowptr[r];
-            len = rowptr[r + 1] - rb;
-        }
-        uint32_t word = 0;
-        for (int b = 0; b < 4; ++b) {
-            const int k = 4 * q + b;
+// moves the slice's values (and, with pb, the dictionary positions of the
+// entries) through LDS in chunks of kChunk entries per row: rows are read
+// contiguously, the value pairs and position words written contiguously.
+// hk/hv: column -> position (used when pb is given)
+__device__ void slice_values_out(int w, const int32_t* rbs, const int32_t* lens, const int32_t* __restrict__ col,
+                                 const double* __restrict__ val, const int32_t* hk, const int32_t* hv, uint8_t* pb,
+                                 double* vt, double* __restrict__ sv, uint32_t* __restrict__ si) {
+    const int lane = threadIdx.x;
+    const int npairs = (w + 1) >> 1, nwords = (w + 3) >> 2;
+    for (int k0 = 0; k0 < w; k0 += kChunk) {
+#pragma unroll
+        for (int it = 0; it < kChunk; ++it) {
+            const int e = it * 64 + lane;
+            const int j = e / kChunk, kk = e % kChunk, k = k0 + kk;
             uint32_t pos = 255;
             double v = 0.0;
-            if (k < len) {
-                const int32_t c = col[rb + k];
-                int lo = 0, hi = u - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (uniq[mid] < c) lo = mid + 1;
-                    else hi = mid;
-                }
-                pos = (uint32_t)lo;
-                v = val[rb + k];
+            if (k < lens[j]) {
+                const int64_t a = (int64_t)rbs[j] + k;
+                v = val[a];
+                if (pb) pos = (uint32_t)hv[hash_find(hk, col[a])];
             }
-            word |= pos << (8 * b);
-            if (k < ((w + 1) & ~1)) sval[vptr[t] + 2 * (64 * (k >> 1) + lane) + (k & 1)] = v;
+            if (pb) pb[kk * 64 + j] = (uint8_t)pos;
+            vt[(kk >> 1) * 128 + 2 * j + (kk & 1)] = v;
         }
-        sidx[iptr[t] + 64 * q + lane] = word;
+        __syncthreads();
+        const int npc = min(kChunk / 2, npairs - k0 / 2);
+        for (int o = lane; o < 128 * npc; o += 64) sv[(int64_t)64 * k0 + o] = vt[o];
+        if (pb) {
+            const int nqc = min(kChunk / 4, nwords - k0 / 4);
+            for (int qq = 0; qq < nqc; ++qq) {
+                const uint8_t* b = pb + 4 * qq * 64 + lane;
+                si[(int64_t)64 * (k0 / 4 + qq) + lane] =
+                    (uint32_t)b[0] | ((uint32_t)b[64] << 8) | ((uint32_t)b[128] << 16) | ((uint32_t)b[192] << 24);
+            }
+        }
+        __syncthreads();
     }
+}
+
+// the dictionary (sorted distinct columns), positions and values of every
+// slice; the count pass has checked that every slice fits
+__global__ __launch_bounds__(64) void k_dict_fill(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                  const int32_t* __restrict__ order, const int32_t* __restrict__ width,
+                                                  const int32_t* __restrict__ dptr, const int64_t* __restrict__ vptr,
+                                                  const int64_t* __restrict__ iptr, int32_t* __restrict__ dict,
+                                                  double* __restrict__ sval, uint32_t* __restrict__ sidx) {
+    __shared__ int32_t hk[kHashSlots], hv[kHashSlots];
+    __shared__ int32_t rbs[64], lens[64], list[256];
+    __shared__ int cnt;
+    __shared__ uint8_t pb[kChunk * 64];
+    __shared__ double vt[kChunk * 64];
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int w = width[t];
+    for (int h = lane; h < kHashSlots; h += 64) hk[h] = -1;
+    if (lane == 0) cnt = 0;
+    slice_rows(nrows, rowptr, order, t, rbs, lens);
+    __syncthreads();
+    const int u = slice_hash(w, rbs, lens, col, hk, &cnt);
+    if (u > kDictSlots) return;   // not a slice that fits (the count pass keeps such layouts from being built)
+    // compact the table (lane l owns slots 8l .. 8l+7), then sort
+    int n = 0;
+    for (int i = 0; i < 8; ++i) n += hk[8 * lane + i] >= 0 ? 1 : 0;
+    int off = n;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(off, d, 64);
+        if (lane >= d) off += o;
+    }
+    off -= n;
+    for (int i = 0; i < 8; ++i) {
+        const int32_t c = hk[8 * lane + i];
+        if (c >= 0) list[off++] = c;
+    }
+    for (int i = u + lane; i < 256; i += 64) list[i] = INT_MAX;
+    __syncthreads();
+    int32_t v[4] = {list[4 * lane], list[4 * lane + 1], list[4 * lane + 2], list[4 * lane + 3]};
+    wave_sort256(v);
+    const int32_t d0 = dptr[t];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int i = 4 * lane + m;
+        if (i < u) {
+            dict[d0 + i] = v[m];
+            hv[hash_find(hk, v[m])] = i;
+        }
+    }
+    __syncthreads();
+    slice_values_out(w, rbs, lens, col, val, hk, hv, pb, vt, sval + vptr[t], sidx + iptr[t]);
+}
+
+// the values alone, for a matrix whose pattern (and so dictionary and
+// positions) is unchanged since k_dict_fill
+__global__ __launch_bounds__(64) void k_dict_values(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                    const double* __restrict__ val, const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ width, const int64_t* __restrict__ vptr,
+                                                    double* __restrict__ sval) {
+    __shared__ int32_t rbs[64], lens[64];
+    __shared__ double vt[kChunk * 64];
+    const int64_t t = blockIdx.x;
+    slice_rows(nrows, rowptr, order, t, rbs, lens);
+    __syncthreads();
+    slice_values_out(width[t], rbs, lens, nullptr, val, nullptr, nullptr, nullptr, vt, sval + vptr[t], nullptr);
 }
 
 // y = alpha*v1 + beta*A*x (mode 0), or y = beta*A*x on rows with entries
@@ -263,7 +375,7 @@ int dict_widths(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int
 int dict_count(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* col, const int32_t* order,
                const int32_t* width, int32_t* ucount, hipStream_t s) {
     if (nslices <= 0) return 0;
-    hipLaunchKernelGGL(k_dict_count, dim3((unsigned)nslices), dim3(256), 0, s, nrows, rowptr, col, order, width,
+    hipLaunchKernelGGL(k_dict_count, dim3((unsigned)nslices), dim3(64), 0, s, nrows, rowptr, col, order, width,
                        ucount);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -272,8 +384,16 @@ int dict_fill(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32
               const int32_t* order, const int32_t* width, const int32_t* dptr, const int64_t* vptr,
               const int64_t* iptr, int32_t* dict, double* sval, uint32_t* sidx, hipStream_t s) {
     if (nslices <= 0) return 0;
-    hipLaunchKernelGGL(k_dict_fill, dim3((unsigned)nslices), dim3(256), 0, s, nrows, rowptr, col, val, order, width,
+    hipLaunchKernelGGL(k_dict_fill, dim3((unsigned)nslices), dim3(64), 0, s, nrows, rowptr, col, val, order, width,
                        dptr, vptr, iptr, dict, sval, sidx);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int dict_values(int64_t nslices, int64_t nrows, const int32_t* rowptr, const double* val, const int32_t* order,
+                const int32_t* width, const int64_t* vptr, double* sval, hipStream_t s) {
+    if (nslices <= 0) return 0;
+    hipLaunchKernelGGL(k_dict_values, dim3((unsigned)nslices), dim3(64), 0, s, nrows, rowptr, val, order, width, vptr,
+                       sval);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -17,7 +17,7 @@ int sell_widths(int64_t nrows, const int32_t* rowptr, const int32_t* order, int3
 template <typename TM>
 int sell_fill(int64_t nrows, int64_t ncols, const int32_t* rowptr, const int32_t* col, const TM* val,
               const int32_t* order, const int64_t* slice_ptr, const int32_t* width, int32_t* scol, TM* sval,
-              hipStream_t s);
+              hipStream_t s, bool vals_only = false);
 template <typename TM, typename TV>
 int sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* width, const int32_t* scol, const TM* sval,
               double alpha, const TV* v1, double beta, const TV* w, TV* dest, int mode, const int32_t* rowptr,
@@ -122,8 +122,12 @@ int q1mf_coeff(int64_t ne, double h, const double* theta, T* c, hipStream_t s);
 template <typename TM, typename TV>
 int dia_jacobi_diag(int64_t n, int nd, const TM* dia, double omega, TV* dinv, int* bad, hipStream_t s);
 
-// tiled diagonal storage of level operators (see kernels.hip)
-inline int64_t dia_tiled_index(int d, int64_t i, int nd) { return ((i >> 6) * nd + d) * 64 + (i & 63); }
+// tiled diagonal storage of level operators: the 64 values of diagonal d for
+// rows 64t..64t+63 are contiguous and the nd diagonals of one tile follow
+// each other
+__host__ __device__ inline int64_t dia_tiled_index(int d, int64_t i, int nd) {
+    return ((i >> 6) * nd + d) * 64 + (i & 63);
+}
 inline int64_t dia_tiled_size(int64_t n, int nd) { return ((n + 63) / 64) * 64 * nd; }
 // P1 transfers between lattice levels. In 3D with Wf = 2 Wc - 1 the Dirichlet
 // flags must mark exactly the lattice boundary (as the multigrid levels' flags
@@ -144,12 +148,6 @@ int lu_solve(int n, const double* LinvT, const double* UinvT, const int32_t* per
 
 int invert_map(int64_t n, const int32_t* dof_of_lat, int32_t* lat_of_dof, hipStream_t s);
 int pattern_count(int dim, int W, int64_t n, const int32_t* lat_of_dof, int32_t* cnt, hipStream_t s);
-int pattern_fill(int dim, int W, int64_t n, const int32_t* lat_of_dof, const int32_t* dof_of_lat, const int32_t* rowptr,
-                 int32_t* col, hipStream_t s);
-template <typename TM>
-int assemble_q1_hex(int n, double h, const double* theta, int nfine, double f_const, const int32_t* dof_of_lat,
-                    const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
-                    hipStream_t s);
 template <typename TM>
 int assemble_fv1_tri(int n, double h, const double* theta, int nfine, const double* f_at_dof, const int32_t* dof_of_lat,
                      const int32_t* rowptr, const int32_t* col, TM* csr_val, TM* dia, int64_t dia_n, double* rhs,
@@ -246,6 +244,12 @@ int scale_values(int64_t n, const double* src, double a, double* dst, hipStream_
 
 // ---- element-loop assembly from connectivity (elem_assembly.hip) ----
 // temporary bytes of the pattern's sort and scan for ne elements of nv DoFs
+// rowptr[0] = 0, rowptr[i+1] = cnt[0] + ... + cnt[i], scanned in 64 bits into
+// incl (n entries; incl[n-1] is the total, which the caller checks against
+// the int32 range of rowptr)
+int rowptr_scan_temp_bytes(int64_t n, size_t* bytes);
+int rowptr_from_counts(int64_t n, const int32_t* cnt, int64_t* incl, void* temp, size_t bytes, int32_t* rowptr,
+                       hipStream_t s);
 int elem_pattern_temp_bytes(int64_t ne, int nv, int64_t ndofs, size_t* sort_bytes, size_t* scan_bytes);
 // sorts the element couplings (keys/idx into key2/idx2), marks run heads,
 // scans them (run_incl), fills rowptr (ndofs + 1) and *nnz_dev; buffers of
@@ -293,6 +297,27 @@ int zero_listed(int64_t n, const int32_t* idx, double* x, hipStream_t s);
 int lattice_connectivity(int kind, int n, const int32_t* dof_of_lat, const uint16_t* cell_tets, const double* theta,
                          int32_t* dofs, double* kel, hipStream_t s);
 
+// ---- lattice operators written row by row (lattice_csr.hip) ----
+// Q1 unit-cube tables, 2x2x2 Gauss points (host): G_q[a][b] =
+// w_q grad phi_a . grad phi_b at 64q + 8a + b, K = sum_q G_q at 512 + 8a + b,
+// F_a = sum_q w_q phi_a at 576 + a
+constexpr int kQ1Tab = 584;
+void q1_lattice_tables(double* tab);
+// rp_lat[p] = rowptr[dof_of_lat[p]] (row start of every lattice point)
+int rowptr_of_lattice(int64_t npts, const int32_t* dof_of_lat, const int32_t* rowptr, int32_t* rp_lat, hipStream_t s);
+// col of the lattice pattern (rows = DoFs; 3D: 27-point, 2D: the triangles'
+// 7-point pattern); rp_lat from rowptr_of_lattice
+int lattice_pattern(int dim, int W, const int32_t* dof_of_lat, const int32_t* rp_lat, int32_t* col, hipStream_t s);
+int exp_values(int64_t n, const double* theta, double* out, hipStream_t s);
+// Q1 top-level operator into the lattice pattern's CSR (every entry written)
+// and the load vector; kel: exp(theta) per element or null
+int q1_lattice_csr(int n, double h, double f_const, const double* tab, const double* kel,
+                   const int32_t* dof_of_lat, const int32_t* rp_lat, double* val, double* rhs, hipStream_t s);
+// Q1 operator of a level (n elements per axis) into its 27 tiled diagonals
+// (every diagonal of every row written); kfine: exp(theta) per fine element
+// (nfine per axis, a multiple of n) or null
+int q1_lattice_dia(int n, int nfine, double h, const double* tab, const double* kfine, double* dia, hipStream_t s);
+
 // ---- sliced ELL with a column dictionary per slice (spmv_dict.hip) ----
 // slot s of the slice order holds row order[s] or -1; a slice is 64 slots
 int dict_widths(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* order, int32_t* width,
@@ -303,6 +328,9 @@ int dict_count(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int3
 int dict_fill(int64_t nslices, int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val,
               const int32_t* order, const int32_t* width, const int32_t* dptr, const int64_t* vptr,
               const int64_t* iptr, int32_t* dict, double* sval, uint32_t* sidx, hipStream_t s);
+// the values alone into an existing dictionary layout of the same pattern
+int dict_values(int64_t nslices, int64_t nrows, const int32_t* rowptr, const double* val, const int32_t* order,
+                const int32_t* width, const int64_t* vptr, double* sval, hipStream_t s);
 // y = alpha*v1 + beta*A*x (mode 0) or y = beta*A*x on rows with entries (mode 1)
 int dict_spmv(int64_t nrows, int64_t nslices, const int32_t* width, const int32_t* dptr, const int32_t* dict,
               const int64_t* vptr, const int64_t* iptr, const double* sval, const uint32_t* sidx,
