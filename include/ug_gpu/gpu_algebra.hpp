@@ -438,12 +438,17 @@ class BiCGStab : public ILinearOperatorInverse {
 // every assembly is one launch per colour without atomics (elem_assembly.hip).
 class ElementDiscretization {
    public:
-    enum ElemType { TRIANGLE_FV1 = 0, TETRAHEDRON_FV1 = 1, HEXAHEDRON_Q1 = 2 };
-    static int vertices_of(int elem_type) { return elem_type == 0 ? 3 : (elem_type == 1 ? 4 : 8); }
+    enum ElemType { TRIANGLE_FV1 = 0, TETRAHEDRON_FV1 = 1, HEXAHEDRON_Q1 = 2, HEXAHEDRON_Q2 = 3 };
+    static int vertices_of(int elem_type) {
+        return elem_type == 0 ? 3 : (elem_type == 1 ? 4 : (elem_type == 2 ? 8 : 27));
+    }
     static int dim_of(int elem_type) { return elem_type == 0 ? 2 : 3; }
     // elem_dofs: num_elems x vertices_of(type) DoFs in the reference element's
-    // local vertex order; dof_coords: num_dofs x dim_of(type). on_device: the
-    // two arrays are device pointers (copied), otherwise host arrays.
+    // local vertex order (Q2: the 27 shapes of LagrangeLSFS<Hexahedron, 2> --
+    // vertices, edge midpoints, face centres, centre; the first 8 DoFs'
+    // coordinates are the element's corners); dof_coords: num_dofs x
+    // dim_of(type). on_device: the two arrays are device pointers (copied),
+    // otherwise host arrays.
     ElementDiscretization(int elem_type, int64_t num_elems, const int32_t* elem_dofs, int64_t num_dofs,
                           const double* dof_coords, bool on_device = false);
     int elem_type() const { return type_; }
@@ -453,6 +458,10 @@ class ElementDiscretization {
     int num_colours() const { return (int)colour_start_.size() - 1; }
     // diffusion k per element (host or device array of num_elems; nullptr: 1)
     void set_diffusion(const double* k, bool on_device = false);
+    // Q2: k at the 3x3x3 Gauss points of every element (num_elems x 27, x
+    // fastest), as the reference evaluates a coefficient function there;
+    // replaces set_diffusion (nullptr: back to it)
+    void set_diffusion_at_points(const double* kq, bool on_device = false);
     void set_source(double f) { f_const_ = f; has_f_dof_ = false; }
     void set_source_dofs(const double* f_host);          // FV1 only
     void set_velocity(double vx, double vy, double vz);  // tetrahedra only
@@ -480,10 +489,10 @@ class ElementDiscretization {
     int type_, nv_, dim_;
     int64_t ne_, ndofs_, nnz_ = 0;
     DeviceBuffer<int32_t> dofs_, rowptr_, col_, order_, slot_, dir_;
-    DeviceBuffer<double> coords_, kel_, f_dof_;
+    DeviceBuffer<double> coords_, kel_, kq_, f_dof_;
     std::vector<int64_t> colour_start_;
     double f_const_ = 0.0;
-    bool has_k_ = false, has_f_dof_ = false;
+    bool has_k_ = false, has_kq_ = false, has_f_dof_ = false;
     double vel_[3] = {0.0, 0.0, 0.0};
     bool conv_ = false;
     int64_t ndir_ = 0;
@@ -616,6 +625,87 @@ class DomainDiscretization {
     DeviceBuffer<int32_t> rp_lat_;
     const double* exp_theta_device();
     const double* q1_tables_device();
+};
+
+// Q2 Lagrange on the unit cube split into one hexahedron and refined
+// num_refs times: -div(k grad u) = f, u = 0 on the boundary, k = exp(theta)
+// per top-level element (evaluated at the Gauss points of every level's
+// elements, as the reference evaluates a coefficient function), f constant.
+// Every level has the reference's LEVEL DoF order (vertices, edges, faces,
+// volumes; subset by subset in creation order), its operator is assembled by
+// the element loop (ElementDiscretization, Q2 element matrices on the matrix
+// cores) and the transfers are StdTransfer's (q2_hierarchy.hip). With
+// MatrixMultiGrid they give the Q2 geometric multigrid.
+class Q2Discretization {
+   public:
+    explicit Q2Discretization(int num_refs);
+    int num_levels() const { return (int)L_.size(); }
+    int64_t num_dofs(int lev) const { return L_.at(lev).N; }
+    void set_theta(const double* theta_host);   // (2^num_refs)^3 values, x fastest; nullptr: k = 1
+    void set_source(double f) { f_ = f; }
+    // DoF coordinates (N x 3) and Dirichlet flags (N) of a level, host arrays
+    void dof_positions(int lev, double* out) const;
+    void dirichlet_flags(int lev, uint8_t* out) const;
+    void assemble_linear(int lev, GPUSparseMatrix& A, GPUVector& b);
+    void prolongation(int lev, GPUSparseMatrix& P) const;                        // lev-1 -> lev
+    void restriction(int lev, const GPUSparseMatrix& P, GPUSparseMatrix& R) const;  // P of the same level
+
+   private:
+    struct Level {
+        int n = 0;
+        int64_t N = 0;
+        DeviceBuffer<int32_t> dof_of_node, node_of_dof;
+        DeviceBuffer<uint8_t> dir;
+        std::unique_ptr<ElementDiscretization> elem;
+        bool coef_valid = false;
+    };
+    std::vector<Level> L_;
+    int refs_ = 0;
+    double f_ = 0.0;
+    DeviceBuffer<double> theta_;
+    bool has_theta_ = false;
+};
+
+// Geometric multigrid over level operators and grid transfers handed in as
+// matrices: the cycle of AssembledMultiGridCycle (mg_solver_impl.hpp:175-275
+// apply, 1686-1816 presmooth + restriction, 1819-1964 prolongation +
+// postsmooth, 1968-2086 base solve, 2090-2136 lmgc) with damped Jacobi
+// smoothers and the dense LU on the base level. Level l = 0 .. top: A[l] the
+// level operator, P[l] (l >= 1) the prolongation from l-1 to l (rows on level
+// l), R[l] the restriction from l to l-1 (StdTransfer's matrices,
+// std_transfer_impl.h:602-806, Dirichlet rows already adjusted). The top level
+// uses the operator passed to init (the solver's), as the reference copies the
+// surface operator to its top level. FP64 throughout; every matrix goes
+// through the SpMV layouts of GPUSparseMatrix.
+class MatrixMultiGrid : public ILinearIterator {
+   public:
+    enum CycleType { _F_ = -1, _V_ = 1, _W_ = 2 };
+    MatrixMultiGrid() = default;
+    const char* name() const override { return "GMG"; }
+    void set_levels(std::vector<std::shared_ptr<GPUSparseMatrix>> A, std::vector<std::shared_ptr<GPUSparseMatrix>> P,
+                    std::vector<std::shared_ptr<GPUSparseMatrix>> R);
+    void set_base_level(int l) { base_ = l; }
+    void set_num_presmooth(int n) { nu1_ = n; }
+    void set_num_postsmooth(int n) { nu2_ = n; }
+    void set_cycle_type(int t) { cycle_ = t; }
+    void set_smoother_damp(double w) { omega_ = w; }
+    bool init(std::shared_ptr<MatrixOperator> J) override;
+    bool apply(GPUVector& c, const GPUVector& d) override;
+
+   private:
+    void lmgc(int lev, int type);
+    void presmooth_and_restriction(int lev);
+    void prolongation_and_postsmooth(int lev);
+    void base_solve(int lev);
+    GPUSparseMatrix& Aop(int lev) { return lev == top_ ? op_->get_matrix() : *A_[lev]; }
+    std::vector<std::shared_ptr<GPUSparseMatrix>> A_, P_, R_;
+    int base_ = 0, top_ = -1, nu1_ = 2, nu2_ = 2, cycle_ = 1;
+    double omega_ = 0.66;
+    std::vector<std::unique_ptr<Jacobi>> smoother_;
+    std::vector<GPUVector> sc_, sd_, st_;
+    int lu_n_ = 0;
+    DeviceBuffer<double> lu_linvT_, lu_uinvT_;
+    DeviceBuffer<int32_t> lu_perm_;
 };
 
 enum class Precision { F64 = 0, F32 = 1 };

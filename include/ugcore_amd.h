@@ -217,7 +217,12 @@ int ugx_disc_assemble_defect(void* disc, void* d, void* u);
  * dof_distribution.cpp:1117-1206); dof_coords: num_dofs x dim coordinates.
  * elem_type 0: P1 triangles, FV1 (nv 3, dim 2); 1: P1 tetrahedra, FV1 with
  * optional full-upwind convection (nv 4, dim 3); 2: Q1 hexahedra, FE with
- * 2x2x2 Gauss (nv 8, dim 3). Host arrays. num_elems * nv^2 < 2^31.
+ * 2x2x2 Gauss (nv 8, dim 3); 3: Q2 hexahedra, FE with 3x3x3 Gauss (nv 27 in
+ * the shape order of LagrangeLSFS<ReferenceHexahedron, 2>, lagrange.cpp:236-253:
+ * the 8 vertices, the 12 edge midpoints, the 6 face centres, the centre;
+ * dim 3; element matrices on the matrix cores, one element per wavefront;
+ * its defect is A u - b of the assembled operator). Host arrays.
+ * num_elems * nv^2 < 2^31.
  * The CSR pattern holds every element coupling (rows sorted by column, zero
  * couplings kept, as AddLocalMatrixToGlobal's insertion, local_algebra.h:766-789);
  * it, the CSR slot of every local entry and an element colouring are built
@@ -229,6 +234,10 @@ int64_t ugx_elemdisc_nnz(void* e);
 int ugx_elemdisc_num_colours(void* e);
 /* diffusion k per element (host array of num_elems; NULL: 1) */
 int ugx_elemdisc_set_diffusion(void* e, const double* k_host);
+/* Q2 only: k at the 3x3x3 Gauss points of every element (host array of
+ * num_elems x 27, x fastest; NULL: back to ugx_elemdisc_set_diffusion), the
+ * coefficient function evaluated where the reference's quadrature evaluates it */
+int ugx_elemdisc_set_diffusion_at_points(void* e, const double* kq_host);
 /* source f: a constant, or per DoF (FV1 types); FV1 takes f |scv|, Q1 f int phi */
 int ugx_elemdisc_set_source_const(void* e, double f);
 int ugx_elemdisc_set_source_dofs(void* e, const double* f_host);
@@ -343,6 +352,46 @@ int ugx_vector_upload(void* v, const double* host);
 int ugx_vector_download(void* v, double* host);
 int ugx_vector_set(void* v, double value);
 
+/* Q2 Lagrange on the unit cube (one hexahedron refined num_refs <= 6 times):
+ * -div(k grad u) = f, u = 0 on the boundary, k = exp(theta) per top-level
+ * element taken at the Gauss points of every level's elements, f constant.
+ * Every level in the reference's LEVEL DoF order (DoFDistribution::reinit,
+ * dof_distribution.cpp:1401-1444: vertices, edges, faces, volumes, each
+ * subset in creation order); operators by the element loop (elem_type 3 of
+ * ugx_elemdisc_*); transfers as StdTransfer builds them for Lagrange elements
+ * (std_transfer_impl.h:214-388) with the Dirichlet adjustments of
+ * lagrange_dirichlet_boundary_impl.h:485-740. No counterpart as one class in
+ * the reference (there: ApproximationSpace + DomainDiscretization +
+ * StdTransfer). */
+void* ugx_q2disc_create(int num_refs);
+void ugx_q2disc_destroy(void* q);
+int ugx_q2disc_num_levels(void* q);
+int64_t ugx_q2disc_num_dofs(void* q, int level);
+/* DoF coordinates (num_dofs x 3) and Dirichlet flags (num_dofs) of a level, host arrays */
+int ugx_q2disc_dof_positions(void* q, int level, double* out_host);
+int ugx_q2disc_dirichlet(void* q, int level, uint8_t* out_host);
+/* theta per top-level element ((2^num_refs)^3, x fastest; host; NULL: k = 1) */
+int ugx_q2disc_set_theta(void* q, const double* theta_host);
+int ugx_q2disc_set_source(void* q, double f);
+/* level operator and rhs (assemble_linear, domain_disc_impl.h:835-948) */
+int ugx_q2disc_assemble_linear(void* q, int level, void* matrix, void* rhs);
+/* StdTransfer::prolongation (level-1 -> level, rows on level) and restriction
+ * (R = P^T of that P with adjust_restriction), std_transfer_impl.h:602-717 */
+int ugx_q2disc_prolongation(void* q, int level, void* P);
+int ugx_q2disc_restriction(void* q, int level, void* P, void* R);
+/* AssembledMultiGridCycle over level operators and transfers given as
+ * matrices (mg_solver_impl.hpp:175-2136): A[l] for l = 0..num_levels-1, P[l]
+ * and R[l] for l >= 1 (P[0], R[0] ignored); damped Jacobi, dense LU on the
+ * base level; the top level uses the operator given to ugx_mgmg_init or the
+ * solver. The multigrid shares the matrices (they stay alive while it does,
+ * whatever happens to the handles). As a preconditioner:
+ * ugx_solver_set_preconditioner(kind 3). */
+void* ugx_mgmg_create(int num_levels, void* const* A, void* const* P, void* const* R, int base_level, int nu1,
+                      int nu2, int cycle, double omega);
+void ugx_mgmg_destroy(void* g);
+int ugx_mgmg_init(void* g, void* matrix);
+int ugx_mgmg_apply(void* g, void* c, void* d);
+
 /* AssembledMultiGridCycle (mg_solver_impl.hpp:175-2136): damped Jacobi
  * smoother, P1 transfer, dense LU on the base level.
  * cycle: 1 = V, 2 = W, gamma > 2, or -1 = F (one F-cycle then one V-cycle on
@@ -400,7 +449,7 @@ void ugx_jacobi_destroy(void* j);
 /* CG (cg.h:103-242) = 0, BiCGStab (bicgstab.h:112-385) = 1, with StdConvCheck */
 void* ugx_solver_create(int type);
 void ugx_solver_destroy(void* s);
-/* precond_kind: 0 none, 1 GMG handle, 2 Jacobi handle */
+/* precond_kind: 0 none, 1 GMG handle, 2 Jacobi handle, 3 ugx_mgmg handle */
 int ugx_solver_set_preconditioner(void* s, int precond_kind, void* precond);
 int ugx_solver_set_conv_check(void* s, int max_steps, double min_defect, double rel_reduction, int verbose);
 int ugx_solver_init(void* s, void* matrix);

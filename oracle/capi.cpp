@@ -27,6 +27,9 @@ struct Problem {
     SolveResult last;
     double t_assemble = 0, t_setup = 0, t_solve = 0;
     std::string err;
+    // Q2 Lagrange instead of Q1 (kind 0): numbering of every level
+    bool q2 = false;
+    std::vector<Q2Numbering> q2num;
 };
 
 thread_local std::string g_err;
@@ -108,6 +111,35 @@ void orc_elements(void* h, int lev, int32_t* out) {
     }
 }
 
+// ---- Q2 (kind 0) ---------------------------------------------------------------------
+// switches the problem to Q2 Lagrange: later orc_assemble calls form the Q2
+// level operators and transfers in the Q2 LEVEL numbering
+int orc_q2_enable(void* h) {
+    auto* p = (Problem*)h;
+    try {
+        if (p->kind != 0) throw std::runtime_error("oracle: Q2 needs the hexahedral hierarchy (kind 0)");
+        p->q2 = true;
+        p->q2num.clear();
+        for (const auto& L : p->mg.levels) p->q2num.push_back(q2_numbering(L));
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int64_t orc_q2_num_dofs(void* h, int lev) { return ((Problem*)h)->q2num[lev].n_dof; }
+
+// DoF positions in doubled finest-lattice units, 3 int64 each, and the
+// Dirichlet flags
+void orc_q2_dof_positions(void* h, int lev, int64_t* pos2, uint8_t* dir) {
+    const auto& Q = ((Problem*)h)->q2num[lev];
+    for (int d = 0; d < Q.n_dof; ++d) {
+        for (int c = 0; c < 3; ++c) pos2[3 * d + c] = Q.pos2[d][c];
+        if (dir) dir[d] = Q.dir[d];
+    }
+}
+
 // ---- synthetic data ---------------------------------------------------------------
 
 void orc_make_theta(int n, uint64_t seed, int passes, double* out) {
@@ -156,7 +188,9 @@ int orc_assemble(void* h, int lev, double f_const) {
         k.dim = p->mg.dim;
         k.theta = p->theta.empty() ? nullptr : p->theta.data();
         double t0 = now();
-        if (p->kind == 0) {
+        if (p->kind == 0 && p->q2) {
+            assemble_q2_hex(p->mg, lev, p->q2num[lev], k, f_const, p->A[lev], p->b[lev]);
+        } else if (p->kind == 0) {
             assemble_q1_hex(p->mg, lev, k, f_const, p->A[lev], p->b[lev]);
         } else if (p->kind == 2) {
             assemble_fv1_tet(p->mg, lev, p->theta, p->tp, f_const, p->A[lev], p->b[lev]);
@@ -167,7 +201,10 @@ int orc_assemble(void* h, int lev, double f_const) {
             assemble_fv1_tri(p->mg, lev, k, f, p->A[lev], p->b[lev]);
         }
         p->t_assemble += now() - t0;
-        if (lev > 0) {
+        if (lev > 0 && p->q2) {
+            p->P[lev] = prolongation_q2(p->mg, lev, p->q2num[lev - 1], p->q2num[lev]);
+            p->R[lev] = restriction_q2(p->mg, lev, p->q2num[lev - 1], p->q2num[lev], p->P[lev]);
+        } else if (lev > 0) {
             p->P[lev] = prolongation_p1(p->mg, lev);
             p->R[lev] = restriction_from(p->mg, lev, p->P[lev]);
         }
@@ -304,7 +341,7 @@ int orc_solve_rhs(void* h, const double* rhs, double* x_out, int base_lev, doubl
         g.R = p->R;
         g.init();
         const int64_t n = p->A[top].n_rows;
-        auto dm = dirichlet_mask(p->mg, top);
+        auto dm = p->q2 ? std::vector<uint8_t>(p->q2num[top].dir) : dirichlet_mask(p->mg, top);
         std::vector<double> b(rhs, rhs + n), x(n, 0.0);
         if (zero_dirichlet)
             for (int64_t i = 0; i < n; ++i)
@@ -369,7 +406,7 @@ int orc_solve_adjoint(void* h, int solver, const double* rhs, double* lambda_out
         g.R = p->R;
         g.init();
         const int64_t n = p->A[top].n_rows;
-        auto dm = dirichlet_mask(p->mg, top);
+        auto dm = p->q2 ? std::vector<uint8_t>(p->q2num[top].dir) : dirichlet_mask(p->mg, top);
         std::vector<double> b(rhs, rhs + n), x(n, 0.0);
         for (int64_t i = 0; i < n; ++i)
             if (dm[i]) b[i] = 0.0;

@@ -233,7 +233,10 @@ static double dlagrange1d(int i, double x) {
     return s;
 }
 
-void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27][27], double fe[27]) {
+// the element loop of both variants: kq(xq) is the coefficient at the
+// quadrature point xq
+template <typename KFn>
+static void q2_hex_element_impl(const double X[8][3], KFn kq_at, double f_const, double Ke[27][27], double fe[27]) {
     int mi[27][3];
     q2_hex_multi_index(mi);
     const double r = 0.5 * std::sqrt(0.6);
@@ -256,6 +259,10 @@ void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27
                 double Ji[3][3];
                 const double det = inv3(J, Ji);
                 const double wd = w * std::fabs(det);
+                double xq[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < 8; ++a)
+                    for (int rr = 0; rr < 3; ++rr) xq[rr] += phi1[a] * X[a][rr];
+                const double k = kq_at(xq);
                 double g[27][3], ph[27];
                 for (int a = 0; a < 27; ++a) {
                     double dref[3];
@@ -273,6 +280,28 @@ void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27
                     fe[a] += wd * f_const * ph[a];
                 }
             }
+}
+
+void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27][27], double fe[27]) {
+    q2_hex_element_impl(X, [k](const double*) { return k; }, f_const, Ke, fe);
+}
+
+// The 3-point Gauss rule puts its middle point on a fine-element boundary
+// whenever an element spans an even number of fine elements, where a
+// piecewise-constant coefficient has two values. Such a point takes the fine
+// element on its upper side: xq is snapped to a 2^-40 grid first (the
+// boundaries are dyadic, the other Gauss points lie far from them), so that
+// the rounding of the trilinear map cannot decide it.
+void q2_hex_element_coef(const double X[8][3], const Coefficient& k, double f_const, double Ke[27][27],
+                         double fe[27]) {
+    q2_hex_element_impl(
+        X,
+        [&k](const double* xq) {
+            const double g = 1099511627776.0;   // 2^40
+            const double xs[3] = {std::round(xq[0] * g) / g, std::round(xq[1] * g) / g, std::round(xq[2] * g) / g};
+            return k.eval(xs);
+        },
+        f_const, Ke, fe);
 }
 
 void assemble_fv1_tri(const MultiGrid& mg, int lev, const Coefficient& k,

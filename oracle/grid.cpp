@@ -63,7 +63,15 @@ Key4 face_key(const int* v, int nv) {
 
 struct Builder {
     Level& L;
+    // the coarse object whose children are being created (recorded as the
+    // parent of every new edge, face and volume)
+    int8_t cur_ptype = -1;
+    int cur_parent = -1;
     explicit Builder(Level& l) : L(l) {}
+    void parent_of(int8_t t, int id) {
+        cur_ptype = t;
+        cur_parent = id;
+    }
 
     int add_vertex(const std::array<int64_t, 3>& p, int si, int8_t ptype, int parent) {
         L.vpos.push_back(p);
@@ -80,6 +88,8 @@ struct Builder {
         int id = (int)L.edges.size();
         L.edges.push_back({a, b});
         L.esub.push_back(si);
+        L.eparent_type.push_back(cur_ptype);
+        L.eparent.push_back(cur_parent);
         L.edge_map.emplace(edge_key(a, b), id);
         return id;
     }
@@ -91,6 +101,8 @@ struct Builder {
         L.faces.push_back(fv);
         L.fnv.push_back((int8_t)nv);
         L.fsub.push_back(si);
+        L.fparent_type.push_back(cur_ptype);
+        L.fparent.push_back(cur_parent);
         L.face_map.emplace(face_key(v, nv), id);
         std::array<int, 4> fe{-1, -1, -1, -1};
         for (int i = 0; i < nv; ++i) {
@@ -114,6 +126,8 @@ struct Builder {
         L.vols.push_back(hv);
         L.volnv.push_back(8);
         L.volsub.push_back(si);
+        L.volparent_type.push_back(cur_ptype);
+        L.volparent.push_back(cur_parent);
         std::array<int, 6> vf;
         for (int i = 0; i < 6; ++i) {
             const int* f = HEX_FACE[i];
@@ -141,6 +155,8 @@ struct Builder {
         L.vols.push_back(tv);
         L.volnv.push_back(4);
         L.volsub.push_back(si);
+        L.volparent_type.push_back(cur_ptype);
+        L.volparent.push_back(cur_parent);
         std::array<int, 6> vf{-1, -1, -1, -1, -1, -1};
         for (int i = 0; i < 4; ++i) {
             const int* f = TET_FACE[i];
@@ -237,6 +253,7 @@ Level refine(const Level& C) {
         int ab[2] = {C.edges[e][0], C.edges[e][1]};
         int m = B.add_vertex(avg(C, ab, 2), C.esub[e], OT_EDGE, (int)e);
         me[e] = m;
+        B.parent_of(OT_EDGE, (int)e);
         B.add_edge(cv[ab[0]], m, C.esub[e]);
         B.add_edge(m, cv[ab[1]], C.esub[e]);
     }
@@ -249,6 +266,7 @@ Level refine(const Level& C) {
             ev[j] = me[C.face_edges[f][j]];
         }
         const int si = C.fsub[f];
+        B.parent_of(OT_FACE, (int)f);
         if (n == 4) {
             int c = B.add_vertex(avg(C, C.faces[f].data(), 4), si, OT_FACE, (int)f);
             cf[f] = c;
@@ -267,6 +285,7 @@ Level refine(const Level& C) {
     }
     // 4. volumes (hexahedra, regular rule)
     for (size_t v = 0; v < C.vols.size(); ++v) {
+        B.parent_of(OT_VOLUME, (int)v);
         if (C.volnv[v] == 4) {
             refine_tet(C, B, cv, me, (int)v);
             continue;

@@ -44,6 +44,10 @@ struct Level {
     // edges
     std::vector<std::array<int, 2>> edges;
     std::vector<int> esub;
+    // parent object (type, id) at level-1 of every edge, face and volume (-1:
+    // level 0), recorded when the refinement creates it
+    std::vector<int8_t> eparent_type, fparent_type, volparent_type;
+    std::vector<int> eparent, fparent, volparent;
     // faces (tri: 3 verts, quad: 4 verts)
     std::vector<std::array<int, 4>> faces;
     std::vector<int8_t> fnv;
@@ -150,6 +154,31 @@ struct Coefficient {
 // the element stiffness k * int grad phi_a . grad phi_b and load f * int phi_a
 void q2_hex_multi_index(int mi[27][3]);
 void q2_hex_element(const double X[8][3], double k, double f_const, double Ke[27][27], double fe[27]);
+// the same with the coefficient taken at each quadrature point
+void q2_hex_element_coef(const double X[8][3], const Coefficient& k, double f_const, double Ke[27][27],
+                         double fe[27]);
+
+// Q2 on the hexahedral hierarchy (base grid kind 0), oracle/q2.cpp:
+// LEVEL DoF numbering (DoFDistribution::reinit, dof_distribution.cpp:
+// 1401-1444: vertices, then edges, faces, volumes, each subset by subset in
+// list order; one DoF per object), the level operators (element loop over the
+// level's hexahedra, local shapes mapped to the DoFs of their vertex / edge /
+// face / volume), the prolongation (StdTransfer::assemble_prolongation for
+// Lagrange, std_transfer_impl.h:214-388: the parent's closure DoFs with the
+// parent's shape values at the child's DoF position; adjust_prolongation,
+// lagrange_dirichlet_boundary_impl.h:485-614) and the restriction (its
+// transpose with adjust_restriction, :617-740).
+struct Q2Numbering {
+    int n_dof = 0;
+    std::vector<int> vdof, edof, fdof, hdof;   // DoF of every vertex, edge, face, hexahedron
+    std::vector<std::array<int64_t, 3>> pos2;  // DoF position in doubled finest-lattice units
+    std::vector<uint8_t> dir;                  // DoF in subset "Boundary"
+};
+Q2Numbering q2_numbering(const Level& L);
+void assemble_q2_hex(const MultiGrid& mg, int lev, const Q2Numbering& Q, const Coefficient& k, double f_const,
+                     CSR& A, std::vector<double>& b);
+CSR prolongation_q2(const MultiGrid& mg, int fine_lev, const Q2Numbering& Qc, const Q2Numbering& Qf);
+CSR restriction_q2(const MultiGrid& mg, int fine_lev, const Q2Numbering& Qc, const Q2Numbering& Qf, const CSR& P);
 void assemble_q1_hex(const MultiGrid& mg, int lev, const Coefficient& k, double f_const,
                      CSR& A, std::vector<double>& b);
 // FV1 (vertex-centred box method) on P1 triangles, diffusion k=1 (or k(x)),

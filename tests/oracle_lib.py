@@ -75,6 +75,9 @@ def lib():
             "orc_iterate_get": (C.c_int, [P, C.c_int, F64P]),
             "orc_csr_solve": (C.c_int, [C.c_int, I64, I64P, I32P, F64P, F64P, F64P, C.c_double, C.c_double,
                                         C.c_int]),
+            "orc_q2_enable": (C.c_int, [P]),
+            "orc_q2_num_dofs": (I64, [P, C.c_int]),
+            "orc_q2_dof_positions": (None, [P, C.c_int, I64P, U8P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -157,6 +160,8 @@ class Problem:
         return lib().orc_num_levels(self.h)
 
     def num_dofs(self, lev):
+        if getattr(self, "q2", False):
+            return lib().orc_q2_num_dofs(self.h, lev)
         return lib().orc_num_dofs(self.h, lev)
 
     def dof_lattice(self, lev):
@@ -284,6 +289,24 @@ class Problem:
     def set_tet_params(self, vel=(0.0, 0.0, 0.0), jitter_seed=0, jitter_amp=0.0):
         lib().orc_set_tet_params(self.h, float(vel[0]), float(vel[1]), float(vel[2]), int(jitter_seed),
                                  float(jitter_amp))
+
+    def enable_q2(self):
+        """switch to Q2 Lagrange (kind 0): later assemble calls form the Q2
+        level operators, prolongations and restrictions in the Q2 numbering"""
+        if lib().orc_q2_enable(self.h) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.q2 = True
+
+    def q2_num_dofs(self, lev):
+        return lib().orc_q2_num_dofs(self.h, lev)
+
+    def q2_dofs(self, lev):
+        """(positions in doubled finest-lattice units [N, 3], Dirichlet flags [N])"""
+        n = self.q2_num_dofs(lev)
+        pos = np.zeros(3 * n, np.int64)
+        dr = np.zeros(n, np.uint8)
+        lib().orc_q2_dof_positions(self.h, lev, pos, dr)
+        return pos.reshape(n, 3), dr
 
     def dof_positions(self, lev):
         n = self.num_dofs(lev)

@@ -133,6 +133,7 @@ _SIGNATURES = {
     "ugx_elemdisc_nnz": (_I64, [_P]),
     "ugx_elemdisc_num_colours": (_I, [_P]),
     "ugx_elemdisc_set_diffusion": (_I, [_P, _P]),
+    "ugx_elemdisc_set_diffusion_at_points": (_I, [_P, _P]),
     "ugx_elemdisc_set_source_const": (_I, [_P, _D]),
     "ugx_elemdisc_set_source_dofs": (_I, [_P, _P]),
     "ugx_elemdisc_set_velocity": (_I, [_P, _D, _D, _D]),
@@ -195,6 +196,21 @@ _SIGNATURES = {
     "ugx_solver_create": (_P, [_I]),
     "ugx_solver_destroy": (None, [_P]),
     "ugx_solver_set_preconditioner": (_I, [_P, _I, _P]),
+    "ugx_q2disc_create": (_P, [_I]),
+    "ugx_q2disc_destroy": (None, [_P]),
+    "ugx_q2disc_num_levels": (_I, [_P]),
+    "ugx_q2disc_num_dofs": (_I64, [_P, _I]),
+    "ugx_q2disc_dof_positions": (_I, [_P, _I, _P]),
+    "ugx_q2disc_dirichlet": (_I, [_P, _I, _P]),
+    "ugx_q2disc_set_theta": (_I, [_P, _P]),
+    "ugx_q2disc_set_source": (_I, [_P, _D]),
+    "ugx_q2disc_assemble_linear": (_I, [_P, _I, _P, _P]),
+    "ugx_q2disc_prolongation": (_I, [_P, _I, _P]),
+    "ugx_q2disc_restriction": (_I, [_P, _I, _P, _P]),
+    "ugx_mgmg_create": (_P, [_I, _P, _P, _P, _I, _I, _I, _I, _D]),
+    "ugx_mgmg_destroy": (None, [_P]),
+    "ugx_mgmg_init": (_I, [_P, _P]),
+    "ugx_mgmg_apply": (_I, [_P, _P, _P]),
     "ugx_solver_set_conv_check": (_I, [_P, _I, _D, _D, _I]),
     "ugx_solver_init": (_I, [_P, _P]),
     "ugx_solver_apply": (_I, [_P, _P, _P]),
@@ -710,6 +726,102 @@ class Jacobi:
             self.h = None
 
 
+class Q2Discretization:
+    """Q2 Lagrange on the unit cube (one hexahedron refined num_refs times):
+    every level in the reference's LEVEL DoF order, operators by the element
+    loop, StdTransfer's prolongations and restrictions (ugx_q2disc_*)."""
+
+    def __init__(self, num_refs):
+        self.num_refs = num_refs
+        self.h = _handle(lib().ugx_q2disc_create(int(num_refs)), "Q2Discretization")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_q2disc_destroy(self.h)
+            self.h = None
+
+    def num_levels(self):
+        return lib().ugx_q2disc_num_levels(self.h)
+
+    def num_dofs(self, level):
+        return lib().ugx_q2disc_num_dofs(self.h, level)
+
+    def dof_positions(self, level):
+        out = np.empty((self.num_dofs(level), 3), np.float64)
+        _check(lib().ugx_q2disc_dof_positions(self.h, level, _ptr(out)), "Q2Discretization.dof_positions")
+        return out
+
+    def dirichlet(self, level):
+        out = np.empty(self.num_dofs(level), np.uint8)
+        _check(lib().ugx_q2disc_dirichlet(self.h, level, _ptr(out)), "Q2Discretization.dirichlet")
+        return out
+
+    def set_theta(self, theta):
+        if theta is None:
+            _check(lib().ugx_q2disc_set_theta(self.h, None), "Q2Discretization.set_theta")
+            return
+        t = np.ascontiguousarray(theta, np.float64)
+        if t.size != 8 ** self.num_refs:
+            raise ValueError(f"theta needs {8 ** self.num_refs} values (one per top-level element)")
+        _check(lib().ugx_q2disc_set_theta(self.h, _ptr(t)), "Q2Discretization.set_theta")
+
+    def set_source(self, f):
+        _check(lib().ugx_q2disc_set_source(self.h, float(f)), "Q2Discretization.set_source")
+
+    def assemble_linear(self, level, A, b):
+        _check(lib().ugx_q2disc_assemble_linear(self.h, level, A.h, b.h), "Q2Discretization.assemble_linear")
+
+    def prolongation(self, level, P):
+        _check(lib().ugx_q2disc_prolongation(self.h, level, P.h), "Q2Discretization.prolongation")
+
+    def restriction(self, level, P, R):
+        _check(lib().ugx_q2disc_restriction(self.h, level, P.h, R.h), "Q2Discretization.restriction")
+
+    def hierarchy(self):
+        """level operators, prolongations and restrictions of every level:
+        (A, P, R, b_top) with P[0] = R[0] = None"""
+        A, P, R = [], [None], [None]
+        b = None
+        for lev in range(self.num_levels()):
+            a = SparseMatrix()
+            bv = Vector(self.num_dofs(lev))
+            self.assemble_linear(lev, a, bv)
+            A.append(a)
+            b = bv
+            if lev > 0:
+                p, r = SparseMatrix(), SparseMatrix()
+                self.prolongation(lev, p)
+                self.restriction(lev, p, r)
+                P.append(p)
+                R.append(r)
+        return A, P, R, b
+
+
+class MatrixMultiGrid:
+    """AssembledMultiGridCycle over level operators and transfers given as
+    matrices (ugx_mgmg_*): A[l] per level, P[l] and R[l] for l >= 1."""
+
+    def __init__(self, A, P, R, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66):
+        n = len(A)
+        if len(P) != n or len(R) != n:
+            raise ValueError("one A, P and R entry per level")
+        arr = lambda ms: (C.c_void_p * n)(*[m.h if m is not None else None for m in ms])
+        self._keep = (A, P, R)
+        self.h = _handle(lib().ugx_mgmg_create(n, arr(A), arr(P), arr(R), int(base_level), int(nu1), int(nu2),
+                                               int(cycle), float(omega)), "MatrixMultiGrid")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ugx_mgmg_destroy(self.h)
+            self.h = None
+
+    def init(self, A):
+        _check(lib().ugx_mgmg_init(self.h, A.h), "MatrixMultiGrid.init")
+
+    def apply(self, c, d):
+        _check(lib().ugx_mgmg_apply(self.h, c.h, d.h), "MatrixMultiGrid.apply")
+
+
 class _Solver:
     _type = 0
 
@@ -726,7 +838,8 @@ class _Solver:
             self.h = None
 
     def set_preconditioner(self, p):
-        kind = 1 if isinstance(p, GeometricMultiGrid) else 2 if isinstance(p, Jacobi) else 0
+        kind = (1 if isinstance(p, GeometricMultiGrid) else 2 if isinstance(p, Jacobi)
+                else 3 if isinstance(p, MatrixMultiGrid) else 0)
         self._precond = p
         _check(lib().ugx_solver_set_preconditioner(self.h, kind, p.h if p is not None else None), "set_preconditioner")
 
@@ -856,7 +969,8 @@ class ElementDiscretization:
     vertex order; dof_coords [num_dofs, dim]. The pattern, the slot of every
     local entry and the element colouring are built on the device here."""
 
-    TYPES = {"triangle_fv1": (0, 3, 2), "tetrahedron_fv1": (1, 4, 3), "hexahedron_q1": (2, 8, 3)}
+    TYPES = {"triangle_fv1": (0, 3, 2), "tetrahedron_fv1": (1, 4, 3), "hexahedron_q1": (2, 8, 3),
+             "hexahedron_q2": (3, 27, 3)}
 
     def __init__(self, elem_type, elem_dofs, dof_coords):
         if elem_type not in self.TYPES:
@@ -891,6 +1005,16 @@ class ElementDiscretization:
         if k.size != self.num_elems:
             raise ValueError(f"one diffusion value per element ({self.num_elems}) expected, got {k.size}")
         _check(lib().ugx_elemdisc_set_diffusion(self.h, _ptr(k)), "set_diffusion")
+
+    def set_diffusion_at_points(self, kq):
+        """Q2: k at the 3x3x3 Gauss points of every element [num_elems, 27] (None: per element)"""
+        if kq is None:
+            _check(lib().ugx_elemdisc_set_diffusion_at_points(self.h, None), "set_diffusion_at_points")
+            return
+        kq = np.ascontiguousarray(kq, np.float64)
+        if kq.size != 27 * self.num_elems:
+            raise ValueError(f"27 values per element ({27 * self.num_elems}) expected, got {kq.size}")
+        _check(lib().ugx_elemdisc_set_diffusion_at_points(self.h, _ptr(kq)), "set_diffusion_at_points")
 
     def set_source(self, f):
         if np.isscalar(f):

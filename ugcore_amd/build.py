@@ -13,7 +13,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libugcore_amd.so")
 
-SOURCES = ["kernels.hip", "tet_kernels.hip", "rap_kernels.hip", "mesh_kernels.hip", "q2_kernels.hip", "matrix_kernels.hip", "elem_assembly.hip", "spmv_dict.hip", "lattice_csr.hip", "hierarchy.cpp", "gpu_algebra.cpp", "io.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "tet_kernels.hip", "rap_kernels.hip", "mesh_kernels.hip", "q2_kernels.hip", "matrix_kernels.hip", "elem_assembly.hip", "spmv_dict.hip", "lattice_csr.hip", "q2_hierarchy.hip", "hierarchy.cpp", "gpu_algebra.cpp", "io.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("UGX_OFFLOAD_ARCH", "gfx950")
 

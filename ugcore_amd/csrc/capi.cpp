@@ -70,6 +70,12 @@ struct JacobiH {
 struct SolverH {
     std::shared_ptr<ILinearOperatorInverse> s;
 };
+struct Q2H {
+    std::unique_ptr<Q2Discretization> q;
+};
+struct MgmgH {
+    std::shared_ptr<MatrixMultiGrid> g;
+};
 
 // ---- the allocation-free dot product (ugx_dot_f64_noalloc) -------------------------
 // Its partial sums live in this file-scope device array instead of the context's
@@ -533,6 +539,12 @@ int ugx_elemdisc_set_diffusion(void* e, const double* k) {
         return 0;
     });
 }
+int ugx_elemdisc_set_diffusion_at_points(void* e, const double* kq) {
+    return guard([&] {
+        ((ElemH*)e)->e->set_diffusion_at_points(kq, false);
+        return 0;
+    });
+}
 int ugx_elemdisc_set_source_const(void* e, double f) {
     ((ElemH*)e)->e->set_source(f);
     return 0;
@@ -824,6 +836,105 @@ int ugx_gmg_set_transposed(void* g, int transposed) {
         return 0;
     });
 }
+// ---- Q2 hierarchy and the multigrid over matrices ----------------------------------
+void* ugx_q2disc_create(int num_refs) {
+    return guard_ptr([&] {
+        std::unique_ptr<Q2H> h(new Q2H());
+        h->q.reset(new Q2Discretization(num_refs));
+        return (void*)h.release();
+    });
+}
+void ugx_q2disc_destroy(void* q) { delete (Q2H*)q; }
+int ugx_q2disc_num_levels(void* q) { return ((Q2H*)q)->q->num_levels(); }
+int64_t ugx_q2disc_num_dofs(void* q, int level) {
+    auto& d = *((Q2H*)q)->q;
+    return level >= 0 && level < d.num_levels() ? d.num_dofs(level) : -1;
+}
+static void q2_level_check(Q2Discretization& d, int level) {
+    if (level < 0 || level >= d.num_levels()) throw UGError("Q2 discretization: level out of range");
+}
+int ugx_q2disc_dof_positions(void* q, int level, double* out) {
+    return guard([&] {
+        auto& d = *((Q2H*)q)->q;
+        q2_level_check(d, level);
+        if (!out) throw UGError("ugx_q2disc_dof_positions: null output");
+        d.dof_positions(level, out);
+        return 0;
+    });
+}
+int ugx_q2disc_dirichlet(void* q, int level, uint8_t* out) {
+    return guard([&] {
+        auto& d = *((Q2H*)q)->q;
+        q2_level_check(d, level);
+        if (!out) throw UGError("ugx_q2disc_dirichlet: null output");
+        d.dirichlet_flags(level, out);
+        return 0;
+    });
+}
+int ugx_q2disc_set_theta(void* q, const double* theta) {
+    return guard([&] {
+        ((Q2H*)q)->q->set_theta(theta);
+        return 0;
+    });
+}
+int ugx_q2disc_set_source(void* q, double f) {
+    ((Q2H*)q)->q->set_source(f);
+    return 0;
+}
+int ugx_q2disc_assemble_linear(void* q, int level, void* matrix, void* rhs) {
+    return guard([&] {
+        auto& d = *((Q2H*)q)->q;
+        q2_level_check(d, level);
+        MatrixH* m = (MatrixH*)matrix;
+        d.assemble_linear(level, *m->A, ((VectorH*)rhs)->v);
+        return 0;
+    });
+}
+int ugx_q2disc_prolongation(void* q, int level, void* P) {
+    return guard([&] {
+        ((Q2H*)q)->q->prolongation(level, *((MatrixH*)P)->A);
+        return 0;
+    });
+}
+int ugx_q2disc_restriction(void* q, int level, void* P, void* R) {
+    return guard([&] {
+        ((Q2H*)q)->q->restriction(level, *((MatrixH*)P)->A, *((MatrixH*)R)->A);
+        return 0;
+    });
+}
+void* ugx_mgmg_create(int num_levels, void* const* A, void* const* P, void* const* R, int base_level, int nu1,
+                      int nu2, int cycle, double omega) {
+    return guard_ptr([&] {
+        if (num_levels < 1 || !A || !P || !R) throw UGError("ugx_mgmg_create: levels missing");
+        std::vector<std::shared_ptr<GPUSparseMatrix>> a(num_levels), p(num_levels), r(num_levels);
+        for (int l = 0; l < num_levels; ++l) {
+            if (!A[l]) throw UGError("ugx_mgmg_create: null level operator");
+            a[l] = ((MatrixH*)A[l])->A;
+            if (l > 0) {
+                if (!P[l] || !R[l]) throw UGError("ugx_mgmg_create: null transfer matrix");
+                p[l] = ((MatrixH*)P[l])->A;
+                r[l] = ((MatrixH*)R[l])->A;
+            }
+        }
+        std::unique_ptr<MgmgH> h(new MgmgH());
+        h->g = std::make_shared<MatrixMultiGrid>();
+        h->g->set_levels(a, p, r);
+        h->g->set_base_level(base_level);
+        h->g->set_num_presmooth(nu1);
+        h->g->set_num_postsmooth(nu2);
+        h->g->set_cycle_type(cycle);
+        h->g->set_smoother_damp(omega);
+        return (void*)h.release();
+    });
+}
+void ugx_mgmg_destroy(void* g) { delete (MgmgH*)g; }
+int ugx_mgmg_init(void* g, void* matrix) {
+    return guard([&] { return ((MgmgH*)g)->g->init(((MatrixH*)matrix)->op) ? 0 : -1; });
+}
+int ugx_mgmg_apply(void* g, void* c, void* d) {
+    return guard([&] { return ((MgmgH*)g)->g->apply(((VectorH*)c)->v, ((VectorH*)d)->v) ? 0 : -1; });
+}
+
 void* ugx_gmg_create(void* disc, int base_level, int nu1, int nu2, int cycle, double omega, int precision) {
     return guard_ptr([&] {
         if (!disc) throw UGError("ugx_gmg_create: null discretization");
@@ -901,6 +1012,7 @@ int ugx_solver_set_preconditioner(void* s, int kind, void* p) {
         if (kind == 0 || !p) solver.set_preconditioner(nullptr);
         else if (kind == 1) solver.set_preconditioner(((GmgH*)p)->g);
         else if (kind == 2) solver.set_preconditioner(((JacobiH*)p)->j);
+        else if (kind == 3) solver.set_preconditioner(((MgmgH*)p)->g);
         else throw UGError("ugx_solver_set_preconditioner: unknown kind");
         return 0;
     });

@@ -431,7 +431,8 @@ void GPUSparseMatrix::clear_retain_structure() {
             for (auto& e : row) e.second = 0.0;
         return;
     }
-    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, val_.get(), 0.0, val_.get(), context().stream));
+    // written, not scaled: 0 * NaN would stay NaN
+    if (nnz_ > 0) UGX_LAUNCH(vec_set<double>(nnz_, val_.get(), 0.0, context().stream));
     values_changed_on_device();
 }
 
@@ -1203,9 +1204,9 @@ void DomainDiscretization::adjoint_gradient(const GPUVector& u, const GPUVector&
 ElementDiscretization::ElementDiscretization(int elem_type, int64_t num_elems, const int32_t* elem_dofs,
                                              int64_t num_dofs, const double* dof_coords, bool on_device)
     : type_(elem_type), ne_(num_elems), ndofs_(num_dofs) {
-    if (elem_type < 0 || elem_type > 2)
-        throw UGError("ElementDiscretization: element type 0 (triangle, FV1), 1 (tetrahedron, FV1) or 2 "
-                      "(hexahedron, Q1 FE)");
+    if (elem_type < 0 || elem_type > 3)
+        throw UGError("ElementDiscretization: element type 0 (triangle, FV1), 1 (tetrahedron, FV1), 2 "
+                      "(hexahedron, Q1 FE) or 3 (hexahedron, Q2 FE)");
     nv_ = vertices_of(elem_type);
     dim_ = dim_of(elem_type);
     if (num_elems < 0 || num_dofs < 0 || num_dofs >= ((int64_t)1 << 31))
@@ -1312,8 +1313,23 @@ void ElementDiscretization::set_diffusion(const double* k, bool on_device) {
     has_k_ = true;
 }
 
+void ElementDiscretization::set_diffusion_at_points(const double* kq, bool on_device) {
+    if (type_ != HEXAHEDRON_Q2) throw UGError("ElementDiscretization: coefficients at Gauss points are for Q2");
+    if (!kq) {
+        has_kq_ = false;
+        return;
+    }
+    kq_.resize(ne_ * 27);
+    if (on_device)
+        UGX_HIP(hipMemcpyAsync(kq_.get(), kq, sizeof(double) * ne_ * 27, hipMemcpyDeviceToDevice, context().stream));
+    else
+        kq_.upload(kq, ne_ * 27);
+    has_kq_ = true;
+}
+
 void ElementDiscretization::set_source_dofs(const double* f_host) {
-    if (type_ == HEXAHEDRON_Q1) throw UGError("ElementDiscretization: the Q1 source is a constant");
+    if (type_ == HEXAHEDRON_Q1 || type_ == HEXAHEDRON_Q2)
+        throw UGError("ElementDiscretization: the Q1/Q2 source is a constant");
     f_dof_.resize(ndofs_);
     f_dof_.upload(f_host, ndofs_);
     has_f_dof_ = true;
@@ -1360,6 +1376,14 @@ void ElementDiscretization::run(double* val, double* rhs, const double* u, doubl
     L.rhs = rhs;
     L.u = u;
     L.d = d;
+    if (type_ == HEXAHEDRON_Q2) {
+        for (int col = 0; col + 1 < (int)colour_start_.size(); ++col)
+            UGX_LAUNCH(elem_assemble_q2_colour(ne_, order_.get(), dofs_.get(), coords_.get(), L.kel,
+                                               has_kq_ ? kq_.get() : nullptr, f_const_, slot_.get(), val, rhs,
+                                               colour_start_[col], colour_start_[col + 1] - colour_start_[col],
+                                               c.stream));
+        return;
+    }
     for (int col = 0; col + 1 < (int)colour_start_.size(); ++col)
         UGX_LAUNCH(elem_assemble_colour(type_, L, colour_start_[col], colour_start_[col + 1] - colour_start_[col],
                                         c.stream));
@@ -1390,7 +1414,7 @@ void ElementDiscretization::finish_matrix(GPUSparseMatrix& A, bool kept) {
 void ElementDiscretization::assemble_linear(GPUSparseMatrix& A, GPUVector& b) {
     auto& c = context();
     const bool kept = give_pattern(A);
-    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, A.val(), 0.0, A.val(), c.stream));
+    if (nnz_ > 0) UGX_LAUNCH(vec_set<double>(nnz_, A.val(), 0.0, c.stream));   // fresh memory may hold NaN
     b.resize(ndofs_);
     b.set(0.0);
     run(A.val(), b.data(), nullptr, nullptr);
@@ -1406,7 +1430,7 @@ void ElementDiscretization::assemble_jacobian(GPUSparseMatrix& J, const GPUVecto
     if (u.size() != ndofs_) throw UGError("ElementDiscretization::assemble_jacobian: size mismatch");
     auto& c = context();
     const bool kept = give_pattern(J);
-    if (nnz_ > 0) UGX_LAUNCH(scale_values(nnz_, J.val(), 0.0, J.val(), c.stream));
+    if (nnz_ > 0) UGX_LAUNCH(vec_set<double>(nnz_, J.val(), 0.0, c.stream));   // fresh memory may hold NaN
     run(J.val(), nullptr, nullptr, nullptr);
     if (ndir_ > 0) UGX_LAUNCH(dirichlet_rows(ndir_, dir_.get(), J.rowptr(), J.col(), J.val(), c.stream));
     sync();
@@ -1417,8 +1441,17 @@ void ElementDiscretization::assemble_defect(GPUVector& d, const GPUVector& u) {
     if (u.size() != ndofs_) throw UGError("ElementDiscretization::assemble_defect: size mismatch");
     if (&d == &u) throw UGError("ElementDiscretization::assemble_defect: d must not alias u");
     d.resize(ndofs_);
-    d.set(0.0);
-    run(nullptr, nullptr, u.data(), d.data());
+    if (type_ == HEXAHEDRON_Q2) {
+        // the Q2 element routine forms matrices only: d = A u - b with the
+        // assembled operator (its non-Dirichlet rows are the element sums)
+        GPUSparseMatrix T;
+        GPUVector b(ndofs_);
+        assemble_linear(T, b);
+        T.axpy(d, -1.0, b, 1.0, u);
+    } else {
+        d.set(0.0);
+        run(nullptr, nullptr, u.data(), d.data());
+    }
     if (ndir_ > 0) UGX_LAUNCH(zero_listed(ndir_, dir_.get(), d.data(), context().stream));
 }
 
@@ -1683,6 +1716,44 @@ bool host_lu(int n, std::vector<double>& a, std::vector<int32_t>& piv) {
     return true;
 }
 
+// The base solve of the multigrid cycles: LU with partial pivoting of the
+// dense matrix (row-major, overwritten), the row permutation of SolveLU (swap
+// x[i], x[piv[i]] in order) and the inverses of the unit lower and of the
+// upper triangular factor, row-major (linvT[i*n+k] = Linv(i,k)), which
+// lu_solve applies as two matrix-vector products. False: singular.
+bool dense_lu_factors(int m, std::vector<double>& dense, std::vector<double>& linvT, std::vector<double>& uinvT,
+                      std::vector<int32_t>& perm) {
+    std::vector<int32_t> piv;
+    if (!host_lu(m, dense, piv)) return false;
+    perm.resize(m);
+    std::iota(perm.begin(), perm.end(), 0);
+    for (int i = 0; i < m; ++i)
+        if (i < piv[i]) std::swap(perm[i], perm[piv[i]]);
+    linvT.assign((size_t)m * m, 0.0);
+    uinvT.assign((size_t)m * m, 0.0);
+    auto LU = [&](int r, int c) { return dense[(size_t)r * m + c]; };
+    for (int k = 0; k < m; ++k) {
+        // column k of L^{-1}: solve L y = e_k (unit diagonal)
+        std::vector<double> y(m, 0.0);
+        y[k] = 1.0;
+        for (int i = k + 1; i < m; ++i) {
+            double acc = 0.0;
+            for (int j = k; j < i; ++j) acc += LU(i, j) * y[j];
+            y[i] = -acc;
+        }
+        for (int i = 0; i < m; ++i) linvT[(size_t)i * m + k] = y[i];
+        // column k of U^{-1}: solve U z = e_k
+        std::vector<double> z(m, 0.0);
+        for (int i = k; i >= 0; --i) {
+            double acc = (i == k) ? 1.0 : 0.0;
+            for (int j = i + 1; j <= k; ++j) acc -= LU(i, j) * z[j];
+            z[i] = acc / LU(i, i);
+        }
+        for (int i = 0; i < m; ++i) uinvT[(size_t)i * m + k] = z[i];
+    }
+    return true;
+}
+
 template <typename T>
 struct GmgLevel {
     int dim = 3, W = 0, nd = 0;
@@ -1842,37 +1913,11 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
                         if (j < 0 || j >= l.N) continue;
                         dense[(size_t)i * lu_n + j] += v;
                     }
-                std::vector<int32_t> piv;
-                if (!host_lu(lu_n, dense, piv)) throw UGError("GMG: base level matrix is singular");
-                // row permutation of SolveLU (swap x[i], x[piv[i]] in order)
-                std::vector<int32_t> perm(lu_n);
-                std::iota(perm.begin(), perm.end(), 0);
-                for (int i = 0; i < lu_n; ++i)
-                    if (i < piv[i]) std::swap(perm[i], perm[piv[i]]);
-                // inverses of the unit lower and the upper triangular factor,
-                // stored row-major: linvT[i*n+k] = Linv(i,k)
+                std::vector<double> linvT, uinvT;
+                std::vector<int32_t> perm;
+                if (!dense_lu_factors(lu_n, dense, linvT, uinvT, perm))
+                    throw UGError("GMG: base level matrix is singular");
                 const int m = lu_n;
-                std::vector<double> linvT((size_t)m * m, 0.0), uinvT((size_t)m * m, 0.0);
-                auto LU = [&](int r, int c) { return dense[(size_t)r * m + c]; };
-                for (int k = 0; k < m; ++k) {
-                    // column k of L^{-1}: solve L y = e_k (unit diagonal)
-                    std::vector<double> y(m, 0.0);
-                    y[k] = 1.0;
-                    for (int i = k + 1; i < m; ++i) {
-                        double acc = 0.0;
-                        for (int j = k; j < i; ++j) acc += LU(i, j) * y[j];
-                        y[i] = -acc;
-                    }
-                    for (int i = 0; i < m; ++i) linvT[(size_t)i * m + k] = y[i];
-                    // column k of U^{-1}: solve U z = e_k
-                    std::vector<double> z(m, 0.0);
-                    for (int i = k; i >= 0; --i) {
-                        double acc = (i == k) ? 1.0 : 0.0;
-                        for (int j = i + 1; j <= k; ++j) acc -= LU(i, j) * z[j];
-                        z[i] = acc / LU(i, i);
-                    }
-                    for (int i = 0; i < m; ++i) uinvT[(size_t)i * m + k] = z[i];
-                }
                 lu_linvT.resize((int64_t)m * m);
                 lu_linvT.upload(linvT.data(), (int64_t)m * m);
                 lu_uinvT.resize((int64_t)m * m);
@@ -2288,5 +2333,276 @@ void AssembledMultiGridCycle::smooth_level(int lev, int nsteps) {
 }
 int AssembledMultiGridCycle::level_num_diags(int lev) const { return impl_ ? impl_->num_diags(lev) : 0; }
 
+// ---------------------------------------------------------------------------
+// MatrixMultiGrid
+// ---------------------------------------------------------------------------
+void MatrixMultiGrid::set_levels(std::vector<std::shared_ptr<GPUSparseMatrix>> A,
+                                 std::vector<std::shared_ptr<GPUSparseMatrix>> P,
+                                 std::vector<std::shared_ptr<GPUSparseMatrix>> R) {
+    if (A.empty() || P.size() != A.size() || R.size() != A.size())
+        throw UGError("MatrixMultiGrid: one A, P and R entry per level (P[0], R[0] unused)");
+    A_ = std::move(A);
+    P_ = std::move(P);
+    R_ = std::move(R);
+    top_ = (int)A_.size() - 1;
+}
+
+bool MatrixMultiGrid::init(std::shared_ptr<MatrixOperator> J) {
+    // the checks of set_cycle_type / init (mg_solver_impl.hpp:379-496)
+    if (cycle_ != _F_ && cycle_ < 1) throw UGError("GMG: cycle type must be 1 (V), 2 (W), gamma > 2 or -1 (F)");
+    if (nu1_ < 0 || nu2_ < 0) throw UGError("GMG: smoothing step counts must be >= 0");
+    if (top_ < 0) throw UGError("MatrixMultiGrid: no levels");
+    if (base_ < 0 || base_ > top_) throw UGError("MatrixMultiGrid: base level out of range");
+    op_ = J;
+    auto rows = [&](int l) { return Aop(l).num_rows(); };
+    if (rows(top_) != A_[top_]->num_rows() || Aop(top_).num_cols() != rows(top_))
+        throw UGError("MatrixMultiGrid: the operator does not match the top level");
+    for (int l = base_; l <= top_; ++l) {
+        if (Aop(l).building()) throw UGError("MatrixMultiGrid: a level operator is still being built");
+        if (l > base_) {
+            if (!P_[l] || !R_[l]) throw UGError("MatrixMultiGrid: missing transfer matrix");
+            if (P_[l]->num_rows() != rows(l) || P_[l]->num_cols() != rows(l - 1) || R_[l]->num_rows() != rows(l - 1) ||
+                R_[l]->num_cols() != rows(l))
+                throw UGError("MatrixMultiGrid: transfer matrix sizes do not match the levels");
+        }
+    }
+    sc_.assign(top_ + 1, GPUVector());
+    sd_.assign(top_ + 1, GPUVector());
+    st_.assign(top_ + 1, GPUVector());
+    smoother_.clear();
+    smoother_.resize(top_ + 1);
+    for (int l = base_; l <= top_; ++l) {
+        sc_[l].resize(rows(l));
+        sd_[l].resize(rows(l));
+        st_[l].resize(rows(l));
+        if (l > base_) {
+            smoother_[l].reset(new Jacobi(omega_));
+            auto opl = l == top_ ? J : std::make_shared<MatrixOperator>(A_[l]);
+            if (!smoother_[l]->init(opl)) throw UGError("GMG: Jacobi found a zero diagonal");
+        }
+    }
+    // base level: dense LU (lu.h:236-269, dense below the sparse threshold)
+    GPUSparseMatrix& B = Aop(base_);
+    const int64_t m = B.num_rows();
+    if (m > 4000) throw UGError("MatrixMultiGrid: the base level is too large for the dense LU (> 4000 rows)");
+    std::vector<int32_t> rp(m + 1), cl(B.total_num_connections());
+    std::vector<double> vl(B.total_num_connections());
+    B.download_csr(rp.data(), cl.data(), vl.data());
+    std::vector<double> dense((size_t)m * m, 0.0), linvT, uinvT;
+    for (int64_t r = 0; r < m; ++r)
+        for (int32_t k = rp[r]; k < rp[r + 1]; ++k) dense[(size_t)r * m + cl[k]] += vl[k];
+    std::vector<int32_t> perm;
+    if (!dense_lu_factors((int)m, dense, linvT, uinvT, perm)) throw UGError("GMG: base level matrix is singular");
+    lu_n_ = (int)m;
+    lu_linvT_.resize(m * m);
+    lu_linvT_.upload(linvT.data(), m * m);
+    lu_uinvT_.resize(m * m);
+    lu_uinvT_.upload(uinvT.data(), m * m);
+    lu_perm_.resize(m);
+    lu_perm_.upload(perm.data(), m);
+    return true;
+}
+
+bool MatrixMultiGrid::apply(GPUVector& c, const GPUVector& d) {
+    if (top_ < 0 || (int)sc_.size() != top_ + 1) throw UGError("MatrixMultiGrid: not initialised");
+    if (d.size() != sd_[top_].size() || c.size() != sd_[top_].size()) throw UGError("GMG::apply: size mismatch");
+    // mg_solver_impl.hpp:175-275
+    sd_[top_] = d;
+    c.set(0.0);
+    sc_[top_].set(0.0);
+    lmgc(top_, cycle_);
+    c += sc_[top_];
+    return true;
+}
+
+void MatrixMultiGrid::lmgc(int lev, int type) {
+    // mg_solver_impl.hpp:2090-2136
+    if (lev == base_) {
+        base_solve(top_);
+        return;
+    }
+    presmooth_and_restriction(lev);
+    if (lev - 1 == base_) {
+        base_solve(lev - 1);
+    } else if (type == _F_) {
+        lmgc(lev - 1, _F_);
+        lmgc(lev - 1, _V_);
+    } else {
+        for (int i = 0; i < type; ++i) lmgc(lev - 1, type);
+    }
+    prolongation_and_postsmooth(lev);
+}
+
+void MatrixMultiGrid::presmooth_and_restriction(int lev) {
+    // mg_solver_impl.hpp:1686-1816
+    GPUVector &sc = sc_[lev], &sd = sd_[lev], &st = st_[lev];
+    for (int nu = 0; nu < nu1_; ++nu) {
+        smoother_[lev]->apply(st, sd);
+        Aop(lev).matmul_minus(sd, st);
+        sc += st;
+    }
+    sc_[lev - 1].set(0.0);
+    // StdTransfer::do_restrict (std_transfer_impl.h:774-806)
+    R_[lev]->apply_ignore_zero_rows(sd_[lev - 1], 1.0, sd);
+}
+
+void MatrixMultiGrid::prolongation_and_postsmooth(int lev) {
+    // mg_solver_impl.hpp:1819-1964; StdTransfer::prolongate (std_transfer_impl.h:719-772)
+    GPUVector &sc = sc_[lev], &sd = sd_[lev], &st = st_[lev];
+    P_[lev]->axpy(st, 0.0, st, 1.0, sc_[lev - 1]);
+    sc += st;
+    for (int nu = 0; nu < nu2_; ++nu) {
+        Aop(lev).matmul_minus(sd, st);
+        smoother_[lev]->apply(st, sd);
+        sc += st;
+    }
+    if (lev == top_) Aop(lev).matmul_minus(sd, st);
+}
+
+void MatrixMultiGrid::base_solve(int lev) {
+    // mg_solver_impl.hpp:1968-2086: the base level is solved once
+    auto& c = context();
+    UGX_LAUNCH(lu_solve<double>(lu_n_, lu_linvT_.get(), lu_uinvT_.get(), lu_perm_.get(), sd_[base_].data(),
+                                sc_[base_].data(), c.stream));
+    if (lev == top_) Aop(lev).matmul_minus(sd_[lev], sc_[lev]);
+}
+
+// ---------------------------------------------------------------------------
+// Q2Discretization
+// ---------------------------------------------------------------------------
+Q2Discretization::Q2Discretization(int num_refs) : refs_(num_refs) {
+    if (num_refs < 0 || num_refs > 6) throw UGError("Q2Discretization: num_refs must be in [0, 6]");
+    auto& c = context();
+    // the vertices of level l+1 of the hexahedral hierarchy are the Q2 nodes
+    // of level l, and its P1 numbering orders them within each subset
+    std::vector<std::unique_ptr<LevelGrid>> grids;
+    build_device_levels(0, num_refs + 1, 0, grids);
+    L_.resize(num_refs + 1);
+    for (int l = 0; l <= num_refs; ++l) {
+        LevelGrid& G = *grids[l + 1];
+        Level& Q = L_[l];
+        Q.n = 1 << l;
+        Q.N = G.N;
+        DeviceBuffer<int32_t> lat_of_dof(G.N);
+        UGX_LAUNCH(invert_map(G.N, G.dof_of_lat.get(), lat_of_dof.get(), c.stream));
+        size_t tb = 0;
+        if (q2_numbering_temp_bytes(G.N, &tb) != 0) throw UGError("Q2Discretization: sort set-up failed");
+        DeviceBuffer<uint64_t> key(G.N), key2(G.N);
+        DeviceBuffer<int32_t> node(G.N);
+        DeviceBuffer<uint8_t> temp((int64_t)std::max<size_t>(tb, 1));
+        Q.node_of_dof.resize(G.N);
+        Q.dof_of_node.resize(G.N);
+        Q.dir.resize(G.N);
+        UGX_LAUNCH(q2_numbering(G.N, G.n_inner, G.W, lat_of_dof.get(), key.get(), key2.get(), node.get(),
+                                Q.node_of_dof.get(), temp.get(), tb, Q.dof_of_node.get(), Q.dir.get(), c.stream));
+        sync();   // the scratch is released next
+    }
+}
+
+void Q2Discretization::set_theta(const double* theta_host) {
+    if (!theta_host) {
+        has_theta_ = false;
+    } else {
+        const int64_t n = (int64_t)1 << refs_;
+        theta_.resize(n * n * n);
+        theta_.upload(theta_host, n * n * n);
+        has_theta_ = true;
+    }
+    for (auto& Q : L_) Q.coef_valid = false;
+}
+
+void Q2Discretization::dof_positions(int lev, double* out) const {
+    const Level& Q = L_.at(lev);
+    std::vector<int32_t> v((size_t)Q.N);
+    Q.node_of_dof.download(v.data(), Q.N);
+    const int64_t W = 2 * (int64_t)Q.n + 1;
+    const double s = 1.0 / (2.0 * Q.n);
+    for (int64_t k = 0; k < Q.N; ++k) {
+        out[3 * k] = (double)(v[(size_t)k] % W) * s;
+        out[3 * k + 1] = (double)((v[(size_t)k] / W) % W) * s;
+        out[3 * k + 2] = (double)(v[(size_t)k] / (W * W)) * s;
+    }
+}
+
+void Q2Discretization::dirichlet_flags(int lev, uint8_t* out) const {
+    const Level& Q = L_.at(lev);
+    Q.dir.download(out, Q.N);
+}
+
+void Q2Discretization::assemble_linear(int lev, GPUSparseMatrix& A, GPUVector& b) {
+    Level& Q = L_.at(lev);
+    auto& c = context();
+    const int64_t ne = (int64_t)Q.n * Q.n * Q.n;
+    if (!Q.elem) {
+        DeviceBuffer<int32_t> conn(ne * 27);
+        DeviceBuffer<double> X(Q.N * 3);
+        UGX_LAUNCH(q2_connectivity(Q.n, Q.dof_of_node.get(), conn.get(), c.stream));
+        UGX_LAUNCH(q2_coords(Q.N, Q.n, Q.node_of_dof.get(), X.get(), c.stream));
+        Q.elem.reset(
+            new ElementDiscretization(ElementDiscretization::HEXAHEDRON_Q2, ne, conn.get(), Q.N, X.get(), true));
+        std::vector<uint8_t> dir((size_t)Q.N);
+        Q.dir.download(dir.data(), Q.N);
+        std::vector<int32_t> dl;
+        for (int64_t i = 0; i < Q.N; ++i)
+            if (dir[(size_t)i]) dl.push_back((int32_t)i);
+        Q.elem->set_dirichlet(dl.data(), (int64_t)dl.size());
+        Q.coef_valid = false;
+    }
+    if (!Q.coef_valid) {
+        if (has_theta_) {
+            DeviceBuffer<double> kq(ne * 27);
+            UGX_LAUNCH(q2_coef_points(Q.n, 1 << refs_, theta_.get(), kq.get(), c.stream));
+            Q.elem->set_diffusion_at_points(kq.get(), true);
+            sync();   // the copy has read kq
+        } else {
+            Q.elem->set_diffusion_at_points(nullptr);
+            Q.elem->set_diffusion(nullptr);
+        }
+        Q.coef_valid = true;
+    }
+    Q.elem->set_source(f_);
+    Q.elem->assemble_linear(A, b);
+}
+
+void Q2Discretization::prolongation(int lev, GPUSparseMatrix& P) const {
+    if (lev < 1 || lev >= (int)L_.size()) throw UGError("Q2Discretization::prolongation: level out of range");
+    const Level& F = L_[lev];
+    const Level& C = L_[lev - 1];
+    auto& c = context();
+    DeviceBuffer<int32_t> cnt(F.N), rowptr(F.N + 1);
+    DeviceBuffer<int64_t> incl(F.N);
+    UGX_LAUNCH(q2_prolong_count(F.N, F.n, F.node_of_dof.get(), cnt.get(), c.stream));
+    size_t tb = 0;
+    if (rowptr_scan_temp_bytes(F.N, &tb) != 0) throw UGError("Q2Discretization: scan set-up failed");
+    DeviceBuffer<uint8_t> temp((int64_t)std::max<size_t>(tb, 1));
+    UGX_LAUNCH(rowptr_from_counts(F.N, cnt.get(), incl.get(), temp.get(), tb, rowptr.get(), c.stream));
+    int64_t nnz = 0;
+    incl.download(&nnz, 1, F.N - 1);
+    P.allocate(F.N, C.N, nnz);
+    P.set_origin(nullptr);
+    UGX_HIP(hipMemcpyAsync(P.rowptr(), rowptr.get(), sizeof(int32_t) * (F.N + 1), hipMemcpyDeviceToDevice, c.stream));
+    UGX_LAUNCH(q2_prolong_fill(F.N, F.n, F.node_of_dof.get(), F.dir.get(), C.dof_of_node.get(), P.rowptr(), P.col(),
+                               P.val(), c.stream));
+    sync();
+    P.build_spmv_layout();
+}
+
+void Q2Discretization::restriction(int lev, const GPUSparseMatrix& P, GPUSparseMatrix& R) const {
+    if (lev < 1 || lev >= (int)L_.size()) throw UGError("Q2Discretization::restriction: level out of range");
+    const Level& F = L_[lev];
+    const Level& C = L_[lev - 1];
+    if (P.num_rows() != F.N || P.num_cols() != C.N) throw UGError("Q2Discretization::restriction: P size mismatch");
+    auto& c = context();
+    R.set_as_transpose_of(P);   // every stored entry, explicit zeros included
+    R.values_edited_externally();
+    DeviceBuffer<int32_t> last(C.N);
+    UGX_HIP(hipMemsetAsync(last.get(), 0xff, sizeof(int32_t) * (size_t)C.N, c.stream));
+    UGX_LAUNCH(q2_restriction_adjust(F.N, F.n, C.N, F.node_of_dof.get(), F.dir.get(), C.dof_of_node.get(), last.get(),
+                                     R.rowptr(), R.col(), R.val(), c.stream));
+    sync();
+    R.finalize();
+}
+
 }  // namespace gpu
 }  // namespace ug
+

@@ -70,7 +70,8 @@ __constant__ Q2RefGrad kQ2Grad = q2_ref_grad();
 constexpr int kQ2Waves = 4;   // elements (wavefronts) per workgroup
 
 // The element computation of one wavefront (lane l): corners from src.corner(l)
-// for l < 24 (valid elements) and src.k(), then sink.ke(i, j, v) for the 729
+// for l < 24 (valid elements) and the coefficient src.k(q) at Gauss point q
+// (x fastest), then sink.ke(i, j, v) for the 729
 // entries and sink.fe(a, v) for the 27 load entries. Every wave of the
 // workgroup calls it once (it holds two barriers); invalid waves work on the
 // unit cube and emit nothing.
@@ -113,7 +114,7 @@ __device__ __forceinline__ void q2_element_wave(int l, bool valid, const Src& sr
         ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
         ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
         const double wd = q2_gauss_w(qx) * q2_gauss_w(qy) * q2_gauss_w(qz) * fabs(det);
-        cs[l] = wd * (valid ? src.k() : 1.0);
+        cs[l] = wd * (valid ? src.k(l) : 1.0);
         fw[l] = wd * f_const;
     }
     __syncthreads();
@@ -169,7 +170,7 @@ struct Q2ArraySrc {
     const double* kc;
     int64_t e;
     __device__ double corner(int l) const { return X[e * 24 + l]; }
-    __device__ double k() const { return kc ? kc[e] : 1.0; }
+    __device__ double k(int) const { return kc ? kc[e] : 1.0; }
 };
 struct Q2ArraySink {
     double* Kout;
@@ -261,7 +262,7 @@ struct Q2LatticeSrc {
         const int e = d == 0 ? ex : d == 1 ? ey : ez;
         return (double)(e + bit) * h;
     }
-    __device__ double k() const { return theta ? exp(theta[((int64_t)ez * n + ey) * n + ex]) : 1.0; }
+    __device__ double k(int) const { return theta ? exp(theta[((int64_t)ez * n + ey) * n + ex]) : 1.0; }
 };
 struct Q2CsrSink {
     int W, ex, ey, ez;
@@ -308,6 +309,56 @@ __global__ void k_q2_lattice_dirichlet(int W, int64_t N, const int32_t* __restri
     }
 }
 
+// ---- Q2 hexahedra of a caller-supplied connectivity (ElementDiscretization) ----
+struct Q2ElemSrc {
+    const double* coords;   // DoF coordinates [N][3]
+    const int32_t* dofs;    // [ne][27] in the local shape order
+    const double* kel;      // per element or null
+    const double* kq;       // per element and Gauss point [ne][27] or null
+    int64_t e;
+    __device__ double corner(int l) const {
+        const int c = l / 3, d = l - 3 * c;
+        return coords[(int64_t)dofs[e * 27 + c] * 3 + d];
+    }
+    __device__ double k(int q) const { return kq ? kq[e * 27 + q] : (kel ? kel[e] : 1.0); }
+};
+// matrix entries through the per-entry slots (slot[(27 i + j) * ne + pos] for
+// the element at colour-sorted position pos), load entries into rhs
+struct Q2SlotSink {
+    const int32_t* slot;
+    const int32_t* dofs;
+    int64_t ne, pos, e;
+    double* val;
+    double* rhs;
+    __device__ void ke(int i, int j, double v) {
+        if (val) val[slot[(int64_t)(27 * i + j) * ne + pos]] += v;
+    }
+    __device__ void fe(int a, double v) { rhs[dofs[e * 27 + a]] += v; }
+};
+
+// one colour's elements (positions pos0 .. pos0+cnt-1 of the colour order),
+// one element per wavefront
+__global__ __launch_bounds__(64 * kQ2Waves) void k_elem_hex_q2(int64_t ne, const int32_t* __restrict__ order,
+                                                               const int32_t* __restrict__ dofs,
+                                                               const double* __restrict__ coords,
+                                                               const double* __restrict__ kel,
+                                                               const double* __restrict__ kq, double f_const,
+                                                               const int32_t* __restrict__ slot, double* val,
+                                                               double* rhs, int64_t pos0, int64_t cnt) {
+    const int w = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * kQ2Waves + w;
+    const bool valid = t < cnt;   // every wave reaches the barriers of the element routine
+    const int64_t pos = pos0 + (valid ? t : 0);
+    const int64_t e = order[pos];
+    __shared__ double Xs_all[kQ2Waves][24];
+    __shared__ double Jis_all[kQ2Waves][27][9];
+    __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    Q2ElemSrc src{coords, dofs, kel, kq, e};
+    Q2SlotSink sink{slot, dofs, ne, pos, e, val, rhs};
+    q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
+                    rhs != nullptr);
+}
+
 }  // namespace
 
 int q2_hex_elements(int64_t n, const double* X, const double* k, double f_const, double* Ke, double* fe,
@@ -343,6 +394,15 @@ int q2_lattice_assemble(int n, const double* theta, double f_const, const int32_
     const int64_t N = (int64_t)W * W * W;
     hipLaunchKernelGGL(k_q2_lattice_dirichlet, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 65536)), dim3(256),
                        0, s, W, N, rowptr, col, val, rhs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int elem_assemble_q2_colour(int64_t ne, const int32_t* order, const int32_t* dofs, const double* coords,
+                            const double* kel, const double* kq, double f_const, const int32_t* slot, double* val,
+                            double* rhs, int64_t pos0, int64_t cnt, hipStream_t s) {
+    if (cnt <= 0) return 0;
+    hipLaunchKernelGGL(k_elem_hex_q2, dim3((unsigned)((cnt + kQ2Waves - 1) / kQ2Waves)), dim3(64 * kQ2Waves), 0, s,
+                       ne, order, dofs, coords, kel, kq, f_const, slot, val, rhs, pos0, cnt);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

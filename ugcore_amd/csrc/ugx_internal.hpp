@@ -292,10 +292,41 @@ struct ElemLaunch {
 // elem_type 0 P1 triangle FV1, 1 P1 tetrahedron FV1, 2 Q1 hexahedron FE
 int elem_assemble_colour(int elem_type, const ElemLaunch& L, int64_t pos0, int64_t cnt, hipStream_t s);
 int zero_listed(int64_t n, const int32_t* idx, double* x, hipStream_t s);
+// Q2 hexahedra (27 DoFs in the local shape order of LagrangeLSFS<Hexahedron,
+// 2>): one colour, one element per wavefront (q2_kernels.hip); the
+// coefficient per element (kel), per element and Gauss point (kq) or 1;
+// val (through the slots) and rhs may be null
+int elem_assemble_q2_colour(int64_t ne, const int32_t* order, const int32_t* dofs, const double* coords,
+                            const double* kel, const double* kq, double f_const, const int32_t* slot, double* val,
+                            double* rhs, int64_t pos0, int64_t cnt, hipStream_t s);
 // element -> DoF table and k = exp(theta) per element of a lattice domain's
 // top level (kind 0 hexahedra, 1 triangles, 2 tetrahedra)
 int lattice_connectivity(int kind, int n, const int32_t* dof_of_lat, const uint16_t* cell_tets, const double* theta,
                          int32_t* dofs, double* kel, hipStream_t s);
+
+// ---- Q2 on the hexahedral lattice hierarchy (q2_hierarchy.hip) ----
+// numbering of level l from the P1 numbering of level l+1 (N1 vertices,
+// n_inner1 Inner ones, lattice width W1 = 2 n_l + 1): node_of_dof and
+// dof_of_node over the level-(l+1) lattice, Dirichlet flag per DoF; key,
+// key2, node: N1 entries of scratch, temp: q2_numbering_temp_bytes
+int q2_numbering_temp_bytes(int64_t N1, size_t* bytes);
+int q2_numbering(int64_t N1, int64_t n_inner1, int W1, const int32_t* lat_of_dof1, uint64_t* key, uint64_t* key2,
+                 int32_t* node, int32_t* node_of_dof, void* temp, size_t temp_bytes, int32_t* dof_of_node,
+                 uint8_t* dir, hipStream_t s);
+// [n^3][27] DoFs of the level's hexahedra (lattice order) in the Q2 shape order
+int q2_connectivity(int n, const int32_t* dof_of_node, int32_t* conn, hipStream_t s);
+int q2_coords(int64_t N, int n, const int32_t* node_of_dof, double* X, hipStream_t s);
+// exp(theta) of the fine element (nfine per axis) at every Gauss point
+int q2_coef_points(int n, int nfine, const double* theta, double* kq, hipStream_t s);
+// prolongation from level l-1 to level l (nf elements per axis): row counts,
+// then columns (ascending) and values incl. the Dirichlet rows
+int q2_prolong_count(int64_t Nf, int nf, const int32_t* node_of_dof_f, int32_t* cnt, hipStream_t s);
+int q2_prolong_fill(int64_t Nf, int nf, const int32_t* node_of_dof_f, const uint8_t* dir_f,
+                    const int32_t* dof_of_node_c, const int32_t* rowptr, int32_t* col, double* val, hipStream_t s);
+// the Dirichlet rows of R = P^T (last: Nc entries, all -1 on entry)
+int q2_restriction_adjust(int64_t Nf, int nf, int64_t Nc, const int32_t* node_of_dof_f, const uint8_t* dir_f,
+                          const int32_t* dof_of_node_c, int32_t* last, const int32_t* rowptr, const int32_t* col,
+                          double* val, hipStream_t s);
 
 // ---- lattice operators written row by row (lattice_csr.hip) ----
 // Q1 unit-cube tables, 2x2x2 Gauss points (host): G_q[a][b] =
