@@ -154,3 +154,44 @@ def test_generic_q2_element_loop_equals_the_lattice_assembly():
     ref = S @ x - b.numpy()
     ref[bnd] = 0.0
     assert np.abs(d.numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_pybind_q2_multigrid_solve_equals_the_ctypes_path():
+    say("case q2 ug4py")
+    u = U()
+    from ugcore_amd import ug4py as P
+    refs = 2
+    theta = np.random.default_rng(9).normal(size=8 ** refs) * 0.3
+    P.init()
+    q = P.Q2Discretization(refs)
+    q.set_theta(theta)
+    q.set_source(1.0)
+    A, Pm, Rm = [], [None], [None]
+    b = None
+    for lev in range(refs + 1):
+        a, bv = P.SparseMatrix(), P.Vector(q.num_dofs(lev))
+        q.assemble_linear(lev, a, bv)
+        A.append(a)
+        b = bv
+        if lev:
+            p, r = P.SparseMatrix(), P.SparseMatrix()
+            q.prolongation(lev, p)
+            q.restriction(lev, p, r)
+            Pm.append(p)
+            Rm.append(r)
+    G = P.MatrixMultiGrid(A, Pm, Rm, base_level=0)
+    S = P.CG(G, 100, 1e-10, 1e-50)
+    S.init(A[refs])
+    x = P.Vector(q.num_dofs(refs))
+    assert S.apply(x, b)
+    # the ctypes path
+    q2 = u.Q2Discretization(refs)
+    q2.set_theta(theta)
+    q2.set_source(1.0)
+    A2, P2, R2, b2 = q2.hierarchy()
+    S2 = u.CG(u.MatrixMultiGrid(A2, P2, R2), max_steps=100, reduction=1e-10)
+    S2.init(A2[refs])
+    x2 = u.Vector(q2.num_dofs(refs))
+    assert S2.apply(x2, b2)
+    assert S.steps() == S2.steps()
+    assert np.array_equal(x.numpy(), x2.numpy())

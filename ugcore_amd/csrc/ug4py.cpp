@@ -180,6 +180,59 @@ struct GeometricMultiGrid {
     void apply(Vector& c, Vector& d) { check(ugx_gmg_apply(h, c.h, d.h), "GeometricMultiGrid.apply"); }
 };
 
+// Q2 hierarchy of the unit cube (ugx_q2disc_*)
+struct Q2Discretization {
+    void* h;
+    int refs;
+    explicit Q2Discretization(int num_refs)
+        : h(check_ptr(ugx_q2disc_create(num_refs), "Q2Discretization")), refs(num_refs) {}
+    ~Q2Discretization() { ugx_q2disc_destroy(h); }
+    Q2Discretization(const Q2Discretization&) = delete;
+    Q2Discretization& operator=(const Q2Discretization&) = delete;
+    int num_levels() const { return ugx_q2disc_num_levels(h); }
+    int64_t num_dofs(int level) const { return ugx_q2disc_num_dofs(h, level); }
+    void set_theta(DArray theta) {
+        if (theta.size() != (int64_t(1) << (3 * refs))) throw py::value_error("one theta per top-level element");
+        check(ugx_q2disc_set_theta(h, theta.data()), "set_theta");
+    }
+    void set_source(double f) { check(ugx_q2disc_set_source(h, f), "set_source"); }
+    void assemble_linear(int level, SparseMatrix& A, Vector& b) {
+        check(ugx_q2disc_assemble_linear(h, level, A.h, b.h), "assemble_linear");
+    }
+    void prolongation(int level, SparseMatrix& P) { check(ugx_q2disc_prolongation(h, level, P.h), "prolongation"); }
+    void restriction(int level, SparseMatrix& P, SparseMatrix& R) {
+        check(ugx_q2disc_restriction(h, level, P.h, R.h), "restriction");
+    }
+};
+
+// AssembledMultiGridCycle over level matrices (ugx_mgmg_*); keeps the
+// matrices' Python objects alive
+struct MatrixMultiGrid {
+    void* h;
+    py::list keep;
+    MatrixMultiGrid(py::list A, py::list P, py::list R, int base_level, int nu1, int nu2, int cycle, double omega) {
+        const size_t n = A.size();
+        if (P.size() != n || R.size() != n) throw py::value_error("one A, P and R entry per level");
+        std::vector<void*> a(n), p(n), r(n);
+        auto handle = [](py::handle o) -> void* { return o.is_none() ? nullptr : o.cast<SparseMatrix&>().h; };
+        for (size_t l = 0; l < n; ++l) {
+            a[l] = handle(A[l]);
+            p[l] = handle(P[l]);
+            r[l] = handle(R[l]);
+        }
+        h = check_ptr(ugx_mgmg_create((int)n, a.data(), p.data(), r.data(), base_level, nu1, nu2, cycle, omega),
+                      "MatrixMultiGrid");
+        keep.append(A);
+        keep.append(P);
+        keep.append(R);
+    }
+    ~MatrixMultiGrid() { ugx_mgmg_destroy(h); }
+    MatrixMultiGrid(const MatrixMultiGrid&) = delete;
+    MatrixMultiGrid& operator=(const MatrixMultiGrid&) = delete;
+    void init(SparseMatrix& A) { check(ugx_mgmg_init(h, A.h), "MatrixMultiGrid.init"); }
+    void apply(Vector& c, Vector& d) { check(ugx_mgmg_apply(h, c.h, d.h), "MatrixMultiGrid.apply"); }
+};
+
 struct Jacobi {
     void* h;
     explicit Jacobi(double omega) : h(check_ptr(ugx_jacobi_create(omega), "Jacobi")) {}
@@ -198,8 +251,10 @@ struct Solver {
                 check(ugx_solver_set_preconditioner(h, 1, pre.cast<GeometricMultiGrid&>().h), "set_preconditioner");
             else if (py::isinstance<Jacobi>(pre))
                 check(ugx_solver_set_preconditioner(h, 2, pre.cast<Jacobi&>().h), "set_preconditioner");
+            else if (py::isinstance<MatrixMultiGrid>(pre))
+                check(ugx_solver_set_preconditioner(h, 3, pre.cast<MatrixMultiGrid&>().h), "set_preconditioner");
             else
-                throw py::type_error("preconditioner: GeometricMultiGrid, Jacobi or None");
+                throw py::type_error("preconditioner: GeometricMultiGrid, MatrixMultiGrid, Jacobi or None");
         }
         check(ugx_solver_set_conv_check(h, max_steps, min_defect, reduction, 0), "set_conv_check");
     }
@@ -228,8 +283,8 @@ struct ElementDiscretization {
     void* h;
     int64_t ne, nd;
     ElementDiscretization(int type, IArray dofs, DArray coords) {
-        const int nv = type == 0 ? 3 : (type == 1 ? 4 : 8), dim = type == 0 ? 2 : 3;
-        if (type < 0 || type > 2) throw py::value_error("element type 0, 1 or 2");
+        const int nv = type == 0 ? 3 : (type == 1 ? 4 : (type == 2 ? 8 : 27)), dim = type == 0 ? 2 : 3;
+        if (type < 0 || type > 3) throw py::value_error("element type 0, 1, 2 or 3");
         if (dofs.ndim() != 2 || dofs.shape(1) != nv || coords.ndim() != 2 || coords.shape(1) != dim)
             throw py::value_error("connectivity shapes do not match the element type");
         ne = dofs.shape(0);
@@ -242,6 +297,10 @@ struct ElementDiscretization {
     void set_diffusion(DArray k) {
         if (k.size() != ne) throw py::value_error("one diffusion value per element");
         check(ugx_elemdisc_set_diffusion(h, k.data()), "set_diffusion");
+    }
+    void set_diffusion_at_points(DArray kq) {
+        if (kq.size() != 27 * ne) throw py::value_error("27 values per element");
+        check(ugx_elemdisc_set_diffusion_at_points(h, kq.data()), "set_diffusion_at_points");
     }
     void set_source(double f) { check(ugx_elemdisc_set_source_const(h, f), "set_source"); }
     void set_dirichlet(IArray d) { check(ugx_elemdisc_set_dirichlet(h, d.data(), d.size()), "set_dirichlet"); }
@@ -309,6 +368,21 @@ PYBIND11_MODULE(ug4py, m) {
         .def("init", &GeometricMultiGrid::init)
         .def("apply", &GeometricMultiGrid::apply);
     py::class_<Jacobi>(m, "Jacobi").def(py::init<double>(), py::arg("omega") = 0.66);
+    py::class_<Q2Discretization>(m, "Q2Discretization")
+        .def(py::init<int>(), py::arg("num_refs"))
+        .def("num_levels", &Q2Discretization::num_levels)
+        .def("num_dofs", &Q2Discretization::num_dofs)
+        .def("set_theta", &Q2Discretization::set_theta)
+        .def("set_source", &Q2Discretization::set_source)
+        .def("assemble_linear", &Q2Discretization::assemble_linear)
+        .def("prolongation", &Q2Discretization::prolongation)
+        .def("restriction", &Q2Discretization::restriction);
+    py::class_<MatrixMultiGrid>(m, "MatrixMultiGrid")
+        .def(py::init<py::list, py::list, py::list, int, int, int, int, double>(), py::arg("A"), py::arg("P"),
+             py::arg("R"), py::arg("base_level") = 0, py::arg("nu1") = 2, py::arg("nu2") = 2, py::arg("cycle") = 1,
+             py::arg("omega") = 0.66)
+        .def("init", &MatrixMultiGrid::init)
+        .def("apply", &MatrixMultiGrid::apply);
     py::class_<Solver>(m, "Solver")
         .def("init", &Solver::init, py::keep_alive<1, 2>())
         .def("apply", &Solver::apply)
@@ -322,6 +396,7 @@ PYBIND11_MODULE(ug4py, m) {
     py::class_<ElementDiscretization>(m, "ElementDiscretization")
         .def(py::init<int, IArray, DArray>(), py::arg("elem_type"), py::arg("elem_dofs"), py::arg("dof_coords"))
         .def("set_diffusion", &ElementDiscretization::set_diffusion)
+        .def("set_diffusion_at_points", &ElementDiscretization::set_diffusion_at_points)
         .def("set_source", &ElementDiscretization::set_source)
         .def("set_dirichlet", &ElementDiscretization::set_dirichlet)
         .def("assemble_linear", &ElementDiscretization::assemble_linear)
