@@ -417,6 +417,14 @@ int ugx_gmg_set_rap(void* g, int on);
  * path replays captured HIP graphs, 0 = launches its kernels one by one (same
  * kernels and results). No counterpart in the reference (launch mechanics). */
 int ugx_gmg_set_graphs(void* g, int on);
+/* before ugx_gmg_init: 1 (default) = on a matrix-free top level, the CG
+ * fast path's cycle runs its last full post-smoothing step together with the
+ * correction update that ends it (mg_solver_impl.hpp:1919-1943, the last
+ * apply_sub and the final c += B d), the conversion of the correction and the
+ * (c, d) product as one pass; 0 = separate passes. The correction is the
+ * same; (c, d) is summed in another order. The switch has no counterpart in
+ * the reference (kernel fusion). */
+int ugx_gmg_set_fused_final_step(void* g, int on);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

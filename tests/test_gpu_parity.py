@@ -316,6 +316,37 @@ def test_jacobi_preconditioned_cg():
     assert np.linalg.norm(x.numpy() - xs) <= 1e-8 * np.linalg.norm(xs)
 
 
+@pytest.mark.parametrize("refs", [4, 7])
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_fused_final_step_equals_separate_passes(precision, refs):
+    """the lattice CG's cycle with its last post-smoothing step fused with the
+    final correction update, conversion and (z, r) against the separate
+    passes, for V-, W- and F-cycles: the same iteration (the product is only
+    summed in another order), the same solution to rounding"""
+    say(f"case fused final step precision={precision} refs={refs}")
+    u = U()
+    from ugcore_amd.synthetic import make_theta
+    n = 2 ** refs
+    theta = make_theta(n, 5, 3)
+    d, A, b = gpu_problem(0, refs, theta, None)
+    for cycle in ("V", "W", "F"):
+        out = {}
+        for fused in (True, False):
+            g = u.GeometricMultiGrid(d, base_level=1, cycle=cycle, precision=precision, fused_final_step=fused)
+            S = u.CG(g, max_steps=100, reduction=1e-10)
+            S.init(A)
+            x = u.Vector(d.num_dofs())
+            assert S.apply(x, b)
+            assert S.used_lattice_path()
+            out[fused] = (S.steps(), x.numpy(), np.asarray(S.defects()))
+        assert out[True][0] == out[False][0], cycle
+        assert np.linalg.norm(out[True][1] - out[False][1]) <= 1e-12 * np.linalg.norm(out[False][1]), cycle
+        # defect norms: the same to rounding, measured against the first one
+        # (late norms are small differences of large sums)
+        d1, d0 = out[True][2], out[False][2]
+        assert np.max(np.abs(d1 - d0)) <= 1e-12 * d0[0], cycle
+
+
 def test_full_size_128_properties():
     """128^3 Q1 (2,146,689 DoFs): pattern size, Dirichlet identity rows,
     symmetric interior couplings, and a solve whose recomputed residual meets

@@ -824,6 +824,12 @@ int ugx_gmg_set_graphs(void* g, int on) {
         return 0;
     });
 }
+int ugx_gmg_set_fused_final_step(void* g, int on) {
+    return guard([&] {
+        ((GmgH*)g)->g->set_fused_final_step(on != 0);
+        return 0;
+    });
+}
 int ugx_gmg_set_rap(void* g, int on) {
     return guard([&] {
         ((GmgH*)g)->g->set_rap(on != 0);

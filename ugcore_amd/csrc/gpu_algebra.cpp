@@ -1789,6 +1789,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     bool sym = true;             // level products through the symmetric storage
     bool rap = false;            // Galerkin coarse operators R A P below the top level
     bool graphs = true;          // replay the lattice cycle from captured HIP graphs
+    bool fuse_final = true;      // matrix-free top level: last smoothing step fused with the final update
     struct CycleGraph {
         const void* key[5];
         hipGraphExec_t exec;
@@ -1818,11 +1819,11 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     }
 
     GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free, bool galerkin,
-            bool use_graphs)
+            bool use_graphs, bool fused_final)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
           sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2),
           mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()), rap(galerkin),
-          graphs(use_graphs) {
+          graphs(use_graphs), fuse_final(fused_final) {
         if (rap && d->kind() == 2)
             throw UGError("GMG: Galerkin coarse operators need the P1 lattice transfer (hexahedral or triangular "
                           "hierarchy)");
@@ -1840,6 +1841,15 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         if (base < 0 || base > top) throw UGError("GMG: base level out of range");
         L.clear();
         L.resize(top + 1);
+        if (mf) {
+            // the matrix-free operator (and the fused prolongation) take every
+            // lattice boundary point as an identity (Dirichlet) row: the
+            // Dirichlet DoFs, all on the boundary, must number the boundary
+            // points
+            const LevelGrid& Gt = disc->level(top);
+            const int64_t Wi = Gt.W - 2;
+            if (Gt.N - Gt.n_inner != Gt.N - Wi * Wi * Wi) mf = false;
+        }
         if (mf) {
             // top level: every quadrature point of element e sees k_e, so the
             // element matrix is h k_e times the unit-cube Q1 stiffness
@@ -1980,6 +1990,13 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     // below it; the lattice entry fuses that update with its conversion
     // (defer_top_final).
     bool defer_top_final = false;
+    // the lattice entry's outputs while its cycle runs (z, r, the partials'
+    // scratch and (z, r)), and whether the fused last step has written them
+    double* last_z = nullptr;
+    const double* last_r = nullptr;
+    double* last_partial = nullptr;
+    double* last_out = nullptr;
+    bool last_done = false;
     // the lattice entry's FP64 defect when the first top-level pre-smoothing
     // step reads it directly (q1mf_jacobi_first) instead of a converted copy,
     // or the caller's FP32 copy of it (q1mf_jacobi_first_t, FP32 levels)
@@ -1987,6 +2004,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     const float* first_defect_f32 = nullptr;
     void clear_cycle_state() {
         defer_top_final = false;
+        last_done = false;
         first_defect = nullptr;
         first_defect_f32 = nullptr;
         for (auto& l : L) l.acc_deferred = false;
@@ -2050,8 +2068,24 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         cl.acc_deferred = false;
         if (nu2 == 0) return;
         UGX_LAUNCH(op_apply_sub(f, f.st, f.sd, f.sd));
+        // matrix-free top level of the lattice entry: the last full smoothing
+        // step, the deferred correction update, its conversion and (z, r) in
+        // one pass (run_lattice then has nothing left to do)
+        const bool fuse_last = fuse_final && defer_top_final && lev == top && sym && mf && nu2 >= 2 &&
+                               q1mf_last_fits(f.W, sizeof(T));
         for (int nu = 0; nu < nu2; ++nu) {
-            if (nu < nu2 - 1) {
+            if (fuse_last && nu == nu2 - 2) {
+                if (last_out) {
+                    int np = AssembledMultiGridCycle::max_partials();
+                    UGX_LAUNCH(q1mf_jacobi_last<T>(f.W, cell_c.get(), f.dinv, f.sd, f.sc, last_z, last_r, c.stream,
+                                                   last_partial, &np));
+                    UGX_LAUNCH(sum_partials(np, last_partial, last_out, c.stream));
+                } else {
+                    UGX_LAUNCH(q1mf_jacobi_last<T>(f.W, cell_c.get(), f.dinv, f.sd, f.sc, last_z, last_r, c.stream,
+                                                   nullptr, nullptr));
+                }
+                last_done = true;
+            } else if (nu < nu2 - 1) {
                 UGX_LAUNCH(op_jacobi_step(f));
                 std::swap(f.sd, f.sd2);
             } else if (lev < top && !tets && last) {
@@ -2131,6 +2165,11 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         // with post-smoothing below a coarser level the cycle ends with the
         // top level's correction update, fused here with the conversion
         defer_top_final = nu2 > 0 && top > base;
+        last_z = c_lat;
+        last_r = d_lat;
+        last_partial = partial;
+        last_out = cd_out;
+        last_done = false;
         try {
             lmgc(top, cycle, true);
         } catch (...) {
@@ -2139,7 +2178,8 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         }
         if (defer_top_final) {
             defer_top_final = false;
-            UGX_LAUNCH(jacobi_convert_dot<T>(t.N, t.dinv, t.sd, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
+            if (!last_done)
+                UGX_LAUNCH(jacobi_convert_dot<T>(t.N, t.dinv, t.sd, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
         } else if (cd_out) {
             UGX_LAUNCH(convert_dot<T>(t.N, t.sc, c_lat, d_lat, partial, cd_out, c.stream));
         } else {
@@ -2309,10 +2349,10 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     impl_ = nullptr;
     if (prec_ == Precision::F64)
         impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                    rap_, graphs_);
+                                    rap_, graphs_, fused_final_);
     else
         impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                   rap_, graphs_);
+                                   rap_, graphs_, fused_final_);
     impl_->init();
     return true;
 }

@@ -1155,6 +1155,13 @@ __host__ __device__ constexpr int mf_slots(size_t elem) { return elem == 4 ? UGX
 #endif
 constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= kMfRows * 256)
 
+// the coarse correction the prolongation reads: sc, or (with dinv) the value
+// the coarse level's deferred last post-smoothing update sc + dinv .* sd would
+// store, with the arithmetic of k_jacobi_accumulate
+template <typename TV>
+__device__ __forceinline__ TV coarse_fold(TV sc, TV dinv, TV sd) {
+    return (TV)((double)sc + (double)dinv * (double)sd);
+}
 // MODE 0: fused Jacobi step (staged st = dinv.*sd; sd_out = sd - A st; sc += st)
 // MODE 1: d_out = d_in - A x;  MODE 2: y = alpha v + beta A x
 // MODE 3: the first Jacobi step of a cycle from the FP64 defect d64 and a zero
@@ -1167,19 +1174,24 @@ constexpr int kMfRows = UGX_MF_ROWS;   // rows per thread and plane (R + 63 <= k
 //         p_new = fma(beta, p_old, z) (a_in = p_old, b_in = z, beta = *d64,
 //         the arithmetic of k_cg_direction), out1 = p_new for the block's rows,
 //         y = A p_new and the (y, p_new) partials as MODE 2 with alpha 0
+// MODE 7: the last post-smoothing step of a cycle fused with the correction
+//         update that ends it, its conversion and the (z, r) partials: MODE 0
+//         (a_in = dinv, b_in = sd_in, c_in = sc) without storing sd_out or
+//         sc, then z = (double)(sc_new + dinv .* sd_out) with the arithmetic
+//         of k_jacobi_convert_dot (z64 = z, d64 = r)
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? UGX_MF_WAVES32 : 1)))
 void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                                                 const T* __restrict__ b_in, const T* c_in, T* out0, T* out1,
                                                 double alpha, double beta, double* dot_partial,
-                                                const double* __restrict__ d64) {
+                                                const double* __restrict__ d64, double* __restrict__ z64) {
     // c_in and out0 may alias (d_out = d_in); every row is read and then
     // written by the same thread. MODE 2 with dot_partial: per-block partial
     // sums of y_i * x_i (fixed order) in dot_partial[blockIdx.x]
     double dot_acc = 0.0;
     __shared__ double dot_sh[4];
     auto finish_dot = [&]() {
-        if ((MODE != 2 && MODE != 5) || !dot_partial) return;
+        if ((MODE != 2 && MODE != 5 && MODE != 7) || !dot_partial) return;
         double v = dot_acc;
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if ((threadIdx.x & 63) == 0) dot_sh[threadIdx.x >> 6] = v;
@@ -1216,7 +1228,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
     };
     const T bcg = MODE == 5 ? (T)*d64 : (T)0;   // the CG's beta (MODE 5)
     auto staged = [&](int64_t j) -> T {
-        return (MODE == 0 || MODE == 4) ? a_in[j] * b_in[j]
+        return (MODE == 0 || MODE == 4 || MODE == 7) ? a_in[j] * b_in[j]
                : MODE == 3             ? a_in[j] * (T)d64[j]
                : MODE == 5             ? (T)fma((double)bcg, (double)a_in[j], (double)b_in[j])
                                        : a_in[j];
@@ -1256,11 +1268,18 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             const int t = tid + u * step;
             const bool ok = more && t < P;
             pa[u] = ok ? a_in[f2 + t] : (T)0;
-            if (MODE == 0 || MODE == 4 || MODE == 5) pb[u] = ok ? b_in[f2 + t] : (T)0;
+            if (MODE == 0 || MODE == 4 || MODE == 5 || MODE == 7) pb[u] = ok ? b_in[f2 + t] : (T)0;
             if (MODE == 3) pb[u] = ok ? (T)d64[f2 + t] : (T)0;
             cv[u] = (more && coff[u] >= 0 && k + 1 < n) ? cell[(int64_t)(k + 1) * n * n + coff[u]] : (T)0;
         }
         T ra[kMfRows], rb[kMfRows];
+        // MODE 7: dinv and r of the rows; FP64 reads them with the other row
+        // inputs, FP32 at the output (its registers are kept for the plane
+        // loop: 7 resident waves instead of 5, measured 148 against 179 us
+        // per step at 257^3; FP64 the other way round, 222 against 253 us)
+        constexpr bool EAGER7 = MODE == 7 && sizeof(T) == 8;
+        T rd[EAGER7 ? kMfRows : 1];
+        double rr[EAGER7 ? kMfRows : 1];
 #pragma unroll
         for (int u = 0; u < kMfRows; ++u) {
             const int r = tid + u * step - a0;
@@ -1272,6 +1291,13 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                 ra[u] = ok ? (T)d64[base + r] : (T)0;   // sd_in
             } else if (MODE == 4) {
                 ra[u] = ok ? b_in[base + r] : (T)0;     // sd_in
+            } else if (MODE == 7) {
+                ra[u] = ok ? b_in[base + r] : (T)0;   // sd_in
+                rb[u] = ok ? c_in[base + r] : (T)0;   // sc
+                if constexpr (EAGER7) {
+                    rd[u] = ok ? a_in[base + r] : (T)0;                     // dinv
+                    rr[u] = (ok && dot_partial) ? d64[base + r] : 0.0;      // r
+                }
             } else if (MODE == 1) {
                 ra[u] = ok ? c_in[base + r] : (T)0;   // d_in
             } else {
@@ -1350,6 +1376,12 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
                 out1[i] = xi;
             } else if (MODE == 1) {
                 out0[i] = ra[u] - sv;
+            } else if (MODE == 7) {
+                const T sd_out = ra[u] - sv, sc_new = rb[u] + xi;
+                const T dv = EAGER7 ? rd[EAGER7 ? u : 0] : a_in[i];
+                const double zi = (double)(T)((double)sc_new + (double)dv * (double)sd_out);
+                z64[i] = zi;
+                if (dot_partial) dot_acc += zi * (EAGER7 ? rr[EAGER7 ? u : 0] : d64[i]);
             } else {
                 const T yv = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
                 out0[i] = yv;
@@ -1368,7 +1400,7 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             for (int u = 0; u < kMfPrefetch; ++u) {
                 const int t = tid + u * step;
                 if (t < P)
-                    dst[t] = (MODE == 0 || MODE == 3 || MODE == 4) ? pa[u] * pb[u]
+                    dst[t] = (MODE == 0 || MODE == 3 || MODE == 4 || MODE == 7) ? pa[u] * pb[u]
                              : MODE == 5 ? (T)fma((double)bcg, (double)pa[u], (double)pb[u])
                                          : pa[u];
                 if (t < CL) cdst[t] = cv[u];
@@ -1495,7 +1527,7 @@ struct CoarseCorr {
     const TV* sd;
     __device__ __forceinline__ TV operator()(int64_t j) const {
         if (!dinv) return sc[j];
-        return (TV)((double)sc[j] + (double)dinv[j] * (double)sd[j]);
+        return coarse_fold<TV>(sc[j], dinv[j], sd[j]);
     }
 };
 
@@ -1524,7 +1556,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
             for (int dz = 0; dz <= oz; ++dz)
                 for (int dy = 0; dy <= oy; ++dy)
                     for (int dx = 0; dx <= ox; ++dx)
-                        v += w * (double)scc(((int64_t)(z0 + dz) * Wc + (y0 + dy)) * Wc + (x0 + dx));
+                        v = fma(w, (double)scc(((int64_t)(z0 + dz) * Wc + (y0 + dy)) * Wc + (x0 + dx)), v);
         }
     } else {
         // triangle mesh: odd coordinates lie on axis edges or on the (1,1) diagonal
@@ -1539,8 +1571,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(int dim, int Wc, int Wf, in
             v = 0.5 * (double)scc((int64_t)y0 * Wc + x0) + 0.5 * (double)scc((int64_t)(y0 + 1) * Wc + (x0 + 1));
         }
     }
-    st[i] = (TV)v;
-    sc[i] = (TV)((double)sc[i] + v);
+    // c += t in the vector precision (mg_solver_impl.hpp:1905)
+    const TV t = (TV)v;
+    st[i] = t;
+    sc[i] = sc[i] + t;
 }
 
 // restriction: sdc_c = sum_f R(c,f) sd_f with R = P^T (Dirichlet coarse rows:
@@ -1579,10 +1613,11 @@ __device__ __forceinline__ void prolong3_point(int x, int y, int z, int Wc, int 
                     const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
                     for (int dz = 0; dz <= oz; ++dz)
                         for (int dy = 0; dy <= oy; ++dy)
-                            for (int dx = 0; dx <= ox; ++dx) val += w * v[dz][dy][dx];
+                            for (int dx = 0; dx <= ox; ++dx) val = fma(w, v[dz][dy][dx], val);
                 }
-                st[f] = (TV)val;
-                sc[f] = (TV)((double)sc[f] + val);
+                const TV t = (TV)val;
+                st[f] = t;
+                sc[f] = sc[f] + t;
             }
 }
 template <typename TV>
@@ -1803,11 +1838,12 @@ __global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nban
             const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
             for (int dz = 0; dz <= oz; ++dz)
                 for (int dy = 0; dy <= oy; ++dy)
-                    for (int dx = 0; dx <= ox; ++dx) val += w * (double)lds[(dz * NL + ly + dy) * LW + x + dx];
+                    for (int dx = 0; dx <= ox; ++dx) val = fma(w, (double)lds[(dz * NL + ly + dy) * LW + x + dx], val);
         }
         const int64_t f = ((int64_t)fz * Wf + fy) * Wf + fx;
-        st[f] = (TV)val;
-        sc[f] = (TV)((double)sc[f] + val);
+        const TV tv = (TV)val;
+        st[f] = tv;
+        sc[f] = sc[f] + tv;
     }
 }
 
@@ -2528,7 +2564,7 @@ int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 0>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 0>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
-                           sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
+                           sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr, (double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -2548,7 +2584,7 @@ int q1mf_jacobi_first(int W, const T* cell, const T* dinv, const double* d64, T*
     const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 3>), zm, zlds);
     if (!zgrid) return -1;
     hipLaunchKernelGGL((k_q1mf_z<T, 3>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, (const T*)nullptr,
-                       (const T*)nullptr, sd_out, sc, 0.0, 0.0, (double*)nullptr, d64);
+                       (const T*)nullptr, sd_out, sc, 0.0, 0.0, (double*)nullptr, d64, (double*)nullptr);
     UGX_CHECK_LAUNCH();
 }
 bool q1mf_first_fits(int W, size_t elem) {
@@ -2565,7 +2601,7 @@ int q1mf_jacobi_first_t(int W, const T* cell, const T* dinv, const T* sd_in, T* 
     const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 4>), zm, zlds);
     if (!zgrid) return -1;
     hipLaunchKernelGGL((k_q1mf_z<T, 4>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, (const T*)nullptr,
-                       sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
+                       sd_out, sc, 0.0, 0.0, (double*)nullptr, (const double*)nullptr, (double*)nullptr);
     UGX_CHECK_LAUNCH();
 }
 template int q1mf_jacobi_first_t<float>(int, const float*, const float*, const float*, float*, float*, hipStream_t);
@@ -2579,7 +2615,7 @@ int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hi
     size_t zlds;
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 1>), zm, zlds)) {
         hipLaunchKernelGGL((k_q1mf_z<T, 1>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, d_in,
-                           d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr, (const double*)nullptr);
+                           d_out, (T*)nullptr, 0.0, 0.0, (double*)nullptr, (const double*)nullptr, (double*)nullptr);
         UGX_CHECK_LAUNCH();
     }
     MfMap m;
@@ -2592,6 +2628,37 @@ int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hi
 template int q1mf_apply_sub<float>(int, const float*, const float*, const float*, float*, hipStream_t);
 template int q1mf_apply_sub<double>(int, const double*, const double*, const double*, double*, hipStream_t);
 
+// the last post-smoothing step of a cycle fused with the correction update
+// that ends it: z = (double)(sc_new + dinv .* sd_out) for sd_out = sd_in - A st,
+// sc_new = sc + st, st = dinv .* sd_in (the arithmetic of q1mf_jacobi_step
+// followed by jacobi_convert_dot; sd_out and sc_new are not stored). With
+// dot_partial, the per-block partials of (z, r) (n_partial in: capacity, out:
+// number written). -1 when the lattice does not fit the z-marching plan
+template <typename T>
+int q1mf_jacobi_last(int W, const T* cell, const T* dinv, const T* sd_in, const T* sc, double* z, const double* r,
+                     hipStream_t s, double* dot_partial, int* n_partial) {
+    MfZMap zm;
+    size_t zlds;
+    const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 7>), zm, zlds);
+    if (!zgrid) return -1;
+    if (dot_partial && (!n_partial || zgrid > *n_partial)) return -1;
+    hipLaunchKernelGGL((k_q1mf_z<T, 7>), dim3(zgrid), dim3(256), zlds, s, zm, cell, dinv, sd_in, sc, (T*)nullptr,
+                       (T*)nullptr, 0.0, 0.0, dot_partial, r, z);
+    if (n_partial) *n_partial = (int)zgrid;
+    UGX_CHECK_LAUNCH();
+}
+bool q1mf_last_fits(int W, size_t elem) {
+    MfZMap zm;
+    size_t lds;
+    const void* k = elem == 4 ? reinterpret_cast<const void*>(&k_q1mf_z<float, 7>)
+                              : reinterpret_cast<const void*>(&k_q1mf_z<double, 7>);
+    return make_mf_zmap(W, elem, k, zm, lds) != 0;
+}
+template int q1mf_jacobi_last<float>(int, const float*, const float*, const float*, const float*, double*,
+                                     const double*, hipStream_t, double*, int*);
+template int q1mf_jacobi_last<double>(int, const double*, const double*, const double*, const double*, double*,
+                                      const double*, hipStream_t, double*, int*);
+
 template <typename T>
 int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const T* x, T* y, hipStream_t s,
               double* dot_partial, int* n_partial) {
@@ -2600,7 +2667,7 @@ int q1mf_axpy(int W, const T* cell, double alpha, const T* v, double beta, const
     if (const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 2>), zm, zlds)) {
         if (dot_partial && n_partial && zgrid > *n_partial) return -1;
         hipLaunchKernelGGL((k_q1mf_z<T, 2>), dim3(zgrid), dim3(256), zlds, s, zm, cell, x, (const T*)nullptr, v, y,
-                           (T*)nullptr, alpha, beta, dot_partial, (const double*)nullptr);
+                           (T*)nullptr, alpha, beta, dot_partial, (const double*)nullptr, (double*)nullptr);
         if (n_partial) *n_partial = (int)zgrid;
         UGX_CHECK_LAUNCH();
     }
@@ -2627,7 +2694,7 @@ int q1mf_direction_axpy(int W, const T* cell, const T* p_old, const T* z, const 
     const int64_t zgrid = make_mf_zmap(W, sizeof(T), reinterpret_cast<const void*>(&k_q1mf_z<T, 5>), zm, zlds);
     if (!zgrid || !dot_partial || !n_partial || zgrid > *n_partial) return -1;
     hipLaunchKernelGGL((k_q1mf_z<T, 5>), dim3(zgrid), dim3(256), zlds, s, zm, cell, p_old, z, (const T*)nullptr, y,
-                       p_new, 0.0, 1.0, dot_partial, beta_dev);
+                       p_new, 0.0, 1.0, dot_partial, beta_dev, (double*)nullptr);
     *n_partial = (int)zgrid;
     UGX_CHECK_LAUNCH();
 }
@@ -2643,6 +2710,8 @@ int q1mf_prepare(int W) {
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 1>), zm, lds);
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 3>), zm, lds);
     make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 4>), zm, lds);
+    make_mf_zmap(W, 4, reinterpret_cast<const void*>(&k_q1mf_z<float, 7>), zm, lds);
+    make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 7>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 3>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 0>), zm, lds);
     make_mf_zmap(W, 8, reinterpret_cast<const void*>(&k_q1mf_z<double, 1>), zm, lds);

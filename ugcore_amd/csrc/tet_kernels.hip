@@ -275,8 +275,10 @@ __global__ __launch_bounds__(256) void k_prolong_add_tet(int Wc, int Wf, int64_t
         const int64_t b = ((int64_t)((z + dz) >> 1) * Wc + ((y + dy) >> 1)) * Wc + ((x + dx) >> 1);
         v = 0.5 * (double)scc[a] + 0.5 * (double)scc[b];
     }
-    st[i] = (TV)v;
-    sc[i] = (TV)((double)sc[i] + v);
+    // c += t in the vector precision (mg_solver_impl.hpp:1905)
+    const TV t = (TV)v;
+    st[i] = t;
+    sc[i] = sc[i] + t;
 }
 
 template <typename TV>

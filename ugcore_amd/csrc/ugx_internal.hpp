@@ -112,6 +112,16 @@ template <typename T>
 int q1mf_jacobi_step(int W, const T* cell, const T* dinv, const T* sd_in, T* sd_out, T* sc, hipStream_t s);
 template <typename T>
 int q1mf_apply_sub(int W, const T* cell, const T* x, const T* d_in, T* d_out, hipStream_t s);
+// the last post-smoothing step fused with the final correction update and
+// its conversion: z = (double)(sc + st + dinv .* (sd_in - A st)), st =
+// dinv .* sd_in (q1mf_jacobi_step then jacobi_convert_dot, without storing the
+// step's outputs); with dot_partial also the (z, r) partials (n_partial in:
+// capacity, out: number written). q1mf_last_fits(W, sizeof(T)) tells whether
+// the plan applies.
+template <typename T>
+int q1mf_jacobi_last(int W, const T* cell, const T* dinv, const T* sd_in, const T* sc, double* z, const double* r,
+                     hipStream_t s, double* dot_partial, int* n_partial);
+bool q1mf_last_fits(int W, size_t elem);
 // dot_partial (optional): per-block partial sums of y.x (n_partial in: capacity,
 // out: number written); needs the z-marching kernel
 template <typename T>
