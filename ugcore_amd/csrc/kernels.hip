@@ -1162,6 +1162,10 @@ template <typename TV>
 __device__ __forceinline__ TV coarse_fold(TV sc, TV dinv, TV sd) {
     return (TV)((double)sc + (double)dinv * (double)sd);
 }
+// The rows' own inputs and outputs (read and written once per launch) use
+// nontemporal loads and stores, which leave the L2 to the staged planes that
+// neighbouring bands share (measured at 257^3: FP32-preconditioned solve
+// 10.34 against 10.40 ms, FP64 15.00 against 15.08 ms).
 // MODE 0: fused Jacobi step (staged st = dinv.*sd; sd_out = sd - A st; sc += st)
 // MODE 1: d_out = d_in - A x;  MODE 2: y = alpha v + beta A x
 // MODE 3: the first Jacobi step of a cycle from the FP64 defect d64 and a zero
@@ -1285,23 +1289,23 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             const int r = tid + u * step - a0;
             const bool ok = r >= 0 && r < R;
             if (MODE == 0) {
-                ra[u] = ok ? b_in[base + r] : (T)0;   // sd_in
-                rb[u] = ok ? out1[base + r] : (T)0;   // sc
+                ra[u] = ok ? ldnt(b_in + base + r) : (T)0;   // sd_in
+                rb[u] = ok ? ldnt(out1 + base + r) : (T)0;   // sc
             } else if (MODE == 3) {
-                ra[u] = ok ? (T)d64[base + r] : (T)0;   // sd_in
+                ra[u] = ok ? (T)ldnt(d64 + base + r) : (T)0;   // sd_in
             } else if (MODE == 4) {
-                ra[u] = ok ? b_in[base + r] : (T)0;     // sd_in
+                ra[u] = ok ? ldnt(b_in + base + r) : (T)0;     // sd_in
             } else if (MODE == 7) {
-                ra[u] = ok ? b_in[base + r] : (T)0;   // sd_in
-                rb[u] = ok ? c_in[base + r] : (T)0;   // sc
+                ra[u] = ok ? ldnt(b_in + base + r) : (T)0;   // sd_in
+                rb[u] = ok ? ldnt(c_in + base + r) : (T)0;   // sc
                 if constexpr (EAGER7) {
                     rd[u] = ok ? a_in[base + r] : (T)0;                     // dinv
                     rr[u] = (ok && dot_partial) ? d64[base + r] : 0.0;      // r
                 }
             } else if (MODE == 1) {
-                ra[u] = ok ? c_in[base + r] : (T)0;   // d_in
+                ra[u] = ok ? ldnt(c_in + base + r) : (T)0;   // d_in
             } else {
-                ra[u] = (ok && alpha != 0.0) ? c_in[base + r] : (T)0;   // v
+                ra[u] = (ok && alpha != 0.0) ? ldnt(c_in + base + r) : (T)0;   // v
             }
         }
         // 2. the operator on this thread's rows of plane k
@@ -1369,23 +1373,23 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             // 3. outputs
             const int64_t i = base + r;
             if (MODE == 0) {
-                out0[i] = ra[u] - sv;
-                out1[i] = rb[u] + xi;
+                stnt(ra[u] - sv, out0 + i);
+                stnt(rb[u] + xi, out1 + i);
             } else if (MODE == 3 || MODE == 4) {
-                out0[i] = ra[u] - sv;
-                out1[i] = xi;
+                stnt(ra[u] - sv, out0 + i);
+                stnt(xi, out1 + i);
             } else if (MODE == 1) {
-                out0[i] = ra[u] - sv;
+                stnt(ra[u] - sv, out0 + i);
             } else if (MODE == 7) {
                 const T sd_out = ra[u] - sv, sc_new = rb[u] + xi;
                 const T dv = EAGER7 ? rd[EAGER7 ? u : 0] : a_in[i];
                 const double zi = (double)(T)((double)sc_new + (double)dv * (double)sd_out);
-                z64[i] = zi;
+                stnt(zi, z64 + i);
                 if (dot_partial) dot_acc += zi * (EAGER7 ? rr[EAGER7 ? u : 0] : d64[i]);
             } else {
                 const T yv = (T)((alpha == 0.0 ? 0.0 : alpha * (double)ra[u]) + beta * (double)sv);
-                out0[i] = yv;
-                if (MODE == 5) out1[i] = xi;   // p_new
+                stnt(yv, out0 + i);
+                if (MODE == 5) stnt(xi, out1 + i);   // p_new
                 dot_acc += (double)yv * (double)xi;
             }
         }
@@ -1624,7 +1628,8 @@ template <typename TV>
 __global__ __launch_bounds__(256) void k_prolong_add3(int Wc, int Wf, int64_t n_c, CoarseCorr<TV> scc,
                                                       const uint8_t* __restrict__ dir_f, TV* __restrict__ st,
                                                       TV* __restrict__ sc, FastDiv fc) {
-    const int64_t c = blockIdx.x * (int64_t)256 + threadIdx.x;
+    // consecutive coarse points (and so fine planes) on one XCD
+    const int64_t c = xcd_block((n_c + 255) / 256) * 256 + threadIdx.x;
     if (c >= n_c) return;
     const uint32_t q1 = fdiv((uint32_t)c, fc);
     const int x = (int)((uint32_t)c - q1 * (uint32_t)Wc);
@@ -1727,29 +1732,34 @@ __global__ __launch_bounds__(256) void k_restrict3_tiled(int Wc, int Wf, int nba
                                                          TV* __restrict__ sdc, TV* __restrict__ zc) {
     constexpr int TY = TransferTile<TV>::TY;
     constexpr int NL = 2 * TY + 1;
-    const int z = blockIdx.x / nbands, band = blockIdx.x - z * nbands;
+    // consecutive coarse planes on one XCD (its L2 then fetches each fine
+    // plane about once)
+    const int64_t lb = xcd_block((int64_t)Wc * nbands);
+    if (lb >= (int64_t)Wc * nbands) return;
+    const int z = (int)(lb / nbands), band = (int)(lb - (int64_t)z * nbands);
     const int y0 = band * TY;
     const int LW = Wf + 2;
     TV* lds = lds_base<TV>();
     const int total = 3 * NL * LW;
-    // (staged line, column) of entry t, advanced without divisions
-    int row = 0, xi = threadIdx.x;
-    while (xi >= LW) {
-        xi -= LW;
-        ++row;
-    }
-    for (int t = threadIdx.x; t < total; t += 256) {
-        const int pl = row >= 2 * NL ? 2 : row >= NL ? 1 : 0, ln = row - pl * NL;
-        const int fz = 2 * z - 1 + pl, fy = 2 * y0 - 1 + ln, fx = xi - 1;
-        TV v = (TV)0;
-        if (fz >= 0 && fz < Wf && fy >= 0 && fy < Wf && fx >= 0 && fx < Wf)
-            v = sdf[((int64_t)fz * Wf + fy) * Wf + fx];
-        lds[t] = v;
-        xi += 256;
-        while (xi >= LW) {
-            xi -= LW;
-            ++row;
+    // the loads of 4 entries per thread are issued before any is stored
+    for (int t0 = threadIdx.x; t0 < total; t0 += 4 * 256) {
+        int64_t j[4];
+        TV v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * 256;
+            const int row = t / LW, xi = t - row * LW;
+            const int pl = row >= 2 * NL ? 2 : row >= NL ? 1 : 0, ln = row - pl * NL;
+            const int fz = 2 * z - 1 + pl, fy = 2 * y0 - 1 + ln, fx = xi - 1;
+            j[u] = (t < total && fz >= 0 && fz < Wf && fy >= 0 && fy < Wf && fx >= 0 && fx < Wf)
+                       ? ((int64_t)fz * Wf + fy) * Wf + fx
+                       : -1;
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = j[u] >= 0 ? sdf[j[u]] : (TV)0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t0 + u * 256 < total) lds[t0 + u * 256] = v[u];
     }
     __syncthreads();
     const int ny = min(TY, Wc - y0);
@@ -1789,61 +1799,94 @@ __global__ __launch_bounds__(256) void k_restrict3_tiled(int Wc, int Wf, int nba
 
 // prolongation st = P scc, sc += st: one workgroup per fine plane fz and the
 // fine lines 2y0 .. 2y0+2TY-1 of a coarse band; LDS holds coarse planes
-// fz/2 and fz/2 + 1, lines y0 .. y0+TY, each line padded with one zero
+// fz/2 and fz/2 + 1, lines y0 .. y0+TY, each line padded with one zero. The
+// Dirichlet rows of P are those of the lattice boundary (every boundary point
+// of the hexahedral hierarchy is Dirichlet), so no flags are read.
 template <typename TV>
 __global__ __launch_bounds__(256) void k_prolong3_tiled(int Wc, int Wf, int nbands, CoarseCorr<TV> scc,
-                                                        TV* __restrict__ st, TV* __restrict__ sc) {
+                                                        TV* __restrict__ st, TV* __restrict__ sc, FastDiv fwf) {
     constexpr int TY = TransferTile<TV>::TY;
     constexpr int NL = TY + 1;
-    const int fz = blockIdx.x / nbands, band = blockIdx.x - fz * nbands;
+    // consecutive fine planes on one XCD: its L2 then fetches each coarse
+    // plane about once instead of every XCD fetching all of them
+    const int64_t lb = xcd_block((int64_t)Wf * nbands);
+    if (lb >= (int64_t)Wf * nbands) return;
+    const int fz = (int)(lb / nbands), band = (int)(lb - (int64_t)fz * nbands);
     const int y0 = band * TY, cz0 = fz >> 1, oz = fz & 1;
     const int LW = Wc + 1;
     TV* lds = lds_base<TV>();
-    const int total = 2 * NL * LW;
-    int row = 0, xs = threadIdx.x;
-    while (xs >= LW) {
-        xs -= LW;
-        ++row;
-    }
-    for (int t = threadIdx.x; t < total; t += 256) {
-        const int pl = row >= NL ? 1 : 0, ln = row - pl * NL;
-        const int cz = cz0 + pl, cy = y0 + ln;
-        TV v = (TV)0;
-        if (cz < Wc && cy < Wc && xs < Wc) v = scc(((int64_t)cz * Wc + cy) * Wc + xs);
-        lds[t] = v;
-        xs += 256;
-        while (xs >= LW) {
-            xs -= LW;
-            ++row;
+    // coarse plane cz0 + 1 only for an odd fine plane; the loads of 4
+    // entries per thread are issued before any is stored
+    const int total = (1 + oz) * NL * LW;
+    for (int t0 = threadIdx.x; t0 < total; t0 += 4 * 256) {
+        int64_t j[4];
+        TV v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * 256;
+            const int row = t / LW, xs = t - row * LW;
+            const int pl = row >= NL ? 1 : 0, ln = row - pl * NL;
+            const int cz = cz0 + pl, cy = y0 + ln;
+            j[u] = (t < total && cz < Wc && cy < Wc && xs < Wc) ? ((int64_t)cz * Wc + cy) * Wc + xs : -1;
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = j[u] >= 0 ? scc(j[u]) : (TV)0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t0 + u * 256 < total) lds[t0 + u * 256] = v[u];
     }
     __syncthreads();
     const int fy0 = 2 * y0, ny = min(2 * TY, Wf - fy0);
-    int lyf = 0, fx = threadIdx.x;
-    while (fx >= Wf) {
-        fx -= Wf;
-        ++lyf;
-    }
-    for (int t = threadIdx.x; t < ny * Wf; t += 256, fx += 256) {
-        while (fx >= Wf) {
-            fx -= Wf;
-            ++lyf;
+    // every fine point: the (up to 8) coarse values around it times w =
+    // 2^-nodd, accumulated in the order (dz, dy, dx) as prolong3_point does
+    // with fma; a power-of-two w commutes with rounding, so that chain equals
+    // w times the plain sum in the same order. Terms a point does not have add
+    // 0 and read a value that is there, so the 8 reads of a point are issued
+    // together and no read waits for the previous sum.
+    const TV* zl = lds;
+    const TV* zh = lds + oz * NL * LW;
+    // the fine points in rounds of 4 per thread: the 4 loads of sc of a round
+    // are issued before any of them is used (8 per round, or loading the
+    // first round before the staging, measured slower: more registers, fewer
+    // resident waves)
+    constexpr int U = 4;
+    const int nf = ny * Wf;
+    for (int t0 = threadIdx.x; t0 < nf; t0 += U * 256) {
+        int64_t f[U];
+        int fx[U], fy[U];
+        TV cold[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * 256;
+            const int lyf = (int)fdiv((uint32_t)t, fwf);
+            fx[u] = t - lyf * Wf;
+            fy[u] = fy0 + lyf;
+            f[u] = ((int64_t)fz * Wf + fy[u]) * Wf + fx[u];
+            cold[u] = t < nf ? sc[f[u]] : (TV)0;
         }
-        const int fy = fy0 + lyf;
-        const int ox = fx & 1, oy = fy & 1, x = fx >> 1, ly = lyf >> 1;
-        const int nodd = ox + oy + oz;
-        const bool bnd = fx == 0 || fy == 0 || fz == 0 || fx == Wf - 1 || fy == Wf - 1 || fz == Wf - 1;
-        double val = 0.0;
-        if (!(bnd && nodd != 0)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t0 + u * 256 >= nf) continue;
+            const int ox = fx[u] & 1, oy = fy[u] & 1, x = fx[u] >> 1, ly = (fy[u] - fy0) >> 1;
+            const int nodd = ox + oy + oz;
+            const bool bnd = fx[u] == 0 || fy[u] == 0 || fz == 0 || fx[u] == Wf - 1 || fy[u] == Wf - 1 || fz == Wf - 1;
+            const int o = ly * LW + x, sy = oy * LW;
+            const TV a0 = zl[o], a1 = zl[o + ox], a2 = zl[o + sy], a3 = zl[o + sy + ox];
+            const TV b0 = zh[o], b1 = zh[o + ox], b2 = zh[o + sy], b3 = zh[o + sy + ox];
+            double sum = (double)a0;
+            sum += ox ? (double)a1 : 0.0;
+            sum += oy ? (double)a2 : 0.0;
+            sum += (ox & oy) ? (double)a3 : 0.0;
+            sum += oz ? (double)b0 : 0.0;
+            sum += (oz & ox) ? (double)b1 : 0.0;
+            sum += (oz & oy) ? (double)b2 : 0.0;
+            sum += (oz & ox & oy) ? (double)b3 : 0.0;
             const double w = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
-            for (int dz = 0; dz <= oz; ++dz)
-                for (int dy = 0; dy <= oy; ++dy)
-                    for (int dx = 0; dx <= ox; ++dx) val = fma(w, (double)lds[(dz * NL + ly + dy) * LW + x + dx], val);
+            const double val = (bnd && nodd != 0) ? 0.0 : w * sum;
+            const TV tv = (TV)val;
+            st[f[u]] = tv;
+            sc[f[u]] = cold[u] + tv;
         }
-        const int64_t f = ((int64_t)fz * Wf + fy) * Wf + fx;
-        const TV tv = (TV)val;
-        st[f] = tv;
-        sc[f] = sc[f] + tv;
     }
 }
 
@@ -2802,18 +2845,22 @@ int prolong_add(int dim, int Wc, int Wf, int64_t n_f, const TV* scc, const uint8
     if ((dinv_c == nullptr) != (sd_c == nullptr)) return -1;
     const CoarseCorr<TV> corr{scc, dinv_c, sd_c};
     if (dim == 3 && Wf == 2 * Wc - 1) {
-        // measured on one MI355X, 129^3 -> 257^3: FP64 tiled 84 us against 118 us
-        // for the per-coarse-point kernel; FP32 the other way round (62 / 46 us)
+        // measured on one MI355X within the 256^3 GMG-CG solve (both kernels
+        // with XCD-grouped blocks): 129^3 -> 257^3 FP32 tiled 76 us against
+        // 81 us for the per-coarse-point kernel, 65^3 -> 129^3 15 against
+        // 13 us, the coarser levels 4.6-5.9 against 7.7-10 us; FP64 tiled
+        // 94 us at the top
         constexpr int TY = TransferTile<TV>::TY;
         const size_t lds = prolong3_tiled_lds(Wc, sizeof(TV), TY);
-        if (sizeof(TV) == 8 && lds <= 64 * 1024) {
+        if (lds <= 64 * 1024) {
             const int nbands = (Wc + TY - 1) / TY;
-            hipLaunchKernelGGL(k_prolong3_tiled<TV>, dim3((unsigned)((int64_t)Wf * nbands)), dim3(256), lds, s, Wc, Wf,
-                               nbands, corr, st, sc);
+            hipLaunchKernelGGL(k_prolong3_tiled<TV>, dim3((unsigned)remapped_grid((int64_t)Wf * nbands)), dim3(256), lds, s,
+                               Wc, Wf, nbands, corr, st, sc, make_fastdiv((uint32_t)Wf));
             UGX_CHECK_LAUNCH();
         }
         const int64_t n_c = (int64_t)Wc * Wc * Wc;
-        hipLaunchKernelGGL(k_prolong_add3<TV>, dim3((n_c + 255) / 256), dim3(256), 0, s, Wc, Wf, n_c, corr, dir_f, st,
+        hipLaunchKernelGGL(k_prolong_add3<TV>, dim3(remapped_grid((n_c + 255) / 256)), dim3(256), 0, s, Wc, Wf, n_c, corr,
+                           dir_f, st,
                            sc, make_fastdiv((uint32_t)Wc));
         UGX_CHECK_LAUNCH();
     }
@@ -2878,7 +2925,8 @@ int restrict_defect(int dim, int Wc, int Wf, int64_t n_c, const TV* sdf, const u
         const size_t lds = restrict3_tiled_lds(Wf, sizeof(TV), TY);
         if (lds <= 64 * 1024) {
             const int nbands = (Wc + TY - 1) / TY;
-            hipLaunchKernelGGL(k_restrict3_tiled<TV>, dim3((unsigned)((int64_t)Wc * nbands)), dim3(256), lds, s, Wc,
+            hipLaunchKernelGGL(k_restrict3_tiled<TV>, dim3((unsigned)remapped_grid((int64_t)Wc * nbands)), dim3(256), lds,
+                               s, Wc,
                                Wf, nbands, sdf, sdc, zero_c);
             UGX_CHECK_LAUNCH();
         }
