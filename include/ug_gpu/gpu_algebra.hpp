@@ -763,7 +763,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     // post-smoothing step together with the correction update that ends the
     // cycle and its conversion (default) or as two passes; the same
     // correction, the (z, r) product summed in another order. Before init.
-    void set_fused_final_step(bool on) { fused_final_ = on; }
+    void set_fused_smoothing(bool on) { fused_smoothing_ = on; }
     bool init(std::shared_ptr<MatrixOperator> J) override;
     bool apply(GPUVector& c, const GPUVector& d) override;
     // testing access: level matrix (FP64 DIA values in lattice numbering)
@@ -814,7 +814,7 @@ class AssembledMultiGridCycle : public ILinearIterator {
     bool matrix_free_ = true;
     bool rap_ = false;
     bool graphs_ = true;
-    bool fused_final_ = true;
+    bool fused_smoothing_ = true;
     Impl* impl_ = nullptr;
     const GPUSparseMatrix* mat_ = nullptr;
     uint64_t mat_version_ = 0;

@@ -424,7 +424,7 @@ int ugx_gmg_set_graphs(void* g, int on);
  * (c, d) product as one pass; 0 = separate passes. The correction is the
  * same; (c, d) is summed in another order. The switch has no counterpart in
  * the reference (kernel fusion). */
-int ugx_gmg_set_fused_final_step(void* g, int on);
+int ugx_gmg_set_fused_smoothing(void* g, int on);
 int ugx_gmg_init(void* g, void* matrix);
 int ugx_gmg_apply(void* g, void* c, void* d);                 /* c = B*d */
 /* nsteps fused damped-Jacobi smoothing steps on one level's current defect

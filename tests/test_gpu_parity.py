@@ -318,7 +318,7 @@ def test_jacobi_preconditioned_cg():
 
 @pytest.mark.parametrize("refs", [4, 7])
 @pytest.mark.parametrize("precision", ["f32", "f64"])
-def test_fused_final_step_equals_separate_passes(precision, refs):
+def test_fused_smoothing_equals_separate_passes(precision, refs):
     """the lattice CG's cycle with its last post-smoothing step fused with the
     final correction update, conversion and (z, r) against the separate
     passes, for V-, W- and F-cycles: the same iteration (the product is only
@@ -332,7 +332,7 @@ def test_fused_final_step_equals_separate_passes(precision, refs):
     for cycle in ("V", "W", "F"):
         out = {}
         for fused in (True, False):
-            g = u.GeometricMultiGrid(d, base_level=1, cycle=cycle, precision=precision, fused_final_step=fused)
+            g = u.GeometricMultiGrid(d, base_level=1, cycle=cycle, precision=precision, fused_smoothing=fused)
             S = u.CG(g, max_steps=100, reduction=1e-10)
             S.init(A)
             x = u.Vector(d.num_dofs())

@@ -97,7 +97,7 @@ _SIGNATURES = {
     "ugx_gmg_set_matrix_free": (_I, [_P, _I]),
     "ugx_gmg_set_rap": (_I, [_P, _I]),
     "ugx_gmg_set_graphs": (_I, [_P, _I]),
-    "ugx_gmg_set_fused_final_step": (_I, [_P, _I]),
+    "ugx_gmg_set_fused_smoothing": (_I, [_P, _I]),
     "ugx_solver_set_restart": (_I, [_P, _I]),
     "ugx_solver_set_min_orthogonality": (_I, [_P, _D]),
     "ugx_disc_create": (_P, [_I, _I]),
@@ -675,7 +675,7 @@ class GeometricMultiGrid:
     builds the coarse levels as Galerkin products R A P (set_rap)."""
 
     def __init__(self, disc, base_level=0, nu1=2, nu2=2, cycle=1, omega=0.66, precision="f64", transposed=False,
-                 matrix_free=True, rap=False, graphs=True, fused_final_step=True):
+                 matrix_free=True, rap=False, graphs=True, fused_smoothing=True):
         self.disc = disc
         if isinstance(cycle, str):
             if cycle.strip() not in _CYCLES:
@@ -694,8 +694,8 @@ class GeometricMultiGrid:
             _check(lib().ugx_gmg_set_rap(self.h, 1), "GMG.set_rap")
         if not graphs:
             _check(lib().ugx_gmg_set_graphs(self.h, 0), "GMG.set_graphs")
-        if not fused_final_step:
-            _check(lib().ugx_gmg_set_fused_final_step(self.h, 0), "GMG.set_fused_final_step")
+        if not fused_smoothing:
+            _check(lib().ugx_gmg_set_fused_smoothing(self.h, 0), "GMG.set_fused_smoothing")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

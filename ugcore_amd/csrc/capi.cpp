@@ -824,9 +824,9 @@ int ugx_gmg_set_graphs(void* g, int on) {
         return 0;
     });
 }
-int ugx_gmg_set_fused_final_step(void* g, int on) {
+int ugx_gmg_set_fused_smoothing(void* g, int on) {
     return guard([&] {
-        ((GmgH*)g)->g->set_fused_final_step(on != 0);
+        ((GmgH*)g)->g->set_fused_smoothing(on != 0);
         return 0;
     });
 }

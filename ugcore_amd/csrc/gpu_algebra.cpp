@@ -1789,7 +1789,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     bool sym = true;             // level products through the symmetric storage
     bool rap = false;            // Galerkin coarse operators R A P below the top level
     bool graphs = true;          // replay the lattice cycle from captured HIP graphs
-    bool fuse_final = true;      // matrix-free top level: last smoothing step fused with the final update
+    bool fuse_smoothing = true;  // matrix-free top level of the lattice entry: fused smoothing passes
     struct CycleGraph {
         const void* key[5];
         hipGraphExec_t exec;
@@ -1819,11 +1819,11 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
     }
 
     GmgImpl(DomainDiscretization* d, int b, int n1, int n2, int cy, double w, bool tr, bool matrix_free, bool galerkin,
-            bool use_graphs, bool fused_final)
+            bool use_graphs, bool fused_smoothing)
         : disc(d), base(b), top(d->top_level()), nu1(n1), nu2(n2), cycle(cy), omega(w),
           sym_ok(d->symmetric()), transposed(tr), tets(d->kind() == 2),
           mf(matrix_free && d->kind() == 0 && d->top_level() >= 1), sym(d->symmetric()), rap(galerkin),
-          graphs(use_graphs), fuse_final(fused_final) {
+          graphs(use_graphs), fuse_smoothing(fused_smoothing) {
         if (rap && d->kind() == 2)
             throw UGError("GMG: Galerkin coarse operators need the P1 lattice transfer (hexahedral or triangular "
                           "hierarchy)");
@@ -2071,7 +2071,7 @@ struct GmgImpl : public AssembledMultiGridCycle::Impl {
         // matrix-free top level of the lattice entry: the last full smoothing
         // step, the deferred correction update, its conversion and (z, r) in
         // one pass (run_lattice then has nothing left to do)
-        const bool fuse_last = fuse_final && defer_top_final && lev == top && sym && mf && nu2 >= 2 &&
+        const bool fuse_last = fuse_smoothing && defer_top_final && lev == top && sym && mf && nu2 >= 2 &&
                                q1mf_last_fits(f.W, sizeof(T));
         for (int nu = 0; nu < nu2; ++nu) {
             if (fuse_last && nu == nu2 - 2) {
@@ -2349,10 +2349,10 @@ bool AssembledMultiGridCycle::init(std::shared_ptr<MatrixOperator> J) {
     impl_ = nullptr;
     if (prec_ == Precision::F64)
         impl_ = new GmgImpl<double>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                    rap_, graphs_, fused_final_);
+                                    rap_, graphs_, fused_smoothing_);
     else
         impl_ = new GmgImpl<float>(disc_.get(), base_lev_, nu1_, nu2_, cycle_, omega_, transposed_, matrix_free_,
-                                   rap_, graphs_, fused_final_);
+                                   rap_, graphs_, fused_smoothing_);
     impl_->init();
     return true;
 }

@@ -1162,6 +1162,53 @@ template <typename TV>
 __device__ __forceinline__ TV coarse_fold(TV sc, TV dinv, TV sd) {
     return (TV)((double)sc + (double)dinv * (double)sd);
 }
+// the Q1 operator at an interior lattice row from staged planes: sm, s0, sp
+// hold the operand on planes k-1, k, k+1 (the row at offset cc0, its
+// neighbours at cc0 +- 1, +- W), cm and c0 the per-element coefficients
+// h k_e of the cell planes below and above it (the row's lower-left cell at
+// offset cl0, lines CX apart). (A x)_i = (1/3) x_i sum_e c_e - (1/12) sum of
+// the 20 diagonal neighbours, each weighted by the coefficients of the cells
+// it shares with the row (one for a body diagonal, two for a face diagonal);
+// the 6 axis neighbours have weight 0 in the Q1 stiffness of a cube.
+template <typename T>
+__device__ __forceinline__ T q1mf_row(const T* sm, const T* s0, const T* sp, const T* cm, const T* c0, int cc0,
+                                      int cl0, int W, int CX, T xi) {
+    auto X = [&](int dx, int dy, int dz) {
+        const T* pl = dz < 0 ? sm : (dz > 0 ? sp : s0);
+        return pl[cc0 + dy * W + dx];
+    };
+    T ce[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) ce[c][bb][aa] = (c ? c0 : cm)[cl0 + bb * CX + aa];
+    T csum = (T)0, off = (T)0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) {
+                csum += ce[c][bb][aa];
+                off = fma(ce[c][bb][aa], X(2 * aa - 1, 2 * bb - 1, 2 * c - 1), off);
+            }
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) off = fma(ce[0][bb][aa] + ce[1][bb][aa], X(2 * aa - 1, 2 * bb - 1, 0), off);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) off = fma(ce[c][0][aa] + ce[c][1][aa], X(2 * aa - 1, 0, 2 * c - 1), off);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) off = fma(ce[c][bb][0] + ce[c][bb][1], X(0, 2 * bb - 1, 2 * c - 1), off);
+    return csum * xi * (T)(1.0 / 3.0) - off * (T)(1.0 / 12.0);
+}
+
 // The rows' own inputs and outputs (read and written once per launch) use
 // nontemporal loads and stores, which leave the L2 to the staged planes that
 // neighbouring bands share (measured at 257^3: FP32-preconditioned solve
@@ -1327,49 +1374,10 @@ void k_q1mf_z(MfZMap m, const T* __restrict__ cell, const T* __restrict__ a_in,
             }
             const int y = j0 + jl;
             const int cc0 = r + W + 1;
-            auto X = [&](int dx, int dy, int dz) {
-                const T* pl = dz < 0 ? sm : (dz > 0 ? sp : s0);
-                return pl[cc0 + dy * W + dx];
-            };
             const T xi = s0[cc0];
-            T sv;
-            if (plane_bnd || x == 0 || x == W - 1 || y == 0 || y == W - 1) {
-                sv = xi;
-            } else {
-                T ce[2][2][2];
-#pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                        for (int aa = 0; aa < 2; ++aa) ce[c][bb][aa] = (c ? c0 : cm)[(jl + bb) * CX + x + aa];
-                T csum = (T)0, off = (T)0;
-#pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                        for (int aa = 0; aa < 2; ++aa) {
-                            csum += ce[c][bb][aa];
-                            off = fma(ce[c][bb][aa], X(2 * aa - 1, 2 * bb - 1, 2 * c - 1), off);
-                        }
-#pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                    for (int aa = 0; aa < 2; ++aa)
-                        off = fma(ce[0][bb][aa] + ce[1][bb][aa], X(2 * aa - 1, 2 * bb - 1, 0), off);
-#pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int aa = 0; aa < 2; ++aa)
-                        off = fma(ce[c][0][aa] + ce[c][1][aa], X(2 * aa - 1, 0, 2 * c - 1), off);
-#pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int bb = 0; bb < 2; ++bb)
-                        off = fma(ce[c][bb][0] + ce[c][bb][1], X(0, 2 * bb - 1, 2 * c - 1), off);
-                sv = csum * xi * (T)(1.0 / 3.0) - off * (T)(1.0 / 12.0);
-            }
+            const T sv = (plane_bnd || x == 0 || x == W - 1 || y == 0 || y == W - 1)
+                             ? xi
+                             : q1mf_row(sm, s0, sp, cm, c0, cc0, jl * CX + x, W, CX, xi);
             // 3. outputs
             const int64_t i = base + r;
             if (MODE == 0) {
