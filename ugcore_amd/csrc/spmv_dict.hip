@@ -254,17 +254,43 @@ __global__ __launch_bounds__(64) void k_dict_fill(int64_t nrows, const int32_t* 
 }
 
 // the values alone, for a matrix whose pattern (and so dictionary and
-// positions) is unchanged since k_dict_fill
+// positions) is unchanged since k_dict_fill: up to 32 columns of all 64 rows
+// at a time through LDS, each row's run read once and in one piece (lane e of
+// a round reads entry e % wc of row e / wc), the loads of 8 rounds issued
+// before their stores; the value pairs then written out contiguously
+constexpr int kValCols = 32;
 __global__ __launch_bounds__(64) void k_dict_values(int64_t nrows, const int32_t* __restrict__ rowptr,
                                                     const double* __restrict__ val, const int32_t* __restrict__ order,
                                                     const int32_t* __restrict__ width, const int64_t* __restrict__ vptr,
                                                     double* __restrict__ sval) {
     __shared__ int32_t rbs[64], lens[64];
-    __shared__ double vt[kChunk * 64];
+    __shared__ double vt[kValCols * 64];
     const int64_t t = blockIdx.x;
     slice_rows(nrows, rowptr, order, t, rbs, lens);
     __syncthreads();
-    slice_values_out(width[t], rbs, lens, nullptr, val, nullptr, nullptr, nullptr, vt, sval + vptr[t], nullptr);
+    const int w = width[t], lane = threadIdx.x;
+    double* sv = sval + vptr[t];
+    for (int k0 = 0; k0 < w; k0 += kValCols) {
+        const int wc = min(kValCols, w - k0), wp = wc + (wc & 1);   // a padded odd entry is written as 0
+        const int total = 64 * wp;
+        for (int e0 = lane; e0 < total; e0 += 8 * 64) {
+            double v[8];
+            int o[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * 64;
+                const int j = e / wp, kk = e - j * wp, k = k0 + kk;
+                o[u] = e < total ? (kk >> 1) * 128 + 2 * j + (kk & 1) : -1;
+                v[u] = (e < total && k < lens[j]) ? val[(int64_t)rbs[j] + k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (o[u] >= 0) vt[o[u]] = v[u];
+        }
+        __syncthreads();
+        for (int q = lane; q < 64 * wp; q += 64) sv[(int64_t)64 * k0 + q] = vt[q];
+        __syncthreads();
+    }
 }
 
 // y = alpha*v1 + beta*A*x (mode 0), or y = beta*A*x on rows with entries
