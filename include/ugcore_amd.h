@@ -435,6 +435,15 @@ int ugx_gmg_smooth_level(void* g, int level, int nsteps);
  * pointers to (n+1)^dim entries; x needs ugx_gmg_lattice_halo zero entries on
  * each side and must vanish on Dirichlet DoFs) */
 int ugx_gmg_surface_apply_lattice(void* g, double* y_dev, const double* x_dev);
+/* the lattice CG's direction update fused with the next operator application
+ * (cg.h:234 p := beta p + z, then cg.h:166-169 q := A p, lambda = (q, p)):
+ * p_new = z + (*beta_dev) p_old, q = A p_new, (q, p_new) into *qp_dev
+ * (device); p_old and p_new are distinct lattice vectors with the halo,
+ * partial_dev device scratch of 65536 doubles. Matrix-free top level only:
+ * an error when it is not available. */
+int ugx_gmg_surface_direction_lattice(void* g, double* q_dev, const double* p_old_dev, const double* z_dev,
+                                      const double* beta_dev, double* p_new_dev, double* qp_dev,
+                                      double* partial_dev);
 /* c = B*d with the cycle applied in lattice order (device pointers to (n+1)^dim
  * FP64 entries, each with ugx_gmg_lattice_halo zero entries on either side; d
  * must vanish on Dirichlet DoFs): the preconditioner of the lattice CG, equal

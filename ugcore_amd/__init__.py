@@ -187,6 +187,7 @@ _SIGNATURES = {
     "ugx_gmg_apply": (_I, [_P, _P, _P]),
     "ugx_gmg_smooth_level": (_I, [_P, _I, _I]),
     "ugx_gmg_surface_apply_lattice": (_I, [_P, _P, _P]),
+    "ugx_gmg_surface_direction_lattice": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ugx_gmg_apply_lattice": (_I, [_P, _P, _P, _P]),
     "ugx_gmg_lattice_uses_f32_defect": (_I, [_P]),
     "ugx_gmg_lattice_halo": (_I64, [_P]),

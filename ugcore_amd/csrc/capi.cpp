@@ -975,6 +975,16 @@ int ugx_gmg_surface_apply_lattice(void* g, double* y, const double* x) {
         return 0;
     });
 }
+int ugx_gmg_surface_direction_lattice(void* g, double* q, const double* p_old, const double* z, const double* beta,
+                                      double* p_new, double* qp, double* partial) {
+    return guard([&] {
+        if (!q || !p_old || !z || !beta || !p_new || !qp || !partial)
+            throw UGError("gmg_surface_direction_lattice: null pointer");
+        if (!((GmgH*)g)->g->surface_direction_axpy_lattice(q, p_old, z, beta, p_new, qp, partial))
+            throw UGError("gmg_surface_direction_lattice: needs the matrix-free top level");
+        return 0;
+    });
+}
 int ugx_gmg_apply_lattice(void* g, double* c, const double* d, const float* d_f32) {
     return guard([&] {
         auto& gm = *((GmgH*)g)->g;

@@ -33,6 +33,7 @@ namespace {
 constexpr int kDictSlots = 255;       // dictionary entries per slice (position 255 = zero)
 constexpr int kDictMaxWidth = 64;     // longest row of a slice that the layout takes
 constexpr int kHashSlots = 512;       // LDS hash of a slice's columns (at most 255 + 64 occupied)
+constexpr int kChunk = 8;             // entries per row moved through LDS at a time
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
@@ -161,51 +162,40 @@ __global__ __launch_bounds__(64) void k_dict_count(int64_t nrows, const int32_t*
     if (lane == 0) ucount[t] = u;
 }
 
-// moves the slice's values and the dictionary positions of its entries
-// through LDS, up to kValCols columns of all 64 rows per pass: lane e of a
-// round reads entry e % wq of row e / wq, so each row's run is read once and
-// in one piece, the loads of 8 rounds issued before their stores; then the
-// value pairs and position words are written contiguously. Entries past a
-// row's end (to the pass's multiple of 4) are value 0, position 255.
-// hk/hv: column -> position
-constexpr int kValCols = 32;
+// moves the slice's values (and, with pb, the dictionary positions of the
+// entries) through LDS in chunks of kChunk entries per row: rows are read
+// contiguously, the value pairs and position words written contiguously.
+// hk/hv: column -> position (used when pb is given)
 __device__ void slice_values_out(int w, const int32_t* rbs, const int32_t* lens, const int32_t* __restrict__ col,
                                  const double* __restrict__ val, const int32_t* hk, const int32_t* hv, uint8_t* pb,
                                  double* vt, double* __restrict__ sv, uint32_t* __restrict__ si) {
     const int lane = threadIdx.x;
     const int npairs = (w + 1) >> 1, nwords = (w + 3) >> 2;
-    for (int k0 = 0; k0 < w; k0 += kValCols) {
-        const int wc = min(kValCols, w - k0), wq = (wc + 3) & ~3;
-        const int total = 64 * wq;
-        for (int e0 = lane; e0 < total; e0 += 8 * 64) {
-            double v[8];
-            int32_t c[8];
-            int kk[8], j[8];
+    for (int k0 = 0; k0 < w; k0 += kChunk) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * 64;
-                j[u] = e / wq;
-                kk[u] = e - j[u] * wq;
-                const bool in = e < total && k0 + kk[u] < lens[j[u]];
-                const int64_t a = in ? (int64_t)rbs[j[u]] + k0 + kk[u] : 0;
-                v[u] = in ? val[a] : 0.0;
-                c[u] = in ? col[a] : -1;
+        for (int it = 0; it < kChunk; ++it) {
+            const int e = it * 64 + lane;
+            const int j = e / kChunk, kk = e % kChunk, k = k0 + kk;
+            uint32_t pos = 255;
+            double v = 0.0;
+            if (k < lens[j]) {
+                const int64_t a = (int64_t)rbs[j] + k;
+                v = val[a];
+                if (pb) pos = (uint32_t)hv[hash_find(hk, col[a])];
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (e0 + u * 64 >= total) continue;
-                pb[kk[u] * 64 + j[u]] = (uint8_t)(c[u] >= 0 ? hv[hash_find(hk, c[u])] : 255);
-                vt[(kk[u] >> 1) * 128 + 2 * j[u] + (kk[u] & 1)] = v[u];
-            }
+            if (pb) pb[kk * 64 + j] = (uint8_t)pos;
+            vt[(kk >> 1) * 128 + 2 * j + (kk & 1)] = v;
         }
         __syncthreads();
-        const int npc = min(kValCols / 2, npairs - k0 / 2);
+        const int npc = min(kChunk / 2, npairs - k0 / 2);
         for (int o = lane; o < 128 * npc; o += 64) sv[(int64_t)64 * k0 + o] = vt[o];
-        const int nqc = min(kValCols / 4, nwords - k0 / 4);
-        for (int qq = 0; qq < nqc; ++qq) {
-            const uint8_t* b = pb + 4 * qq * 64 + lane;
-            si[(int64_t)64 * (k0 / 4 + qq) + lane] =
-                (uint32_t)b[0] | ((uint32_t)b[64] << 8) | ((uint32_t)b[128] << 16) | ((uint32_t)b[192] << 24);
+        if (pb) {
+            const int nqc = min(kChunk / 4, nwords - k0 / 4);
+            for (int qq = 0; qq < nqc; ++qq) {
+                const uint8_t* b = pb + 4 * qq * 64 + lane;
+                si[(int64_t)64 * (k0 / 4 + qq) + lane] =
+                    (uint32_t)b[0] | ((uint32_t)b[64] << 8) | ((uint32_t)b[128] << 16) | ((uint32_t)b[192] << 24);
+            }
         }
         __syncthreads();
     }
@@ -222,8 +212,8 @@ __global__ __launch_bounds__(64) void k_dict_fill(int64_t nrows, const int32_t* 
     __shared__ int32_t hk[kHashSlots], hv[kHashSlots];
     __shared__ int32_t rbs[64], lens[64], list[256];
     __shared__ int cnt;
-    __shared__ uint8_t pb[kValCols * 64];
-    __shared__ double vt[kValCols * 64];
+    __shared__ uint8_t pb[kChunk * 64];
+    __shared__ double vt[kChunk * 64];
     const int64_t t = blockIdx.x;
     const int lane = threadIdx.x;
     const int w = width[t];
@@ -268,6 +258,7 @@ __global__ __launch_bounds__(64) void k_dict_fill(int64_t nrows, const int32_t* 
 // at a time through LDS, each row's run read once and in one piece (lane e of
 // a round reads entry e % wc of row e / wc), the loads of 8 rounds issued
 // before their stores; the value pairs then written out contiguously
+constexpr int kValCols = 32;
 __global__ __launch_bounds__(64) void k_dict_values(int64_t nrows, const int32_t* __restrict__ rowptr,
                                                     const double* __restrict__ val, const int32_t* __restrict__ order,
                                                     const int32_t* __restrict__ width, const int64_t* __restrict__ vptr,
