@@ -100,3 +100,47 @@ def test_lattice_cycle_matches_surface_cycle(refs, precision, graphs):
             u.synchronize()
             c = c_lat.numpy()[H:H + N]
             assert np.array_equal(c, c_surf.numpy()[dol])
+
+
+@pytest.mark.parametrize("refs", [4, 6])
+def test_surface_direction_lattice_equals_its_parts(refs):
+    """ugx_gmg_surface_direction_lattice (p_new = z + beta p_old, q = A p_new,
+    (q, p_new) in one pass) against the direction update formed on the host,
+    the lattice surface operator applied to the kernel's p_new (the same
+    operator: bit for bit) and the product formed on the host"""
+    say(f"case surface direction lattice refs={refs}")
+    u = U()
+    L = u.lib()
+    d, A, b = problem(0, refs)
+    g = u.GeometricMultiGrid(d, base_level=0, precision="f32")
+    g.init(A)
+    N = d.num_dofs()
+    H = L.ugx_gmg_lattice_halo(g.h)
+    rng = np.random.default_rng(5)
+    n = 2 ** refs
+    W = n + 1
+    lat = np.zeros((W, W, W), bool)
+    lat[1:-1, 1:-1, 1:-1] = True          # the vectors vanish on the Dirichlet boundary
+    inner = lat.ravel()
+    p_old = np.where(inner, rng.normal(size=N), 0.0)
+    z = np.where(inner, rng.normal(size=N), 0.0)
+    beta = 0.37
+    pad = lambda v: np.concatenate([np.zeros(H), v, np.zeros(H)])
+    d_pold, d_z = u.DeviceArray(pad(p_old)), u.DeviceArray(pad(z))
+    d_pnew, d_q = u.DeviceArray(np.zeros(N + 2 * H)), u.DeviceArray(np.zeros(N))
+    d_beta, d_qp, d_part = u.DeviceArray(np.array([beta])), u.DeviceArray(np.zeros(1)), u.DeviceArray(np.zeros(65536))
+    assert L.ugx_gmg_surface_direction_lattice(g.h, d_q.ptr, d_pold.ptr + 8 * H, d_z.ptr + 8 * H, d_beta.ptr,
+                                               d_pnew.ptr + 8 * H, d_qp.ptr, d_part.ptr) == 0
+    u.synchronize()
+    p_new = d_pnew.numpy()[H:H + N]
+    # the update (the kernel fuses the multiply-add; equal to rounding)
+    expect_p = beta * p_old + z
+    assert np.allclose(p_new, expect_p, rtol=0, atol=1e-15 * np.abs(expect_p).max())
+    d_y = u.DeviceArray(np.zeros(N))
+    d_pn = u.DeviceArray(pad(p_new))
+    assert L.ugx_gmg_surface_apply_lattice(g.h, d_y.ptr, d_pn.ptr + 8 * H) == 0
+    u.synchronize()
+    q = d_q.numpy()
+    assert np.array_equal(q, d_y.numpy())
+    qp = d_qp.numpy()[0]
+    assert abs(qp - float(np.dot(q, p_new))) <= 1e-12 * abs(float(np.dot(np.abs(q), np.abs(p_new))))
