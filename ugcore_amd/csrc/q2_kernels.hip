@@ -67,7 +67,16 @@ constexpr Q2RefGrad q2_ref_grad() {
 }
 __constant__ Q2RefGrad kQ2Grad = q2_ref_grad();
 
-constexpr int kQ2Waves = 4;   // elements (wavefronts) per workgroup
+constexpr int kQ2Waves = 8;   // elements (wavefronts) per workgroup
+
+// the reference gradient and shape tables copied into the workgroup's LDS
+// (their per-lane reads in the matrix steps then cost an LDS access instead
+// of a cached global one); visible after the element routine's first barrier
+__device__ __forceinline__ void q2_stage_table(Q2RefGrad& tab) {
+    const double* src = reinterpret_cast<const double*>(&kQ2Grad);
+    double* dst = reinterpret_cast<double*>(&tab);
+    for (int i = threadIdx.x; i < (int)(sizeof(Q2RefGrad) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+}
 
 // The element computation of one wavefront (lane l): corners from src.corner(l)
 // for l < 24 (valid elements) and the coefficient src.k(q) at Gauss point q
@@ -77,7 +86,8 @@ constexpr int kQ2Waves = 4;   // elements (wavefronts) per workgroup
 // unit cube and emit nothing.
 template <class Src, class Sink>
 __device__ __forceinline__ void q2_element_wave(int l, bool valid, const Src& src, Sink& sink, double f_const,
-                                                double* Xs, double (*Jis)[9], double* cs, double* fw, bool want_fe) {
+                                                double* Xs, double (*Jis)[9], double* cs, double* fw, bool want_fe,
+                                                const Q2RefGrad& tab) {
     if (l < 24) {
         const int c = l / 3, d = l - 3 * c;
         const int bit = d == 0 ? (c == 1 || c == 2 || c == 5 || c == 6) : d == 1 ? (c == 2 || c == 3 || c == 6 || c == 7)
@@ -121,7 +131,7 @@ __device__ __forceinline__ void q2_element_wave(int l, bool valid, const Src& sr
     // load vector: fe[a] = sum_q w_q |det J_q| f phi_a(xi_q)
     if (want_fe && valid && l < 27) {
         double s = 0.0;
-        for (int q = 0; q < 27; ++q) s += fw[q] * kQ2Grad.phi[q][l];
+        for (int q = 0; q < 27; ++q) s += fw[q] * tab.phi[q][l];
         sink.fe(l, s);
     }
     // 2. Ke = (C B)^T B in the 16 x 16 tiles (0,0), (0,1), (1,1); (1,0) is the
@@ -142,8 +152,8 @@ __device__ __forceinline__ void q2_element_wave(int l, bool valid, const Src& sr
         double v0 = 0.0, v1 = 0.0, c = 0.0;
         if (k < 81) {
             const double j0 = Jis[q][r], j1 = Jis[q][3 + r], j2 = Jis[q][6 + r];
-            const double* g0 = kQ2Grad.g[q][li];
-            const double* g1 = kQ2Grad.g[q][a1];
+            const double* g0 = tab.g[q][li];
+            const double* g1 = tab.g[q][a1];
             c = cs[q];
             v0 = j0 * g0[0] + j1 * g0[1] + j2 * g0[2];
             if (has1) v1 = j0 * g1[0] + j1 * g1[1] + j2 * g1[2];
@@ -188,10 +198,12 @@ __global__ __launch_bounds__(64 * kQ2Waves) void k_q2_hex_elements(int64_t n, co
     __shared__ double Xs_all[kQ2Waves][24];
     __shared__ double Jis_all[kQ2Waves][27][9];
     __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    __shared__ Q2RefGrad tab;
+    q2_stage_table(tab);
     Q2ArraySrc src{X, kcoef, e};
     Q2ArraySink sink{Ke + (valid ? e : 0) * 729, fe ? fe + (valid ? e : 0) * 27 : nullptr};
     q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
-                    fe != nullptr);
+                    fe != nullptr, tab);
 }
 
 // ---- assembly on the unit-cube lattice of n^3 elements ----
@@ -291,10 +303,12 @@ __global__ __launch_bounds__(64 * kQ2Waves) void k_q2_lattice_assemble(int n, in
     __shared__ double Xs_all[kQ2Waves][24];
     __shared__ double Jis_all[kQ2Waves][27][9];
     __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    __shared__ Q2RefGrad tab;
+    q2_stage_table(tab);
     Q2LatticeSrc src{n, 1.0 / n, ex, ey, ez, theta};
     Q2CsrSink sink{2 * n + 1, ex, ey, ez, rowptr, col, val, rhs};
     q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
-                    rhs != nullptr);
+                    rhs != nullptr, tab);
 }
 
 // Dirichlet rows (u = 0 on the boundary of the cube): row := unit row on its
@@ -353,10 +367,12 @@ __global__ __launch_bounds__(64 * kQ2Waves) void k_elem_hex_q2(int64_t ne, const
     __shared__ double Xs_all[kQ2Waves][24];
     __shared__ double Jis_all[kQ2Waves][27][9];
     __shared__ double cs_all[kQ2Waves][28], fw_all[kQ2Waves][27];
+    __shared__ Q2RefGrad tab;
+    q2_stage_table(tab);
     Q2ElemSrc src{coords, dofs, kel, kq, e};
     Q2SlotSink sink{slot, dofs, ne, pos, e, val, rhs};
     q2_element_wave(threadIdx.x & 63, valid, src, sink, f_const, Xs_all[w], Jis_all[w], cs_all[w], fw_all[w],
-                    rhs != nullptr);
+                    rhs != nullptr, tab);
 }
 
 }  // namespace
