@@ -1097,11 +1097,12 @@ __device__ __forceinline__ void q1mf_block(const MfMap& m, const T* __restrict__
 }
 
 // z-marching variant: a block owns TY lines of a chunk of ZC planes and walks
-// it plane by plane. The staged values of 4 planes live in an LDS ring (3 in
-// use, 1 being filled), the block's cells of 3 cell planes in a second ring;
-// the loads of plane k+2 are issued into registers before plane k is
-// computed and stored after it, so every plane is read once per block and
-// its load latency overlaps the arithmetic.
+// it plane by plane. The staged values of the planes k-1, k, k+1 live in an
+// LDS ring of mf_slots() planes, the block's cells in a second ring of one
+// plane fewer; the loads of plane k+2 are issued into registers before plane
+// k is computed and stored after it (into the slot of plane k-1, behind a
+// second barrier with the 3-plane ring), so every plane is read once per
+// block and its load latency overlaps the arithmetic.
 struct MfZMap {
     int32_t W, TY, nchunks;
     int32_t glo[8], gsz[8];
