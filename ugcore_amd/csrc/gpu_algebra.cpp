@@ -880,8 +880,9 @@ bool CG::apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCyc
             UGX_LAUNCH(cg_direction(N, sc, pl, zl, c.stream));
         }
     }
-    UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), xl, x.data(), c.stream)));
-    if (defect_out) UGX_LAUNCH((permute_from<double, double>(N, gmg.lattice_of_dof(), rl, defect_out->data(), c.stream)));
+    UGX_LAUNCH((permute_scatter<double, double>(N, gmg.dof_of_lattice(), xl, x.data(), c.stream)));
+    if (defect_out)
+        UGX_LAUNCH((permute_scatter<double, double>(N, gmg.dof_of_lattice(), rl, defect_out->data(), c.stream)));
     sync();
     return ok && conv_->post();
 }

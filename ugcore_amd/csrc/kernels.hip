@@ -411,32 +411,96 @@ __global__ void k_jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_
 // surface (DoF order) <-> level (lattice order) copies with precision
 // conversion, written as gathers so the stores are coalesced:
 //   dst_lat[l] = src_dof[dof_of_lat[l]]   and   dst_dof[i] += src_lat[lat_of_dof[i]]
+// In the streaming layout above: a pass loads its kStreamE indices, then
+// issues its gathers (plain loads: neighbouring points share cache lines
+// that other waves read), then stores.
 template <typename TS, typename TL>
-__global__ void k_permute_to(int64_t n, const int32_t* __restrict__ dof_of_lat, const TS* __restrict__ src,
-                             TL* __restrict__ dst) {
-    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x)
-        dst[l] = (TL)src[dof_of_lat[l]];
+__global__ __launch_bounds__(256) void k_permute_to(int64_t n, const int32_t* __restrict__ dof_of_lat,
+                                                    const TS* __restrict__ src, TL* __restrict__ dst) {
+    UGX_STREAM_LOOP(n) {
+        int32_t k[kStreamE];
+        TS v[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) k[e] = ldnt(dof_of_lat + UGX_STREAM_SAFE(e, n));
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) v[e] = src[k[e]];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt((TL)v[e], dst + UGX_STREAM_IDX(e));
+    }
 }
 // the same into an FP64 and an FP32 lattice vector
-__global__ void k_permute_to_dual(int64_t n, const int32_t* __restrict__ dof_of_lat, const double* __restrict__ src,
-                                  double* __restrict__ dst, float* __restrict__ dst32) {
-    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
-        const double v = src[dof_of_lat[l]];
-        dst[l] = v;
-        dst32[l] = (float)v;
+__global__ __launch_bounds__(256) void k_permute_to_dual(int64_t n, const int32_t* __restrict__ dof_of_lat,
+                                                         const double* __restrict__ src, double* __restrict__ dst,
+                                                         float* __restrict__ dst32) {
+    UGX_STREAM_LOOP(n) {
+        int32_t k[kStreamE];
+        double v[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) k[e] = ldnt(dof_of_lat + UGX_STREAM_SAFE(e, n));
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) v[e] = src[k[e]];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) {
+                stnt(v[e], dst + UGX_STREAM_IDX(e));
+                stnt((float)v[e], dst32 + UGX_STREAM_IDX(e));
+            }
     }
 }
 template <typename TL, typename TS>
-__global__ void k_permute_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
-                               TS* __restrict__ dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = (TS)src[lat_of_dof[i]];
+__global__ __launch_bounds__(256) void k_permute_from(int64_t n, const int32_t* __restrict__ lat_of_dof,
+                                                      const TL* __restrict__ src, TS* __restrict__ dst) {
+    UGX_STREAM_LOOP(n) {
+        int32_t k[kStreamE];
+        TL v[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) k[e] = ldnt(lat_of_dof + UGX_STREAM_SAFE(e, n));
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) v[e] = src[k[e]];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt((TS)v[e], dst + UGX_STREAM_IDX(e));
+    }
+}
+// dst_dof[dof_of_lat[l]] = src_lat[l], the scatter form of k_permute_from for
+// a bijective numbering: the reads are coalesced, and the writes of a wave
+// fall into a few runs of consecutive DoFs (the lattice points of one line
+// alternate between a handful of DoF ranges, each numbered along the line)
+template <typename TL, typename TS>
+__global__ __launch_bounds__(256) void k_permute_scatter(int64_t n, const int32_t* __restrict__ dof_of_lat,
+                                                         const TL* __restrict__ src, TS* __restrict__ dst) {
+    UGX_STREAM_LOOP(n) {
+        int32_t k[kStreamE];
+        TL v[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            k[e] = ldnt(dof_of_lat + UGX_STREAM_SAFE(e, n));
+            v[e] = ldnt(src + UGX_STREAM_SAFE(e, n));
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) dst[k[e]] = (TS)v[e];
+    }
 }
 template <typename TL, typename TS>
-__global__ void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of_dof, const TL* __restrict__ src,
-                                   TS* __restrict__ dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = (TS)((double)dst[i] + (double)src[lat_of_dof[i]]);
+__global__ __launch_bounds__(256) void k_permute_add_from(int64_t n, const int32_t* __restrict__ lat_of_dof,
+                                                          const TL* __restrict__ src, TS* __restrict__ dst) {
+    UGX_STREAM_LOOP(n) {
+        int32_t k[kStreamE];
+        TL v[kStreamE];
+        TS d[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) {
+            k[e] = ldnt(lat_of_dof + UGX_STREAM_SAFE(e, n));
+            d[e] = ldnt(dst + UGX_STREAM_SAFE(e, n));
+        }
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) v[e] = src[k[e]];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n) stnt((TS)((double)d[e] + (double)v[e]), dst + UGX_STREAM_IDX(e));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2384,21 +2448,21 @@ template int dot<float>(int64_t, const float*, const float*, double*, double*, b
 template <typename TS, typename TL>
 int permute_to(int64_t n, const int32_t* dof_of_lat, const TS* src, TL* dst, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_permute_to<TS, TL>), dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst);
+    hipLaunchKernelGGL((k_permute_to<TS, TL>), dim3(stream_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst);
     UGX_CHECK_LAUNCH();
 }
 template int permute_to<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
 template int permute_to<double, float>(int64_t, const int32_t*, const double*, float*, hipStream_t);
 int permute_to_dual(int64_t n, const int32_t* dof_of_lat, const double* src, double* dst, float* dst32, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_permute_to_dual, dim3(vec_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst, dst32);
+    hipLaunchKernelGGL(k_permute_to_dual, dim3(stream_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst, dst32);
     UGX_CHECK_LAUNCH();
 }
 
 template <typename TL, typename TS>
 int permute_add_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_permute_add_from<TL, TS>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    hipLaunchKernelGGL((k_permute_add_from<TL, TS>), dim3(stream_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
     UGX_CHECK_LAUNCH();
 }
 template int permute_add_from<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
@@ -2588,10 +2652,17 @@ template int dias_axpy<double, double>(int64_t, int, const int32_t*, const doubl
 template <typename TL, typename TS>
 int permute_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_permute_from<TL, TS>), dim3(vec_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
+    hipLaunchKernelGGL((k_permute_from<TL, TS>), dim3(stream_grid(n)), dim3(256), 0, s, n, lat_of_dof, src, dst);
     UGX_CHECK_LAUNCH();
 }
 template int permute_from<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
+template <typename TL, typename TS>
+int permute_scatter(int64_t n, const int32_t* dof_of_lat, const TL* src, TS* dst, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL((k_permute_scatter<TL, TS>), dim3(stream_grid(n)), dim3(256), 0, s, n, dof_of_lat, src, dst);
+    UGX_CHECK_LAUNCH();
+}
+template int permute_scatter<double, double>(int64_t, const int32_t*, const double*, double*, hipStream_t);
 
 template <typename TM, typename TV>
 int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,

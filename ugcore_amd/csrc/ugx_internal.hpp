@@ -103,6 +103,10 @@ int dias_axpy(int64_t n, int nd, const int32_t* off, const TM* U, double alpha, 
 // dst_dof[i] = src_lat[lat_of_dof[i]]
 template <typename TL, typename TS>
 int permute_from(int64_t n, const int32_t* lat_of_dof, const TL* src, TS* dst, hipStream_t s);
+// dst_dof[dof_of_lat[l]] = src_lat[l]: the same copy when dof_of_lat is a
+// bijection of [0, n) (every DoF numbers exactly one lattice point)
+template <typename TL, typename TS>
+int permute_scatter(int64_t n, const int32_t* dof_of_lat, const TL* src, TS* dst, hipStream_t s);
 template <typename TM, typename TV>
 int dias_jacobi_step(int64_t n, int nd, const int32_t* off, const TM* U, const TV* dinv, const TV* sd_in, TV* sd_out,
                      TV* sc, hipStream_t s);
