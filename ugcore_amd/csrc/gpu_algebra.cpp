@@ -860,7 +860,6 @@ bool CG::apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCyc
             UGX_LAUNCH(dot<double>(N, ql, pl, c.red_partial, sc + 1, false, c.stream));
         }
         q_ready = false;
-        UGX_LAUNCH(cg_alpha(sc, c.stream));
         UGX_LAUNCH(cg_update(N, sc, pl, ql, xl, rl, part, c.stream, rfl));
         UGX_HIP(hipMemcpyAsync(c.red_host, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         sync();

@@ -573,29 +573,25 @@ __global__ __launch_bounds__(1024) void k_sum_partials(int np, const double* par
 // (sc[0] rho, sc[1] lambda, sc[2] alpha, sc[3] beta, sc[4] rho_new, sc[5] |r|^2,
 //  sc[6] = 1 after a breakdown lambda == 0)
 // ---------------------------------------------------------------------------
-__global__ void k_cg_alpha(double* sc) {
-    if (threadIdx.x != 0) return;
-    const double lambda = sc[1];
-    if (lambda == 0.0) {
-        sc[6] = 1.0;
-        sc[2] = 0.0;
-    } else {
-        sc[2] = sc[0] / lambda;
-    }
-}
-
 __global__ void k_cg_beta(double* sc) {
     if (threadIdx.x != 0) return;
     sc[3] = sc[4] / sc[0];
     sc[0] = sc[4];
 }
 
-// x += alpha p ; r -= alpha q ; partial sums of r.r
-__global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __restrict__ sc,
+// alpha = rho / lambda (0, with the breakdown flag, when lambda == 0), formed
+// by every thread and recorded by the first; x += alpha p ; r -= alpha q ;
+// partial sums of r.r
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, double* __restrict__ sc,
                                                    const double* __restrict__ p, const double* __restrict__ q,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ partial, float* __restrict__ rf) {
-    const double alpha = sc[2];
+    const double lambda = sc[1];
+    const double alpha = lambda == 0.0 ? 0.0 : sc[0] / lambda;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc[2] = alpha;
+        if (lambda == 0.0) sc[6] = 1.0;
+    }
     double acc = 0.0;
     UGX_STREAM_LOOP(n) {
         double xv[kStreamE], pv[kStreamE], rv[kStreamE], qv[kStreamE];
@@ -2101,6 +2097,7 @@ __device__ __forceinline__ void lu_solve_block(int n, const double* __restrict__
     const int g = threadIdx.x & 7, rows = blockDim.x >> 3;
     for (int i = threadIdx.x >> 3; i < n; i += rows) {
         double s = 0.0;
+#pragma unroll 8
         for (int k = g; k <= i; k += 8) s += Linv[(int64_t)i * n + k] * pb[k];
         s += __shfl_xor(s, 1, 8);
         s += __shfl_xor(s, 2, 8);
@@ -2110,6 +2107,7 @@ __device__ __forceinline__ void lu_solve_block(int n, const double* __restrict__
     __syncthreads();
     for (int i = threadIdx.x >> 3; i < n; i += rows) {
         double s = 0.0;
+#pragma unroll 8
         for (int k = i + g; k < n; k += 8) s += Uinv[(int64_t)i * n + k] * t[k];
         s += __shfl_xor(s, 1, 8);
         s += __shfl_xor(s, 2, 8);
@@ -2415,19 +2413,15 @@ int sum_partials(int np, const double* partial, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, np, partial, out, 0);
     UGX_CHECK_LAUNCH();
 }
-int cg_alpha(double* sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(64), 0, s, sc);
-    UGX_CHECK_LAUNCH();
-}
 int cg_beta(double* sc, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, s, sc);
     UGX_CHECK_LAUNCH();
 }
-int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
+int cg_update(int64_t n, double* sc, const double* p, const double* q, double* x, double* r, double* partial,
               hipStream_t s, float* rf) {
     const int g = red_grid(n);
     hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(256), 0, s, n, sc, p, q, x, r, partial, rf);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, (double*)(sc + 5), 0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, sc + 5, 0);
     UGX_CHECK_LAUNCH();
 }
 int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s) {

@@ -41,10 +41,10 @@ int dot_partials(int64_t n);
 // CG with device scalars sc[0..6] = rho, lambda, alpha, beta, rho_new, |r|^2,
 // breakdown flag
 int sum_partials(int np, const double* partial, double* out, hipStream_t s);
-int cg_alpha(double* sc, hipStream_t s);
 int cg_beta(double* sc, hipStream_t s);
-// rf (nullable): also writes (float) r, the FP32 cycle's copy of the defect
-int cg_update(int64_t n, const double* sc, const double* p, const double* q, double* x, double* r, double* partial,
+// alpha = sc[0] / sc[1] into sc[2] (0 and sc[6] = 1 when sc[1] == 0), then the
+// update; rf (nullable): also writes (float) r, the FP32 cycle's copy of the defect
+int cg_update(int64_t n, double* sc, const double* p, const double* q, double* x, double* r, double* partial,
               hipStream_t s,
               float* rf = nullptr);
 int cg_direction(int64_t n, const double* sc, double* p, const double* z, hipStream_t s);
