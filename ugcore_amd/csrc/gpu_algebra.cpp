@@ -841,10 +841,11 @@ bool CG::apply_lattice(GPUVector& x, const GPUVector& src, AssembledMultiGridCyc
         UGX_LAUNCH((permute_to<double, double>(N, gmg.dof_of_lattice(), src.data(), rl, c.stream)));
     }
     if (kind == 2) gmg.surface_axpy_lattice(rl, 1.0, rl, -1.0, xl);   // r = b - A x
-    gmg.apply_lattice(zl, rl, sc + 0, part, rf_first);   // z = B r, rho = (z, r)
+    // p = z = B r, rho = (z, r): the first z is written where p lives (the
+    // loop's z buffer is written afresh before it is read)
+    gmg.apply_lattice(pl, rl, sc + 0, part, rf_first);
     conv_->set_name(name());
     conv_->start_defect(dev_dot(N, rl, rl, true));
-    UGX_HIP(hipMemcpyAsync(pl, zl, sizeof(double) * N, hipMemcpyDeviceToDevice, c.stream));
     // second direction buffer: the fused update writes p_new beside p_old
     if (ws_p2_.size() != N + 2 * H) {
         ws_p2_.resize(N + 2 * H);
