@@ -750,18 +750,26 @@ static double dev_dot(int64_t n, const double* a, const double* b, bool take_sqr
     return reduce_to_host(0);
 }
 
+// number of nonzero entries of a[0, n)
+static double dev_count_nonzero(int64_t n, const double* a) {
+    auto& c = context();
+    UGX_LAUNCH(count_nonzero(n, a, c.red_partial, c.red_out, c.stream));
+    return reduce_to_host(0);
+}
+
 // How the lattice path can start for x and b (one host synchronisation for
-// three sums): 1 = x is zero and b vanishes on the Dirichlet DoFs (r = b);
+// three nonzero counts): 1 = x is zero and b vanishes on the Dirichlet DoFs (r = b);
 // 2 = x and b both vanish there (r = b - A x with the Dirichlet-eliminated
 // lattice operator, which equals A x for such x); 0 = neither.
 int CG::lattice_start_kind(const GPUVector& x, const GPUVector& b, int64_t ni) {
     auto& c = context();
     const int64_t N = x.size(), nt = N - ni;
     if (b.size() != N || nt < 0) return 0;
-    UGX_LAUNCH(dot<double>(N, x.data(), x.data(), c.red_partial, c.red_out + 0, false, c.stream));
+    // nonzero counts (x == 0 exactly, not x.x == 0, which underflows)
+    UGX_LAUNCH(count_nonzero(N, x.data(), c.red_partial, c.red_out + 0, c.stream));
     if (nt > 0) {
-        UGX_LAUNCH(dot<double>(nt, x.data() + ni, x.data() + ni, c.red_partial, c.red_out + 1, false, c.stream));
-        UGX_LAUNCH(dot<double>(nt, b.data() + ni, b.data() + ni, c.red_partial, c.red_out + 2, false, c.stream));
+        UGX_LAUNCH(count_nonzero(nt, x.data() + ni, c.red_partial, c.red_out + 1, c.stream));
+        UGX_LAUNCH(count_nonzero(nt, b.data() + ni, c.red_partial, c.red_out + 2, c.stream));
     } else {
         UGX_LAUNCH(vec_set<double>(2, c.red_out + 1, 0.0, c.stream));
     }
@@ -904,7 +912,7 @@ bool CG::apply_return_defect(GPUVector& x, GPUVector& b) {
     op_->apply_sub(r, x);
     if (gmg) {
         const int64_t ni = gmg->num_inner_dofs();
-        const double tail = r.size() > ni ? dev_dot(r.size() - ni, r.data() + ni, r.data() + ni, false) : 0.0;
+        const double tail = r.size() > ni ? dev_count_nonzero(r.size() - ni, r.data() + ni) : 0.0;
         if (tail == 0.0) {
             used_fast_ = true;
             return apply_lattice(x, r, *gmg, 0, &r);

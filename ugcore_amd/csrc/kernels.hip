@@ -543,6 +543,22 @@ __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const T* __restr
     block_partial(acc, partial);
 }
 
+// partial counts of the nonzero entries (exact in double below 2^53; unlike
+// a.a, entries whose square underflows still count)
+__global__ __launch_bounds__(256) void k_nonzero_partial(int64_t n, const double* __restrict__ a,
+                                                         double* partial) {
+    double acc = 0.0;
+    UGX_STREAM_LOOP(n) {
+        double av[kStreamE];
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e) av[e] = ldnt(a + UGX_STREAM_SAFE(e, n));
+#pragma unroll
+        for (int e = 0; e < kStreamE; ++e)
+            if (UGX_STREAM_IDX(e) < n && av[e] != 0.0) acc += 1.0;
+    }
+    block_partial(acc, partial);
+}
+
 // one workgroup of 1024: thread t sums partial[t], partial[t + 1024], ... in
 // that order (eight loads in flight), then the waves and the 16 wave sums are
 // combined in a fixed order, so the result does not depend on timing
@@ -2406,6 +2422,13 @@ int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool ta
     UGX_CHECK_LAUNCH();
 }
 template int dot<double>(int64_t, const double*, const double*, double*, double*, bool, hipStream_t);
+
+int count_nonzero(int64_t n, const double* a, double* partial, double* out, hipStream_t s) {
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_nonzero_partial, dim3(g), dim3(256), 0, s, n, a, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, g, partial, out, 0);
+    UGX_CHECK_LAUNCH();
+}
 
 int dot_partials(int64_t n) { return red_grid(n); }
 

@@ -35,6 +35,8 @@ int jacobi_diag_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const 
                     int* bad, hipStream_t s);
 template <typename T>
 int dot(int64_t n, const T* a, const T* b, double* partial, double* out, bool take_sqrt, hipStream_t s);
+// out[0] = number of nonzero entries of a[0, n)
+int count_nonzero(int64_t n, const double* a, double* partial, double* out, hipStream_t s);
 // number of partial sums dot() writes for n entries (one per workgroup of its
 // grid, at most 65536)
 int dot_partials(int64_t n);
